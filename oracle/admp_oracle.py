@@ -1,0 +1,958 @@
+"""FP64 CPU restatement of DMFF's ADMP multipolar PME path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the oracle: the HIP product path is
+checked against it.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker or
+as the timed CPU baseline.  The product package never imports it.
+
+It restates, function by function, the reference's JAX code in torch float64 on
+the CPU, so that torch's reverse-mode autodiff takes the place ``jax.grad``
+takes in the reference (forces, virial and parameter gradients are gradients of
+the restated energy, exactly as in the reference).  Every function cites the
+reference file:line it follows.  Pinning of each function (golden vectors,
+closed forms, brute force, finite differences) is listed in
+``PINNING`` at the bottom of this file and exercised by ``tests/test_oracle_*.py``.
+
+Units follow the reference backend: Angstrom, elementary charge, kJ/mol.
+"""
+import math
+
+import numpy as np
+import torch
+
+F64 = torch.float64
+
+# dmff/common/constants.py:3-4
+DIELECTRIC = 1389.35455846
+SQRT_PI = math.sqrt(math.pi)
+# dmff/admp/multipole.py:12-13 (10-digit literal kept on purpose)
+RT3 = 1.73205080757
+INV_RT3 = 1.0 / RT3
+# dmff/admp/settings.py:2-3
+POL_CONV = 1.0
+MAX_N_POL = 30
+# dmff/admp/pme.py:33
+DEFAULT_THOLE_WIDTH = 5.0
+
+ZThenX, Bisector, ZBisect, ThreeFold, ZOnly, NoAxisType = 0, 1, 2, 3, 4, 5
+
+
+def _t(x, dtype=F64):
+    if isinstance(x, torch.Tensor):
+        return x.to(dtype) if x.is_floating_point() else x
+    return torch.as_tensor(np.asarray(x), dtype=dtype)
+
+
+# ---------------------------------------------------------------------------
+# Ewald parameters  (dmff/admp/pme.py:426-492)
+# ---------------------------------------------------------------------------
+def setup_ewald_parameters(rc, ethresh, box=None, spacing=None, method="openmm"):
+    """kappa (1/A) and K1..K3; 'openmm' rule pme.py:460-466, 'gromacs' :467-488."""
+    if method == "openmm":
+        box = np.asarray(box, dtype=np.float64)
+        kappa = math.sqrt(-math.log(2 * ethresh)) / rc
+        K = [int(math.ceil(2 * kappa * box[d, d] / 3 / ethresh ** 0.2)) for d in range(3)]
+        return kappa, K[0], K[1], K[2]
+    if method == "gromacs":
+        from scipy.special import erfc
+        kappa = 5.0
+        i = 0
+        while erfc(kappa * rc) > ethresh:
+            i += 1
+            kappa *= 2
+        low, high = 0.0, kappa
+        for _ in range(i + 60):
+            kappa = (low + high) / 2
+            if erfc(kappa * rc) > ethresh:
+                low = kappa
+            else:
+                high = kappa
+        box = np.asarray(box, dtype=np.float64)
+        K = [int(math.ceil(box[d, d] / spacing)) for d in range(3)]
+        return kappa, K[0], K[1], K[2]
+    raise ValueError(f"Invalid method: {method}.Valid methods: 'openmm', 'gromacs'")
+
+
+# ---------------------------------------------------------------------------
+# Geometry  (dmff/admp/spatial.py)
+# ---------------------------------------------------------------------------
+def pbc_shift(drvecs, box, box_inv):
+    """spatial.py:13-32: single minimum image through fractional rounding."""
+    s = drvecs @ box_inv
+    s = s - torch.floor(s + 0.5)
+    return s @ box
+
+
+def _normalize(v):
+    # spatial.py:36-41
+    return v / torch.linalg.norm(v, dim=-1, keepdim=True)
+
+
+def construct_local_frames(positions, box, axis_types, axis_indices):
+    """spatial.py:44-156.  Returns (N,3,3), rows = local x, y, z axes."""
+    axis_types = np.asarray(axis_types)
+    axis_indices = np.asarray(axis_indices)
+    n = positions.shape[0]
+    box_inv = torch.linalg.inv(box)
+    zf = torch.as_tensor(axis_types == ZOnly).reshape(-1, 1)
+    bis = torch.as_tensor(axis_types == Bisector).reshape(-1, 1)
+    zbi = torch.as_tensor(axis_types == ZBisect).reshape(-1, 1)
+    thr = torch.as_tensor(axis_types == ThreeFold).reshape(-1, 1)
+    nat = torch.as_tensor(axis_types == NoAxisType).reshape(-1, 1)
+    idx = torch.arange(n)
+    # index -1 would gather the last atom; use self for rows whose result is discarded
+    za = torch.where(torch.as_tensor(axis_indices[:, 0] >= 0), torch.as_tensor(axis_indices[:, 0]), idx)
+    xa = torch.where(torch.as_tensor(axis_indices[:, 1] >= 0), torch.as_tensor(axis_indices[:, 1]), idx)
+    ya = torch.where(torch.as_tensor(axis_indices[:, 2] >= 0), torch.as_tensor(axis_indices[:, 2]), idx)
+    e_x = torch.tensor([1.0, 0.0, 0.0], dtype=F64)
+    e_y = torch.tensor([0.0, 1.0, 0.0], dtype=F64)
+    e_z = torch.tensor([0.0, 0.0, 1.0], dtype=F64)
+
+    def safe_norm(v, mask):
+        # rows outside mask may be zero vectors: keep them finite (their value is discarded)
+        v = torch.where(mask, v, e_x.expand_as(v))
+        return _normalize(v)
+
+    ok_z = ~nat
+    vec_z = pbc_shift(positions[za] - positions, box, box_inv)
+    vec_z = safe_norm(vec_z, ok_z)
+    # Z-only: x = (1-round|z_x|, round|z_x|, 0)  (spatial.py:99-102)
+    xz = torch.round(torch.abs(vec_z[:, 0:1])).detach()
+    vec_x_zonly = torch.cat([1.0 - xz, xz, torch.zeros_like(xz)], dim=1)
+    ok_x = ~(zf | nat)
+    vec_x_raw = pbc_shift(positions[xa] - positions, box, box_inv)
+    vec_x = torch.where(ok_x, safe_norm(vec_x_raw, ok_x), vec_x_zonly)
+    # Bisector (spatial.py:108-111)
+    vec_z = torch.where(bis, safe_norm(vec_z + vec_x, bis), vec_z)
+    # Z-bisect (spatial.py:113-118)
+    vy_raw = pbc_shift(positions[ya] - positions, box, box_inv)
+    yzb = zbi | thr
+    vec_y_n = safe_norm(vy_raw, yzb)
+    vec_x = torch.where(zbi, safe_norm(vec_x + vec_y_n, zbi), vec_x)
+    # Three-fold (spatial.py:120-132)
+    vec_z = torch.where(thr, safe_norm(vec_z + vec_x + vec_y_n, thr), vec_z)
+    # Gram-Schmidt and y = z x x (spatial.py:134-139)
+    proj = torch.sum(vec_x * vec_z, dim=1, keepdim=True)
+    vec_x = _normalize(vec_x - vec_z * proj)
+    vec_y = torch.cross(vec_z, vec_x, dim=1)
+    # NoAxisType (spatial.py:141-145)
+    vec_x = torch.where(nat, e_x.expand_as(vec_x), vec_x)
+    vec_y = torch.where(nat, e_y.expand_as(vec_y), vec_y)
+    vec_z = torch.where(nat, e_z.expand_as(vec_z), vec_z)
+    return torch.stack((vec_x, vec_y, vec_z), dim=1)
+
+
+def build_quasi_internal(r1, r2, dr, norm_dr):
+    """spatial.py:158-187: z from r2 to r1; x from (1,0,0) unless r1,r2 share y and z."""
+    vz = dr / norm_dr.reshape(-1, 1)
+    cond = ((r1[:, 1] != r2[:, 1]) | (r1[:, 2] != r2[:, 2])).reshape(-1, 1)
+    ex = torch.tensor([1.0, 0.0, 0.0], dtype=F64)
+    ey = torch.tensor([0.0, 1.0, 0.0], dtype=F64)
+    vx = torch.where(cond, vz + ex, vz + ey)
+    dot = torch.sum(vz * vx, dim=1, keepdim=True)
+    vx = vx - vz * dot
+    vx = vx / torch.linalg.norm(vx, dim=1, keepdim=True)
+    vy = torch.cross(vz, vx, dim=1)
+    return torch.stack([vx, vy, vz], dim=1)
+
+
+# ---------------------------------------------------------------------------
+# Multipole conversions and rotations  (dmff/admp/multipole.py)
+# ---------------------------------------------------------------------------
+C1_h2c = torch.tensor([[0, 1, 0], [0, 0, 1], [1, 0, 0]], dtype=F64)   # multipole.py:15-17
+C1_c2h = C1_h2c.T.contiguous()                                         # :18
+C2_c2h = torch.tensor([                                                # :20-24
+    [0, 0, 1, 0, 0, 0],
+    [0, 0, 0, 0, 2 * INV_RT3, 0],
+    [0, 0, 0, 0, 0, 2 * INV_RT3],
+    [INV_RT3, -INV_RT3, 0, 0, 0, 0],
+    [0, 0, 0, 2 * INV_RT3, 0, 0]], dtype=F64)
+C2_h2c = torch.tensor([                                                # :26-31
+    [-0.5, 0, 0, RT3 / 2, 0],
+    [-0.5, 0, 0, -RT3 / 2, 0],
+    [1, 0, 0, 0, 0],
+    [0, 0, 0, 0, RT3 / 2],
+    [0, RT3 / 2, 0, 0, 0],
+    [0, 0, RT3 / 2, 0, 0]], dtype=F64)
+
+
+def convert_cart2harm(theta, lmax):
+    """multipole.py:34-74.  theta (n, 1|4|10) Cartesian (xx,yy,zz,xy,xz,yz)."""
+    if lmax > 2:
+        raise ValueError("l > 2 (beyond quadrupole) not supported")
+    out = [theta[:, 0:1]]
+    if lmax >= 1:
+        out.append(theta[:, 1:4] @ C1_c2h.T)
+    if lmax >= 2:
+        out.append(theta[:, 4:10] @ C2_c2h.T)
+    return torch.cat(out, dim=1)
+
+
+def convert_harm2cart(Q, lmax):
+    """multipole.py:77-114."""
+    if lmax > 2:
+        raise ValueError("l > 2 (beyond quadrupole) not supported")
+    out = [Q[:, 0:1]]
+    if lmax >= 1:
+        out.append(Q[:, 1:4] @ C1_h2c.T)
+    if lmax >= 2:
+        out.append(Q[:, 4:9] @ C2_h2c.T)
+    return torch.cat(out, dim=1)
+
+
+_ZXY = [2, 0, 1]
+
+
+def rot_ind_global2local(U_g, frames):
+    """multipole.py:117-126: dipole (harmonic order z,x,y) rotation."""
+    R1 = frames[:, _ZXY][:, :, _ZXY]
+    return torch.einsum("nij,nj->ni", R1, U_g)
+
+
+def rot_global2local(Q_gh, frames, lmax=2):
+    """multipole.py:129-214: rotate harmonic multipoles global -> local."""
+    if lmax > 2:
+        raise NotImplementedError("l > 2 (beyond quadrupole) not supported")
+    out = [Q_gh[:, 0:1]]
+    if lmax >= 1:
+        R1 = frames[:, _ZXY][:, :, _ZXY]
+        out.append(torch.einsum("nij,nj->ni", R1, Q_gh[:, 1:4]))
+    if lmax >= 2:
+        xx, xy, xz = frames[:, 0, 0], frames[:, 0, 1], frames[:, 0, 2]
+        yx, yy, yz = frames[:, 1, 0], frames[:, 1, 1], frames[:, 1, 2]
+        zx, zy, zz = frames[:, 2, 0], frames[:, 2, 1], frames[:, 2, 2]
+        c = {}
+        c[0, 0] = (3 * zz ** 2 - 1) / 2
+        c[0, 1] = RT3 * zx * zz
+        c[0, 2] = RT3 * zy * zz
+        c[0, 3] = (RT3 * (-2 * zy ** 2 - zz ** 2 + 1)) / 2
+        c[0, 4] = RT3 * zx * zy
+        c[1, 0] = RT3 * xz * zz
+        c[1, 1] = 2 * xx * zz - yy
+        c[1, 2] = yx + 2 * xy * zz
+        c[1, 3] = -2 * xy * zy - xz * zz
+        c[1, 4] = xx * zy + zx * xy
+        c[2, 0] = RT3 * yz * zz
+        c[2, 1] = 2 * yx * zz + xy
+        c[2, 2] = -xx + 2 * yy * zz
+        c[2, 3] = -2 * yy * zy - yz * zz
+        c[2, 4] = yx * zy + zx * yy
+        c[3, 0] = RT3 * (-2 * yz ** 2 - zz ** 2 + 1) / 2
+        c[3, 1] = -2 * yx * yz - zx * zz
+        c[3, 2] = -2 * yy * yz - zy * zz
+        c[3, 3] = (4 * yy ** 2 + 2 * zy ** 2 + 2 * yz ** 2 + zz ** 2 - 3) / 2
+        c[3, 4] = -2 * yx * yy - zx * zy
+        c[4, 0] = RT3 * xz * yz
+        c[4, 1] = xx * yz + yx * xz
+        c[4, 2] = xy * yz + yy * xz
+        c[4, 3] = -2 * xy * yy - xz * yz
+        c[4, 4] = xx * yy + yx * xy
+        # the literal table at :202-210 lists c[k, j] in row j; after .swapaxes(0,1)
+        # C2_gl[j, k] = c[j, k], and Q_l2 = C2_gl @ Q_g2 (:211)
+        q2 = Q_gh[:, 4:9]
+        rows = []
+        for j in range(5):
+            rows.append(sum(c[j, k] * q2[:, k] for k in range(5)))
+        out.append(torch.stack(rows, dim=1))
+    return torch.cat(out, dim=1)
+
+
+def rot_local2global(Q_lh, frames, lmax=2):
+    """multipole.py:218-237: global2local with transposed frames."""
+    return rot_global2local(Q_lh, frames.transpose(-2, -1), lmax)
+
+
+# ---------------------------------------------------------------------------
+# Real space  (dmff/admp/pme.py:632-1180)
+# ---------------------------------------------------------------------------
+def _ewald_common(dr, kappa):
+    rinv = 1.0 / dr
+    rInvVec = [DIELECTRIC * (rinv + 1e-16) ** i for i in range(9)]
+    aR = [(kappa * dr + 1e-16) ** i for i in range(10)]
+    X = 2 * torch.exp(-aR[2]) / SQRT_PI
+    tmp = aR[1]
+    dfac = 1
+    fac = 1
+    b = [None] * 6
+    b[1] = -torch.erf(aR[1])
+    for i in range(2, 6):
+        b[i] = b[i - 1] + tmp * X / dfac
+        fac += 2
+        dfac *= fac
+        tmp = tmp * 2 * aR[2]
+    return rInvVec, aR, X, b
+
+
+def calc_e_perm(dr, mscales, kappa, lmax=2):
+    """pme.py:632-710."""
+    rInvVec, aR, X, b = _ewald_common(dr, kappa)
+    m = mscales
+    cc = rInvVec[1] * (m + b[2] - aR[1] * X)
+    z = torch.zeros_like(dr)
+    cd = dd_m0 = dd_m1 = cq = dq_m0 = dq_m1 = qq_m0 = qq_m1 = qq_m2 = z
+    if lmax >= 1:
+        cd = rInvVec[2] * (m + b[2])
+        dd_m0 = -2 / 3 * rInvVec[3] * (3 * (m + b[3]) + aR[3] * X)
+        dd_m1 = rInvVec[3] * (m + b[3] - (2 / 3) * aR[3] * X)
+    if lmax >= 2:
+        cq = (m + b[3]) * rInvVec[3]
+        dq_m0 = rInvVec[4] * (3 * (m + b[3]) + (4 / 3) * aR[5] * X)
+        dq_m1 = -math.sqrt(3) * rInvVec[4] * (m + b[3])
+        qq_m0 = rInvVec[5] * (6 * (m + b[4]) + (4 / 45) * (-3 + 10 * aR[2]) * aR[5] * X)
+        qq_m1 = -(4 / 15) * rInvVec[5] * (15 * (m + b[4]) + aR[5] * X)
+        qq_m2 = rInvVec[5] * (m + b[4] - (4 / 15) * aR[5] * X)
+    return cc, cd, dd_m0, dd_m1, cq, dq_m0, dq_m1, qq_m0, qq_m1, qq_m2
+
+
+def trim_val_0(x, thresh=1e-8):
+    """pme.py:727-741: x < thresh -> thresh."""
+    return torch.where(x < thresh, torch.full_like(x, thresh), x)
+
+
+def trim_val_infty(x, thresh=1e8):
+    """pme.py:744-759: x >= thresh -> thresh."""
+    return torch.where(x < thresh, x, torch.full_like(x, thresh))
+
+
+def calc_e_ind(dr, thole1, thole2, dmp, pscales, dscales, kappa, lmax=2):
+    """pme.py:765-879 (Thole widths :794-795, trims :797-811)."""
+    w = (pscales > 0).to(F64)        # jnp.heaviside(pscales, 0)
+    a = w * DEFAULT_THOLE_WIDTH + (1 - w) * (thole1 + thole2)
+    dmp = trim_val_0(dmp)
+    u = trim_val_infty(dr / dmp)
+    au = a * u
+    expau = torch.where(au < 50, torch.exp(-torch.clamp(au, max=50.0)), torch.zeros_like(au))
+    au2 = trim_val_infty(au * au)
+    au3 = trim_val_infty(au2 * au)
+    au4 = trim_val_infty(au3 * au)
+    thole_c = 1.0 - expau * (1.0 + au + 0.5 * au2)
+    thole_d0 = 1.0 - expau * (1.0 + au + 0.5 * au2 + au3 / 4.0)
+    thole_d1 = 1.0 - expau * (1.0 + au + 0.5 * au2)
+    thole_q0 = 1.0 - expau * (1.0 + au + 0.5 * au2 + au3 / 6.0 + au4 / 18.0)
+    thole_q1 = 1.0 - expau * (1.0 + au + 0.5 * au2 + au3 / 6.0)
+    rInvVec, aR, X, b = _ewald_common(dr, kappa)
+    p, d = pscales, dscales
+    cud = 2.0 * rInvVec[2] * (p * thole_c + b[2])
+    z = torch.zeros_like(dr)
+    dud_m0 = dud_m1 = udq_m0 = udq_m1 = z
+    if lmax >= 1:
+        dud_m0 = -2.0 * 2.0 / 3.0 * rInvVec[3] * (3.0 * (p * thole_d0 + b[3]) + aR[3] * X)
+        dud_m1 = 2.0 * rInvVec[3] * (p * thole_d1 + b[3] - 2.0 / 3.0 * aR[3] * X)
+    if lmax >= 2:
+        udq_m0 = 2.0 * rInvVec[4] * (3.0 * (p * thole_q0 + b[3]) + 4 / 3 * aR[5] * X)
+        udq_m1 = -2.0 * math.sqrt(3) * rInvVec[4] * (p * thole_q1 + b[3])
+    udud_m0 = -2.0 / 3.0 * rInvVec[3] * (3.0 * (d * thole_d0 + b[3]) + aR[3] * X)
+    udud_m1 = rInvVec[3] * (d * thole_d1 + b[3] - 2.0 / 3.0 * aR[3] * X)
+    return cud, dud_m0, dud_m1, udq_m0, udq_m1, udud_m0, udud_m1
+
+
+def pme_real_kernel(dr, qiQI, qiQJ, qiUindI, qiUindJ, thole1, thole2, dmp,
+                    mscales, pscales, dscales, kappa, lmax=2, lpol=False):
+    """pme.py:882-1048, vectorised over pairs."""
+    cc, cd, dd_m0, dd_m1, cq, dq_m0, dq_m1, qq_m0, qq_m1, qq_m2 = calc_e_perm(dr, mscales, kappa, lmax)
+    if lpol:
+        cud, dud_m0, dud_m1, udq_m0, udq_m1, udud_m0, udud_m1 = calc_e_ind(
+            dr, thole1, thole2, dmp, pscales, dscales, kappa, lmax)
+    QI, QJ = qiQI, qiQJ
+    Vij = [None] * 9
+    Vji = [None] * 9
+    Vij[0] = cc * QI[:, 0]
+    Vji[0] = cc * QJ[:, 0]
+    if lpol:
+        UI, UJ = qiUindI, qiUindJ
+        Vij[0] = Vij[0] - cud * UI[:, 0]
+        Vji[0] = Vji[0] + cud * UJ[:, 0]
+    if lmax >= 1:
+        Vij[0] = Vij[0] - cd * QI[:, 1]
+        Vji[1] = -cd * QJ[:, 0]
+        Vij[1] = cd * QI[:, 0]
+        Vji[0] = Vji[0] + cd * QJ[:, 1]
+        Vij[1] = Vij[1] + dd_m0 * QI[:, 1]
+        Vji[1] = Vji[1] + dd_m0 * QJ[:, 1]
+        Vij[2] = dd_m1 * QI[:, 2]
+        Vji[2] = dd_m1 * QJ[:, 2]
+        Vij[3] = dd_m1 * QI[:, 3]
+        Vji[3] = dd_m1 * QJ[:, 3]
+        if lpol:
+            Vij[1] = Vij[1] + dud_m0 * UI[:, 0]
+            Vji[1] = Vji[1] + dud_m0 * UJ[:, 0]
+            Vij[2] = Vij[2] + dud_m1 * UI[:, 1]
+            Vji[2] = Vji[2] + dud_m1 * UJ[:, 1]
+            Vij[3] = Vij[3] + dud_m1 * UI[:, 2]
+            Vji[3] = Vji[3] + dud_m1 * UJ[:, 2]
+    if lmax >= 2:
+        Vij[0] = Vij[0] + cq * QI[:, 4]
+        Vji[4] = cq * QJ[:, 0]
+        Vij[4] = cq * QI[:, 0]
+        Vji[0] = Vji[0] + cq * QJ[:, 4]
+        Vij[1] = Vij[1] + dq_m0 * QI[:, 4]
+        Vji[4] = Vji[4] + dq_m0 * QJ[:, 1]
+        Vij[4] = Vij[4] - dq_m0 * QI[:, 1]
+        Vji[1] = Vji[1] - dq_m0 * QJ[:, 4]
+        Vij[2] = Vij[2] + dq_m1 * QI[:, 5]
+        Vji[5] = dq_m1 * QJ[:, 2]
+        Vij[3] = Vij[3] + dq_m1 * QI[:, 6]
+        Vji[6] = dq_m1 * QJ[:, 3]
+        Vij[5] = -(dq_m1 * QI[:, 2])
+        Vji[2] = Vji[2] - dq_m1 * QJ[:, 5]
+        Vij[6] = -(dq_m1 * QI[:, 3])
+        Vji[3] = Vji[3] - dq_m1 * QJ[:, 6]
+        Vij[4] = Vij[4] + qq_m0 * QI[:, 4]
+        Vji[4] = Vji[4] + qq_m0 * QJ[:, 4]
+        Vij[5] = Vij[5] + qq_m1 * QI[:, 5]
+        Vji[5] = Vji[5] + qq_m1 * QJ[:, 5]
+        Vij[6] = Vij[6] + qq_m1 * QI[:, 6]
+        Vji[6] = Vji[6] + qq_m1 * QJ[:, 6]
+        Vij[7] = qq_m2 * QI[:, 7]
+        Vji[7] = qq_m2 * QJ[:, 7]
+        Vij[8] = qq_m2 * QI[:, 8]
+        Vji[8] = qq_m2 * QJ[:, 8]
+        if lpol:
+            Vji[4] = Vji[4] + udq_m0 * UJ[:, 0]
+            Vij[4] = Vij[4] - udq_m0 * UI[:, 0]
+            Vji[5] = Vji[5] + udq_m1 * UJ[:, 1]
+            Vji[6] = Vji[6] + udq_m1 * UJ[:, 2]
+            Vij[5] = Vij[5] - udq_m1 * UI[:, 1]
+            Vij[6] = Vij[6] - udq_m1 * UI[:, 2]
+    nh = (lmax + 1) ** 2
+    Vij = torch.stack(Vij[:nh], dim=1)
+    Vji = torch.stack(Vji[:nh], dim=1)
+    e = 0.5 * (torch.sum(QJ * Vij, dim=1) + torch.sum(QI * Vji, dim=1))
+    if lpol:
+        Vijdd = torch.stack((udud_m0 * UI[:, 0], udud_m1 * UI[:, 1], udud_m1 * UI[:, 2]), dim=1)
+        Vjidd = torch.stack((udud_m0 * UJ[:, 0], udud_m1 * UJ[:, 1], udud_m1 * UJ[:, 2]), dim=1)
+        e = e + 0.5 * (torch.sum(UJ * Vijdd, dim=1) + torch.sum(UI * Vjidd, dim=1))
+    return e
+
+
+def regularize_pairs(p):
+    """dmff/utils.py:65-74."""
+    dp = p[:, 1] - p[:, 0]
+    flag = (dp <= 0).to(p.dtype)
+    return torch.stack((p[:, 0] - flag, p[:, 1] - 2 * flag), dim=1)
+
+
+def pair_buffer_scales(p):
+    """dmff/utils.py:77-84."""
+    return ((p[:, 0] - p[:, 1]) < 0).to(F64)
+
+
+def _live_pairs(pairs):
+    """Rows the reference keeps (buffer scale 1): i < j.  Masked rows contribute 0
+    in the reference (utils.py:65-84 + pme.py:1158-1170); they are dropped here
+    so that a masked row can never inject a NaN."""
+    pairs = torch.as_tensor(np.asarray(pairs), dtype=torch.int64)
+    keep = pairs[:, 0] < pairs[:, 1]
+    return pairs[keep]
+
+
+def get_pair_dmp(pol1, pol2):
+    """pme.py:1176-1180."""
+    return (pol1 * pol2 + 1e-50) ** (1 / 6)
+
+
+def pme_real(positions, box, pairs, Q_global, Uind_global, pol, tholes,
+             mScales, pScales, dScales, kappa, lmax, lpol):
+    """pme.py:1052-1173.  Uind_global in harmonic order (as energy_pme passes it)."""
+    pairs = _live_pairs(pairs)
+    box_inv = torch.linalg.inv(box)
+    i, j, nb = pairs[:, 0], pairs[:, 1], pairs[:, 2]
+    r1, r2 = positions[i], positions[j]
+    ind = nb - 1
+    mscales = _t(mScales)[ind]
+    if lpol:
+        pscales = _t(pScales)[ind]
+        dscales = _t(dScales)[ind]
+        thole1, thole2 = tholes[i], tholes[j]
+        dmp = get_pair_dmp(pol[i], pol[j])
+        UI, UJ = Uind_global[i], Uind_global[j]
+    else:
+        pscales = dscales = thole1 = thole2 = dmp = None
+    dr = pbc_shift(r1 - r2, box, box_inv)
+    norm_dr = torch.linalg.norm(dr + 1e-64, dim=-1)
+    Ri = build_quasi_internal(r1, r2, dr, norm_dr)
+    qiQI = rot_global2local(Q_global[i], Ri, lmax)
+    qiQJ = rot_global2local(Q_global[j], Ri, lmax)
+    if lpol:
+        qiUI = rot_ind_global2local(UI, Ri)
+        qiUJ = rot_ind_global2local(UJ, Ri)
+    else:
+        qiUI = qiUJ = None
+    e = pme_real_kernel(norm_dr, qiQI, qiQJ, qiUI, qiUJ, thole1, thole2, dmp,
+                        mscales, pscales, dscales, kappa, lmax, lpol)
+    return torch.sum(e)
+
+
+def pme_self(Q_h, kappa, lmax=2):
+    """pme.py:1183-1222."""
+    nh = (lmax + 1) ** 2
+    l_list = np.array([0] + [1] * 3 + [2] * 5)[:nh]
+    l_fac2 = np.array([1] + [3] * 3 + [15] * 5)[:nh]
+    factor = kappa / np.sqrt(np.pi) * (2 * kappa ** 2) ** l_list / l_fac2
+    return -torch.sum(_t(factor)[None, :] * Q_h ** 2) * DIELECTRIC
+
+
+def pol_penalty(U_ind, pol):
+    """pme.py:1225-1240."""
+    pol_pi = trim_val_0(pol)
+    return torch.sum(0.5 / pol_pi * ((U_ind + 1e-16) ** 2).sum(dim=1)) * DIELECTRIC
+
+
+# ---------------------------------------------------------------------------
+# Reciprocal space  (dmff/admp/recip.py)
+# ---------------------------------------------------------------------------
+def _bspline6(u):
+    """recip.py:74-112: cardinal B-spline of order 6 (piecewise quintics)."""
+    u2, u3, u4, u5 = u ** 2, u ** 3, u ** 4, u ** 5
+    l1, l2, l3 = (u - 1) ** 5, (u - 2) ** 5, (u - 3) ** 5
+    outs = [u5 / 120,
+            u5 / 120 - l1 / 20,
+            u5 / 120 + l2 / 8 - l1 / 20,
+            u5 / 120 - l3 / 6 + l2 / 8 - l1 / 20,
+            u5 / 24 - u4 + 19 * u3 / 2 - 89 * u2 / 2 + 409 * u / 4 - 1829 / 20,
+            -u5 / 120 + u4 / 4 - 3 * u3 + 18 * u2 - 54 * u + 324 / 5]
+    return _piecewise6(u, outs)
+
+
+def _bspline6_p(u):
+    """recip.py:114-145: first derivative."""
+    u2, u3, u4 = u ** 2, u ** 3, u ** 4
+    l1, l2 = (u - 1) ** 4, (u - 2) ** 4
+    outs = [u4 / 24,
+            u4 / 24 - l1 / 4,
+            u4 / 24 + 5 * l2 / 8 - l1 / 4,
+            -5 * u4 / 12 + 6 * u3 - 63 * u2 / 2 + 71 * u - 231 / 4,
+            5 * u4 / 24 - 4 * u3 + 57 * u2 / 2 - 89 * u + 409 / 4,
+            -u4 / 24 + u3 - 9 * u2 + 36 * u - 54]
+    return _piecewise6(u, outs)
+
+
+def _bspline6_p2(u):
+    """recip.py:147-177: second derivative."""
+    u2, u3 = u ** 2, u ** 3
+    outs = [u3 / 6,
+            u3 / 6 - (u - 1) ** 3,
+            5 * u3 / 3 - 12 * u2 + 27 * u - 19,
+            -5 * u3 / 3 + 18 * u2 - 63 * u + 71,
+            5 * u3 / 6 - 12 * u2 + 57 * u - 89,
+            -u3 / 6 + 3 * u2 - 18 * u + 36]
+    return _piecewise6(u, outs)
+
+
+def _piecewise6(u, outs):
+    total = torch.zeros_like(u)
+    for k, o in enumerate(outs):
+        cond = ((u >= k) & (u < k + 1)).to(u.dtype)
+        total = total + cond * o
+    return total
+
+
+def _shifts():
+    r = np.arange(-3, 3)
+    g = np.array(np.meshgrid(r, r, r)).T.reshape(-1, 3)   # recip.py:15-17
+    return torch.as_tensor(g, dtype=torch.int64)
+
+
+def _kpts_int(K):
+    """recip.py:371-381 (FFT order)."""
+    ks = []
+    for k in K:
+        a = np.arange(-(k - 1) // 2, (k + 1) // 2)
+        ks.append(np.roll(a, -(k - 1) // 2))
+    g = np.meshgrid(*ks, indexing="ij")
+    return torch.as_tensor(np.stack([x.flatten() for x in g], axis=1), dtype=F64)
+
+
+def Ck_1(ksq, kappa, V):
+    """recip.py:474-475."""
+    return 2 * math.pi / V / ksq * torch.exp(-ksq / 4 / kappa ** 2)
+
+
+def _erfc(x):
+    return torch.special.erfc(x)
+
+
+def Ck_6(ksq, kappa, V):
+    """recip.py:477-486."""
+    ksq = torch.where(ksq < 1e-16, torch.full_like(ksq, 1e-16), ksq)
+    x2 = ksq / 4 / kappa ** 2
+    x = torch.sqrt(x2)
+    x3 = x2 * x
+    f = (1 - 2 * x2) * torch.exp(-x2) + 2 * x3 * SQRT_PI * _erfc(x)
+    return SQRT_PI * math.pi / 2 / V * kappa ** 3 * f / 3
+
+
+def Ck_8(ksq, kappa, V):
+    """recip.py:488-497."""
+    ksq = torch.where(ksq < 1e-16, torch.full_like(ksq, 1e-16), ksq)
+    x2 = ksq / 4 / kappa ** 2
+    x = torch.sqrt(x2)
+    x4 = x2 * x2
+    x5 = x4 * x
+    f = (3 - 2 * x2 + 4 * x4) * torch.exp(-x2) - 4 * x5 * SQRT_PI * _erfc(x)
+    return SQRT_PI * math.pi / 2 / V * kappa ** 5 * f / 45
+
+
+def Ck_10(ksq, kappa, V):
+    """recip.py:499-508."""
+    ksq = torch.where(ksq < 1e-16, torch.full_like(ksq, 1e-16), ksq)
+    x2 = ksq / 4 / kappa ** 2
+    x = torch.sqrt(x2)
+    x4 = x2 * x2
+    x6 = x4 * x2
+    x7 = x6 * x
+    f = (15 - 6 * x2 + 4 * x4 - 8 * x6) * torch.exp(-x2) + 8 * x7 * SQRT_PI * _erfc(x)
+    return SQRT_PI * math.pi / 2 / V * kappa ** 7 * f / 1260
+
+
+def spread_Q(positions, box, Q, K, lmax):
+    """recip.py:30-71, 179-368: B-spline weights, harmonic gradient operators, scatter-add."""
+    N = torch.tensor(K, dtype=F64)
+    A = (N.reshape(1, 3) * torch.linalg.inv(box)).T          # recip.py:46
+    R = positions @ A.T                                       # einsum("ij,kj->ki") :67
+    m_u0 = torch.ceil(R).detach()
+    u0 = (m_u0 - R) + 3.0
+    sh = _shifts()
+    na = positions.shape[0]
+    u = (u0[:, None, :] + sh[None].to(F64)).reshape(na * 216, 3)
+    M = _bspline6(u)
+    theta = M.prod(dim=-1)
+    harms = [theta]
+    if lmax >= 1:
+        Mp = _bspline6_p(u)
+        div = torch.stack([Mp[:, 0] * M[:, 1] * M[:, 2],
+                           Mp[:, 1] * M[:, 2] * M[:, 0],
+                           Mp[:, 2] * M[:, 0] * M[:, 1]], dim=1)
+        tp = torch.einsum("ij,kj->ki", -A, div)            # recip.py:207
+        harms += [tp[:, 2], tp[:, 0], tp[:, 1]]
+    if lmax >= 2:
+        Mpp = _bspline6_p2(u)
+        d00 = Mpp[:, 0] * M[:, 1] * M[:, 2]
+        d11 = Mpp[:, 1] * M[:, 0] * M[:, 2]
+        d22 = Mpp[:, 2] * M[:, 0] * M[:, 1]
+        d01 = Mp[:, 0] * Mp[:, 1] * M[:, 2]
+        d02 = Mp[:, 0] * Mp[:, 2] * M[:, 1]
+        d12 = Mp[:, 1] * Mp[:, 2] * M[:, 0]
+        div2 = torch.stack([torch.stack([d00, d01, d02], 1),
+                            torch.stack([d01, d11, d12], 1),
+                            torch.stack([d02, d12, d22], 1)], 1)   # (n,3,3)
+        t2 = torch.einsum("im,jn,kmn->kij", -A, -A, div2)      # recip.py:251
+        rt3 = math.sqrt(3)
+        tr = t2[:, 0, 0] + t2[:, 1, 1] + t2[:, 2, 2]
+        harms += [(3 * t2[:, 2, 2] - tr) / 2, rt3 * t2[:, 0, 2], rt3 * t2[:, 1, 2],
+                  rt3 / 2 * (t2[:, 0, 0] - t2[:, 1, 1]), rt3 * t2[:, 0, 1]]
+    nh = (lmax + 1) ** 2
+    sph = torch.stack(harms, dim=1).reshape(na, 216, nh)
+    Qd = [Q[:, 0:1]]
+    if lmax >= 1:
+        Qd.append(Q[:, 1:4])
+    if lmax >= 2:
+        Qd.append(Q[:, 4:9] / 3)                                # recip.py:344
+    Qd = torch.cat(Qd, dim=1)
+    per = torch.sum(Qd[:, None, :] * sph, dim=2)               # (na,216)
+    Kt = torch.tensor(K, dtype=torch.int64)
+    idx = torch.remainder(m_u0.to(torch.int64)[:, None, :] + sh[None], Kt)   # recip.py:363
+    flat = (idx[..., 0] * K[1] + idx[..., 1]) * K[2] + idx[..., 2]
+    mesh = torch.zeros(K[0] * K[1] * K[2], dtype=F64)
+    mesh = mesh.index_add(0, flat.reshape(-1), per.reshape(-1))
+    return mesh.reshape(K)
+
+
+def theta_k(K):
+    """recip.py:440-448: Euler-spline modulus product over the 3 dimensions."""
+    kint = _kpts_int(K)
+    m = torch.linspace(-2, 2, 5, dtype=F64).reshape(5, 1, 1)
+    Kt = torch.tensor(K, dtype=F64)
+    bs = _bspline6(m + 3.0)
+    return torch.prod(torch.sum(bs * torch.cos(2 * math.pi * m * kint[None] / Kt), dim=0), dim=1)
+
+
+def pme_recip(positions, box, Q, kappa, K, lmax, Ck_fn=Ck_1, gamma=False):
+    """recip.py:24-466 (generate_pme_recip's closure)."""
+    if K[0] == K[1] == K[2] == 0:
+        return torch.zeros((), dtype=F64)
+    mesh = spread_Q(positions, box, Q, K, lmax)
+    kint = _kpts_int(K)
+    kpts = 2 * math.pi * kint @ torch.linalg.inv(box).T
+    ksq = torch.sum(kpts ** 2, dim=1)
+    th = theta_k(K)
+    V = torch.linalg.det(box)
+    S = torch.fft.fftn(mesh).flatten()
+    if not gamma:
+        C = Ck_fn(ksq[1:], kappa, V)
+        Ek = C * torch.abs(S[1:] / th[1:]) ** 2
+        return torch.sum(Ek) * DIELECTRIC
+    C = Ck_fn(ksq, kappa, V)
+    return torch.sum(C * torch.abs(S / th) ** 2)
+
+
+# ---------------------------------------------------------------------------
+# Top level  (dmff/admp/pme.py:36-628)
+# ---------------------------------------------------------------------------
+def energy_pme(positions, box, pairs, Q_local, Uind_global, pol, tholes,
+               mScales, pScales, dScales, axis_types, axis_indices,
+               kappa, K, lmax, lpol, lpme=True):
+    """pme.py:496-628.  Uind_global is global Cartesian (N,3)."""
+    if lmax > 0:
+        frames = construct_local_frames(positions, box, axis_types, axis_indices)
+        Q_global = rot_local2global(Q_local, frames, lmax)
+    else:
+        if lpol:
+            Q_global = torch.cat((Q_local, torch.zeros((Q_local.shape[0], 3), dtype=F64)), dim=1)
+            lmax = 1
+        else:
+            Q_global = Q_local
+    if lpol:
+        U_ind = Uind_global @ C1_c2h.T                # pme.py:573
+        Q_tot = torch.cat((Q_global[:, 0:1], Q_global[:, 1:4] + U_ind, Q_global[:, 4:]), dim=1)
+    else:
+        U_ind = None
+        Q_tot = Q_global
+    if not lpme:
+        kappa = 0.0
+    e_real = pme_real(positions, box, pairs, Q_global, U_ind, pol, tholes,
+                      mScales, pScales, dScales, kappa, lmax, lpol)
+    if lpme:
+        e_recip = pme_recip(positions, box, Q_tot, kappa, K, lmax)
+        e_self = pme_self(Q_tot, kappa, lmax)
+        if lpol:
+            e_self = e_self + pol_penalty(U_ind, pol)
+        return e_real + e_recip + e_self
+    e_self = pol_penalty(U_ind, pol) if lpol else 0.0
+    return e_real + e_self
+
+
+class ADMPPmeForce:
+    """Oracle counterpart of pme.py:36-423 (same constructor arguments)."""
+
+    def __init__(self, box, axis_type, axis_indices, rc, ethresh, lmax,
+                 lpol=False, lpme=True, steps_pol=None):
+        self.axis_type = np.asarray(axis_type)
+        self.axis_indices = np.asarray(axis_indices) if axis_indices is not None else np.full((len(axis_type), 3), -1)
+        self.rc, self.ethresh, self.lmax = rc, ethresh, int(lmax)
+        self.lpol, self.lpme, self.steps_pol = lpol, lpme, steps_pol
+        if lpme:
+            self.kappa, K1, K2, K3 = setup_ewald_parameters(rc, ethresh, np.asarray(box))
+            self.K = (K1, K2, K3)
+        else:
+            self.kappa, self.K = 0.0, (0, 0, 0)
+        self.n_atoms = len(axis_type)
+
+    def energy_fn(self, positions, box, pairs, Q_local, U, pol, tholes, mS, pS, dS):
+        lm = self.lmax
+        K = self.K
+        if self.lmax == 0 and self.lpol:
+            lm = 0   # energy_pme promotes to 1 itself; the recip generator uses lmax 1 (pme.py:330-333)
+        return energy_pme(positions, box, pairs, Q_local, U, pol, tholes, mS, pS, dS,
+                          self.axis_type, self.axis_indices, self.kappa, K, lm, self.lpol, self.lpme)
+
+    def grad_U(self, positions, box, pairs, Q_local, U, pol, tholes, mS, pS, dS):
+        U = U.detach().clone().requires_grad_(True)
+        e = self.energy_fn(positions.detach(), box.detach(), pairs, Q_local.detach(), U,
+                           pol.detach(), tholes.detach(), mS, pS, dS)
+        (g,) = torch.autograd.grad(e, U)
+        return g.detach()
+
+    def optimize_Uind(self, positions, box, pairs, Q_local, pol, tholes, mS, pS, dS,
+                      U_init=None, steps_pol=None, maxiter=MAX_N_POL, thresh=POL_CONV):
+        """pme.py:342-423: Jacobi U <- U - grad*pol/DIELECTRIC (Hellmann-Feynman: no grad)."""
+        U = torch.zeros((self.n_atoms, 3), dtype=F64) if U_init is None else U_init.detach().clone()
+        pol_d = pol.detach()
+        if steps_pol is None:
+            site = pol_d > 0.001
+            i = 0
+            for i in range(maxiter):
+                g = self.grad_U(positions, box, pairs, Q_local, U, pol, tholes, mS, pS, dS)
+                if torch.max(torch.abs(g[site])) < thresh:
+                    break
+                U = U - g * pol_d[:, None] / DIELECTRIC
+            flag = i != maxiter - 1
+            return U, flag, i
+        for _ in range(steps_pol):
+            g = self.grad_U(positions, box, pairs, Q_local, U, pol, tholes, mS, pS, dS)
+            U = U - g * pol_d[:, None] / DIELECTRIC
+        return U, True, steps_pol
+
+    def solve_Uind_cg(self, positions, box, pairs, Q_local, pol, tholes, mS, pS, dS,
+                      U_init=None, tol=1e-6, maxiter=500):
+        """The fixed point of optimize_Uind reached to max|dE/dU| < tol by preconditioned
+        conjugate gradients on the (exactly quadratic) energy; same fixed point as pme.py:380-401."""
+        n = self.n_atoms
+        args = (positions, box, pairs, Q_local)
+        rest = (pol, tholes, mS, pS, dS)
+        g0 = self.grad_U(*args, torch.zeros((n, 3), dtype=F64), *rest)
+
+        def matvec(v):
+            return self.grad_U(*args, v, *rest) - g0
+
+        U = torch.zeros((n, 3), dtype=F64) if U_init is None else U_init.detach().clone()
+        Minv = (pol.detach() / DIELECTRIC)[:, None]
+        site = pol.detach() > 0.001
+        r = -(g0 + matvec(U))
+        z = Minv * r
+        p = z.clone()
+        rz = torch.sum(r * z)
+        it = 0
+        for it in range(maxiter):
+            if torch.max(torch.abs(r[site])) < tol:
+                break
+            Ap = matvec(p)
+            alpha = rz / torch.sum(p * Ap)
+            U = U + alpha * p
+            r = r - alpha * Ap
+            z = Minv * r
+            rz_new = torch.sum(r * z)
+            p = z + (rz_new / rz) * p
+            rz = rz_new
+        res = self.grad_U(*args, U, *rest)
+        return U, bool(torch.max(torch.abs(res[site])) < tol), it
+
+    def get_energy(self, positions, box, pairs, Q_local, pol=None, tholes=None,
+                   mScales=None, pScales=None, dScales=None, U_init=None, solver="reference",
+                   tol=1e-6, aux=None):
+        """pme.py:120-141 (non-pol) / 186-232 (pol).  U_init in e*nm as the reference."""
+        if not self.lpol:
+            return self.energy_fn(positions, box, pairs, Q_local, None, None, None, mScales, None, None)
+        U0 = None if U_init is None else _t(U_init) * 10.0
+        if solver == "reference":
+            U, flag, ncyc = self.optimize_Uind(positions, box, pairs, Q_local, pol, tholes,
+                                               mScales, pScales, dScales, U_init=U0, steps_pol=self.steps_pol)
+        else:
+            U, flag, ncyc = self.solve_Uind_cg(positions, box, pairs, Q_local, pol, tholes,
+                                               mScales, pScales, dScales, U_init=U0, tol=tol)
+        e = self.energy_fn(positions, box, pairs, Q_local, U.detach(), pol, tholes, mScales, pScales, dScales)
+        if aux is not None:
+            aux["U_ind"] = U.detach() * 0.1
+            aux["lconverg"] = flag
+            aux["n_cycle"] = ncyc
+            return e, aux
+        return e
+
+
+# ---------------------------------------------------------------------------
+# Dispersion PME  (dmff/admp/disp_pme.py)
+# ---------------------------------------------------------------------------
+def g_p(x2, pmax):
+    """disp_pme.py:229-261."""
+    x4 = x2 * x2
+    x8 = x4 * x4
+    e = torch.exp(-x2)
+    g6 = 1 + x2 + 0.5 * x4
+    gs = [g6]
+    if pmax >= 8:
+        g8 = g6 + x4 * x2 / 6
+        gs.append(g8)
+    if pmax >= 10:
+        gs.append(g8 + x8 / 24)
+    return [g * e for g in gs]
+
+
+def disp_pme_real(positions, box, pairs, c_list, mScales, kappa, pmax):
+    """disp_pme.py:133-226."""
+    pairs = _live_pairs(pairs)
+    box_inv = torch.linalg.inv(box)
+    i, j, nb = pairs[:, 0], pairs[:, 1], pairs[:, 2]
+    m = _t(mScales)[nb - 1]
+    ci, cj = c_list[i], c_list[j]
+    dr = pbc_shift(positions[i] - positions[j], box, box_inv)
+    dr2 = torch.sum(dr * dr, dim=1)
+    g = g_p(kappa * kappa * dr2, pmax)
+    dr6 = dr2 * dr2 * dr2
+    e = (m + g[0] - 1) * ci[:, 0] * cj[:, 0] / dr6
+    if pmax >= 8:
+        dr8 = dr6 * dr2
+        e = e + (m + g[1] - 1) * ci[:, 1] * cj[:, 1] / dr8
+    if pmax >= 10:
+        dr10 = dr8 * dr2
+        e = e + (m + g[2] - 1) * ci[:, 2] * cj[:, 2] / dr10
+    return torch.sum(e)
+
+
+def disp_pme_self(c_list, kappa, pmax):
+    """disp_pme.py:264-289."""
+    e = -kappa ** 6 / 12 * torch.sum(c_list[:, 0] ** 2)
+    if pmax >= 8:
+        e = e - kappa ** 8 / 48 * torch.sum(c_list[:, 1] ** 2)
+    if pmax >= 10:
+        e = e - kappa ** 10 / 240 * torch.sum(c_list[:, 2] ** 2)
+    return e
+
+
+def energy_disp_pme(positions, box, pairs, c_list, mScales, kappa, K, pmax, lpme=True):
+    """disp_pme.py:80-130 (backend sign: the generator returns the negative, generators/admp.py:373-384)."""
+    if not lpme:
+        kappa = 0.0
+    e = disp_pme_real(positions, box, pairs, c_list, mScales, kappa, pmax)
+    if lpme:
+        e = e + pme_recip(positions, box, c_list[:, 0:1], kappa, K, 0, Ck_6, True)
+        if pmax >= 8:
+            e = e + pme_recip(positions, box, c_list[:, 1:2], kappa, K, 0, Ck_8, True)
+        if pmax >= 10:
+            e = e + pme_recip(positions, box, c_list[:, 2:3], kappa, K, 0, Ck_10, True)
+        e = e + disp_pme_self(c_list, kappa, pmax)
+    return e
+
+
+# ---------------------------------------------------------------------------
+# Slater-ISA short range  (dmff/admp/pairwise.py:47-95,156-169)
+# ---------------------------------------------------------------------------
+def slater_sr_energy(positions, box, pairs, mScales, a_list, b_list):
+    """generate_pairwise_interaction(slater_sr_kernel): sum m*Ai*Aj*(1/3 (Br)^2+Br+1) e^-Br."""
+    pairs = _live_pairs(pairs)
+    box_inv = torch.linalg.inv(box)
+    i, j, nb = pairs[:, 0], pairs[:, 1], pairs[:, 2]
+    m = _t(mScales)[nb - 1]
+    dr = torch.linalg.norm(pbc_shift(positions[i] - positions[j], box, box_inv), dim=1)
+    b = torch.sqrt(b_list[i] * b_list[j])
+    a = a_list[i] * a_list[j]
+    br = b * dr
+    P = 1 / 3 * br * br + br + 1
+    return torch.sum(a * P * torch.exp(-br) * m)
+
+
+# ---------------------------------------------------------------------------
+# Neighbour list: exact FP64 brute force with a fixed operation order
+# (pair-set semantics of dmff/common/nblist.py:27-55: i<j, min-image d < rc)
+# ---------------------------------------------------------------------------
+def brute_force_pairs(positions, box, rc):
+    """All i<j with minimum-image distance < rc.  The squared distance is formed
+    with single-rounded IEEE operations in the order
+        s_c = ((dx*Bi[0,c] + dy*Bi[1,c]) + dz*Bi[2,c]); s_c -= floor(s_c + 0.5)
+        d_c = ((s0*B[0,c] + s1*B[1,c]) + s2*B[2,c]); d2 = (d0*d0 + d1*d1) + d2*d2
+    (no fused multiply-add), the same order the GPU build uses, so the two sets
+    must agree bit for bit.  Returns an (Np,2) int64 array sorted by (i, j)."""
+    pos = np.asarray(positions, dtype=np.float64)
+    B = np.asarray(box, dtype=np.float64)
+    Bi = np.linalg.inv(B)
+    n = pos.shape[0]
+    rc2 = float(rc) * float(rc)
+    out = []
+    for i in range(n - 1):
+        d = pos[i] - pos[i + 1:]
+        s = [(d[:, 0] * Bi[0, c] + d[:, 1] * Bi[1, c]) + d[:, 2] * Bi[2, c] for c in range(3)]
+        s = [x - np.floor(x + 0.5) for x in s]
+        dd = [(s[0] * B[0, c] + s[1] * B[1, c]) + s[2] * B[2, c] for c in range(3)]
+        d2 = (dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]
+        js = np.nonzero(d2 < rc2)[0] + i + 1
+        out.append(np.stack([np.full(js.shape, i), js], axis=1))
+    if not out:
+        return np.zeros((0, 2), dtype=np.int64)
+    return np.concatenate(out, axis=0).astype(np.int64)
+
+
+PINNING = {
+    "setup_ewald_parameters": "BASELINE table kappa/K values (tests/test_oracle_pins.py)",
+    "pbc_shift": "reference test vectors tests/test_admp/test_sptial.py:43-68",
+    "construct_local_frames": "reference test vectors test_sptial.py:70-142, test_noaxistype.py:12-49",
+    "build_quasi_internal": "reference test vectors test_sptial.py:11-41",
+    "convert_cart2harm/rot_*": "reference test vectors test_multipole.py:9-189; Cartesian rotation identity",
+    "calc_e_perm/pme_real (kappa=0)": "Stone Cartesian T-tensors on random pairs; water-dimer energies test_compute.py",
+    "calc_e_perm (kappa>0)": "erfc closed form for charges; 2988-atom box energies examples/water_fullpol",
+    "calc_e_ind/optimize_Uind": "examples/water_fullpol/ref_out U_ind + energies; fixed-point residual",
+    "pme_recip": "direct Ewald k-sum on a small cell; two-charge -332.9216; numpy FFT",
+    "pme_self/pol_penalty": "closed forms",
+    "disp_pme": "direct dispersion Ewald sum; g_p series; self closed form",
+    "slater_sr_energy": "closed form on a pair",
+    "brute_force_pairs": "pair count 15 for water dimer at 4 A (test_nblist.py:61-69); O(N^2) loop",
+    "gradients": "central finite differences of the restated energy",
+}

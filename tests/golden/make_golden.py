@@ -1,0 +1,155 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own test
+data and example outputs.
+
+Runs only where the reference tree is mounted (/root/reference).  It reads the
+reference's files as text: literal arrays are pulled out of the test modules
+with ``ast`` (nothing is imported or executed), numbers are parsed out of the
+example ``ref_out`` files, and the PDB/XML data files the tests load are copied
+verbatim (they are inputs, not code).  The fixtures it writes:
+
+  data/*.pdb, data/*.xml         inputs the reference tests and examples load
+  vectors.json                   literal vectors from tests/test_admp/test_multipole.py,
+                                 test_sptial.py, test_noaxistype.py
+  energies.json                  known answers (test_compute.py, examples/*/ref_out)
+  water_fullpol_uind.npy         per-atom induced dipoles, DMFF column of
+                                 examples/water_fullpol/ref_out (e*nm)
+"""
+import ast
+import json
+import os
+import re
+import shutil
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+DATA = os.path.join(HERE, "data")
+
+
+def lit(node):
+    """Evaluate the literal subset used by the reference's parametrize tables."""
+    if isinstance(node, ast.Constant):
+        return node.value
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+        return -lit(node.operand)
+    if isinstance(node, (ast.List, ast.Tuple)):
+        return [lit(e) for e in node.elts]
+    if isinstance(node, ast.Call):          # jnp.array([...]) / np.array([...])
+        return lit(node.args[0])
+    raise ValueError(ast.dump(node))
+
+
+def parametrize_tables(path):
+    tree = ast.parse(open(path).read())
+    out = {}
+    for fn in ast.walk(tree):
+        if not isinstance(fn, ast.FunctionDef):
+            continue
+        for dec in fn.decorator_list:
+            if isinstance(dec, ast.Call) and getattr(dec.func, "attr", "") == "parametrize":
+                names = [s.strip() for s in lit(dec.args[0]).split(",")]
+                rows = lit(dec.args[1])
+                out[fn.name] = [dict(zip(names, row)) for row in rows]
+    return out
+
+
+def asserted_values(path):
+    """assert_almost_equal(energy, VALUE, decimal=D) calls, in file order."""
+    tree = ast.parse(open(path).read())
+    vals = []
+    for node in ast.walk(tree):
+        if isinstance(node, ast.Call) and getattr(node.func, "attr", "") == "assert_almost_equal":
+            dec = [k for k in node.keywords if k.arg == "decimal"]
+            try:
+                v = lit(node.args[1])
+            except ValueError:
+                continue
+            vals.append({"line": node.lineno, "value": v, "decimal": lit(dec[0].value) if dec else 7})
+    return sorted(vals, key=lambda d: d["line"])
+
+
+def main():
+    if not os.path.isdir(REF):
+        sys.exit("reference tree not mounted; fixtures are already committed")
+    os.makedirs(DATA, exist_ok=True)
+    copies = {
+        "tests/data/admp.xml": "admp.xml",
+        "tests/data/admp_mono.xml": "admp_mono.xml",
+        "tests/data/admp_nonpol.xml": "admp_nonpol.xml",
+        "tests/data/water_dimer.pdb": "water_dimer.pdb",
+        "examples/water_fullpol/forcefield.xml": "fullpol_forcefield.xml",
+        "examples/water_fullpol/residues.xml": "fullpol_residues.xml",
+        "examples/water_fullpol/waterbox_31ang.pdb": "waterbox_31ang.pdb",
+        "examples/water_fullpol/quadrupole_nonpol/forcefield.xml": "quadnonpol_forcefield.xml",
+        "examples/water_fullpol/monopole_polarizable/forcefield.xml": "monopol_forcefield.xml",
+        "examples/water_fullpol/monopole_nonpol/forcefield.xml": "mononpol_forcefield.xml",
+        "examples/water_fullpol/monopole_nonpol/pair.pdb": "mononpol_pair.pdb",
+        "examples/fluctuated_leading_term_waterff/forcefield.xml": "fluct_forcefield.xml",
+        "examples/fluctuated_leading_term_waterff/residues.xml": "fluct_residues.xml",
+        "examples/fluctuated_leading_term_waterff/water_dimer.pdb": "fluct_water_dimer.pdb",
+    }
+    for src, dst in copies.items():
+        shutil.copyfile(os.path.join(REF, src), os.path.join(DATA, dst))
+
+    vec = {}
+    vec["test_multipole"] = parametrize_tables(os.path.join(REF, "tests/test_admp/test_multipole.py"))
+    vec["test_sptial"] = parametrize_tables(os.path.join(REF, "tests/test_admp/test_sptial.py"))
+    vec["test_noaxistype"] = parametrize_tables(os.path.join(REF, "tests/test_admp/test_noaxistype.py"))
+    with open(os.path.join(HERE, "vectors.json"), "w") as f:
+        json.dump(vec, f)
+
+    en = {"source": {}}
+    comp = asserted_values(os.path.join(REF, "tests/test_admp/test_compute.py"))
+    en["test_compute"] = comp
+    en["source"]["test_compute"] = "tests/test_admp/test_compute.py"
+
+    def first_number(path, pattern):
+        txt = open(os.path.join(REF, path)).read()
+        return float(re.search(pattern, txt).group(1))
+
+    num = r"(-?\d+\.\d+(?:[eE][-+]?\d+)?)"
+    en["fullpol_E"] = first_number("examples/water_fullpol/ref_out", r"Polarization Energy:\s*#\s*" + num)
+    en["quadnonpol_E"] = first_number("examples/water_fullpol/quadrupole_nonpol/ref_out",
+                                      r"Polarization Energy:\s*#\s*" + num)
+    en["monopol_E"] = first_number("examples/water_fullpol/monopole_polarizable/ref_out",
+                                   r"Energy \(kJ/mol\):\s*" + num)
+    en["mononpol_E"] = first_number("examples/water_fullpol/monopole_nonpol/ref_out",
+                                    r"Energy \(kJ/mol\):\s*" + num)
+    txt = open(os.path.join(REF, "examples/fluctuated_leading_term_waterff/ref_out")).read()
+    en["fluct_E"] = float(re.search(r"Energy:\s*#\s*" + num, txt).group(1))
+    rows = re.findall(r"\[\s*(-?[\d.eE+-]+)\s+(-?[\d.eE+-]+)\s+(-?[\d.eE+-]+)\]", txt)
+    en["fluct_grad"] = [[float(x) for x in r] for r in rows]
+    mpid = []
+    for line in open(os.path.join(REF, "examples/water_fullpol/mpid_ref/results.txt")):
+        parts = line.split()
+        if len(parts) == 3 and parts[0].isdigit():
+            mpid.append([float(p) for p in parts])
+    en["mpid_rc_es_disp"] = mpid
+    en["source"].update({
+        "fullpol_E": "examples/water_fullpol/ref_out:2",
+        "quadnonpol_E": "examples/water_fullpol/quadrupole_nonpol/ref_out:2",
+        "monopol_E": "examples/water_fullpol/monopole_polarizable/ref_out:2",
+        "mononpol_E": "examples/water_fullpol/monopole_nonpol/ref_out:2",
+        "fluct_E": "examples/fluctuated_leading_term_waterff/ref_out:2",
+        "fluct_grad": "examples/fluctuated_leading_term_waterff/ref_out:4-9",
+        "mpid_rc_es_disp": "examples/water_fullpol/mpid_ref/results.txt:2-9",
+    })
+    with open(os.path.join(HERE, "energies.json"), "w") as f:
+        json.dump(en, f, indent=1)
+
+    u = []
+    for line in open(os.path.join(REF, "examples/water_fullpol/ref_out")):
+        if line.startswith("#"):
+            continue
+        parts = line.split()
+        if len(parts) == 3:
+            u.append([float(p) for p in parts])
+    u = np.asarray(u)
+    np.save(os.path.join(HERE, "water_fullpol_uind.npy"), u[:, [0, 2]])   # (MPID, DMFF) columns
+    print("wrote fixtures:", len(copies), "data files,", u.shape[0], "U_ind components")
+
+
+if __name__ == "__main__":
+    main()

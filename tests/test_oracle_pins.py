@@ -1,0 +1,437 @@
+"""Pins the oracle (oracle/admp_oracle.py) before it is trusted as the checker.
+
+Each check is chosen so a plausible restatement mistake (dropped term, wrong
+sign, wrong index, transposed operand) fails it:
+  * the reference's own test vectors (tests/test_admp/test_multipole.py,
+    test_sptial.py, test_noaxistype.py) and known energies (test_compute.py,
+    examples/water_fullpol/**/ref_out, examples/fluctuated_leading_term_waterff);
+  * first principles: the kappa=0 real-space energy equals the Coulomb energy of
+    the multipole densities q - mu.grad + (1/3) Theta:grad grad, obtained by
+    nested autodiff of 1/|a-b| (independent of the quasi-internal-frame code);
+  * Ewald invariance: the total energy does not depend on kappa once real and
+    reciprocal sums are converged (pins erf terms, spreading, C_k, self term);
+  * central finite differences of the restated energy for every gradient.
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import admp_oracle as O
+from tests.systems import GOLD, load_system, reference_pairs
+
+F64 = torch.float64
+VEC = json.load(open(f"{GOLD}/vectors.json"))
+EN = json.load(open(f"{GOLD}/energies.json"))
+
+
+def T(x):
+    return torch.tensor(np.asarray(x, dtype=np.float64))
+
+
+# ----------------------------------------------------------------- vectors
+def test_pbc_shift_vectors():
+    for c in VEC["test_sptial"]["test_pbc_shift"]:
+        out = O.pbc_shift(T(c["drvecs"]), T(c["box"]), T(c["box_inv"]))
+        np.testing.assert_allclose(out.numpy(), np.asarray(c["expected"], dtype=float), atol=0)
+
+
+def test_quasi_internal_vectors():
+    for c in VEC["test_sptial"]["test_build_quasi_internal"]:
+        out = O.build_quasi_internal(T(c["r1"]), T(c["r2"]), T(c["dr"]), T(c["norm_dr"]))
+        np.testing.assert_allclose(out.numpy(), np.asarray(c["expected"]), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("src", ["test_sptial", "test_noaxistype"])
+def test_local_frames_vectors(src):
+    c = VEC[src]["test_generate_construct_local_frames"][0]
+    out = O.construct_local_frames(T(c["positions"]), T(c["box"]), np.array(c["axis_types"]),
+                                   np.array(c["axis_indices"]))
+    np.testing.assert_allclose(out.numpy(), np.asarray(c["expected_local_frames"], dtype=float),
+                               rtol=1e-4, atol=1e-6)
+
+
+def test_cart2harm_and_rotation_vectors():
+    c = VEC["test_multipole"]["test_convert_cart2harm"][0]
+    np.testing.assert_allclose(O.convert_cart2harm(T(c["theta"]), 2).numpy(), np.asarray(c["expected"]),
+                               rtol=1e-6, atol=1e-8)
+    c = VEC["test_multipole"]["test_rot_global_local"][0]
+    fr = T(c["local_frames"])
+    np.testing.assert_allclose(O.rot_local2global(T(c["Q_local"]), fr, 2).numpy(), np.asarray(c["Q_global"]),
+                               rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(O.rot_global2local(T(c["Q_global"]), fr, 2).numpy(), np.asarray(c["Q_local"]),
+                               rtol=1e-6, atol=1e-6)
+
+
+def test_harmonic_rotation_equals_cartesian_rotation():
+    """rot_local2global on harmonics == R^T Theta R on Cartesian (to the rt3 literal)."""
+    g = torch.Generator().manual_seed(1)
+    q, _ = torch.linalg.qr(torch.randn(5, 3, 3, generator=g, dtype=F64))
+    q = q * torch.sign(torch.det(q)).reshape(-1, 1, 1)
+    cart = torch.randn(5, 10, generator=g, dtype=F64)
+    cart[:, 6] = -(cart[:, 4] + cart[:, 5])
+    h = O.convert_cart2harm(cart, 2)
+    hg = O.rot_local2global(h, q, 2)
+    cg = O.convert_harm2cart(hg, 2)
+    th = torch.zeros(5, 3, 3, dtype=F64)
+    idx = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    for k, (a, b) in enumerate(idx):
+        th[:, a, b] = cart[:, 4 + k]
+        th[:, b, a] = cart[:, 4 + k]
+    th_g = q.transpose(1, 2) @ th @ q
+    mu_g = torch.einsum("nji,nj->ni", q, cart[:, 1:4])
+    np.testing.assert_allclose(cg[:, 1:4].numpy(), mu_g.numpy(), atol=1e-12)
+    for k, (a, b) in enumerate(idx):
+        np.testing.assert_allclose(cg[:, 4 + k].numpy(), th_g[:, a, b].numpy(), atol=1e-10)
+
+
+def test_ewald_parameters_rule():
+    # BASELINE table: cfg1 L=49.67 A, rc 8 -> kappa 0.328533, K 50; cfg3 L=99.30, rc 10 -> 0.262826, 80
+    k, K1, _, _ = O.setup_ewald_parameters(8.0, 5e-4, np.eye(3) * 49.67)
+    assert abs(k - 0.328533) < 1e-6 and K1 == 50
+    k, K1, _, _ = O.setup_ewald_parameters(10.0, 5e-4, np.eye(3) * 99.30)
+    assert abs(k - 0.262826) < 1e-6 and K1 == 80
+    k, K1, _, _ = O.setup_ewald_parameters(8.0, 5e-4, np.eye(3) * 18.64)
+    assert K1 == 19
+
+
+# ----------------------------------------------------------------- known energies
+def _run(ff, pdb, res, rc_nm, lpme, steps_pol=None, solver="reference"):
+    s = load_system(ff, pdb, res)
+    pos, box = T(s.pos_nm * 10), T(s.box_nm * 10)
+    pairs = reference_pairs(s.pos_nm * 10, s.box_nm * 10, rc_nm * 10, s.bonded)
+    f = O.ADMPPmeForce(box.numpy(), s.axis_types, s.axis_idx, rc_nm * 10, 5e-4, s.pme.lmax,
+                       s.pme.lpol, lpme, steps_pol)
+    Q = T(s.pme.Q_local[s.map_atomtype])
+    if s.pme.lpol:
+        pol, th = T(s.pme.pol[s.map_poltype]), T(s.pme.thole[s.map_poltype])
+        return f.get_energy(pos, box, pairs, Q, pol, th, s.pme.mScales, s.pme.pScales,
+                            s.pme.dScales, solver=solver, aux={})
+    return f.get_energy(pos, box, pairs, Q, mScales=s.pme.mScales), None
+
+
+def test_water_dimer_energies_test_compute():
+    """tests/test_admp/test_compute.py:55,124,145 (kappa=0, rc 4 A, step_pol 5)."""
+    tc = {d["line"]: d for d in EN["test_compute"]}
+    e, _ = _run("admp.xml", "water_dimer.pdb", None, 0.4, False, 5)
+    assert abs(float(e) - tc[55]["value"]) < 1.5 * 10 ** -tc[55]["decimal"]
+    e, _ = _run("admp_mono.xml", "water_dimer.pdb", None, 0.4, False, 5)
+    assert abs(float(e) - tc[124]["value"]) < 1e-9
+    e, _ = _run("admp_nonpol.xml", "water_dimer.pdb", None, 0.4, False, 5)
+    assert abs(float(e) - tc[145]["value"]) < 1e-7
+
+
+def test_two_charge_pme():
+    """examples/water_fullpol/monopole_nonpol/ref_out:2 (printed in float32)."""
+    e, _ = _run("mononpol_forcefield.xml", "mononpol_pair.pdb", None, 0.6, True)
+    assert abs(float(e) - EN["mononpol_E"]) / abs(EN["mononpol_E"]) < 1e-5
+
+
+@pytest.mark.slow
+def test_water_box_2988_energies_and_uind():
+    """examples/water_fullpol/{quadrupole_nonpol,monopole_polarizable,.}/ref_out (float32 runs)."""
+    e, _ = _run("quadnonpol_forcefield.xml", "waterbox_31ang.pdb", "fullpol_residues.xml", 0.6, True)
+    assert abs(float(e) / EN["quadnonpol_E"] - 1) < 1e-5
+    e, _ = _run("monopol_forcefield.xml", "waterbox_31ang.pdb", "fullpol_residues.xml", 0.6, True)
+    assert abs(float(e) / EN["monopol_E"] - 1) < 1e-4
+    e, aux = _run("fullpol_forcefield.xml", "waterbox_31ang.pdb", "fullpol_residues.xml", 0.6, True)
+    assert abs(float(e) / EN["fullpol_E"] - 1) < 1e-5
+    U = aux["U_ind"].numpy().ravel() * 10      # the example printed e*A
+    ref = np.load(f"{GOLD}/water_fullpol_uind.npy")
+    assert np.linalg.norm(U - ref[:, 1]) / np.linalg.norm(ref[:, 1]) < 5e-4
+    assert np.abs(U - ref[:, 1]).max() < 5e-4
+
+
+def test_fluctuating_charge_gradient():
+    """examples/fluctuated_leading_term_waterff/ref_out: E and dE/dpos with the
+    charges a function of the geometry (run.py:22-56), step_pol 5.  The stored
+    numbers are reproduced to ~1e-11 with PME on (kappa, K from ethresh 5e-4 and
+    the 31.289 A box); this is the full-precision pin of the whole path."""
+    s = load_system("fluct_forcefield.xml", "fluct_water_dimer.pdb", "fluct_residues.xml")
+    box = T(s.box_nm * 10)
+    pos = T(s.pos_nm * 10).requires_grad_(True)
+    pairs = reference_pairs(s.pos_nm * 10, s.box_nm * 10, 4.0, s.bonded)
+    f = O.ADMPPmeForce(box.numpy(), s.axis_types, s.axis_idx, 4.0, 5e-4, 2, True, True, 5)
+    Q = T(s.pme.Q_local[s.map_atomtype])
+    c0 = fluct_charges(pos, box)
+    Q = torch.cat([c0[:, None], Q[:, 1:]], dim=1)
+    pol, th = T(s.pme.pol[s.map_poltype]), T(s.pme.thole[s.map_poltype])
+    e = f.get_energy(pos, box, pairs, Q, pol, th, s.pme.mScales, s.pme.pScales, s.pme.dScales)
+    (g,) = torch.autograd.grad(e, pos)
+    assert abs(float(e.detach()) - EN["fluct_E"]) < 1e-8
+    np.testing.assert_allclose(g.numpy(), np.asarray(EN["fluct_grad"]), rtol=1e-6, atol=1e-6)
+
+
+def fluct_charges(pos, box):
+    """Geometry-dependent charges of examples/fluctuated_leading_term_waterff/run.py:22-46
+    (the example's own model, restated; atoms ordered O, H1, H2 per water)."""
+    box_inv = torch.linalg.inv(box)
+    O_, H1, H2 = pos[0::3], pos[1::3], pos[2::3]
+    r1 = O.pbc_shift(H1 - O_, box, box_inv)
+    r2 = O.pbc_shift(H2 - O_, box, box_inv)
+    d1, d2 = torch.linalg.norm(r1, dim=1), torch.linalg.norm(r2, dim=1)
+    ang = torch.arccos(torch.sum(r1 * r2, dim=1) / (d1 * d2)) * 180 / math.pi
+    dip1 = -0.016858755 + 0.002287251 * ang + 0.239667591 * d1 + (-0.070483437) * d2
+    dip2 = -0.016858755 + 0.002287251 * ang + 0.239667591 * d2 + (-0.070483437) * d1
+    qh1, qh2 = dip1 / d1, dip2 / d2
+    q = torch.zeros(pos.shape[0], dtype=F64)
+    q = q.index_put((torch.arange(0, pos.shape[0], 3),), -(qh1 + qh2))
+    q = q.index_put((torch.arange(1, pos.shape[0], 3),), qh1)
+    q = q.index_put((torch.arange(2, pos.shape[0], 3),), qh2)
+    return q
+
+
+def test_nblist_pair_count_water_dimer():
+    """tests/test_common/test_nblist.py:61-69: 15 pairs at 4 A, 19 rows with padding."""
+    s = load_system("admp.xml", "water_dimer.pdb")
+    p = reference_pairs(s.pos_nm * 10, s.box_nm * 10, 4.0, s.bonded)
+    assert p.shape == (15, 3)
+    assert int(15 * 1.3) == 19
+
+
+# ----------------------------------------------------------------- first principles
+def _cart_theta(c):
+    th = torch.zeros(3, 3, dtype=F64)
+    idx = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    for k, (a, b) in enumerate(idx):
+        th[a, b] = c[4 + k]
+        th[b, a] = c[4 + k]
+    return th
+
+
+def _coulomb_multipole_pair(ra, rb, ca, cb):
+    """D * L_a L_b 1/|a-b| with L = q + mu.grad + (1/3) Theta:grad grad (nested autodiff)."""
+    a = ra.clone().requires_grad_(True)
+    b = rb.clone().requires_grad_(True)
+    tha, thb = _cart_theta(ca), _cart_theta(cb)
+
+    def Lop(fn_val, x, c, th):
+        g = torch.autograd.grad(fn_val, x, create_graph=True)[0]
+        H = torch.stack([torch.autograd.grad(g[k], x, create_graph=True)[0] for k in range(3)])
+        return c[0] * fn_val + torch.dot(c[1:4], g) + torch.sum(th * H) / 3.0
+
+    f = 1.0 / torch.linalg.norm(a - b)
+    inner = Lop(f, b, cb, thb)
+    return O.DIELECTRIC * Lop(inner, a, ca, tha)
+
+
+def test_kappa0_real_space_equals_coulomb_of_multipole_densities():
+    g = torch.Generator().manual_seed(7)
+    n = 6
+    pos = torch.rand(n, 3, generator=g, dtype=F64) * 8.0 + 5.0
+    cart = torch.randn(n, 10, generator=g, dtype=F64) * torch.tensor([1, .3, .3, .3, .2, .2, .2, .2, .2, .2], dtype=F64)
+    cart[:, 6] = -(cart[:, 4] + cart[:, 5])
+    Qh = O.convert_cart2harm(cart, 2)
+    box = torch.eye(3, dtype=F64) * 100.0
+    pairs = np.array([[i, j, 0] for i in range(n) for j in range(i + 1, n)])
+    e = O.pme_real(pos, box, pairs, Qh, None, None, None, np.ones(6), None, None, 0.0, 2, False)
+    ref = sum(_coulomb_multipole_pair(pos[i], pos[j], cart[i], cart[j]) for i, j, _ in pairs)
+    # the harmonic round trip uses the 10-digit rt3 literal
+    assert abs(float(e) - float(ref.detach())) < 1e-9 * abs(float(ref.detach())) + 1e-9
+
+
+def _ewald_total(pos, box, pairs, Qh, U, pol, th, kappa, K, lmax=2):
+    m = np.array([0.0, 0.0, 1, 1, 1, 1])
+    e = O.pme_real(pos, box, pairs, Qh, U, pol, th, m, m, m, kappa, lmax, U is not None)
+    Qt = Qh.clone()
+    if U is not None:
+        Qt[:, 1:4] = Qt[:, 1:4] + U
+    e = e + O.pme_recip(pos, box, Qt, kappa, K, lmax) + O.pme_self(Qt, kappa, lmax)
+    return float(e)
+
+
+def test_ewald_total_is_kappa_independent():
+    """Real + reciprocal + self must not depend on kappa (pins bVec, spreading, C_k, self)."""
+    g = torch.Generator().manual_seed(3)
+    L = 13.0
+    box = torch.eye(3, dtype=F64) * L
+    n = 8
+    pos = torch.rand(n, 3, generator=g, dtype=F64) * L
+    cart = torch.randn(n, 10, generator=g, dtype=F64) * torch.tensor([1, .3, .3, .3, .3, .3, .3, .3, .3, .3], dtype=F64)
+    cart[:, 0] -= cart[:, 0].mean()
+    cart[:, 6] = -(cart[:, 4] + cart[:, 5])
+    Qh = O.convert_cart2harm(cart, 2)
+    U = torch.randn(n, 3, generator=g, dtype=F64) * 0.2
+    pol = torch.rand(n, generator=g, dtype=F64) + 0.5
+    th = torch.rand(n, generator=g, dtype=F64) + 0.3
+    rc = 6.4
+    pairs = reference_pairs(pos.numpy(), box.numpy(), rc, {(0, 1): 1, (1, 0): 1, (2, 3): 2, (3, 2): 2})
+    # the residual kappa dependence is the order-6 interpolation error (~2e-6 rel,
+    # not monotonic in K); a dropped or mis-weighted term moves E by O(10) kJ/mol
+    for UU in (U, None):
+        e1 = _ewald_total(pos, box, pairs, Qh, UU, pol, th, 0.60, (56, 56, 56))
+        e2 = _ewald_total(pos, box, pairs, Qh, UU, pol, th, 0.70, (64, 64, 64))
+        assert abs(e1 - e2) < 5e-6 * max(1.0, abs(e1)), (e1, e2)
+
+
+def test_pme_recip_matches_direct_kspace_sum():
+    """PME reciprocal energy vs the direct sum with S(k) = sum_j (q - i k.mu - k.Theta.k/3) e^{-ik.r}."""
+    g = torch.Generator().manual_seed(5)
+    L = 10.0
+    box = torch.diag(torch.tensor([L, L * 1.1, L * 0.9], dtype=F64))
+    n = 4
+    pos = torch.rand(n, 3, generator=g, dtype=F64) * 9.0
+    cart = torch.randn(n, 10, generator=g, dtype=F64) * 0.3
+    cart[:, 6] = -(cart[:, 4] + cart[:, 5])
+    Qh = O.convert_cart2harm(cart, 2)
+    kappa = 0.35
+    e_pme = float(O.pme_recip(pos, box, Qh, kappa, (40, 44, 36), 2))
+    binv = torch.linalg.inv(box)
+    rng = torch.arange(-12, 13, dtype=F64)
+    m = torch.stack(torch.meshgrid(rng, rng, rng, indexing="ij"), -1).reshape(-1, 3)
+    m = m[(m != 0).any(1)]
+    k = 2 * math.pi * m @ binv.T
+    ksq = (k * k).sum(1)
+    S = torch.zeros(k.shape[0], dtype=torch.complex128)
+    mu = O.convert_harm2cart(Qh, 2)
+    for j in range(n):
+        th = _cart_theta(mu[j])
+        amp = mu[j, 0] - 1j * (k @ mu[j, 1:4]) - torch.einsum("ka,ab,kb->k", k, th, k) / 3.0
+        S = S + amp * torch.exp(-1j * (k @ pos[j]))
+    V = float(torch.det(box))
+    e_dir = float(O.DIELECTRIC * torch.sum(2 * math.pi / V / ksq * torch.exp(-ksq / 4 / kappa ** 2) * S.abs() ** 2))
+    assert abs(e_pme - e_dir) < 1e-7 * abs(e_dir), (e_pme, e_dir)
+
+
+# ----------------------------------------------------------------- gradients (FD)
+def _small_polar_system():
+    """Waters of the 2988-atom box whose three atoms lie in a 12 A sub-cube, placed
+    in a 13.2 A periodic box (the gap keeps periodic images apart)."""
+    s = load_system("fullpol_forcefield.xml", "waterbox_31ang.pdb", "fullpol_residues.xml")
+    p = s.pos_nm * 10
+    mol = p.reshape(-1, 3, 3)
+    ok = np.all((mol > 1.0) & (mol < 13.0), axis=(1, 2))
+    idx = np.nonzero(np.repeat(ok, 3))[0]
+    remap = {int(a): k for k, a in enumerate(idx)}
+    pos = p[idx] - 0.4
+    bonded = {(remap[i], remap[j]): v for (i, j), v in s.bonded.items() if i in remap and j in remap}
+    s.axis_types = s.axis_types[idx]
+    s.axis_idx = np.array([[remap.get(int(a), -1) if a >= 0 else -1 for a in row] for row in s.axis_idx[idx]])
+    s.map_atomtype = s.map_atomtype[idx]
+    s.map_poltype = s.map_poltype[idx]
+    return s, pos, np.eye(3) * 13.2, bonded, len(idx)
+
+
+def test_oracle_gradients_central_differences():
+    s, pos, box, bonded, n = _small_polar_system()
+    rc = 6.0
+    pairs = reference_pairs(pos, box, rc, bonded)
+    f = O.ADMPPmeForce(box, s.axis_types[:n], s.axis_idx[:n], rc, 5e-4, 2, True, True, None)
+    pos_t = T(pos).requires_grad_(True)
+    box_t = T(box).requires_grad_(True)
+    Q = T(s.pme.Q_local[s.map_atomtype[:n]]).requires_grad_(True)
+    pol = T(s.pme.pol[s.map_poltype[:n]]).requires_grad_(True)
+    th = T(s.pme.thole[s.map_poltype[:n]]).requires_grad_(True)
+    U, _, _ = f.solve_Uind_cg(pos_t, box_t, pairs, Q, pol, th, s.pme.mScales, s.pme.pScales, s.pme.dScales, tol=1e-9)
+
+    def E(p, b, q, a, t):
+        return f.energy_fn(p, b, pairs, q, U, a, t, s.pme.mScales, s.pme.pScales, s.pme.dScales)
+
+    e = E(pos_t, box_t, Q, pol, th)
+    gp, gb, gq, ga, gt = torch.autograd.grad(e, (pos_t, box_t, Q, pol, th))
+    rng = np.random.default_rng(0)
+    h = 1e-5
+    with torch.no_grad():
+        for _ in range(6):
+            i, c = rng.integers(n), rng.integers(3)
+            p1, p2 = pos_t.detach().clone(), pos_t.detach().clone()
+            p1[i, c] += h
+            p2[i, c] -= h
+            fd = (E(p1, box_t, Q, pol, th) - E(p2, box_t, Q, pol, th)) / (2 * h)
+            assert abs(float(fd) - float(gp[i, c])) < 1e-5 * max(1, abs(float(gp[i, c]))) + 2e-4
+        for (a, b) in [(0, 0), (1, 1), (2, 2)]:
+            b1, b2 = box_t.detach().clone(), box_t.detach().clone()
+            b1[a, b] += h
+            b2[a, b] -= h
+            fd = (E(pos_t, b1, Q, pol, th) - E(pos_t, b2, Q, pol, th)) / (2 * h)
+            assert abs(float(fd) - float(gb[a, b])) < 1e-5 * max(1, abs(float(gb[a, b]))) + 2e-3
+        for _ in range(4):
+            i, c = rng.integers(n), rng.integers(9)
+            q1, q2 = Q.detach().clone(), Q.detach().clone()
+            q1[i, c] += h
+            q2[i, c] -= h
+            fd = (E(pos_t, box_t, q1, pol, th) - E(pos_t, box_t, q2, pol, th)) / (2 * h)
+            assert abs(float(fd) - float(gq[i, c])) < 1e-5 * max(1, abs(float(gq[i, c]))) + 2e-4
+        for i in rng.integers(n, size=3):
+            a1, a2 = pol.detach().clone(), pol.detach().clone()
+            a1[i] += h
+            a2[i] -= h
+            fd = (E(pos_t, box_t, Q, a1, th) - E(pos_t, box_t, Q, a2, th)) / (2 * h)
+            assert abs(float(fd) - float(ga[i])) < 1e-5 * max(1, abs(float(ga[i]))) + 2e-4
+            t1, t2 = th.detach().clone(), th.detach().clone()
+            t1[i] += h
+            t2[i] -= h
+            fd = (E(pos_t, box_t, Q, pol, t1) - E(pos_t, box_t, Q, pol, t2)) / (2 * h)
+            assert abs(float(fd) - float(gt[i])) < 1e-5 * max(1, abs(float(gt[i]))) + 2e-4
+
+
+def test_solver_fixed_point_and_warm_start():
+    """At the solution dE/dU = 0, and CG and the reference Jacobi reach the same fixed point.
+    Uses the benchmark scales (d-scales 0 for 1-2..1-4): with the example's d-scale 1 on
+    intramolecular pairs the Jacobi map is not contractive and, run to a tight threshold,
+    diverges (the reference stops it after a few steps at POL_CONV=1)."""
+    s, pos, box, bonded, n = _small_polar_system()
+    pairs = reference_pairs(pos, box, 6.0, bonded)
+    f = O.ADMPPmeForce(box, s.axis_types[:n], s.axis_idx[:n], 6.0, 5e-4, 2, True, True, None)
+    dS = np.array([0.0, 0.0, 0.0, 0.0, 1.0, 1.0])
+    args = (T(pos), T(box), pairs, T(s.pme.Q_local[s.map_atomtype[:n]]), T(s.pme.pol[s.map_poltype[:n]]),
+            T(s.pme.thole[s.map_poltype[:n]]), s.pme.mScales, s.pme.pScales, dS)
+    U1, ok, _ = f.solve_Uind_cg(*args, tol=1e-8)
+    assert ok
+    U2, _, _ = f.optimize_Uind(*args, maxiter=400, thresh=1e-8)
+    assert float(torch.max(torch.abs(U1 - U2))) < 1e-9
+    U3, ok, _ = f.solve_Uind_cg(*args, U_init=U1 * 0.5, tol=1e-8)
+    assert float(torch.max(torch.abs(U1 - U3))) < 1e-9
+
+
+# ----------------------------------------------------------------- dispersion / Slater
+def _direct_dispersion(P, C, L, excl, R=6):
+    """1/2 sum_{i,j,n} c_i^k c_j^k / |r_ij + nL|^(6+2k), omitting i=j at n=0 and the
+    minimum-image copy of each excluded pair (the copy the reference's pair list holds)."""
+    import itertools
+    sh = np.array(list(itertools.product(range(-R, R + 1), repeat=3)), dtype=np.float64) * L
+    tot = 0.0
+    n = len(P)
+    for i in range(n):
+        for j in range(n):
+            d = P[i] - P[j] + sh
+            r2 = (d * d).sum(1)
+            w = np.ones(len(sh))
+            if i == j:
+                w[np.argmin(r2)] = 0
+            elif (min(i, j), max(i, j)) in excl:
+                w[np.argmin(r2)] = 0
+            r2 = np.where(w > 0, r2, 1.0)
+            tot += 0.5 * np.sum(w * sum(C[i, k] * C[j, k] / r2 ** (3 + k) for k in range(3)))
+    return tot
+
+
+def test_dispersion_pme_kappa_independent_and_direct_sum():
+    """Dispersion PME (real + C_6/8/10 reciprocal incl. k=0 + self) vs the direct lattice sum.
+    kappa is kept small so the self and near-pair parts the reciprocal sum carries do not
+    dwarf the lattice energy (their cancellation magnifies the interpolation error)."""
+    g = torch.Generator().manual_seed(9)
+    L = 20.0
+    box = torch.eye(3, dtype=F64) * L
+    n = 10
+    pos = torch.rand(n, 3, generator=g, dtype=F64) * L
+    c = torch.rand(n, 3, generator=g, dtype=F64) * torch.tensor([30.0, 60.0, 120.0], dtype=F64) + 5.0
+    m = np.array([0.0, 0.0, 1.0, 1.0, 1.0, 1.0])
+    pairs = reference_pairs(pos.numpy(), box.numpy(), 9.9, {(0, 1): 1, (1, 0): 1})
+    e1 = float(O.energy_disp_pme(pos, box, pairs, c, m, 0.35, (40, 40, 40), 10))
+    e2 = float(O.energy_disp_pme(pos, box, pairs, c, m, 0.40, (48, 48, 48), 10))
+    assert abs(e1 - e2) < 2e-5 * abs(e1), (e1, e2)
+    ref = _direct_dispersion(pos.numpy(), c.numpy(), L, {(0, 1)})
+    assert abs(e1 - ref) < 2e-5 * abs(ref), (e1, ref)
+
+
+def test_slater_closed_form():
+    pos = T([[0.0, 0.0, 0.0], [1.7, 0.4, -0.3]])
+    box = torch.eye(3, dtype=F64) * 30
+    A, B = T([30.0, 45.0]), T([3.1, 3.9])
+    e = float(O.slater_sr_energy(pos, box, np.array([[0, 1, 0]]), np.ones(6), A, B))
+    r = float(torch.linalg.norm(pos[1]))
+    b = math.sqrt(3.1 * 3.9)
+    assert abs(e - 30 * 45 * (b * b * r * r / 3 + b * r + 1) * math.exp(-b * r)) < 1e-10
