@@ -1,0 +1,209 @@
+"""Drop-in counterpart of DMFF's ADMPPmeForce (dmff/admp/pme.py:36-423) on the
+MI355X HIP library.
+
+Same constructor arguments and get_energy signatures as the reference backend
+(Angstrom, e, kJ/mol; U_init and aux["U_ind"] in e*nm as the reference).
+Energies are torch scalars that carry gradients with respect to positions,
+box, Q_local, pol and tholes through hand-written backward kernels (the
+induced dipoles are held fixed, as the reference's stop_gradient does).
+"""
+import math
+
+import numpy as np
+import torch
+
+from .engine import Engine, _dev
+
+F64 = torch.float64
+# dmff/admp/settings.py:2-3
+POL_CONV = 1.0
+MAX_N_POL = 30
+
+
+def setup_ewald_parameters(rc, ethresh, box=None, spacing=None, method="openmm"):
+    """kappa (1/A) and K1..K3 (dmff/admp/pme.py:426-492)."""
+    if method == "openmm":
+        box = np.asarray(box.detach().cpu() if isinstance(box, torch.Tensor) else box, dtype=np.float64)
+        kappa = math.sqrt(-math.log(2 * ethresh)) / rc
+        K = [int(math.ceil(2 * kappa * box[d, d] / 3 / ethresh ** 0.2)) for d in range(3)]
+        return kappa, K[0], K[1], K[2]
+    if method == "gromacs":
+        kappa = 5.0
+        i = 0
+        while math.erfc(kappa * rc) > ethresh:
+            i += 1
+            kappa *= 2
+        low, high = 0.0, kappa
+        for _ in range(i + 60):
+            kappa = (low + high) / 2
+            if math.erfc(kappa * rc) > ethresh:
+                low = kappa
+            else:
+                high = kappa
+        box = np.asarray(box, dtype=np.float64)
+        K = [int(math.ceil(box[d, d] / spacing)) for d in range(3)]
+        return kappa, K[0], K[1], K[2]
+    raise ValueError(f"Invalid method: {method}.Valid methods: 'openmm', 'gromacs'")
+
+
+class _ADMPEnergy(torch.autograd.Function):
+    """E(pos, box, Q_local, pol, thole) with the gradients computed by the library
+    in the same call (value-and-grad), returned on backward."""
+
+    @staticmethod
+    def forward(ctx, pos, box, Q_local, pol, thole, run):
+        need = any(ctx.needs_input_grad[:5])
+        E, g = run(need)
+        ctx.g = g
+        ctx.shapes = (pos.shape, box.shape, Q_local.shape,
+                      pol.shape if pol is not None else None, thole.shape if thole is not None else None)
+        return E
+
+    @staticmethod
+    def backward(ctx, gE):
+        g = ctx.g
+        sp, sb, sq, sa, st = ctx.shapes
+        if not g:
+            return None, None, None, None, None, None
+        F = gE.reshape(-1).shape[0]
+        w = gE.reshape(F, 1, 1)
+        gp = (w * g["pos"]).reshape(sp) if ctx.needs_input_grad[0] else None
+        gb = (w * g["box"]).reshape(sb) if ctx.needs_input_grad[1] else None
+        gq = None
+        if ctx.needs_input_grad[2]:
+            q = (w * g["Q"]).sum(0)
+            gq = torch.zeros(sq, dtype=q.dtype, device=q.device)
+            gq[:, : q.shape[1]] = q
+        ga = (gE.reshape(F, 1) * g["pol"]).sum(0).reshape(sa) if (sa is not None and ctx.needs_input_grad[3]) else None
+        gt = (gE.reshape(F, 1) * g["thole"]).sum(0).reshape(st) if (st is not None and ctx.needs_input_grad[4]) else None
+        return gp, gb, gq, ga, gt, None
+
+
+class ADMPPmeForce:
+    """Multipolar PME calculator (dmff/admp/pme.py:36-114 constructor arguments).
+
+    Extra keyword arguments (not in the reference): precision ("fp64" | "mixed"),
+    solver ("reference" | "pcg") and tol for the induced-dipole solve, n_frames
+    for batched evaluation, device.  With the defaults the solver is the
+    reference's: Jacobi to POL_CONV = 1.0 / MAX_N_POL = 30 sweeps, or exactly
+    steps_pol sweeps when steps_pol is set.
+    """
+
+    def __init__(self, box, map_atomtype, axis_type, axis_indices, rc, ethresh, lmax, lpol=False, lpme=True,
+                 steps_pol=None, has_aux=False, *, precision="fp64", solver="reference", tol=None, maxiter=None,
+                 n_frames=1, device="cuda"):
+        self.map_atomtype = map_atomtype
+        self.axis_type = np.asarray(axis_type)
+        self.axis_indices = None if axis_indices is None else np.asarray(axis_indices)
+        self.rc = float(rc)
+        self.ethresh = ethresh
+        self.lmax = int(lmax)
+        self.lpme = lpme
+        if lpme:
+            self.kappa, self.K1, self.K2, self.K3 = setup_ewald_parameters(rc, ethresh, box)
+        else:
+            self.kappa, self.K1, self.K2, self.K3 = 0.0, 0, 0, 0
+        self.pme_order = 6
+        self.lpol = lpol
+        self.steps_pol = steps_pol
+        self.n_atoms = len(axis_type)
+        self.has_aux = has_aux
+        self.precision = precision
+        self.solver = solver
+        self.tol = tol
+        self.maxiter = maxiter
+        self.n_frames = n_frames
+        self.device = device
+        self._engine = None
+        self.last_info = None
+
+    # -------------------------------------------------------------- engine
+    @property
+    def engine(self):
+        if self._engine is None:
+            e = Engine(self.n_atoms, self.lmax, self.lpol, self.lpme, self.kappa, (self.K1, self.K2, self.K3),
+                       self.rc, n_frames=self.n_frames, precision=self.precision, device=self.device)
+            ai = self.axis_indices if self.axis_indices is not None else np.full((self.n_atoms, 3), -1)
+            e.set_topology(self.axis_type, ai)
+            self._engine = e
+        return self._engine
+
+    def update_env(self, attr, val):
+        """pme.py:313-319: change an environment attribute and rebuild."""
+        setattr(self, attr, val)
+        if attr in ("rc", "ethresh", "lpme") and self.lpme and getattr(self, "_box0", None) is not None:
+            self.kappa, self.K1, self.K2, self.K3 = setup_ewald_parameters(self.rc, self.ethresh, self._box0)
+        self._engine = None
+
+    def _solver_args(self):
+        if not self.lpol:
+            return {}
+        if self.steps_pol is not None:
+            return dict(solver="steps", steps=int(self.steps_pol), tol=0.0, maxiter=int(self.steps_pol))
+        if self.solver == "reference":
+            return dict(solver="reference", tol=POL_CONV if self.tol is None else self.tol,
+                        maxiter=MAX_N_POL if self.maxiter is None else self.maxiter)
+        return dict(solver="pcg", tol=1e-6 if self.tol is None else self.tol,
+                    maxiter=500 if self.maxiter is None else self.maxiter)
+
+    def _energy(self, positions, box, pairs, Q_local, pol, tholes, mScales, pScales, dScales, U_init_A):
+        eng = self.engine
+        dev = eng.device
+        if pairs is not None:
+            eng.set_pairs(pairs)
+        pos_t = positions if isinstance(positions, torch.Tensor) else torch.as_tensor(np.asarray(positions), dtype=F64)
+        box_t = box if isinstance(box, torch.Tensor) else torch.as_tensor(np.asarray(box), dtype=F64)
+        Q_t = Q_local if isinstance(Q_local, torch.Tensor) else torch.as_tensor(np.asarray(Q_local), dtype=F64)
+        pol_t = thole_t = None
+        if self.lpol:
+            pol_t = pol if isinstance(pol, torch.Tensor) else torch.as_tensor(np.asarray(pol), dtype=F64)
+            thole_t = tholes if isinstance(tholes, torch.Tensor) else torch.as_tensor(np.asarray(tholes), dtype=F64)
+        sargs = self._solver_args()
+        out = {}
+
+        def run(want_grad):
+            E, g, U, info = eng.compute(pos_t, box_t, Q_t, pol_t, thole_t, mScales, pScales, dScales,
+                                        U_init=U_init_A, want_grad=want_grad, **sargs)
+            out["U"], out["info"] = U, info
+            return (E if self.n_frames > 1 else E.reshape(())), g
+
+        E = _ADMPEnergy.apply(pos_t, box_t, Q_t, pol_t, thole_t, run)
+        self.last_info = out["info"]
+        if E.device != pos_t.device:
+            E = E.to(pos_t.device)
+        return E, out
+
+    # -------------------------------------------------------------- reference API
+    def get_energy(self, positions, box, pairs, Q_local, *args, U_init=None, aux=None):
+        """Non-polarisable: get_energy(positions, box, pairs, Q_local, mScales) (pme.py:120-141).
+        Polarisable: get_energy(positions, box, pairs, Q_local, pol, tholes, mScales, pScales,
+        dScales, U_init=None, aux=None) (pme.py:186-232); U_init in e*nm."""
+        if not self.lpol:
+            (mScales,) = args[:1]
+            E, _ = self._energy(positions, box, pairs, Q_local, None, None, mScales, None, None, None)
+            return E
+        pol, tholes, mScales, pScales, dScales = args[:5]
+        if len(args) > 5:
+            U_init = args[5]
+        U0 = None
+        if U_init is not None:
+            U0 = _dev(U_init, self.engine.device) * 10.0
+        E, out = self._energy(positions, box, pairs, Q_local, pol, tholes, mScales, pScales, dScales, U0)
+        if aux is not None:
+            info = out["info"]
+            U = out["U"] * 0.1
+            aux["U_ind"] = U[0] if self.n_frames == 1 else U
+            aux["lconverg"] = all(info["converged"])
+            aux["n_cycle"] = info["n_iter"][0] if self.n_frames == 1 else info["n_iter"]
+            return E, aux
+        return E
+
+    def get_forces(self, *args, **kwargs):
+        """value_and_grad of get_energy with respect to positions (pme.py:339)."""
+        positions = args[0]
+        pos = (positions if isinstance(positions, torch.Tensor) else torch.as_tensor(np.asarray(positions), dtype=F64))
+        pos = pos.detach().clone().requires_grad_(True)
+        E = self.get_energy(pos, *args[1:], **kwargs)
+        e = E[0] if isinstance(E, tuple) else E
+        (g,) = torch.autograd.grad(e.sum(), pos)
+        return E, g

@@ -1,0 +1,87 @@
+// Shared device definitions for the ADMP multipolar PME kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace admp {
+
+// dmff/common/constants.py:3
+constexpr double DIELECTRIC = 1389.35455846;
+constexpr double SQRT_PI = 1.7724538509055160273;
+constexpr double TWO_OVER_SQRT_PI = 1.1283791670955125739;
+constexpr double PI = 3.14159265358979323846;
+// 2/sqrt(3): weight of the m=1 quadrupole components in the invariant form
+constexpr double K1_INV = 1.1547005383792515290;
+constexpr double SQRT3 = 1.7320508075688772935;
+
+constexpr int NB_SHIFT = 28;                 // neighbour word: j | nbond << 28
+constexpr int NB_MASK = (1 << NB_SHIFT) - 1;
+
+// Per-frame box quantities (all double).  B rows are lattice vectors.
+struct BoxInfo {
+    double B[9];      // B[p*3+q]
+    double Binv[9];   // inverse of B
+    double A[9];      // A[j*3+c] = K_j * Binv[c*3+j]  (fractional grid coordinate R = A r)
+    double V;         // det(B)
+    double kb[9];     // k = kb m : kb[c*3+j] = 2*pi*Binv[c*3+j]
+    double dA[9];     // same as A for the dispersion grid
+    int ortho;        // 1 if B is diagonal
+    int pad;
+};
+
+template <class R> struct Cplx { R x, y; };
+
+template <class R> __device__ __forceinline__ Cplx<R> cmul(Cplx<R> a, Cplx<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// Minimum image exactly as dmff/admp/spatial.py:13-32, in double:
+//   s = dr . Binv ; s -= floor(s + 0.5) ; dr = s . B   (returns n = floor(s+0.5))
+__device__ __forceinline__ void pbc_shift(const BoxInfo& b, double& dx, double& dy, double& dz,
+                                          double& n0, double& n1, double& n2) {
+    double s0 = dx * b.Binv[0] + dy * b.Binv[3] + dz * b.Binv[6];
+    double s1 = dx * b.Binv[1] + dy * b.Binv[4] + dz * b.Binv[7];
+    double s2 = dx * b.Binv[2] + dy * b.Binv[5] + dz * b.Binv[8];
+    n0 = floor(s0 + 0.5);
+    n1 = floor(s1 + 0.5);
+    n2 = floor(s2 + 0.5);
+    s0 -= n0;
+    s1 -= n1;
+    s2 -= n2;
+    dx = s0 * b.B[0] + s1 * b.B[3] + s2 * b.B[6];
+    dy = s0 * b.B[1] + s1 * b.B[4] + s2 * b.B[7];
+    dz = s0 * b.B[2] + s1 * b.B[5] + s2 * b.B[8];
+}
+
+// scale factor for a pair with bonded distance nbond: S[nbond-1] with the
+// reference's Python-style wrap for nbond = 0 (dmff/admp/pme.py:1111-1125)
+__device__ __forceinline__ int scale_index(int nb, int nscale) {
+    int idx = nb - 1;
+    if (idx < 0) idx += nscale;
+    if (idx >= nscale) idx = nscale - 1;
+    if (idx < 0) idx = 0;
+    return idx;
+}
+
+// sum over groups of `width` consecutive lanes (width = power of two <= 64)
+template <class T> __device__ __forceinline__ T group_sum(T v, int width) {
+    for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, width);
+    return v;
+}
+
+// block-wide sum of one double (blockDim.x multiple of 64, <= 1024); result valid in thread 0
+__device__ inline double block_sum(double v, double* sh) {
+    v = group_sum(v, 64);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        for (int k = 0; k < nw; ++k) t += sh[k];
+    }
+    __syncthreads();
+    return t;
+}
+
+}  // namespace admp

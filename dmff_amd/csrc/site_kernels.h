@@ -1,0 +1,610 @@
+// Per-site kernels: box set-up, local frames (forward and backward), site
+// packing, self energy / polarisation penalty, solver vector algebra and
+// deterministic reductions.
+#pragma once
+#include "admp_common.h"
+#include "pair_kernels.h"
+
+namespace admp {
+
+// 10-digit sqrt(3) of the reference's harmonic conversions (dmff/admp/multipole.py:12)
+constexpr double RT3_LIT = 1.73205080757;
+
+__global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K1, int K2, int dK0, int dK1,
+                            int dK2, BoxInfo* __restrict__ out) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const double* B = box + (size_t)f * 9;
+    BoxInfo b;
+    for (int k = 0; k < 9; ++k) b.B[k] = B[k];
+    const double det = B[0] * (B[4] * B[8] - B[5] * B[7]) - B[1] * (B[3] * B[8] - B[5] * B[6]) +
+                       B[2] * (B[3] * B[7] - B[4] * B[6]);
+    const double id = 1.0 / det;
+    b.Binv[0] = (B[4] * B[8] - B[5] * B[7]) * id;
+    b.Binv[1] = (B[2] * B[7] - B[1] * B[8]) * id;
+    b.Binv[2] = (B[1] * B[5] - B[2] * B[4]) * id;
+    b.Binv[3] = (B[5] * B[6] - B[3] * B[8]) * id;
+    b.Binv[4] = (B[0] * B[8] - B[2] * B[6]) * id;
+    b.Binv[5] = (B[2] * B[3] - B[0] * B[5]) * id;
+    b.Binv[6] = (B[3] * B[7] - B[4] * B[6]) * id;
+    b.Binv[7] = (B[1] * B[6] - B[0] * B[7]) * id;
+    b.Binv[8] = (B[0] * B[4] - B[1] * B[3]) * id;
+    b.V = det;
+    const int K[3] = {K0, K1, K2}, dK[3] = {dK0, dK1, dK2};
+    for (int j = 0; j < 3; ++j)
+        for (int c = 0; c < 3; ++c) {
+            b.A[j * 3 + c] = K[j] * b.Binv[c * 3 + j];
+            b.dA[j * 3 + c] = dK[j] * b.Binv[c * 3 + j];
+            b.kb[c * 3 + j] = 2.0 * PI * b.Binv[c * 3 + j];
+        }
+    b.ortho = (B[1] == 0 && B[2] == 0 && B[3] == 0 && B[5] == 0 && B[6] == 0 && B[7] == 0) ? 1 : 0;
+    b.pad = 0;
+    out[f] = b;
+}
+
+__global__ void k_pack_pos(size_t n, const double* __restrict__ p, double4* __restrict__ o) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = make_double4(p[i * 3], p[i * 3 + 1], p[i * 3 + 2], 0.0);
+}
+
+// ----------------------------------------------------------------- local frames
+struct FrameFwd {
+    double vz[3], vx[3], vy[3];        // raw (minimum-image) vectors
+    double nvz, nvx, nvy;              // their norms
+    double z0[3], x0[3], y0[3];        // normalised
+    double zs[3], nzs;                 // un-normalised z sum (bisector / three-fold)
+    double xs[3], nxs;                 // un-normalised x sum (z-bisect)
+    double z1[3], x1[3];
+    double xg[3], nxg, p;
+    double X[3], Y[3];
+    double sz[3], sx[3], sy[3];        // pbc shifts of the raw vectors
+};
+
+__device__ inline void d_norm3(const double v[3], double o[3], double& n) {
+    n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    o[0] = v[0] / n; o[1] = v[1] / n; o[2] = v[2] / n;
+}
+
+__device__ inline void raw_vec(const BoxInfo& b, const double4* pos, size_t fo, int from, int to, double v[3], double s[3]) {
+    double4 a = pos[fo + to], c = pos[fo + from];
+    v[0] = a.x - c.x; v[1] = a.y - c.y; v[2] = a.z - c.z;
+    pbc_shift(b, v[0], v[1], v[2], s[0], s[1], s[2]);
+}
+
+// dmff/admp/spatial.py:77-151 for one site
+__device__ inline void frame_forward(int type, const BoxInfo& b, const double4* pos, size_t fo, int i, int zi, int xi,
+                                     int yi, FrameFwd& F) {
+    for (int k = 0; k < 3; ++k) { F.sz[k] = F.sx[k] = F.sy[k] = 0; F.vy[k] = F.y0[k] = 0; }
+    if (type == 5) {  // NoAxisType
+        F.X[0] = 1; F.X[1] = 0; F.X[2] = 0;
+        F.Y[0] = 0; F.Y[1] = 1; F.Y[2] = 0;
+        F.z1[0] = 0; F.z1[1] = 0; F.z1[2] = 1;
+        return;
+    }
+    raw_vec(b, pos, fo, i, zi, F.vz, F.sz);
+    d_norm3(F.vz, F.z0, F.nvz);
+    if (type == 4) {  // Z-only
+        double xz = rint(fabs(F.z0[0]));
+        F.x0[0] = 1.0 - xz; F.x0[1] = xz; F.x0[2] = 0.0;
+    } else {
+        raw_vec(b, pos, fo, i, xi, F.vx, F.sx);
+        d_norm3(F.vx, F.x0, F.nvx);
+    }
+    for (int k = 0; k < 3; ++k) { F.z1[k] = F.z0[k]; F.x1[k] = F.x0[k]; }
+    if (type == 2 || type == 3) {
+        raw_vec(b, pos, fo, i, yi, F.vy, F.sy);
+        d_norm3(F.vy, F.y0, F.nvy);
+    }
+    if (type == 1) {  // Bisector
+        for (int k = 0; k < 3; ++k) F.zs[k] = F.z0[k] + F.x0[k];
+        d_norm3(F.zs, F.z1, F.nzs);
+    } else if (type == 2) {  // Z-bisect
+        for (int k = 0; k < 3; ++k) F.xs[k] = F.x0[k] + F.y0[k];
+        d_norm3(F.xs, F.x1, F.nxs);
+    } else if (type == 3) {  // Three-fold
+        for (int k = 0; k < 3; ++k) F.zs[k] = F.z0[k] + F.x0[k] + F.y0[k];
+        d_norm3(F.zs, F.z1, F.nzs);
+    }
+    F.p = F.x1[0] * F.z1[0] + F.x1[1] * F.z1[1] + F.x1[2] * F.z1[2];
+    for (int k = 0; k < 3; ++k) F.xg[k] = F.x1[k] - F.z1[k] * F.p;
+    d_norm3(F.xg, F.X, F.nxg);
+    F.Y[0] = F.z1[1] * F.X[2] - F.z1[2] * F.X[1];
+    F.Y[1] = F.z1[2] * F.X[0] - F.z1[0] * F.X[2];
+    F.Y[2] = F.z1[0] * F.X[1] - F.z1[1] * F.X[0];
+}
+
+__device__ inline void norm_bwd(const double n[3], double len, const double g[3], double out[3]) {
+    double d = n[0] * g[0] + n[1] * g[1] + n[2] * g[2];
+    for (int k = 0; k < 3; ++k) out[k] = (g[k] - n[k] * d) / len;
+}
+
+// harmonic local multipoles -> Cartesian local (C2_h2c with the rt3 literal)
+__device__ inline void harm2cart_local(const double* h, int nh, double& q, double mu[3], double T[3][3]) {
+    q = h[0];
+    mu[0] = mu[1] = mu[2] = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) T[a][c] = 0;
+    if (nh >= 4) { mu[0] = h[2]; mu[1] = h[3]; mu[2] = h[1]; }
+    if (nh >= 9) {
+        const double r2 = RT3_LIT / 2;
+        T[0][0] = -0.5 * h[4] + r2 * h[7];
+        T[1][1] = -0.5 * h[4] - r2 * h[7];
+        T[2][2] = h[4];
+        T[0][1] = T[1][0] = r2 * h[8];
+        T[0][2] = T[2][0] = r2 * h[5];
+        T[1][2] = T[2][1] = r2 * h[6];
+    }
+}
+
+// Q_global (Cartesian, 10 words) = rotate(Q_local) with rows of Rm = local axes
+__global__ void k_frames_fwd(int N, int nh, int use_frames, const double4* __restrict__ pos,
+                             const BoxInfo* __restrict__ boxes, const int* __restrict__ axis_type,
+                             const int* __restrict__ axis_idx, const double* __restrict__ Q_local,
+                             double* __restrict__ frames, double* __restrict__ Qc) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t fo = (size_t)f * N, a = fo + i;
+    double Rm[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    if (use_frames) {
+        FrameFwd F;
+        frame_forward(axis_type[i], boxes[f], pos, fo, i, axis_idx[i * 3], axis_idx[i * 3 + 1], axis_idx[i * 3 + 2], F);
+        for (int k = 0; k < 3; ++k) { Rm[0][k] = F.X[k]; Rm[1][k] = F.Y[k]; Rm[2][k] = F.z1[k]; }
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) frames[a * 9 + r * 3 + c] = Rm[r][c];
+    double q, mu[3], T[3][3];
+    harm2cart_local(Q_local + (size_t)i * nh, nh, q, mu, T);
+    double mg[3], Tg[3][3];
+    for (int c = 0; c < 3; ++c) mg[c] = Rm[0][c] * mu[0] + Rm[1][c] * mu[1] + Rm[2][c] * mu[2];
+    double TR[3][3];
+    for (int x = 0; x < 3; ++x)
+        for (int d = 0; d < 3; ++d) TR[x][d] = T[x][0] * Rm[0][d] + T[x][1] * Rm[1][d] + T[x][2] * Rm[2][d];
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d) Tg[c][d] = Rm[0][c] * TR[0][d] + Rm[1][c] * TR[1][d] + Rm[2][c] * TR[2][d];
+    double* o = Qc + a * 10;
+    o[0] = q; o[1] = mg[0]; o[2] = mg[1]; o[3] = mg[2];
+    o[4] = Tg[0][0]; o[5] = Tg[1][1]; o[6] = Tg[2][2]; o[7] = Tg[0][1]; o[8] = Tg[0][2]; o[9] = Tg[1][2];
+}
+
+// Backward through rotation and frame construction.  gQc (10): dE/d(q, mu, Theta
+// matrix entries).  Writes dE/dQ_local (nh), per-site contributions to the
+// gradient of the positions of (site, z, x, y) atoms (12 words) and the frame
+// part of dE/dbox (9 words).
+__global__ void k_frames_bwd(int N, int nh, int use_frames, const double4* __restrict__ pos,
+                             const BoxInfo* __restrict__ boxes, const int* __restrict__ axis_type,
+                             const int* __restrict__ axis_idx, const double* __restrict__ Q_local,
+                             const double* __restrict__ gQc, double* __restrict__ gQl,
+                             double* __restrict__ contrib, double* __restrict__ vir) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t fo = (size_t)f * N, a = fo + i;
+    FrameFwd F;
+    double Rm[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const int type = use_frames ? axis_type[i] : 5;
+    if (use_frames) {
+        frame_forward(type, boxes[f], pos, fo, i, axis_idx[i * 3], axis_idx[i * 3 + 1], axis_idx[i * 3 + 2], F);
+        for (int k = 0; k < 3; ++k) { Rm[0][k] = F.X[k]; Rm[1][k] = F.Y[k]; Rm[2][k] = F.z1[k]; }
+    }
+    double q, mu[3], T[3][3];
+    harm2cart_local(Q_local + (size_t)i * nh, nh, q, mu, T);
+    const double* g = gQc + a * 10;
+    double gmu_g[3] = {g[1], g[2], g[3]};
+    double G[3][3];
+    G[0][0] = g[4]; G[1][1] = g[5]; G[2][2] = g[6];
+    G[0][1] = G[1][0] = g[7]; G[0][2] = G[2][0] = g[8]; G[1][2] = G[2][1] = g[9];
+    // local gradients
+    double gmu_l[3];
+    for (int r = 0; r < 3; ++r) gmu_l[r] = Rm[r][0] * gmu_g[0] + Rm[r][1] * gmu_g[1] + Rm[r][2] * gmu_g[2];
+    double RG[3][3], Gl[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int d = 0; d < 3; ++d) RG[r][d] = Rm[r][0] * G[0][d] + Rm[r][1] * G[1][d] + Rm[r][2] * G[2][d];
+    for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s) Gl[r][s] = RG[r][0] * Rm[s][0] + RG[r][1] * Rm[s][1] + RG[r][2] * Rm[s][2];
+    double* ol = gQl + a * nh;
+    ol[0] = g[0];
+    if (nh >= 4) { ol[1] = gmu_l[2]; ol[2] = gmu_l[0]; ol[3] = gmu_l[1]; }
+    if (nh >= 9) {
+        const double r2 = RT3_LIT / 2;
+        ol[4] = -0.5 * Gl[0][0] - 0.5 * Gl[1][1] + Gl[2][2];
+        ol[7] = r2 * (Gl[0][0] - Gl[1][1]);
+        ol[8] = 2.0 * r2 * Gl[0][1];
+        ol[5] = 2.0 * r2 * Gl[0][2];
+        ol[6] = 2.0 * r2 * Gl[1][2];
+    }
+    double* cb = contrib + a * 12;
+    double* vb = vir + a * 9;
+    for (int k = 0; k < 12; ++k) cb[k] = 0;
+    for (int k = 0; k < 9; ++k) vb[k] = 0;
+    if (!use_frames || type == 5) return;
+    // dE/dRm[r][c] = mu_l[r] gmu_g[c] + 2 (T Rm G)[r][c]
+    double TRG[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) TRG[r][c] = T[r][0] * RG[0][c] + T[r][1] * RG[1][c] + T[r][2] * RG[2][c];
+    double gX[3], gY[3], gZ[3];
+    for (int c = 0; c < 3; ++c) {
+        gX[c] = mu[0] * gmu_g[c] + 2.0 * TRG[0][c];
+        gY[c] = mu[1] * gmu_g[c] + 2.0 * TRG[1][c];
+        gZ[c] = mu[2] * gmu_g[c] + 2.0 * TRG[2][c];
+    }
+    // Y = z1 x X
+    double gz1[3], gXt[3];
+    gz1[0] = F.X[1] * gY[2] - F.X[2] * gY[1];
+    gz1[1] = F.X[2] * gY[0] - F.X[0] * gY[2];
+    gz1[2] = F.X[0] * gY[1] - F.X[1] * gY[0];
+    gXt[0] = gX[0] + (gY[1] * F.z1[2] - gY[2] * F.z1[1]);
+    gXt[1] = gX[1] + (gY[2] * F.z1[0] - gY[0] * F.z1[2]);
+    gXt[2] = gX[2] + (gY[0] * F.z1[1] - gY[1] * F.z1[0]);
+    for (int k = 0; k < 3; ++k) gz1[k] += gZ[k];
+    double gxg[3];
+    norm_bwd(F.X, F.nxg, gXt, gxg);
+    // xg = x1 - z1 (x1.z1)
+    const double gz = gxg[0] * F.z1[0] + gxg[1] * F.z1[1] + gxg[2] * F.z1[2];
+    double gx1[3];
+    for (int k = 0; k < 3; ++k) {
+        gx1[k] = gxg[k] - F.z1[k] * gz;
+        gz1[k] += -F.p * gxg[k] - F.x1[k] * gz;
+    }
+    double gz0[3] = {0, 0, 0}, gx0[3] = {0, 0, 0}, gy0[3] = {0, 0, 0};
+    if (type == 1) {
+        double gzs[3];
+        norm_bwd(F.z1, F.nzs, gz1, gzs);
+        for (int k = 0; k < 3; ++k) { gz0[k] = gzs[k]; gx0[k] = gx1[k] + gzs[k]; }
+    } else if (type == 2) {
+        double gxs[3];
+        norm_bwd(F.x1, F.nxs, gx1, gxs);
+        for (int k = 0; k < 3; ++k) { gz0[k] = gz1[k]; gx0[k] = gxs[k]; gy0[k] = gxs[k]; }
+    } else if (type == 3) {
+        double gzs[3];
+        norm_bwd(F.z1, F.nzs, gz1, gzs);
+        for (int k = 0; k < 3; ++k) { gz0[k] = gzs[k]; gx0[k] = gx1[k] + gzs[k]; gy0[k] = gzs[k]; }
+    } else {
+        for (int k = 0; k < 3; ++k) { gz0[k] = gz1[k]; gx0[k] = gx1[k]; }
+    }
+    double gvz[3], gvx[3] = {0, 0, 0}, gvy[3] = {0, 0, 0};
+    norm_bwd(F.z0, F.nvz, gz0, gvz);
+    if (type != 4) norm_bwd(F.x0, F.nvx, gx0, gvx);
+    if (type == 2 || type == 3) norm_bwd(F.y0, F.nvy, gy0, gvy);
+    // raw vector v = pos[other] - pos[i] - n B
+    for (int k = 0; k < 3; ++k) {
+        cb[0 + k] = -(gvz[k] + gvx[k] + gvy[k]);
+        cb[3 + k] = gvz[k];
+        cb[6 + k] = gvx[k];
+        cb[9 + k] = gvy[k];
+    }
+    for (int p = 0; p < 3; ++p)
+        for (int c = 0; c < 3; ++c) vb[p * 3 + c] = -(F.sz[p] * gvz[c] + F.sx[p] * gvx[c] + F.sy[p] * gvy[c]);
+}
+
+// dpos[j] += own contribution + contributions of sites whose frame uses j
+// (rev_off/rev_ent: static reverse map, entry = site*4 + slot).
+__global__ void k_frames_gather(int N, const int* __restrict__ rev_off, const int* __restrict__ rev_ent,
+                                const double* __restrict__ contrib, double* __restrict__ gpos) {
+    const int f = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const size_t fo = (size_t)f * N;
+    double s0 = contrib[(fo + j) * 12 + 0], s1 = contrib[(fo + j) * 12 + 1], s2 = contrib[(fo + j) * 12 + 2];
+    for (int e = rev_off[j]; e < rev_off[j + 1]; ++e) {
+        const int ent = rev_ent[e];
+        const int site = ent >> 2, slot = ent & 3;
+        const double* c = contrib + (fo + site) * 12 + slot * 3;
+        s0 += c[0]; s1 += c[1]; s2 += c[2];
+    }
+    gpos[(fo + j) * 3 + 0] += s0;
+    gpos[(fo + j) * 3 + 1] += s1;
+    gpos[(fo + j) * 3 + 2] += s2;
+}
+
+// ----------------------------------------------------------------- site packing
+// site record for the pair kernels (see pair_kernels.h)
+template <class R>
+__global__ void k_pack_site(int N, const double* __restrict__ Qc, const double* __restrict__ pol,
+                            const double* __restrict__ thole, const double* __restrict__ U, R* __restrict__ site) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    R* s = site + a * SITE;
+    const double* c = Qc + a * 10;
+    for (int k = 0; k < 10; ++k) s[k] = (R)c[k];
+    const double al = pol ? pol[i] : 0.0;
+    s[10] = (R)(al > 0 ? pow(al, 1.0 / 6.0) : 0.0);
+    s[11] = (R)(thole ? thole[i] : 0.0);
+    s[12] = (R)al;
+    s[13] = (R)(U ? U[a * 3 + 0] : 0.0);
+    s[14] = (R)(U ? U[a * 3 + 1] : 0.0);
+    s[15] = (R)(U ? U[a * 3 + 2] : 0.0);
+}
+
+template <class R> __global__ void k_cast(size_t n, const double* __restrict__ in, R* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (R)in[i];
+}
+
+// Self energy (pme.py:1183-1222) on Q + U and polarisation penalty (pme.py:1225-1240):
+// per-site energy, and their gradients added into gQc / gpol.
+__global__ void k_self(int N, double kappa, int lpme, int lpol, const double* __restrict__ Qc,
+                       const double* __restrict__ U, const double* __restrict__ pol, double* __restrict__ e_self,
+                       double* __restrict__ e_pol, double* __restrict__ gQc, double* __restrict__ gpol) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    const double* c = Qc + a * 10;
+    double mu[3] = {c[1], c[2], c[3]};
+    if (lpol) { mu[0] += U[a * 3]; mu[1] += U[a * 3 + 1]; mu[2] += U[a * 3 + 2]; }
+    double es = 0;
+    if (lpme) {
+        const double f0 = kappa / SQRT_PI;
+        const double f1 = f0 * 2.0 * kappa * kappa / 3.0;
+        const double f2 = f0 * 4.0 * kappa * kappa * kappa * kappa / 15.0;
+        const double ThTh = c[4] * c[4] + c[5] * c[5] + c[6] * c[6] + 2.0 * (c[7] * c[7] + c[8] * c[8] + c[9] * c[9]);
+        const double m2 = mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2];
+        es = -DIELECTRIC * (f0 * c[0] * c[0] + f1 * m2 + f2 * (2.0 / 3.0) * ThTh);
+        double* g = gQc + a * 10;
+        g[0] += -2.0 * DIELECTRIC * f0 * c[0];
+        for (int k = 0; k < 3; ++k) g[1 + k] += -2.0 * DIELECTRIC * f1 * mu[k];
+        for (int k = 0; k < 6; ++k) g[4 + k] += -(4.0 / 3.0) * DIELECTRIC * f2 * c[4 + k];
+    }
+    e_self[a] = es;
+    if (lpol) {
+        const double al = pol[i];
+        const double ap = al < 1e-8 ? 1e-8 : al;
+        const double u2 = U[a * 3] * U[a * 3] + U[a * 3 + 1] * U[a * 3 + 1] + U[a * 3 + 2] * U[a * 3 + 2];
+        e_pol[a] = DIELECTRIC * 0.5 * u2 / ap;
+        if (gpol && al >= 1e-8) gpol[a] += -DIELECTRIC * 0.5 * u2 / (al * al);
+    } else {
+        e_pol[a] = 0.0;
+    }
+}
+
+// ----------------------------------------------------------------- reductions
+// out[f*ncol + c] = sum_i in[(f*N + i)*ncol + c]   (one block per (column, frame), fixed order)
+__global__ void k_reduce_cols(int N, int ncol, const double* __restrict__ in, double* __restrict__ out, int accumulate) {
+    __shared__ double sh[16];
+    const int c = blockIdx.x, f = blockIdx.y;
+    double s = 0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) s += in[((size_t)f * N + i) * ncol + c];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        if (accumulate) out[(size_t)f * ncol + c] += s;
+        else out[(size_t)f * ncol + c] = s;
+    }
+}
+
+// ----------------------------------------------------------------- solver algebra (FP64)
+// y = sum of terms: out = real + recip + diag * v   where diag_i = D/max(pol,1e-8) - 2 D f1
+__global__ void k_apply_diag(int N, double f1, const double* __restrict__ pol, const double* __restrict__ v,
+                             const double* __restrict__ real, const double* __restrict__ recip, double* __restrict__ out) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
+    const double d = DIELECTRIC / ap - 2.0 * DIELECTRIC * f1;
+    for (int k = 0; k < 3; ++k) {
+        double r = real ? real[a * 3 + k] : 0.0;
+        double q = recip ? recip[a * 3 + k] : 0.0;
+        out[a * 3 + k] = r + q + d * v[a * 3 + k];
+    }
+}
+
+// per-frame partial dot products / max norms: part[(f*nb + b)*3 + {0,1,2}]
+// = (sum x.y, sum x.z, max over polarisable sites |x component|)
+__global__ void k_dots(int N, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                       const double* __restrict__ pol, double* __restrict__ part) {
+    __shared__ double sh[16];
+    __shared__ double shm[16];
+    const int f = blockIdx.y;
+    double sxy = 0, sxz = 0, mx = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const size_t a = (size_t)f * N + i;
+        for (int k = 0; k < 3; ++k) {
+            const double xv = x[a * 3 + k];
+            if (y) sxy += xv * y[a * 3 + k];
+            if (z) sxz += xv * z[a * 3 + k];
+            if (pol[i] > 0.001) mx = fmax(mx, fabs(xv));
+        }
+    }
+    sxy = block_sum(sxy, sh);
+    sxz = block_sum(sxz, sh);
+    // block max
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m = fmax(m, shm[k]);
+        double* p = part + ((size_t)f * gridDim.x + blockIdx.x) * 3;
+        p[0] = sxy; p[1] = sxz; p[2] = m;
+    }
+}
+
+__global__ void k_dots_final(int F, int nb, const double* __restrict__ part, double* __restrict__ out) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    double a = 0, b = 0, m = 0;
+    for (int k = 0; k < nb; ++k) {
+        const double* p = part + ((size_t)f * nb + k) * 3;
+        a += p[0]; b += p[1]; m = fmax(m, p[2]);
+    }
+    out[f * 3 + 0] = a; out[f * 3 + 1] = b; out[f * 3 + 2] = m;
+}
+
+// PCG update step for frame f using coefficients in coef[f*4..]: see admp_api.hip
+//   mode 0: x += alpha p ; r -= alpha Ap ; z = Minv r          (alpha = coef[0])
+//   mode 1: p = z + beta p                                       (beta = coef[1])
+__global__ void k_pcg_update(int N, int mode, const double* __restrict__ coef, const int* __restrict__ active,
+                             const double* __restrict__ pol, double* __restrict__ x, double* __restrict__ r,
+                             double* __restrict__ z, double* __restrict__ p, const double* __restrict__ Ap) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || !active[f]) return;
+    const size_t a = (size_t)f * N + i;
+    const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
+    if (mode == 0) {
+        const double al = coef[f * 4 + 0];
+        for (int k = 0; k < 3; ++k) {
+            x[a * 3 + k] += al * p[a * 3 + k];
+            r[a * 3 + k] -= al * Ap[a * 3 + k];
+            z[a * 3 + k] = ap / DIELECTRIC * r[a * 3 + k];
+        }
+    } else {
+        const double be = coef[f * 4 + 1];
+        for (int k = 0; k < 3; ++k) p[a * 3 + k] = z[a * 3 + k] + be * p[a * 3 + k];
+    }
+}
+
+// generic elementwise: out = a*x + b*y (+ c*z)
+__global__ void k_lincomb(size_t n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                          double* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i] + (y ? b * y[i] : 0.0);
+}
+
+// Jacobi update (pme.py:395): U -= g * pol / DIELECTRIC  for active frames
+__global__ void k_jacobi(int N, const int* __restrict__ active, const double* __restrict__ pol,
+                         const double* __restrict__ g, double* __restrict__ U) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || !active[f]) return;
+    const size_t a = (size_t)f * N + i;
+    for (int k = 0; k < 3; ++k) U[a * 3 + k] -= g[a * 3 + k] * pol[i] / DIELECTRIC;
+}
+
+// minus-in-place
+__global__ void k_sub(size_t n, const double* __restrict__ b, double* __restrict__ x) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] -= b[i];
+}
+
+// dE/dbox assembly: vir (F,9) holds the real+frame+A-term sums (A term as dE/dA);
+// here: dbox = vir_direct - Binv^T H Binv^T + (-E_rec Binv^T + 2 Binv^T W)
+// H = (diag(K) dEdA)^T.
+__global__ void k_box_grad(int F, const BoxInfo* __restrict__ boxes, const double* __restrict__ vir_direct,
+                           const double* __restrict__ dEdA, const double* __restrict__ erec,
+                           const double* __restrict__ W6, int K0, int K1, int K2, int use_dA, double* __restrict__ dbox,
+                           int accumulate) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const BoxInfo& b = boxes[f];
+    double BiT[3][3];
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) BiT[p][q] = b.Binv[q * 3 + p];
+    double out[3][3];
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) out[p][q] = vir_direct ? vir_direct[f * 9 + p * 3 + q] : 0.0;
+    if (dEdA) {
+        const int K[3] = {K0, K1, K2};
+        double H[3][3];  // H[c][j] = dEdA[j][c] K_j
+        for (int c = 0; c < 3; ++c)
+            for (int j = 0; j < 3; ++j) H[c][j] = dEdA[f * 9 + j * 3 + c] * K[j];
+        double HB[3][3];
+        for (int c = 0; c < 3; ++c)
+            for (int q = 0; q < 3; ++q) HB[c][q] = H[c][0] * BiT[0][q] + H[c][1] * BiT[1][q] + H[c][2] * BiT[2][q];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) out[p][q] -= BiT[p][0] * HB[0][q] + BiT[p][1] * HB[1][q] + BiT[p][2] * HB[2][q];
+    }
+    if (W6) {
+        const double* w = W6 + f * 7;  // e, w00, w11, w22, w01, w02, w12
+        const double E = w[0];
+        double Wm[3][3];
+        Wm[0][0] = w[1]; Wm[1][1] = w[2]; Wm[2][2] = w[3];
+        Wm[0][1] = Wm[1][0] = w[4]; Wm[0][2] = Wm[2][0] = w[5]; Wm[1][2] = Wm[2][1] = w[6];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q)
+                out[p][q] += -E * BiT[p][q] + 2.0 * (BiT[p][0] * Wm[0][q] + BiT[p][1] * Wm[1][q] + BiT[p][2] * Wm[2][q]);
+    }
+    for (int k = 0; k < 9; ++k) {
+        if (accumulate) dbox[f * 9 + k] += out[k / 3][k % 3];
+        else dbox[f * 9 + k] = out[k / 3][k % 3];
+    }
+}
+
+
+// b += 2 D f1 mu_perm (self-energy part of -dE/dU at U = 0)
+__global__ void k_add_self_b(int N, double f1, const double* __restrict__ Qc, double* __restrict__ b) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    for (int k = 0; k < 3; ++k) b[a * 3 + k] += 2.0 * DIELECTRIC * f1 * Qc[a * 10 + 1 + k];
+}
+
+// z = Minv r, p = z   (Minv = pol/DIELECTRIC)
+__global__ void k_precond(int N, const double* __restrict__ pol, const double* __restrict__ r, double* __restrict__ z,
+                          double* __restrict__ p) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
+    for (int k = 0; k < 3; ++k) {
+        const double v = ap / DIELECTRIC * r[a * 3 + k];
+        z[a * 3 + k] = v;
+        if (p) p[a * 3 + k] = v;
+    }
+}
+
+// energy[f] = sum of the per-frame components in red[f*16 + 0..3] (+ rec[f*7])
+__global__ void k_energy_sum(int F, const double* __restrict__ red, const double* __restrict__ rec, double* __restrict__ e) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    e[f] = red[f * 16 + 0] + red[f * 16 + 1] + red[f * 16 + 2] + (rec ? rec[f * 7] : 0.0);
+}
+
+// out = x + y (+ z)
+__global__ void k_add3(size_t n, const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                       double* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = x[i] + (y ? y[i] : 0.0) + (z ? z[i] : 0.0);
+}
+
+// strided copy of one column-block: out[a*w + k] = in[a*win + off + k]
+__global__ void k_copy_cols(size_t n, int w, int win, int off, const double* __restrict__ in, double* __restrict__ out) {
+    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    for (int k = 0; k < w; ++k) out[a * w + k] = in[a * win + off + k];
+}
+
+// Qd[a*10] = (c[a*3 + col], 0 ...)  : a dispersion coefficient as a charge-only source
+__global__ void k_disp_source(int N, int col, const double* __restrict__ c, double* __restrict__ Qd) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    Qd[a * 10] = c[(size_t)i * 3 + col];
+    for (int k = 1; k < 10; ++k) Qd[a * 10 + k] = 0.0;
+}
+
+// dispersion self energy (disp_pme.py:264-289) per site and its c-gradient
+__global__ void k_disp_self(int N, double kappa, int pmax, const double* __restrict__ c, double* __restrict__ e_self,
+                            double* __restrict__ gc) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    const double k2 = kappa * kappa, k6 = k2 * k2 * k2, k8 = k6 * k2, k10 = k8 * k2;
+    const double c6 = c[i * 3], c8 = c[i * 3 + 1], c10 = c[i * 3 + 2];
+    double e = -k6 / 12.0 * c6 * c6;
+    gc[a * 3 + 0] += -k6 / 6.0 * c6;
+    if (pmax >= 8) { e += -k8 / 48.0 * c8 * c8; gc[a * 3 + 1] += -k8 / 24.0 * c8; }
+    if (pmax >= 10) { e += -k10 / 240.0 * c10 * c10; gc[a * 3 + 2] += -k10 / 120.0 * c10; }
+    e_self[a] = e;
+}
+
+// gc[a*3 + col] += gq[a*10]   (recip charge gradient of one dispersion grid)
+__global__ void k_disp_collect(size_t n, int col, const double* __restrict__ gq, double* __restrict__ gc) {
+    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) gc[a * 3 + col] += gq[a * 10];
+}
+
+// generic per-frame summed vector add: out[f*w+k] += in[f*w+k]
+__global__ void k_vadd(size_t n, const double* __restrict__ in, double* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += in[i];
+}
+
+}  // namespace admp

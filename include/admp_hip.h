@@ -1,0 +1,166 @@
+/*
+ * admp_hip.h -- C ABI of the MI355X (gfx950) ADMP multipolar PME library.
+ *
+ * The library evaluates DMFF's ADMP multipolar PME energy, the induced-dipole
+ * solve, dispersion PME and the Slater short-range term, plus their gradients,
+ * with hand-written HIP kernels.  It is the drop-in boundary for the reference's
+ * ADMPPmeForce / ADMPDispPmeForce backends:
+ *
+ *   reference                                    this ABI
+ *   dmff/admp/pme.py:36-114   ADMPPmeForce(...)    admp_ctx_create + admp_set_topology
+ *   dmff/common/nblist.py:27-55 NeighborList.allocate  admp_build_neighbors + admp_get_pairs
+ *   dmff/admp/pme.py:186-232  get_energy (pol)     admp_set_pairs + admp_compute
+ *   dmff/admp/pme.py:120-141  get_energy (non-pol) admp_set_pairs + admp_compute
+ *   dmff/admp/pme.py:339      get_forces = value_and_grad   admp_compute(want_grad=1)
+ *   dmff/admp/pme.py:342-423  optimize_Uind        admp_compute (solver options)
+ *   dmff/admp/disp_pme.py:13-77 ADMPDispPmeForce.get_energy  admp_disp_compute
+ *   dmff/admp/pairwise.py:47-95,156-169 slater_sr_kernel       admp_slater_compute
+ *
+ * Conventions
+ *   - Units are the reference backend's: Angstrom, e, kJ/mol (DIELECTRIC =
+ *     1389.35455846).  Conversion from nm happens in the Python host layer.
+ *   - Every array argument of the compute entry points is a DEVICE pointer owned
+ *     by the caller, in row-major order, float64 unless stated.  Host pointers
+ *     are marked (host).
+ *   - Boxes are 3x3 with the lattice vectors in rows (reference convention).
+ *   - A context owns all scratch (neighbour lists, grids, FFT twiddles, solver
+ *     vectors).  Calls on one context are asynchronous on its stream except where
+ *     a function documents a host synchronisation.  One context per thread.
+ *   - Every function returns an admp_status; 0 is success.  No C++ exception
+ *     crosses the ABI; admp_last_error() describes the last failure.
+ *   - A batch of n_frames configurations of the same topology can be evaluated
+ *     in one call: per-frame arrays are stacked on a leading frame axis.
+ */
+#ifndef ADMP_HIP_H
+#define ADMP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ADMP_OK = 0,
+    ADMP_ERR_ARG = 1,          /* bad argument / shape                         */
+    ADMP_ERR_CAPACITY = 2,     /* neighbour capacity exceeded (count reported)  */
+    ADMP_ERR_HIP = 3,          /* HIP runtime error                             */
+    ADMP_ERR_STATE = 4,        /* call order (e.g. compute before topology)     */
+    ADMP_WARN_NOT_CONVERGED = 5/* solver hit max iterations; results are valid  */
+} admp_status;
+
+typedef enum { ADMP_FP64 = 0, ADMP_MIXED = 1 } admp_precision;
+
+/* Induced-dipole solver.
+ *  ADMP_SOLVER_REFERENCE: the reference's Jacobi update U <- U - dE/dU*pol/DIELECTRIC,
+ *      stop when max|dE/dU| over sites with pol > 0.001 A^3 is below tol (reference
+ *      POL_CONV = 1.0) or after maxiter (MAX_N_POL = 30) sweeps (pme.py:380-401).
+ *  ADMP_SOLVER_STEPS: exactly `steps` Jacobi sweeps (pme.py:404-422, steps_pol).
+ *  ADMP_SOLVER_PCG: preconditioned conjugate gradients to the same fixed point,
+ *      stop on the same max-norm criterion (used for tight tolerances). */
+typedef enum { ADMP_SOLVER_REFERENCE = 0, ADMP_SOLVER_STEPS = 1, ADMP_SOLVER_PCG = 2 } admp_solver_kind;
+
+typedef struct {
+    int32_t n_atoms;
+    int32_t n_frames;          /* batch size (>= 1)                           */
+    int32_t lmax;              /* 0, 1 or 2                                   */
+    int32_t lpol;              /* induced dipoles on/off                      */
+    int32_t lpme;              /* 0: kappa = 0 and no reciprocal/self (pme.py:83-88) */
+    int32_t precision;         /* admp_precision                              */
+    double  kappa;             /* 1/A (from the ethresh rule, pme.py:460-466) */
+    int32_t K1, K2, K3;        /* PME grid                                    */
+    int32_t pme_order;         /* must be 6 (recip.py:13)                     */
+    double  rc;                /* neighbour cutoff, A                         */
+    int32_t max_neighbors;     /* per-atom full-list capacity (0 = estimate)  */
+    /* dispersion PME (0 grid = disabled) */
+    double  disp_kappa;
+    int32_t dK1, dK2, dK3;
+    int32_t disp_pmax;         /* 6, 8 or 10                                  */
+} admp_config;
+
+typedef struct {
+    int32_t kind;              /* admp_solver_kind                            */
+    int32_t maxiter;
+    int32_t steps;
+    double  tol;               /* max |dE/dU| threshold, kJ/mol/(e*A)         */
+    int32_t warm_start;        /* 1: use U_inout as the initial guess         */
+} admp_solver_opts;
+
+typedef struct {
+    /* filled on return of admp_compute (host-visible; admp_compute synchronises) */
+    int32_t n_iter[64];        /* per frame (first 64 frames)                 */
+    int32_t converged[64];
+    double  residual[64];      /* final max |dE/dU| over polarisable sites    */
+    double  e_real, e_recip, e_self, e_pol;   /* frame-0 components         */
+} admp_stats;
+
+typedef struct admp_ctx admp_ctx;
+
+const char* admp_version(void);
+
+/* ADMPPmeForce.__init__ (pme.py:43-114): allocate grids, FFT tables, scratch. */
+admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out);
+admp_status admp_ctx_destroy(admp_ctx* ctx);
+const char* admp_last_error(const admp_ctx* ctx);
+/* stream: a hipStream_t (NULL = the legacy default stream). */
+admp_status admp_set_stream(admp_ctx* ctx, void* stream);
+
+/* Local frames (axis_type (N,), axis_idx (N,3) [z,x,y], -1 = none; spatial.py:44-156)
+ * and the covalent exclusions (i, j, nbond) for nbond in 1..6
+ * (dmff/api/topology.py:391-418, each unordered pair once).  All (host). */
+admp_status admp_set_topology(admp_ctx* ctx, const int32_t* axis_type, const int32_t* axis_idx,
+                              int64_t n_excl, const int32_t* excl_i, const int32_t* excl_j,
+                              const int32_t* excl_nbond);
+
+/* Reference-format pair list (Np,3) [i, j, nbond] of frame `frame`; rows with
+ * i >= j (padding) are ignored (dmff/utils.py:65-84).  pairs_is_int64 selects
+ * the element type.  No distance cutoff is applied to listed pairs (pme.py:1052-1173).
+ * Synchronises the stream once (capacity check). */
+admp_status admp_set_pairs(admp_ctx* ctx, int32_t frame, const void* pairs, int64_t n_pairs,
+                           int32_t pairs_is_int64);
+
+/* Cell-list neighbour build on the GPU for all frames: the pair set
+ * {i<j : min-image |r_i - r_j| < rc} (nblist.py:27-55), distances in FP64 with
+ * a fixed, unfused operation order.  pos (F,N,3), box (F,3,3).  Synchronises. */
+admp_status admp_build_neighbors(admp_ctx* ctx, const double* pos, const double* box);
+
+/* Write the i<j pairs of `frame` in the reference format (int32 (n,3) [i,j,nbond])
+ * to device buffer `out` with room for `cap` rows; *n_out receives the count.
+ * Synchronises. */
+admp_status admp_get_pairs(admp_ctx* ctx, int32_t frame, int32_t* out, int64_t cap, int64_t* n_out);
+
+/* Energy and (optionally) all gradients for every frame.
+ *   pos (F,N,3) A, box (F,3,3) A, Q_local (N,9) harmonic per atom, pol (N,) A^3,
+ *   thole (N,), m/p/d scales (host, n_scales each; index nbond-1 with
+ *   Python wrap for nbond = 0, pme.py:1111-1125).
+ *   U_inout (F,N,3) e*A global Cartesian: warm start in, induced dipoles out.
+ *   energy (F,) out.  With want_grad: dpos (F,N,3), dbox (F,3,3), dQ_local (F,N,9),
+ *   dpol (F,N), dthole (F,N) out (any may be NULL to skip writing it).
+ *   Gradients hold U fixed (Hellmann-Feynman, pme.py:364-371).
+ *   Synchronises the stream once at the end (solver statistics). */
+admp_status admp_compute(admp_ctx* ctx, const double* pos, const double* box,
+                         const double* Q_local, const double* pol, const double* thole,
+                         const double* mScales, const double* pScales, const double* dScales,
+                         int32_t n_scales, double* U_inout, const admp_solver_opts* opts,
+                         int32_t want_grad, double* energy, double* dpos, double* dbox,
+                         double* dQ_local, double* dpol, double* dthole, admp_stats* stats);
+
+/* Dispersion PME (disp_pme.py:80-130): backend energy (positive convention; the
+ * generator negates it) for c_list (N,3) = sqrt(C6, C8, C10) per atom.
+ * Uses the context's neighbour list.  Gradients: dpos, dbox, dc (N,3). */
+admp_status admp_disp_compute(admp_ctx* ctx, const double* pos, const double* box,
+                              const double* c_list, const double* mScales, int32_t n_scales,
+                              int32_t want_grad, double* energy, double* dpos, double* dbox,
+                              double* dc);
+
+/* Slater-ISA short range (pairwise.py:156-169): sum m*Ai*Aj*(1/3 (Br)^2+Br+1)e^-Br,
+ * B = sqrt(Bi*Bj).  a (N,), b (N,) in kJ/mol and 1/A.  Gradients dpos, dbox, da, db. */
+admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* box,
+                                const double* a, const double* b, const double* mScales,
+                                int32_t n_scales, int32_t want_grad, double* energy,
+                                double* dpos, double* dbox, double* da, double* db);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADMP_HIP_H */

@@ -1,0 +1,301 @@
+"""GPU parity: the HIP library (through the C ABI) against the FP64 oracle.
+
+Tolerances (north-star bar for the FP64 path: energy 1e-6 relative, force
+relative RMS 1e-5; the FP64 kernels land near 1e-10):
+  FP64   energy rel 1e-9, gradient rel RMS 1e-7 (pos, box, Q_local, pol, thole)
+  mixed  energy rel 2e-5, force rel RMS 1e-4
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import admp_oracle as O
+from tests.systems import load_system, reference_pairs
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+DEV = "cuda:0"
+
+
+def T(x):
+    return torch.tensor(np.asarray(x, dtype=np.float64))
+
+
+def rel_rms(a, b, floor=1e-6):
+    """||a - b|| / max(||b||, floor): relative RMS with an absolute floor (kJ/mol per
+    unit) so gradients that vanish analytically (e.g. dE/dbox without PME and without
+    boundary-crossing pairs, where both sides hold rounding noise) compare sensibly."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), floor))
+
+
+def oracle_eval(s, pos, box, pairs, rc, lpme, steps_pol=None, solver="reference", tol=1e-6, idx=None, scales=None):
+    n = pos.shape[0]
+    at = s.axis_types if idx is None else s.axis_types
+    f = O.ADMPPmeForce(box, at, s.axis_idx, rc, 5e-4, s.pme.lmax, s.pme.lpol, lpme, steps_pol)
+    p = T(pos).requires_grad_(True)
+    b = T(box).requires_grad_(True)
+    Q = T(s.pme.Q_local[s.map_atomtype]).requires_grad_(True)
+    mS, pS, dS = scales if scales is not None else (s.pme.mScales, s.pme.pScales, s.pme.dScales)
+    out = {}
+    if s.pme.lpol:
+        a = T(s.pme.pol[s.map_poltype]).requires_grad_(True)
+        t = T(s.pme.thole[s.map_poltype]).requires_grad_(True)
+        aux = {}
+        e, aux = f.get_energy(p, b, pairs, Q, a, t, mS, pS, dS, solver=solver, tol=tol, aux=aux)
+        gp, gb, gq, ga, gt = torch.autograd.grad(e, (p, b, Q, a, t))
+        out.update(pol=ga.numpy(), thole=gt.numpy(), U=aux["U_ind"].numpy() * 10.0)
+    else:
+        e = f.get_energy(p, b, pairs, Q, mScales=mS)
+        gp, gb, gq = torch.autograd.grad(e, (p, b, Q))
+    out.update(E=float(e.detach()), pos=gp.numpy(), box=gb.numpy(), Q=gq.numpy())
+    return out, f
+
+
+def gpu_eval(s, pos, box, pairs, rc, lpme, steps_pol=None, solver="reference", tol=None, precision="fp64",
+             scales=None, maxiter=None):
+    from dmff_amd.admp.pme import ADMPPmeForce
+    f = ADMPPmeForce(box, s.map_atomtype, s.axis_types, s.axis_idx, rc, 5e-4, s.pme.lmax, s.pme.lpol, lpme,
+                     steps_pol, precision=precision, solver=solver, tol=tol, maxiter=maxiter, device=DEV)
+    p = torch.tensor(pos, dtype=F64, device=DEV, requires_grad=True)
+    b = torch.tensor(box, dtype=F64, device=DEV, requires_grad=True)
+    Q = torch.tensor(s.pme.Q_local[s.map_atomtype], dtype=F64, device=DEV, requires_grad=True)
+    mS, pS, dS = scales if scales is not None else (s.pme.mScales, s.pme.pScales, s.pme.dScales)
+    pr = torch.as_tensor(pairs, device=DEV)
+    out = {}
+    if s.pme.lpol:
+        a = torch.tensor(s.pme.pol[s.map_poltype], dtype=F64, device=DEV, requires_grad=True)
+        t = torch.tensor(s.pme.thole[s.map_poltype], dtype=F64, device=DEV, requires_grad=True)
+        e, aux = f.get_energy(p, b, pr, Q, a, t, mS, pS, dS, aux={})
+        gp, gb, gq, ga, gt = torch.autograd.grad(e, (p, b, Q, a, t))
+        out.update(pol=ga.cpu().numpy(), thole=gt.cpu().numpy(), U=aux["U_ind"].cpu().numpy() * 10.0, aux=aux)
+    else:
+        e = f.get_energy(p, b, pr, Q, mS)
+        gp, gb, gq = torch.autograd.grad(e, (p, b, Q))
+    out.update(E=float(e.detach()), pos=gp.cpu().numpy(), box=gb.cpu().numpy(), Q=gq.cpu().numpy(),
+               info=f.last_info)
+    return out
+
+
+def compare(g, o, etol=1e-9, gtol=1e-7, keys=("pos", "box", "Q", "pol", "thole")):
+    msgs = []
+    de = abs(g["E"] - o["E"]) / max(1.0, abs(o["E"]))
+    msgs.append(f"E gpu={g['E']:.12f} oracle={o['E']:.12f} rel={de:.2e}")
+    ok = de < etol
+    for k in keys:
+        if k in o:
+            r = rel_rms(g[k], o[k])
+            msgs.append(f"{k}: relrms={r:.2e}")
+            ok = ok and r < gtol
+    print("; ".join(msgs), flush=True)
+    return ok, msgs
+
+
+@pytest.mark.parametrize("ff,lpme,steps", [
+    ("admp_nonpol.xml", False, None),
+    ("admp_mono.xml", False, 5),
+    ("admp.xml", False, 5),
+])
+def test_water_dimer_cutoff(ff, lpme, steps):
+    s = load_system(ff, "water_dimer.pdb")
+    pos, box = s.pos_nm * 10, s.box_nm * 10
+    pairs = reference_pairs(pos, box, 4.0, s.bonded, pad=4)
+    o, _ = oracle_eval(s, pos, box, pairs, 4.0, lpme, steps)
+    g = gpu_eval(s, pos, box, pairs, 4.0, lpme, steps)
+    ok, msgs = compare(g, o)
+    assert ok, msgs
+
+
+def test_fluctuating_dimer_pme_full_precision_pin():
+    """examples/fluctuated_leading_term_waterff/ref_out: E = -41.437696011272465 and the
+    6x3 gradient, with charges that depend on the geometry (run.py:22-56): the position
+    gradient flows through dE/dQ_local of the library into the charge model (autograd)."""
+    import json
+    from dmff_amd.admp.pme import ADMPPmeForce
+    from tests.systems import GOLD
+    from tests.test_oracle_pins import fluct_charges
+    en = json.load(open(f"{GOLD}/energies.json"))
+    s = load_system("fluct_forcefield.xml", "fluct_water_dimer.pdb", "fluct_residues.xml")
+    pos, box = s.pos_nm * 10, s.box_nm * 10
+    pairs = reference_pairs(pos, box, 4.0, s.bonded)
+    f = ADMPPmeForce(box, s.map_atomtype, s.axis_types, s.axis_idx, 4.0, 5e-4, 2, True, True, 5, device=DEV)
+    p = torch.tensor(pos, dtype=F64, device=DEV, requires_grad=True)
+    b = torch.tensor(box, dtype=F64, device=DEV)
+    Q = torch.tensor(s.pme.Q_local[s.map_atomtype], dtype=F64, device=DEV)
+    Q = torch.cat([fluct_charges(p, b)[:, None], Q[:, 1:]], dim=1)
+    e = f.get_energy(p, b, torch.as_tensor(pairs, device=DEV), Q, s.pme.pol[s.map_poltype],
+                     s.pme.thole[s.map_poltype], s.pme.mScales, s.pme.pScales, s.pme.dScales)
+    (g,) = torch.autograd.grad(e, p)
+    print(f"fluct: E {float(e.detach()):.12f} ref {en['fluct_E']:.12f}", flush=True)
+    assert abs(float(e.detach()) - en["fluct_E"]) < 1e-8
+    assert rel_rms(g.cpu().numpy(), np.asarray(en["fluct_grad"])) < 1e-7
+
+
+def _small_waters(ff="fullpol_forcefield.xml"):
+    s = load_system(ff, "waterbox_31ang.pdb", "fullpol_residues.xml")
+    p = s.pos_nm * 10
+    mol = p.reshape(-1, 3, 3)
+    ok = np.all((mol > 1.0) & (mol < 13.0), axis=(1, 2))
+    idx = np.nonzero(np.repeat(ok, 3))[0]
+    remap = {int(a): k for k, a in enumerate(idx)}
+    pos = p[idx] - 0.4
+    s.bonded = {(remap[i], remap[j]): v for (i, j), v in s.bonded.items() if i in remap and j in remap}
+    s.axis_types = s.axis_types[idx]
+    s.axis_idx = np.array([[remap.get(int(a), -1) if a >= 0 else -1 for a in row] for row in s.axis_idx[idx]])
+    s.map_atomtype = s.map_atomtype[idx]
+    s.map_poltype = s.map_poltype[idx]
+    return s, pos, np.eye(3) * 13.2
+
+
+def test_small_box_pme_polar_pcg():
+    s, pos, box = _small_waters()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    o, _ = oracle_eval(s, pos, box, pairs, 6.0, True, solver="cg", tol=1e-9)
+    g = gpu_eval(s, pos, box, pairs, 6.0, True, solver="pcg", tol=1e-9)
+    ok, msgs = compare(g, o)
+    du = np.abs(g["U"] - o["U"]).max()
+    print(f"U max abs diff {du:.2e}; solver info {g['info']['n_iter']} {g['info']['residual']}")
+    assert ok and du < 1e-8, msgs
+
+
+def test_small_box_reference_jacobi_matches_oracle_iteration_count():
+    s, pos, box = _small_waters()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    o, fo = oracle_eval(s, pos, box, pairs, 6.0, True)
+    g = gpu_eval(s, pos, box, pairs, 6.0, True)
+    ok, msgs = compare(g, o)
+    assert ok, msgs
+    assert np.abs(g["U"] - o["U"]).max() < 1e-8
+
+
+@pytest.mark.parametrize("ff,key", [
+    ("quadnonpol_forcefield.xml", "quadnonpol_E"),
+    ("fullpol_forcefield.xml", "fullpol_E"),
+])
+def test_waterbox_2988(ff, key):
+    """examples/water_fullpol: 2988 atoms, rc 6 A, PME; reference Jacobi (POL_CONV 1.0)."""
+    import json
+    from tests.systems import GOLD
+    en = json.load(open(f"{GOLD}/energies.json"))
+    s = load_system(ff, "waterbox_31ang.pdb", "fullpol_residues.xml")
+    pos, box = s.pos_nm * 10, s.box_nm * 10
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    o, _ = oracle_eval(s, pos, box, pairs, 6.0, True)
+    g = gpu_eval(s, pos, box, pairs, 6.0, True)
+    ok, msgs = compare(g, o, keys=("pos", "box", "Q", "pol", "thole"))
+    assert ok, msgs
+    assert abs(g["E"] / en[key] - 1) < 1e-5
+    if "U" in o:
+        ref = np.load(f"{GOLD}/water_fullpol_uind.npy")
+        assert np.abs(g["U"].ravel() - o["U"].ravel()).max() < 1e-8
+        assert np.linalg.norm(g["U"].ravel() - ref[:, 1]) / np.linalg.norm(ref[:, 1]) < 5e-4
+
+
+def test_neighbor_list_bit_exact():
+    from dmff_amd.admp.engine import Engine
+    s = load_system("fullpol_forcefield.xml", "waterbox_31ang.pdb", "fullpol_residues.xml")
+    pos, box = s.pos_nm * 10, s.box_nm * 10
+    for rc in (6.0, 8.0, 4.0):
+        ref = reference_pairs(pos, box, rc, s.bonded)
+        e = Engine(len(pos), 0, False, False, 0.0, (0, 0, 0), rc, device=DEV)
+        e.set_topology(None, None, s.bonded)
+        e.build_neighbors(pos, box)
+        got = e.get_pairs(0).cpu().numpy().astype(np.int64)
+        a = ref[np.lexsort((ref[:, 1], ref[:, 0]))]
+        b = got[np.lexsort((got[:, 1], got[:, 0]))]
+        print(f"rc={rc}: oracle {len(a)} pairs, gpu {len(b)} pairs", flush=True)
+        assert a.shape == b.shape and np.array_equal(a, b)
+    # random points with pairs at the cutoff to the last bit, small box (nc < 3)
+    rng = np.random.default_rng(5)
+    L = 11.0
+    p = rng.uniform(0, L, size=(300, 3))
+    p[1] = p[0] + np.array([4.0, 0, 0])
+    p[3] = p[2] + np.array([0, 3.9999999999, 0])
+    for rc in (4.0, 5.4):
+        ref = reference_pairs(p, np.eye(3) * L, rc, {})
+        e = Engine(len(p), 0, False, False, 0.0, (0, 0, 0), rc, device=DEV)
+        e.set_topology(None, None, {})
+        e.build_neighbors(p, np.eye(3) * L)
+        got = e.get_pairs(0).cpu().numpy().astype(np.int64)
+        a = ref[np.lexsort((ref[:, 1], ref[:, 0]))]
+        b = got[np.lexsort((got[:, 1], got[:, 0]))]
+        assert np.array_equal(a, b), (len(a), len(b))
+
+
+def test_mixed_precision_error_bounds():
+    s, pos, box = _small_waters()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    o, _ = oracle_eval(s, pos, box, pairs, 6.0, True, solver="cg", tol=1e-9)
+    g = gpu_eval(s, pos, box, pairs, 6.0, True, solver="pcg", tol=1e-4, precision="mixed")
+    de = abs(g["E"] - o["E"]) / abs(o["E"])
+    df = rel_rms(g["pos"], o["pos"])
+    print(f"mixed: E rel {de:.2e}, force relrms {df:.2e}, box {rel_rms(g['box'], o['box']):.2e}", flush=True)
+    assert de < 2e-5 and df < 1e-4
+
+
+def test_dispersion_and_slater_vs_oracle():
+    from dmff_amd.admp.engine import Engine
+    rng = np.random.default_rng(9)
+    L = 20.0
+    n = 60
+    pos = rng.uniform(0, L, size=(n, 3))
+    c = rng.uniform(5, 40, size=(n, 3))
+    m = np.array([0.0, 0.0, 1.0, 1.0, 1.0, 1.0])
+    bonded = {(0, 1): 1, (1, 0): 1, (2, 3): 2, (3, 2): 2}
+    rc = 9.0
+    pairs = reference_pairs(pos, np.eye(3) * L, rc, bonded)
+    kappa, K1, K2, K3 = O.setup_ewald_parameters(rc, 5e-4, np.eye(3) * L)
+    p = T(pos).requires_grad_(True)
+    b = T(np.eye(3) * L).requires_grad_(True)
+    ct = T(c).requires_grad_(True)
+    e = O.energy_disp_pme(p, b, pairs, ct, m, kappa, (K1, K2, K3), 10)
+    gp, gb, gc = torch.autograd.grad(e, (p, b, ct))
+    eng = Engine(n, 0, False, False, 0.0, (0, 0, 0), rc, disp_kappa=kappa, dK=(K1, K2, K3), disp_pmax=10, device=DEV)
+    eng.set_topology(None, None, bonded)
+    eng.set_pairs(torch.as_tensor(pairs))
+    E, g = eng.disp_compute(pos, np.eye(3) * L, c, m)
+    print(f"disp: E gpu {float(E[0]):.12f} oracle {float(e):.12f}", flush=True)
+    assert abs(float(E[0]) - float(e)) < 1e-9 * abs(float(e))
+    assert rel_rms(g["pos"].cpu().numpy(), gp.numpy()) < 1e-8
+    assert rel_rms(g["box"].cpu().numpy(), gb.numpy()) < 1e-8
+    assert rel_rms(g["c"].cpu().numpy(), gc.numpy()) < 1e-8
+    A = rng.uniform(1e2, 3e2, size=n)
+    B = rng.uniform(3.0, 4.5, size=n)
+    At, Bt = T(A).requires_grad_(True), T(B).requires_grad_(True)
+    p2 = T(pos).requires_grad_(True)
+    b2 = T(np.eye(3) * L).requires_grad_(True)
+    es = O.slater_sr_energy(p2, b2, pairs, m, At, Bt)
+    gp2, gb2, ga2, gbb2 = torch.autograd.grad(es, (p2, b2, At, Bt))
+    E2, g2 = eng.slater_compute(pos, np.eye(3) * L, A, B, m)
+    print(f"slater: E gpu {float(E2[0]):.12f} oracle {float(es):.12f}", flush=True)
+    assert abs(float(E2[0]) - float(es)) < 1e-10 * max(1, abs(float(es)))
+    assert rel_rms(g2["pos"].cpu().numpy(), gp2.numpy()) < 1e-9
+    assert rel_rms(g2["a"].cpu().numpy(), ga2.numpy()) < 1e-9
+    assert rel_rms(g2["b"].cpu().numpy(), gbb2.numpy()) < 1e-9
+    assert rel_rms(g2["box"].cpu().numpy(), gb2.numpy()) < 1e-9
+
+
+def test_batched_frames_equal_single():
+    from dmff_amd.admp.pme import ADMPPmeForce
+    s, pos, box = _small_waters()
+    rng = np.random.default_rng(3)
+    frames = np.stack([pos + rng.normal(0, 0.005, pos.shape) for _ in range(3)])
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    Q = s.pme.Q_local[s.map_atomtype]
+    pol = s.pme.pol[s.map_poltype]
+    th = s.pme.thole[s.map_poltype]
+    args = (s.pme.mScales, s.pme.pScales, s.pme.dScales)
+    fb = ADMPPmeForce(box, s.map_atomtype, s.axis_types, s.axis_idx, 6.0, 5e-4, 2, True, True, None,
+                      solver="pcg", tol=1e-8, n_frames=3, device=DEV)
+    eng = fb.engine
+    for k in range(3):
+        eng.set_pairs(torch.as_tensor(pairs, device=DEV), frame=k)
+    Eb = fb.get_energy(torch.tensor(frames), np.stack([box] * 3), None, Q, pol, th, *args)
+    for k in range(3):
+        f1 = ADMPPmeForce(box, s.map_atomtype, s.axis_types, s.axis_idx, 6.0, 5e-4, 2, True, True, None,
+                          solver="pcg", tol=1e-8, device=DEV)
+        e1 = f1.get_energy(frames[k], box, pairs, Q, pol, th, *args)
+        assert abs(float(Eb[k]) - float(e1)) < 1e-9 * abs(float(e1))
