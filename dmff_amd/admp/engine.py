@@ -64,6 +64,20 @@ class Engine:
     def _stream(self):
         self.lib.admp_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
 
+    def profile(self, enable=True, reset=True):
+        L.check(self.h, self.lib.admp_profile(self.h, int(enable), int(reset)), "admp_profile")
+
+    def profile_get(self):
+        """{family: (total_ms, launches)} accumulated since the last reset."""
+        out = {}
+        for k in range(16):
+            ms, n, name = ctypes.c_double(), ctypes.c_int64(), ctypes.c_char_p()
+            L.check(self.h, self.lib.admp_profile_get(self.h, k, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(name)),
+                    "admp_profile_get")
+            if name.value:
+                out[name.value.decode()] = (ms.value, n.value)
+        return out
+
     def set_topology(self, axis_type=None, axis_idx=None, exclusions=None):
         n = self.n_atoms
         at = np.ascontiguousarray(np.asarray(axis_type if axis_type is not None else np.full(n, 5)), dtype=np.int32)

@@ -74,7 +74,62 @@ struct admp_ctx {
     int* d_active = nullptr;
     int dots_blocks = 64;
     std::vector<void*> allocs;
+    // per-kernel timing (HIP events around launches on the context stream)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double prof_ms[16] = {0};
+    long prof_n[16] = {0};
 };
+
+static const char* PROF_NAMES[16] = {"real_final", "real_permfield", "real_matvec", "spline_prep", "spread",
+                                     "fft", "influence", "gather", "nbr_build", "frames", "pair_short",
+                                     "solver_vec", "", "", "", ""};
+
+static hipEvent_t ev_get(admp_ctx* ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// RAII bracket: records a start/stop event pair around the launches in scope
+struct ProfScope {
+    admp_ctx* ctx;
+    int key;
+    hipEvent_t a = nullptr;
+    ProfScope(admp_ctx* c, int k) : ctx(c), key(k) {
+        if (ctx->prof) {
+            a = ev_get(ctx);
+            (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~ProfScope() {
+        if (ctx->prof) {
+            hipEvent_t b = ev_get(ctx);
+            (void)hipEventRecord(b, ctx->stream);
+            ctx->ev_pending.push_back({key, {a, b}});
+        }
+    }
+};
+
+// resolve completed event pairs (call after a stream synchronisation)
+static void prof_resolve(admp_ctx* ctx) {
+    for (auto& p : ctx->ev_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            ctx->prof_ms[p.first] += ms;
+            ctx->prof_n[p.first] += 1;
+        }
+        ctx->ev_pool.push_back(p.second.first);
+        ctx->ev_pool.push_back(p.second.second);
+    }
+    ctx->ev_pending.clear();
+}
 
 #define HIPCHK(x)                                                                  \
     do {                                                                           \
@@ -174,8 +229,31 @@ extern "C" admp_status admp_set_stream(admp_ctx* ctx, void* stream) {
     return ADMP_OK;
 }
 
+extern "C" admp_status admp_profile(admp_ctx* ctx, int32_t enable, int32_t reset) {
+    if (!ctx) return ADMP_ERR_ARG;
+    if (reset) {
+        for (int k = 0; k < 16; ++k) { ctx->prof_ms[k] = 0; ctx->prof_n[k] = 0; }
+    }
+    ctx->prof = enable != 0;
+    return ADMP_OK;
+}
+
+extern "C" admp_status admp_profile_get(admp_ctx* ctx, int32_t key, double* total_ms, int64_t* count,
+                                        const char** name) {
+    if (!ctx || key < 0 || key >= 16) return ADMP_ERR_ARG;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return ADMP_ERR_HIP;
+    prof_resolve(ctx);
+    if (total_ms) *total_ms = ctx->prof_ms[key];
+    if (count) *count = ctx->prof_n[key];
+    if (name) *name = PROF_NAMES[key];
+    return ADMP_OK;
+}
+
 extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     if (!ctx) return ADMP_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->ev_pending) { ctx->ev_pool.push_back(p.second.first); ctx->ev_pool.push_back(p.second.second); }
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (void* p : ctx->allocs) (void)hipFree(p);
     delete ctx;
     return ADMP_OK;
@@ -467,9 +545,9 @@ extern "C" admp_status admp_build_neighbors(admp_ctx* ctx, const double* pos, co
     }
     for (int attempt = 0; attempt < 2; ++attempt) {
         HIPCHK(hipMemsetAsync(ctx->d_maxcnt, 0, sizeof(int), ctx->stream));
-        hipLaunchKernelGGL(k_nbr_build, dim3(nblk(N, 4), F), dim3(256), 0, ctx->stream, N, ctx->cap, ctx->d_pos,
+        { ProfScope ps_(ctx, 8); hipLaunchKernelGGL(k_nbr_build, dim3(nblk(N, 4), F), dim3(256), 0, ctx->stream, N, ctx->cap, ctx->d_pos,
                            ctx->d_box, cg, rc * rc, ctx->d_cell, ctx->d_cstart, ctx->d_catoms, ctx->d_xoff,
-                           ctx->d_xent, ctx->d_nbr, ctx->d_ncnt, ctx->d_nhalf, ctx->d_maxcnt);
+                           ctx->d_xent, ctx->d_nbr, ctx->d_ncnt, ctx->d_nhalf, ctx->d_maxcnt); }
         HIPCHK(hipGetLastError());
         int mx = 0;
         HIPCHK(hipMemcpyAsync(&mx, ctx->d_maxcnt, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -516,10 +594,10 @@ static admp_status recip_run(admp_ctx* ctx, Grid& g, int use_dA, const double* Q
     R* fq = reinterpret_cast<R*>(ctx->d_fq);
     const size_t tot = g.G * F;
     hipLaunchKernelGGL(k_zero_cplx<R>, dim3(nblk(tot, 256)), dim3(256), 0, st, tot, grid);
-    hipLaunchKernelGGL(k_spline_prep<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_pos, ctx->d_box, Qc, U, qmode,
-                       use_dA, ctx->d_mbase, wfrac, fq);
-    hipLaunchKernelGGL(k_spread_atomic<R>, dim3(nblk((size_t)N * 36, 256), F), dim3(256), 0, st, N, g.K[0], g.K[1],
-                       g.K[2], ctx->d_mbase, wfrac, fq, lmax_eff, grid);
+    { ProfScope ps_(ctx, 3); hipLaunchKernelGGL(k_spline_prep<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_pos, ctx->d_box, Qc, U, qmode,
+                       use_dA, ctx->d_mbase, wfrac, fq); }
+    { ProfScope ps_(ctx, 4); hipLaunchKernelGGL(k_spread_atomic<R>, dim3(nblk((size_t)N * 36, 256), F), dim3(256), 0, st, N, g.K[0], g.K[1],
+                       g.K[2], ctx->d_mbase, wfrac, fq, lmax_eff, grid); }
     const int maxpts = sizeof(R) == 8 ? 2048 : 4096;
     auto fft = [&](int inverse) {
         for (int dd = 0; dd < 3; ++dd) {
@@ -528,19 +606,19 @@ static admp_status recip_run(admp_ctx* ctx, Grid& g, int use_dA, const double* Q
             const int nlines = (int)(g.G / K);
             int LB = std::max(1, std::min(maxpts / K, nlines));
             const size_t sm = (size_t)2 * LB * K * sizeof(Cplx<R>);
-            hipLaunchKernelGGL(k_fft_pass<R>, dim3(nblk(nlines, LB), F), dim3(256), sm, st, grid, g.K[0], g.K[1], g.K[2],
-                               d, g.plan[d], reinterpret_cast<const Cplx<R>*>(g.tw[d]), inverse, LB);
+            { ProfScope ps_(ctx, 5); hipLaunchKernelGGL(k_fft_pass<R>, dim3(nblk(nlines, LB), F), dim3(256), sm, st, grid, g.K[0], g.K[1], g.K[2],
+                               d, g.plan[d], reinterpret_cast<const Cplx<R>*>(g.tw[d]), inverse, LB); }
         }
     };
     fft(0);
     const int nbI = (int)nblk(g.G, 256);
-    hipLaunchKernelGGL(k_influence<R>, dim3(nbI, F), dim3(256), 0, st, g.K[0], g.K[1], g.K[2], ctx->d_box, use_dA,
-                       g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind, pref, grid, ctx->d_part);
+    { ProfScope ps_(ctx, 6); hipLaunchKernelGGL(k_influence<R>, dim3(nbI, F), dim3(256), 0, st, g.K[0], g.K[1], g.K[2], ctx->d_box, use_dA,
+                       g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind, pref, grid, ctx->d_part); }
     if (want_energy)
         hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbI, 7, ctx->d_part, ctx->d_rec, 0);
     fft(1);
-    hipLaunchKernelGGL(k_gather<R>, dim3(nblk(N, 64), F), dim3(64), 0, st, N, g.K[0], g.K[1], g.K[2], ctx->d_pos,
-                       ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, grid, gather_mode, out3, gq, gpos, vir);
+    { ProfScope ps_(ctx, 7); hipLaunchKernelGGL(k_gather<R>, dim3(nblk(N, 64), F), dim3(64), 0, st, N, g.K[0], g.K[1], g.K[2], ctx->d_pos,
+                       ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, grid, gather_mode, out3, gq, gpos, vir); }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
@@ -559,9 +637,9 @@ static admp_status matvec(admp_ctx* ctx, const double* v, double* out, const dou
         hipLaunchKernelGGL(k_cast<R>, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, v, reinterpret_cast<R*>(ctx->d_vR));
         vR = reinterpret_cast<const R*>(ctx->d_vR);
     }
-    hipLaunchKernelGGL((k_real_matvec<R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
+    { ProfScope ps_(ctx, 2); hipLaunchKernelGGL((k_real_matvec<R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
                        ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
-                       reinterpret_cast<const R*>(ctx->d_pcoef), vR, ctx->d_t1);
+                       reinterpret_cast<const R*>(ctx->d_pcoef), vR, ctx->d_t1); }
     double* rec = nullptr;
     if (ctx->cfg.lpme) {
         admp_status s = recip_run<R>(ctx, ctx->eg, 0, nullptr, v, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1, ctx->d_t2,
@@ -616,8 +694,8 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
         HIPCHK(hipMemsetAsync(U, 0, NF * 3 * sizeof(double), st));
     hipLaunchKernelGGL(k_pack_site<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_Qc, pol, thole,
                        (const double*)nullptr, site);
-    hipLaunchKernelGGL((k_real_permfield<R>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box, site,
-                       ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_b, reinterpret_cast<R*>(ctx->d_pcoef));
+    { ProfScope ps_(ctx, 1); hipLaunchKernelGGL((k_real_permfield<R>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box, site,
+                       ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_b, reinterpret_cast<R*>(ctx->d_pcoef)); }
     HIPCHK(hipGetLastError());
     if (lpme) {
         admp_status s = recip_run<R>(ctx, ctx->eg, 0, ctx->d_Qc, nullptr, 1, lmax_eff, 0, DIELECTRIC, kappa, 1, ctx->d_t2,
@@ -740,8 +818,8 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     const int lmax_eff = (lpol && c.lmax == 0) ? 1 : c.lmax;
     R* site = reinterpret_cast<R*>(ctx->d_site);
     const dim3 pgrid(nblk((size_t)N * TPA, PAIR_BLOCK), F);
-    hipLaunchKernelGGL(k_frames_fwd, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->nh, ctx->use_frames, ctx->d_pos,
-                       ctx->d_box, ctx->d_axis_type, ctx->d_axis_idx, Q_local, ctx->d_frames, ctx->d_Qc);
+    { ProfScope ps_(ctx, 9); hipLaunchKernelGGL(k_frames_fwd, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->nh, ctx->use_frames, ctx->d_pos,
+                       ctx->d_box, ctx->d_axis_type, ctx->d_axis_idx, Q_local, ctx->d_frames, ctx->d_Qc); }
     HIPCHK(hipGetLastError());
     SolveInfo info;
     admp_status s;
@@ -754,13 +832,13 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     // ---------------- final pass: energy and gradients with U fixed ----------------
     hipLaunchKernelGGL(k_pack_site<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_Qc, pol, thole, U, site);
     if (lpol)
-        hipLaunchKernelGGL((k_real_final<R, true>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box,
+        { ProfScope ps_(ctx, 0); hipLaunchKernelGGL((k_real_final<R, true>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box,
                            site, ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_e_atom, ctx->d_gpos, ctx->d_gq, ctx->d_vir_atom,
-                           ctx->d_gpol, ctx->d_gth);
+                           ctx->d_gpol, ctx->d_gth); }
     else
-        hipLaunchKernelGGL((k_real_final<R, false>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box,
+        { ProfScope ps_(ctx, 0); hipLaunchKernelGGL((k_real_final<R, false>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box,
                            site, ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_e_atom, ctx->d_gpos, ctx->d_gq, ctx->d_vir_atom,
-                           ctx->d_gpol, ctx->d_gth);
+                           ctx->d_gpol, ctx->d_gth); }
     HIPCHK(hipGetLastError());
     if (!lpol) {
         HIPCHK(hipMemsetAsync(ctx->d_gpol, 0, NF * sizeof(double), st));
@@ -790,9 +868,9 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         // dE/dQ_global total -> frames backward
         hipLaunchKernelGGL(k_add3, dim3(nblk(NF * 10, 256)), dim3(256), 0, st, NF * 10, ctx->d_gq, ctx->d_gq_rec,
                            (const double*)nullptr, ctx->d_gq);
-        hipLaunchKernelGGL(k_frames_bwd, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->nh, ctx->use_frames, ctx->d_pos,
+        { ProfScope ps_(ctx, 9); hipLaunchKernelGGL(k_frames_bwd, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->nh, ctx->use_frames, ctx->d_pos,
                            ctx->d_box, ctx->d_axis_type, ctx->d_axis_idx, Q_local, ctx->d_gq, ctx->d_gQl,
-                           ctx->d_contrib, ctx->d_vir_fr);
+                           ctx->d_contrib, ctx->d_vir_fr); }
         hipLaunchKernelGGL(k_add3, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_gpos, ctx->d_gpos_rec,
                            (const double*)nullptr, ctx->d_gpos);
         hipLaunchKernelGGL(k_frames_gather, dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->d_rev_off, ctx->d_rev_ent,
@@ -873,9 +951,9 @@ static admp_status disp_impl(admp_ctx* ctx, const double* c_list, const PairPara
     const double kappa = lpme ? c.disp_kappa : 0.0;
     PairParams p2 = pp;
     p2.kappa = kappa;
-    hipLaunchKernelGGL((k_pair_short<R, 0>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
+    { ProfScope ps_(ctx, 10); hipLaunchKernelGGL((k_pair_short<R, 0>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
                        ctx->cap, ctx->d_pos, ctx->d_box, c_list, ctx->d_nbr, ctx->d_ncnt, p2, c.disp_pmax, ctx->d_e_atom,
-                       ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec /* (F,N,3) dE/dc */);
+                       ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec /* (F,N,3) dE/dc */); }
     double* gc = ctx->d_gpos_rec;
     HIPCHK(hipMemsetAsync(ctx->d_red9b, 0, F * 9 * sizeof(double), st));
     std::vector<double> erec(F, 0.0);
@@ -961,13 +1039,13 @@ extern "C" admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, con
     for (int i = 0; i < N; ++i) { ab[2 * i] = ha[i]; ab[2 * i + 1] = hb[i]; }
     HIPCHK(hipMemcpyAsync(ctx->d_t1, ab.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, st));
     if (ctx->cfg.precision == ADMP_MIXED)
-        hipLaunchKernelGGL((k_pair_short<float, 1>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st,
+        { ProfScope ps_(ctx, 10); hipLaunchKernelGGL((k_pair_short<float, 1>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st,
                            N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_t1, ctx->d_nbr, ctx->d_ncnt, pp, 0, ctx->d_e_atom,
-                           ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec);
+                           ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec); }
     else
-        hipLaunchKernelGGL((k_pair_short<double, 1>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st,
+        { ProfScope ps_(ctx, 10); hipLaunchKernelGGL((k_pair_short<double, 1>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st,
                            N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_t1, ctx->d_nbr, ctx->d_ncnt, pp, 0, ctx->d_e_atom,
-                           ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec);
+                           ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec); }
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(256), 0, st, N, 1, ctx->d_e_atom, ctx->d_red9a, 0);
     std::vector<double> e(F);

@@ -160,6 +160,16 @@ admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* 
                                 int32_t n_scales, int32_t want_grad, double* energy,
                                 double* dpos, double* dbox, double* da, double* db);
 
+/* Instrumentation (no counterpart in the reference): per-kernel-family timing with
+ * HIP events recorded around the library's launches on the context stream.
+ * admp_profile switches recording on/off (reset != 0 clears the totals);
+ * admp_profile_get synchronises the stream and returns the accumulated milliseconds
+ * and launch count of family `key` (0 real_final, 1 real_permfield, 2 real_matvec,
+ * 3 spline_prep, 4 spread, 5 fft, 6 influence, 7 gather, 8 nbr_build, 9 frames,
+ * 10 pair_short) and its name. */
+admp_status admp_profile(admp_ctx* ctx, int32_t enable, int32_t reset);
+admp_status admp_profile_get(admp_ctx* ctx, int32_t key, double* total_ms, int64_t* count, const char** name);
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,10 +176,11 @@ def fluct_charges(pos, box):
     dip1 = -0.016858755 + 0.002287251 * ang + 0.239667591 * d1 + (-0.070483437) * d2
     dip2 = -0.016858755 + 0.002287251 * ang + 0.239667591 * d2 + (-0.070483437) * d1
     qh1, qh2 = dip1 / d1, dip2 / d2
-    q = torch.zeros(pos.shape[0], dtype=F64)
-    q = q.index_put((torch.arange(0, pos.shape[0], 3),), -(qh1 + qh2))
-    q = q.index_put((torch.arange(1, pos.shape[0], 3),), qh1)
-    q = q.index_put((torch.arange(2, pos.shape[0], 3),), qh2)
+    dev = pos.device
+    q = torch.zeros(pos.shape[0], dtype=F64, device=dev)
+    q = q.index_put((torch.arange(0, pos.shape[0], 3, device=dev),), -(qh1 + qh2))
+    q = q.index_put((torch.arange(1, pos.shape[0], 3, device=dev),), qh1)
+    q = q.index_put((torch.arange(2, pos.shape[0], 3, device=dev),), qh2)
     return q
 
 
