@@ -88,6 +88,26 @@ def time_path(eng, D, steps, warmup):
     return (t1 - t0) / steps, out, prof
 
 
+def kernel_profile(D, precision, steps=2):
+    """Per-kernel average launch durations with HIP events (graph capture off so each
+    launch is bracketed individually)."""
+    os.environ["ADMP_NO_GRAPH"] = "1"
+    try:
+        eng = make_engine(D, precision)
+    finally:
+        os.environ.pop("ADMP_NO_GRAPH", None)
+    evaluate(eng, D)
+    import torch
+    torch.cuda.synchronize()
+    eng.profile(True, True)
+    for _ in range(steps):
+        evaluate(eng, D)
+    torch.cuda.synchronize()
+    prof = eng.profile_get()
+    eng.profile(False, True)
+    return prof
+
+
 def cpu_baseline(D, n_iter):
     """FP64 oracle on the host cores: one E+grad pass and one dE/dU pass at full size."""
     import numpy as np
@@ -153,12 +173,18 @@ def main():
             if n:
                 log(f"   {k:16s} {ms / args.steps:9.3f} ms/eval  {n // args.steps:5d} launches/eval  "
                     f"{ms / n * 1e3:9.1f} us/launch")
+    kprof = kernel_profile(D, "mixed")
+    log("per-kernel (events, graphs off, mixed):")
+    for k, (ms, n) in sorted(kprof.items(), key=lambda x: -x[1][0]):
+        if n:
+            log(f"   {k:16s} {ms / 2:9.3f} ms/eval  {n // 2:5d} launches/eval  {ms / n * 1e3:9.1f} us/launch")
     f64, mix = res["fp64"], res["mixed"]
     e_rel = abs(mix["E"] - f64["E"]) / abs(f64["E"])
     fr = float(torch.linalg.norm(mix["g"]["pos"] - f64["g"]["pos"]) / torch.linalg.norm(f64["g"]["pos"]))
     # dominant kernel family of the mixed path and its roofline
     prof = mix["prof"]
-    dom = max(prof.items(), key=lambda x: x[1][0])
+    cand = {k: v for k, v in kprof.items() if v[1] and k not in ("solver_vec", "solver_graph")}
+    dom = max(cand.items(), key=lambda x: x[1][0])
     roof = roofline(dom[0], dom[1], D, "mixed")
     out = {
         "metric": "ADMP E+F+grad evals/s @12,288 atoms (mixed precision); FP64 beside it",
@@ -182,6 +208,7 @@ def main():
                           "solver_iters": mix["info"]["n_iter"][0], "residual": mix["info"]["residual"][0]},
         "roofline": roof,
         "kernels_ms_per_eval": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
+        "kernel_us_per_launch": {k: v[0] / v[1] * 1e3 for k, v in kprof.items() if v[1]},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(D, f64["info"]["n_iter"][0])

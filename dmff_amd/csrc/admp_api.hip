@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -14,6 +15,7 @@
 #include "nblist_kernels.h"
 #include "pair_kernels.h"
 #include "recip_kernels.h"
+#include "recip2_kernels.h"
 #include "site_kernels.h"
 
 using namespace admp;
@@ -33,7 +35,9 @@ struct Grid {
 
 struct admp_ctx {
     admp_config cfg;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // the context's own stream (capturable)
+    hipStream_t user_stream = nullptr;  // caller's stream: ordered with events on entry and exit
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
     std::string err;
     int N = 0, F = 1, nh = 1;
     int cap = 0;
@@ -56,7 +60,12 @@ struct admp_ctx {
     void *d_wfrac = nullptr, *d_fq = nullptr;
     // grids
     Grid eg, dg;
-    void* d_grid = nullptr;     // complex, F * max(G)
+    void* d_grid = nullptr;     // complex, F * max(G)   (small-grid fallback path)
+    void* d_rgrid = nullptr;    // real grid, F * max(G)
+    void* d_hgrid = nullptr;    // half spectrum, F * max(K0 K1 (K2/2+1)) complex
+    double* d_wd = nullptr;     // per-site grid offsets w (double)
+    int *d_gcell = nullptr, *d_gccount = nullptr, *d_gcstart = nullptr, *d_gcfill = nullptr, *d_gcatoms = nullptr;
+    const Grid* prepped = nullptr;  // grid whose site bases / sort are current
     double* d_part = nullptr;   // influence partials
     int part_blocks = 0;
     double* d_rec = nullptr;    // F x 7 reduced influence sums
@@ -70,10 +79,21 @@ struct admp_ctx {
     // solver
     double *d_b = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_Ap = nullptr, *d_t1 = nullptr,
            *d_t2 = nullptr, *d_U = nullptr;
-    double *d_dpart = nullptr, *d_dots = nullptr, *d_coef = nullptr;
+    double *d_dpart = nullptr, *d_dots = nullptr, *d_coef = nullptr, *d_res = nullptr, *d_dU = nullptr;
+    PcgState* d_pcg = nullptr;
     int* d_active = nullptr;
     int dots_blocks = 64;
     std::vector<void*> allocs;
+    // captured PCG iteration chunks (replayed until the frames converge)
+    struct ChunkGraph {
+        hipGraphExec_t exec = nullptr;
+        int type = -1, nrun = 0;
+        const void *rhs = nullptr, *x = nullptr;
+        double tol = 0, f1 = 0;
+    };
+    ChunkGraph graphs[4];
+    bool use_graphs = true;   // ADMP_NO_GRAPH=1 runs the chunk launches eagerly
+    double* d_pol = nullptr;  // context-owned copy of the polarisabilities (stable graph argument)
     // per-kernel timing (HIP events around launches on the context stream)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -84,7 +104,14 @@ struct admp_ctx {
 
 static const char* PROF_NAMES[16] = {"real_final", "real_permfield", "real_matvec", "spline_prep", "spread",
                                      "fft", "influence", "gather", "nbr_build", "frames", "pair_short",
-                                     "solver_vec", "", "", "", ""};
+                                     "solver_vec", "solver_graph", "", "", ""};
+
+static void clear_graphs(admp_ctx* ctx) {
+    for (auto& g : ctx->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = admp_ctx::ChunkGraph();
+    }
+}
 
 static hipEvent_t ev_get(admp_ctx* ctx) {
     if (!ctx->ev_pool.empty()) {
@@ -202,19 +229,22 @@ static admp_status setup_grid(admp_ctx* ctx, Grid& g, int K0, int K1, int K2, bo
         }
         double* t64;
         HIPCHK(dalloc(ctx, &t64, 2 * K));
-        HIPCHK(hipMemcpy(t64, tw.data(), 2 * K * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(t64, tw.data(), 2 * K * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
         g.tw64[d] = t64;
         if (mixed) {
             float* t32;
             HIPCHK(dalloc(ctx, &t32, 2 * K));
-            HIPCHK(hipMemcpy(t32, twf.data(), 2 * K * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpyAsync(t32, twf.data(), 2 * K * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
             g.tw[d] = t32;
         } else {
             g.tw[d] = t64;
         }
         std::vector<double> bm = bspline_moduli(K);
         HIPCHK(dalloc(ctx, &g.bmod[d], K));
-        HIPCHK(hipMemcpy(g.bmod[d], bm.data(), K * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(g.bmod[d], bm.data(), K * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     return ADMP_OK;
 }
@@ -225,7 +255,21 @@ extern "C" const char* admp_last_error(const admp_ctx* ctx) { return ctx ? ctx->
 
 extern "C" admp_status admp_set_stream(admp_ctx* ctx, void* stream) {
     if (!ctx) return ADMP_ERR_ARG;
-    ctx->stream = reinterpret_cast<hipStream_t>(stream);
+    ctx->user_stream = reinterpret_cast<hipStream_t>(stream);
+    return ADMP_OK;
+}
+
+// work on the context stream starts after everything queued on the caller's stream
+static admp_status enter(admp_ctx* ctx) {
+    HIPCHK(hipEventRecord(ctx->ev_in, ctx->user_stream));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_in, 0));
+    return ADMP_OK;
+}
+
+// and the caller's stream continues after the context's work
+static admp_status leave(admp_ctx* ctx) {
+    HIPCHK(hipEventRecord(ctx->ev_out, ctx->stream));
+    HIPCHK(hipStreamWaitEvent(ctx->user_stream, ctx->ev_out, 0));
     return ADMP_OK;
 }
 
@@ -251,10 +295,14 @@ extern "C" admp_status admp_profile_get(admp_ctx* ctx, int32_t key, double* tota
 
 extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     if (!ctx) return ADMP_OK;
-    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& p : ctx->ev_pending) { ctx->ev_pool.push_back(p.second.first); ctx->ev_pool.push_back(p.second.second); }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    clear_graphs(ctx);
     for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
+    if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return ADMP_OK;
 }
@@ -279,11 +327,21 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     if (c.n_atoms >= (1 << NB_SHIFT)) { ctx->err = "too many atoms"; return fail(ADMP_ERR_ARG); }
     ctx->N = c.n_atoms;
     ctx->F = c.n_frames;
+    {
+        const char* ng = getenv("ADMP_NO_GRAPH");
+        ctx->use_graphs = !(ng && ng[0] == '1');
+    }
     ctx->nh = (c.lmax + 1) * (c.lmax + 1);
     ctx->use_frames = c.lmax > 0 ? 1 : 0;
     const size_t NF = (size_t)ctx->N * ctx->F;
     const bool mixed = c.precision == ADMP_MIXED;
     admp_status st;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess) {
+        ctx->err = "stream/event creation failed";
+        return fail(ADMP_ERR_HIP);
+    }
     if (c.lpme) {
         if ((st = setup_grid(ctx, ctx->eg, c.K1, c.K2, c.K3, mixed)) != ADMP_OK) return fail(st);
     }
@@ -310,6 +368,28 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         ALLOC(ctx->d_part, (size_t)ctx->part_blocks * ctx->F * 7);
     }
     ALLOC(ctx->d_rec, ctx->F * 7);
+    if (Gmax) {
+        double* g;
+        ALLOC(g, Gmax * ctx->F);
+        ctx->d_rgrid = g;
+        size_t Hmax = 0, Lmax = 0;
+        for (const Grid* gg : {&ctx->eg, &ctx->dg})
+            if (gg->G) {
+                Hmax = std::max(Hmax, (size_t)gg->K[0] * gg->K[1] * (gg->K[2] / 2 + 1));
+                Lmax = std::max(Lmax, (size_t)gg->K[1] * (gg->K[2] / 2 + 1));
+            }
+        ALLOC(g, Hmax * 2 * ctx->F);
+        ctx->d_hgrid = g;
+        ctx->part_blocks = std::max<int>(ctx->part_blocks, (int)Lmax);
+        dfree(ctx, ctx->d_part);
+        ALLOC(ctx->d_part, (size_t)ctx->part_blocks * ctx->F * 7);
+        ALLOC(ctx->d_gccount, Gmax * ctx->F);
+        ALLOC(ctx->d_gcfill, Gmax * ctx->F);
+        ALLOC(ctx->d_gcstart, (Gmax + 1) * ctx->F);
+    }
+    ALLOC(ctx->d_wd, NF * 3);
+    ALLOC(ctx->d_gcell, NF);
+    ALLOC(ctx->d_gcatoms, NF);
     ALLOC(ctx->d_e_atom, NF); ALLOC(ctx->d_gpos, NF * 3); ALLOC(ctx->d_gq, NF * 10); ALLOC(ctx->d_vir_atom, NF * 9);
     ALLOC(ctx->d_gpol, NF); ALLOC(ctx->d_gth, NF); ALLOC(ctx->d_e_self, NF); ALLOC(ctx->d_e_pol, NF);
     ALLOC(ctx->d_gq_rec, NF * 10); ALLOC(ctx->d_gpos_rec, NF * 3); ALLOC(ctx->d_vir_rec, NF * 9);
@@ -320,6 +400,10 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_Ap, NF * 3); ALLOC(ctx->d_t1, NF * 3); ALLOC(ctx->d_t2, NF * 3); ALLOC(ctx->d_U, NF * 3);
     ALLOC(ctx->d_dpart, (size_t)ctx->dots_blocks * ctx->F * 3);
     ALLOC(ctx->d_dots, ctx->F * 3);
+    ALLOC(ctx->d_res, NF * 3);
+    ALLOC(ctx->d_dU, NF * 3);
+    ALLOC(ctx->d_pcg, ctx->F);
+    ALLOC(ctx->d_pol, ctx->N);
     ALLOC(ctx->d_coef, ctx->F * 4);
     ALLOC(ctx->d_active, ctx->F);
     ALLOC(ctx->d_maxcnt, 1);
@@ -387,6 +471,7 @@ extern "C" admp_status admp_set_topology(admp_ctx* ctx, const int32_t* axis_type
         roff[j + 1] = roff[j] + (int)rv[j].size();
         rent.insert(rent.end(), rv[j].begin(), rv[j].end());
     }
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // no queued work may still read the old tables
     dfree(ctx, ctx->d_axis_type); dfree(ctx, ctx->d_axis_idx); dfree(ctx, ctx->d_xoff); dfree(ctx, ctx->d_xent);
     dfree(ctx, ctx->d_rev_off); dfree(ctx, ctx->d_rev_ent);
     HIPCHK(dalloc(ctx, &ctx->d_axis_type, N));
@@ -395,18 +480,20 @@ extern "C" admp_status admp_set_topology(admp_ctx* ctx, const int32_t* axis_type
     HIPCHK(dalloc(ctx, &ctx->d_xent, xent.size()));
     HIPCHK(dalloc(ctx, &ctx->d_rev_off, N + 1));
     HIPCHK(dalloc(ctx, &ctx->d_rev_ent, rent.size()));
-    HIPCHK(hipMemcpy(ctx->d_axis_type, at.data(), N * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->d_axis_idx, ai.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->d_xoff, xoff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
-    if (!xent.empty()) HIPCHK(hipMemcpy(ctx->d_xent, xent.data(), xent.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->d_rev_off, roff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
-    if (!rent.empty()) HIPCHK(hipMemcpy(ctx->d_rev_ent, rent.data(), rent.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(ctx->d_axis_type, at.data(), N * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_axis_idx, ai.data(), 3 * N * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_xoff, xoff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    if (!xent.empty()) HIPCHK(hipMemcpyAsync(ctx->d_xent, xent.data(), xent.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_rev_off, roff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    if (!rent.empty()) HIPCHK(hipMemcpyAsync(ctx->d_rev_ent, rent.data(), rent.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // host vectors are released on return
     ctx->has_topo = true;
     return ADMP_OK;
 }
 
 static admp_status ensure_cap(admp_ctx* ctx, int need) {
     if (need <= ctx->cap && ctx->d_nbr) return ADMP_OK;
+    clear_graphs(ctx);  // captured launches hold the old buffers and capacity
     int cap = std::max(64, ((int)(need * 1.25) + 63) / 64 * 64);
     dfree(ctx, ctx->d_nbr);
     dfree(ctx, ctx->d_pcoef);
@@ -423,6 +510,7 @@ static admp_status ensure_cap(admp_ctx* ctx, int need) {
 
 static admp_status upload_geometry(admp_ctx* ctx, const double* pos, const double* box) {
     const size_t NF = (size_t)ctx->N * ctx->F;
+    ctx->prepped = nullptr;  // site grid bases / sort follow the positions
     hipLaunchKernelGGL(k_pack_pos, dim3(nblk(NF, 256)), dim3(256), 0, ctx->stream, NF, pos, ctx->d_pos);
     const admp_config& c = ctx->cfg;
     hipLaunchKernelGGL(k_box_setup, dim3(nblk(ctx->F, 64)), dim3(64), 0, ctx->stream, ctx->F, box, c.lpme ? c.K1 : 0,
@@ -431,7 +519,7 @@ static admp_status upload_geometry(admp_ctx* ctx, const double* pos, const doubl
     return ADMP_OK;
 }
 
-extern "C" admp_status admp_set_pairs(admp_ctx* ctx, int32_t frame, const void* pairs, int64_t n_pairs,
+static admp_status admp_set_pairs_impl(admp_ctx* ctx, int32_t frame, const void* pairs, int64_t n_pairs,
                                       int32_t pairs_is_int64) {
     if (!ctx || frame < 0 || frame >= ctx->F || (n_pairs > 0 && !pairs)) return ADMP_ERR_ARG;
     const int N = ctx->N;
@@ -480,7 +568,7 @@ extern "C" admp_status admp_set_pairs(admp_ctx* ctx, int32_t frame, const void* 
     return ADMP_OK;
 }
 
-extern "C" admp_status admp_build_neighbors(admp_ctx* ctx, const double* pos, const double* box) {
+static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, const double* box) {
     if (!ctx || !pos || !box) return ADMP_ERR_ARG;
     if (!ctx->has_topo) { ctx->err = "admp_set_topology first"; return ADMP_ERR_STATE; }
     const int N = ctx->N, F = ctx->F;
@@ -559,7 +647,7 @@ extern "C" admp_status admp_build_neighbors(admp_ctx* ctx, const double* pos, co
     return ADMP_OK;
 }
 
-extern "C" admp_status admp_get_pairs(admp_ctx* ctx, int32_t frame, int32_t* out, int64_t cap, int64_t* n_out) {
+static admp_status admp_get_pairs_impl(admp_ctx* ctx, int32_t frame, int32_t* out, int64_t cap, int64_t* n_out) {
     if (!ctx || frame < 0 || frame >= ctx->F || !n_out) return ADMP_ERR_ARG;
     if (!ctx->has_nbr) { ctx->err = "no neighbour list"; return ADMP_ERR_STATE; }
     const int N = ctx->N;
@@ -581,51 +669,141 @@ extern "C" admp_status admp_get_pairs(admp_ctx* ctx, int32_t frame, int32_t* out
 }
 
 // ---------------------------------------------------------------------------
-// Reciprocal-space pipeline on grid g for the sources given by (Qc, U, mode).
+// Reciprocal space.  recip_positions() computes every site's grid base and the
+// counting sort of sites by base grid point; it depends only on positions and
+// the box, so it runs once per evaluation and serves every spread of the solve.
 // ---------------------------------------------------------------------------
+static bool small_grid(const Grid& g) { return g.K[0] < 6 || g.K[1] < 6 || g.K[2] < 6; }
+
+static admp_status recip_positions(admp_ctx* ctx, const Grid& g, int use_dA) {
+    const int N = ctx->N, F = ctx->F;
+    hipStream_t st = ctx->stream;
+    { ProfScope ps_(ctx, 3);
+    hipLaunchKernelGGL(k_grid_base, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_pos, ctx->d_box, use_dA,
+                       ctx->d_mbase, ctx->d_wd);
+    if (!small_grid(g)) {
+        const size_t G = g.G;
+        HIPCHK(hipMemsetAsync(ctx->d_gccount, 0, G * F * sizeof(int), st));
+        HIPCHK(hipMemsetAsync(ctx->d_gcfill, 0, G * F * sizeof(int), st));
+        hipLaunchKernelGGL(k_grid_cell, dim3(nblk(N, 256), F), dim3(256), 0, st, N, g.K[0], g.K[1], g.K[2],
+                           ctx->d_mbase, ctx->d_gcell, ctx->d_gccount);
+        hipLaunchKernelGGL(k_scan, dim3(F), dim3(1024), 0, st, (int)G, ctx->d_gccount, ctx->d_gcstart);
+        hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256), F), dim3(256), 0, st, N, (int)G, ctx->d_gcell,
+                           ctx->d_gcstart, ctx->d_gcfill, ctx->d_gcatoms);
+        hipLaunchKernelGGL(k_cell_sort, dim3(nblk(G, 256), F), dim3(256), 0, st, (int)G, N, ctx->d_gcstart,
+                           ctx->d_gcatoms);
+    }
+    }
+    HIPCHK(hipGetLastError());
+    ctx->prepped = &g;
+    return ADMP_OK;
+}
+
+template <class R> static const Cplx<R>* twid(const Grid& g, int d) {
+    return reinterpret_cast<const Cplx<R>*>(sizeof(R) == 8 ? g.tw64[d] : g.tw[d]);
+}
+
+// Spread the sources (Qc, U, qmode), transform, apply the influence function,
+// transform back and gather.  gather_mode 1: dE/dmu only into out3; 0: full
+// gradients into gq, gpos, vir (dE/dA per site).  Energy/box sums land in d_rec.
 template <class R>
-static admp_status recip_run(admp_ctx* ctx, Grid& g, int use_dA, const double* Qc, const double* U, int qmode,
+static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const double* Qc, const double* U, int qmode,
                              int lmax_eff, int kind, double pref, double kappa, int gather_mode, double* out3,
                              double* gq, double* gpos, double* vir, bool want_energy) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
-    Cplx<R>* grid = reinterpret_cast<Cplx<R>*>(ctx->d_grid);
+    if (ctx->prepped != &g) {
+        admp_status s = recip_positions(ctx, g, use_dA);
+        if (s != ADMP_OK) return s;
+    }
     R* wfrac = reinterpret_cast<R*>(ctx->d_wfrac);
     R* fq = reinterpret_cast<R*>(ctx->d_fq);
-    const size_t tot = g.G * F;
-    hipLaunchKernelGGL(k_zero_cplx<R>, dim3(nblk(tot, 256)), dim3(256), 0, st, tot, grid);
-    { ProfScope ps_(ctx, 3); hipLaunchKernelGGL(k_spline_prep<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_pos, ctx->d_box, Qc, U, qmode,
-                       use_dA, ctx->d_mbase, wfrac, fq); }
-    { ProfScope ps_(ctx, 4); hipLaunchKernelGGL(k_spread_atomic<R>, dim3(nblk((size_t)N * 36, 256), F), dim3(256), 0, st, N, g.K[0], g.K[1],
-                       g.K[2], ctx->d_mbase, wfrac, fq, lmax_eff, grid); }
-    const int maxpts = sizeof(R) == 8 ? 2048 : 4096;
-    auto fft = [&](int inverse) {
-        for (int dd = 0; dd < 3; ++dd) {
-            const int d = inverse ? dd : 2 - dd;
-            const int K = g.K[d];
-            const int nlines = (int)(g.G / K);
-            int LB = std::max(1, std::min(maxpts / K, nlines));
-            const size_t sm = (size_t)2 * LB * K * sizeof(Cplx<R>);
-            { ProfScope ps_(ctx, 5); hipLaunchKernelGGL(k_fft_pass<R>, dim3(nblk(nlines, LB), F), dim3(256), sm, st, grid, g.K[0], g.K[1], g.K[2],
-                               d, g.plan[d], reinterpret_cast<const Cplx<R>*>(g.tw[d]), inverse, LB); }
-        }
-    };
-    fft(0);
-    const int nbI = (int)nblk(g.G, 256);
-    { ProfScope ps_(ctx, 6); hipLaunchKernelGGL(k_influence<R>, dim3(nbI, F), dim3(256), 0, st, g.K[0], g.K[1], g.K[2], ctx->d_box, use_dA,
-                       g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind, pref, grid, ctx->d_part); }
+    { ProfScope ps_(ctx, 3);
+    hipLaunchKernelGGL(k_frac_mult<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_box, use_dA, Qc, U, qmode,
+                       ctx->d_wd, wfrac, fq); }
+    const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
+    if (small_grid(g)) {
+        // grids below 6 points per axis: scatter with atomics on the complex C2C path
+        Cplx<R>* grid = reinterpret_cast<Cplx<R>*>(ctx->d_grid);
+        const size_t tot = g.G * F;
+        hipLaunchKernelGGL(k_zero_cplx<R>, dim3(nblk(tot, 256)), dim3(256), 0, st, tot, grid);
+        { ProfScope ps_(ctx, 4);
+        hipLaunchKernelGGL(k_spread_atomic<R>, dim3(nblk((size_t)N * 36, 256), F), dim3(256), 0, st, N, K0, K1, K2,
+                           ctx->d_mbase, wfrac, fq, lmax_eff, grid); }
+        const int maxpts = sizeof(R) == 8 ? 2048 : 4096;
+        auto fft = [&](int inverse) {
+            for (int dd = 0; dd < 3; ++dd) {
+                const int d = inverse ? dd : 2 - dd;
+                const int K = g.K[d];
+                const int nlines = (int)(g.G / K);
+                int LB = std::max(1, std::min(maxpts / K, nlines));
+                const size_t sm = (size_t)2 * LB * K * sizeof(Cplx<R>);
+                { ProfScope ps_(ctx, 5);
+                hipLaunchKernelGGL(k_fft_pass<R>, dim3(nblk(nlines, LB), F), dim3(256), sm, st, grid, K0, K1, K2, d,
+                                   g.plan[d], twid<R>(g, d), inverse, LB); }
+            }
+        };
+        fft(0);
+        const int nbI = (int)nblk(g.G, 256);
+        { ProfScope ps_(ctx, 6);
+        hipLaunchKernelGGL(k_influence<R>, dim3(nbI, F), dim3(256), 0, st, K0, K1, K2, ctx->d_box, use_dA, g.bmod[0],
+                           g.bmod[1], g.bmod[2], kappa, kind, pref, grid, ctx->d_part); }
+        if (want_energy)
+            hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbI, 7, ctx->d_part, ctx->d_rec, 0);
+        fft(1);
+        { ProfScope ps_(ctx, 7);
+        hipLaunchKernelGGL(k_gather<R>, dim3(nblk(N, 64), F), dim3(64), 0, st, N, K0, K1, K2, ctx->d_pos, ctx->d_box,
+                           use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, grid, gather_mode, out3, gq, gpos, vir); }
+        HIPCHK(hipGetLastError());
+        return ADMP_OK;
+    }
+    R* rg = reinterpret_cast<R*>(ctx->d_rgrid);
+    Cplx<R>* hg = reinterpret_cast<Cplx<R>*>(ctx->d_hgrid);
+    const int H = K2 / 2 + 1;
+    const int nbricks = ((K0 + BRX - 1) / BRX) * ((K1 + BRY - 1) / BRY) * ((K2 + BRZ - 1) / BRZ);
+    { ProfScope ps_(ctx, 4);
+    if (lmax_eff >= 2)
+        hipLaunchKernelGGL((k_spread_brick<R, 2>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, ctx->d_mbase, wfrac,
+                           fq, ctx->d_gcstart, ctx->d_gcatoms, rg);
+    else if (lmax_eff == 1)
+        hipLaunchKernelGGL((k_spread_brick<R, 1>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, ctx->d_mbase, wfrac,
+                           fq, ctx->d_gcstart, ctx->d_gcatoms, rg);
+    else
+        hipLaunchKernelGGL((k_spread_brick<R, 0>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, ctx->d_mbase, wfrac,
+                           fq, ctx->d_gcstart, ctx->d_gcatoms, rg);
+    }
+    const int LB = 4, TH = 64;
+    const int npairs = (K0 * K1 + 1) / 2;
+    const int nly = K0 * H, nlx = K1 * H;
+    { ProfScope ps_(ctx, 5);
+    hipLaunchKernelGGL(k_fft_z_r2c<R>, dim3(nblk(npairs, LB), F), dim3(TH), 2 * LB * K2 * sizeof(Cplx<R>), st, K0, K1,
+                       K2, g.plan[2], twid<R>(g, 2), rg, hg, LB);
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(nblk(nly, LB), F), dim3(TH), 2 * LB * K1 * sizeof(Cplx<R>), st, K0, K1, H,
+                       g.plan[1], twid<R>(g, 1), hg, 0, LB); }
+    const int nbx = (int)nblk(nlx, LB);
+    { ProfScope ps_(ctx, 6);
+    hipLaunchKernelGGL(k_fft_x_infl<R>, dim3(nbx, F), dim3(TH), 2 * LB * K0 * sizeof(Cplx<R>), st, K0, K1, K2,
+                       g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind, pref,
+                       ctx->d_part, LB); }
     if (want_energy)
-        hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbI, 7, ctx->d_part, ctx->d_rec, 0);
-    fft(1);
-    { ProfScope ps_(ctx, 7); hipLaunchKernelGGL(k_gather<R>, dim3(nblk(N, 64), F), dim3(64), 0, st, N, g.K[0], g.K[1], g.K[2], ctx->d_pos,
-                       ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, grid, gather_mode, out3, gq, gpos, vir); }
+        hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbx, 7, ctx->d_part, ctx->d_rec, 0);
+    { ProfScope ps_(ctx, 5);
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(nblk(nly, LB), F), dim3(TH), 2 * LB * K1 * sizeof(Cplx<R>), st, K0, K1, H,
+                       g.plan[1], twid<R>(g, 1), hg, 1, LB);
+    hipLaunchKernelGGL(k_fft_z_c2r<R>, dim3(nblk(npairs, LB), F), dim3(TH), 2 * LB * K2 * sizeof(Cplx<R>), st, K0, K1,
+                       K2, g.plan[2], twid<R>(g, 2), hg, rg, LB); }
+    { ProfScope ps_(ctx, 7);
+    hipLaunchKernelGGL(k_gather_ml<R>, dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2, ctx->d_pos,
+                       ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, gather_mode, out3, gq, gpos,
+                       vir); }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
 
-// A v for the induced-dipole system (everything but the stored right-hand side).
-// The real-space coupling uses the per-pair coefficients stored by k_real_permfield.
-template <class R>
+// A v for the induced-dipole system (everything but the right-hand side).
+// RC: type of the stored pair coefficients; R: arithmetic of the real-space and
+// reciprocal products (the mixed solver's FP64 residual uses RC=float, R=double).
+template <class RC, class R>
 static admp_status matvec(admp_ctx* ctx, const double* v, double* out, const double* pol, double f1) {
     const int N = ctx->N, F = ctx->F;
     const size_t NF = (size_t)N * F;
@@ -637,9 +815,10 @@ static admp_status matvec(admp_ctx* ctx, const double* v, double* out, const dou
         hipLaunchKernelGGL(k_cast<R>, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, v, reinterpret_cast<R*>(ctx->d_vR));
         vR = reinterpret_cast<const R*>(ctx->d_vR);
     }
-    { ProfScope ps_(ctx, 2); hipLaunchKernelGGL((k_real_matvec<R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
+    { ProfScope ps_(ctx, 2);
+    hipLaunchKernelGGL((k_real_matvec<RC, R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
                        ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
-                       reinterpret_cast<const R*>(ctx->d_pcoef), vR, ctx->d_t1); }
+                       reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1); }
     double* rec = nullptr;
     if (ctx->cfg.lpme) {
         admp_status s = recip_run<R>(ctx, ctx->eg, 0, nullptr, v, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1, ctx->d_t2,
@@ -647,17 +826,25 @@ static admp_status matvec(admp_ctx* ctx, const double* v, double* out, const dou
         if (s != ADMP_OK) return s;
         rec = ctx->d_t2;
     }
-    hipLaunchKernelGGL(k_apply_diag, dim3(nblk(N, 256), F), dim3(256), 0, st, N, f1, pol, v, ctx->d_t1, rec, out);
+    { ProfScope ps_(ctx, 11);
+    hipLaunchKernelGGL(k_apply_diag, dim3(nblk(N, 256), F), dim3(256), 0, st, N, f1, pol, v, ctx->d_t1, rec, out); }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
+}
+
+// per-frame (x.y, x.z, max |x| over polarisable sites) into ctx->d_dots (device only)
+static void dots_dev(admp_ctx* ctx, const double* x, const double* y, const double* z, const double* pol) {
+    const int F = ctx->F;
+    ProfScope ps_(ctx, 11);
+    hipLaunchKernelGGL(k_dots, dim3(ctx->dots_blocks, F), dim3(256), 0, ctx->stream, ctx->N, x, y, z, pol, ctx->d_dpart);
+    hipLaunchKernelGGL(k_dots_final, dim3(nblk(F, 64)), dim3(64), 0, ctx->stream, F, ctx->dots_blocks, ctx->d_dpart,
+                       ctx->d_dots);
 }
 
 static admp_status dots(admp_ctx* ctx, const double* x, const double* y, const double* z, const double* pol,
                         std::vector<double>& h) {
     const int F = ctx->F;
-    hipLaunchKernelGGL(k_dots, dim3(ctx->dots_blocks, F), dim3(256), 0, ctx->stream, ctx->N, x, y, z, pol, ctx->d_dpart);
-    hipLaunchKernelGGL(k_dots_final, dim3(nblk(F, 64)), dim3(64), 0, ctx->stream, F, ctx->dots_blocks, ctx->d_dpart,
-                       ctx->d_dots);
+    dots_dev(ctx, x, y, z, pol);
     h.resize(3 * F);
     HIPCHK(hipMemcpyAsync(h.data(), ctx->d_dots, 3 * F * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -669,8 +856,131 @@ struct SolveInfo {
     std::vector<double> res;
 };
 
-// Induced-dipole solve: U (ctx->d_U) from the linear system A U = b,
-// b = -dE/dU at U = 0 (real permanent field, reciprocal field, self term).
+// Preconditioned CG on A x = rhs with all scalars on the device; the host looks
+// at the per-frame done flags once per chunk of iterations.  x holds the initial
+// guess on entry.  Returns iterations per frame in it_out and the final
+// recursively updated residual max-norm in res_out.
+template <class RC, class R>
+static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double* pol, double f1, double tol,
+                       int maxiter, std::vector<int>& it_out, std::vector<double>& res_out) {
+    const int N = ctx->N, F = ctx->F;
+    const size_t NF = (size_t)N * F;
+    hipStream_t st = ctx->stream;
+    admp_status s;
+    if ((s = matvec<RC, R>(ctx, x, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
+    hipLaunchKernelGGL(k_lincomb, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, 1.0, rhs, -1.0, ctx->d_Ap, ctx->d_r);
+    hipLaunchKernelGGL(k_precond, dim3(nblk(N, 256), F), dim3(256), 0, st, N, pol, ctx->d_r, ctx->d_z, ctx->d_p);
+    dots_dev(ctx, ctx->d_r, ctx->d_z, nullptr, pol);
+    hipLaunchKernelGGL(k_pcg_init, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_dots, tol, ctx->d_pcg);
+    const int chunk = 4;
+    std::vector<PcgState> hs(F);
+    int done_iters = 0;
+    for (;;) {
+        HIPCHK(hipMemcpyAsync(hs.data(), ctx->d_pcg, F * sizeof(PcgState), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        bool all = true;
+        for (int f = 0; f < F; ++f) all = all && hs[f].done;
+        if (all || done_iters >= maxiter) break;
+        const int nrun = std::min(chunk, maxiter - done_iters);
+        auto iterate = [&]() -> admp_status {
+            for (int c = 0; c < nrun; ++c) {
+                admp_status s2;
+                if ((s2 = matvec<RC, R>(ctx, ctx->d_p, ctx->d_Ap, pol, f1)) != ADMP_OK) return s2;
+                dots_dev(ctx, ctx->d_p, ctx->d_Ap, nullptr, pol);
+                { ProfScope ps_(ctx, 11);
+                hipLaunchKernelGGL(k_pcg_xr, dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->d_pcg, ctx->d_dots, pol, x,
+                                   ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap); }
+                dots_dev(ctx, ctx->d_r, ctx->d_z, nullptr, pol);
+                { ProfScope ps_(ctx, 11);
+                hipLaunchKernelGGL(k_pcg_beta, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_dots, tol, ctx->d_pcg);
+                hipLaunchKernelGGL(k_pcg_p, dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->d_pcg, ctx->d_z, ctx->d_p); }
+            }
+            return ADMP_OK;
+        };
+        if (!ctx->use_graphs) {
+            if ((s = iterate()) != ADMP_OK) return s;
+            done_iters += nrun;
+            continue;
+        }
+        // a chunk of iterations is captured once as a HIP graph and replayed
+        const int type = (sizeof(RC) == 8 ? 1 : 0) + (sizeof(R) == 8 ? 2 : 0);
+        admp_ctx::ChunkGraph* cg = nullptr;
+        for (auto& g : ctx->graphs)
+            if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1)
+                cg = &g;
+        if (!cg) {
+            admp_ctx::ChunkGraph* slot = nullptr;
+            for (auto& g : ctx->graphs)
+                if (!g.exec) { slot = &g; break; }
+            if (!slot) { clear_graphs(ctx); slot = &ctx->graphs[0]; }
+            const bool prof = ctx->prof;
+            ctx->prof = false;
+            hipGraph_t graph = nullptr;
+            hipError_t be = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (be != hipSuccess) {
+                ctx->prof = prof;
+                ctx->err = std::string("graph capture begin: ") + hipGetErrorString(be);
+                return ADMP_ERR_HIP;
+            }
+            admp_status cs = iterate();
+            hipError_t ce = hipStreamEndCapture(st, &graph);
+            ctx->prof = prof;
+            if (cs != ADMP_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return cs;
+            }
+            if (ce != hipSuccess) { ctx->err = std::string("graph capture: ") + hipGetErrorString(ce); return ADMP_ERR_HIP; }
+            hipGraphExec_t exec = nullptr;
+            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) { ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ie); return ADMP_ERR_HIP; }
+            slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
+            slot->tol = tol; slot->f1 = f1;
+            cg = slot;
+        }
+        { ProfScope ps_(ctx, 12);
+        HIPCHK(hipGraphLaunch(cg->exec, st)); }
+        done_iters += nrun;
+    }
+    it_out.resize(F);
+    res_out.resize(F);
+    for (int f = 0; f < F; ++f) { it_out[f] = hs[f].iter; res_out[f] = hs[f].res; }
+    HIPCHK(hipGetLastError());
+    return ADMP_OK;
+}
+
+// right-hand side b = -dE/dU at U = 0: real permanent field (+ stored pair
+// coefficients), minus the reciprocal dE/dmu of the permanent multipoles, plus
+// the self term 2 D f1 mu.  RB: precision of the reciprocal part.
+template <class R, class RB>
+static admp_status solver_rhs(admp_ctx* ctx, const double* pol, const double* thole, const PairParams& pp) {
+    const admp_config& c = ctx->cfg;
+    const int N = ctx->N, F = ctx->F;
+    const size_t NF = (size_t)N * F;
+    hipStream_t st = ctx->stream;
+    const double kappa = c.lpme ? c.kappa : 0.0;
+    const double f1 = c.lpme ? kappa / SQRT_PI * 2.0 * kappa * kappa / 3.0 : 0.0;
+    const int lmax_eff = c.lmax == 0 ? 1 : c.lmax;
+    R* site = reinterpret_cast<R*>(ctx->d_site);
+    hipLaunchKernelGGL(k_pack_site<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_Qc, pol, thole,
+                       (const double*)nullptr, site);
+    { ProfScope ps_(ctx, 1);
+    hipLaunchKernelGGL((k_real_permfield<R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
+                       ctx->cap, ctx->d_pos, ctx->d_box, site, ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_b,
+                       reinterpret_cast<R*>(ctx->d_pcoef)); }
+    HIPCHK(hipGetLastError());
+    if (c.lpme) {
+        admp_status s = recip_run<RB>(ctx, ctx->eg, 0, ctx->d_Qc, nullptr, 1, lmax_eff, 0, DIELECTRIC, kappa, 1,
+                                      ctx->d_t2, nullptr, nullptr, nullptr, false);
+        if (s != ADMP_OK) return s;
+        hipLaunchKernelGGL(k_sub, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_t2, ctx->d_b);
+        hipLaunchKernelGGL(k_add_self_b, dim3(nblk(N, 256), F), dim3(256), 0, st, N, f1, ctx->d_Qc, ctx->d_b);
+    }
+    HIPCHK(hipGetLastError());
+    return ADMP_OK;
+}
+
+// Induced-dipole solve: U (ctx->d_U) from A U = b.
 template <class R>
 static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, const PairParams& pp,
                          const admp_solver_opts* o, const double* U_in, SolveInfo& info) {
@@ -678,88 +988,56 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     const int N = ctx->N, F = ctx->F;
     const size_t NF = (size_t)N * F;
     hipStream_t st = ctx->stream;
-    const int lpme = c.lpme;
-    const double kappa = lpme ? c.kappa : 0.0;
-    const double f1 = lpme ? kappa / SQRT_PI * 2.0 * kappa * kappa / 3.0 : 0.0;
-    const int lmax_eff = c.lmax == 0 ? 1 : c.lmax;
-    R* site = reinterpret_cast<R*>(ctx->d_site);
-    const dim3 pgrid(nblk((size_t)N * TPA, PAIR_BLOCK), F);
+    const double kappa = c.lpme ? c.kappa : 0.0;
+    const double f1 = c.lpme ? kappa / SQRT_PI * 2.0 * kappa * kappa / 3.0 : 0.0;
     double* U = ctx->d_U;
     const int kind = o ? o->kind : ADMP_SOLVER_PCG;
     const double tol = o ? o->tol : 1e-6;
     const int maxiter = o ? (kind == ADMP_SOLVER_STEPS ? o->steps : o->maxiter) : 500;
+    const bool mixed = sizeof(R) == 4;
     if (o && o->warm_start && U_in)
         HIPCHK(hipMemcpyAsync(U, U_in, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     else
         HIPCHK(hipMemsetAsync(U, 0, NF * 3 * sizeof(double), st));
-    hipLaunchKernelGGL(k_pack_site<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_Qc, pol, thole,
-                       (const double*)nullptr, site);
-    { ProfScope ps_(ctx, 1); hipLaunchKernelGGL((k_real_permfield<R>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box, site,
-                       ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_b, reinterpret_cast<R*>(ctx->d_pcoef)); }
-    HIPCHK(hipGetLastError());
-    if (lpme) {
-        admp_status s = recip_run<R>(ctx, ctx->eg, 0, ctx->d_Qc, nullptr, 1, lmax_eff, 0, DIELECTRIC, kappa, 1, ctx->d_t2,
-                                     nullptr, nullptr, nullptr, false);
-        if (s != ADMP_OK) return s;
-        hipLaunchKernelGGL(k_sub, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_t2, ctx->d_b);
-        hipLaunchKernelGGL(k_add_self_b, dim3(nblk(N, 256), F), dim3(256), 0, st, N, f1, ctx->d_Qc, ctx->d_b);
-    }
+    admp_status s;
+    // right-hand side: reciprocal part in FP64 even on the mixed path (FP64 residual)
+    if ((s = solver_rhs<R, double>(ctx, pol, thole, pp)) != ADMP_OK) return s;
     info.n_iter.assign(F, 0);
     info.conv.assign(F, 0);
     info.res.assign(F, 0.0);
     std::vector<int> act(F, 1);
-    std::vector<double> h, rz(F, 0.0);
-    admp_status s;
-    if (kind == ADMP_SOLVER_PCG) {
-        if ((s = matvec<R>(ctx, U, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
-        hipLaunchKernelGGL(k_lincomb, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, 1.0, ctx->d_b, -1.0, ctx->d_Ap,
-                           ctx->d_r);
-        hipLaunchKernelGGL(k_precond, dim3(nblk(N, 256), F), dim3(256), 0, st, N, pol, ctx->d_r, ctx->d_z, ctx->d_p);
-        if ((s = dots(ctx, ctx->d_r, ctx->d_z, nullptr, pol, h)) != ADMP_OK) return s;
-        for (int f = 0; f < F; ++f) rz[f] = h[3 * f];
-        int it = 0;
-        for (it = 0; it < maxiter; ++it) {
-            int nact = 0;
-            for (int f = 0; f < F; ++f) {
-                info.res[f] = h[3 * f + 2];
-                if (act[f] && h[3 * f + 2] < tol) { act[f] = 0; info.conv[f] = 1; info.n_iter[f] = it; }
-                nact += act[f];
-            }
-            if (!nact) break;
-            HIPCHK(hipMemcpyAsync(ctx->d_active, act.data(), F * sizeof(int), hipMemcpyHostToDevice, st));
-            if ((s = matvec<R>(ctx, ctx->d_p, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
-            if ((s = dots(ctx, ctx->d_p, ctx->d_Ap, nullptr, pol, h)) != ADMP_OK) return s;
-            std::vector<double> coef(4 * F, 0.0);
-            for (int f = 0; f < F; ++f) coef[4 * f] = h[3 * f] != 0 ? rz[f] / h[3 * f] : 0.0;
-            HIPCHK(hipMemcpyAsync(ctx->d_coef, coef.data(), 4 * F * sizeof(double), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_pcg_update, dim3(nblk(N, 256), F), dim3(256), 0, st, N, 0, ctx->d_coef, ctx->d_active,
-                               pol, U, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap);
-            if ((s = dots(ctx, ctx->d_r, ctx->d_z, nullptr, pol, h)) != ADMP_OK) return s;
-            for (int f = 0; f < F; ++f) {
-                coef[4 * f + 1] = rz[f] != 0 ? h[3 * f] / rz[f] : 0.0;
-                if (act[f]) rz[f] = h[3 * f];
-            }
-            HIPCHK(hipMemcpyAsync(ctx->d_coef, coef.data(), 4 * F * sizeof(double), hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_pcg_update, dim3(nblk(N, 256), F), dim3(256), 0, st, N, 1, ctx->d_coef, ctx->d_active,
-                               pol, U, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap);
-        }
-        for (int f = 0; f < F; ++f)
-            if (act[f]) { info.n_iter[f] = it; info.res[f] = h[3 * f + 2]; }
-        // true residual of the final iterate
-        if ((s = matvec<R>(ctx, U, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
-        hipLaunchKernelGGL(k_lincomb, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, 1.0, ctx->d_Ap, -1.0, ctx->d_b,
-                           ctx->d_r);
-        if ((s = dots(ctx, ctx->d_r, nullptr, nullptr, pol, h)) != ADMP_OK) return s;
-        for (int f = 0; f < F; ++f) {
-            info.res[f] = h[3 * f + 2];
-            info.conv[f] = h[3 * f + 2] < tol ? 1 : 0;
+    std::vector<double> h;
+    if (kind == ADMP_SOLVER_PCG && !mixed) {
+        std::vector<int> it;
+        std::vector<double> rr;
+        if ((s = pcg<double, double>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr)) != ADMP_OK) return s;
+        for (int f = 0; f < F; ++f) info.n_iter[f] = it[f];
+    } else if (kind == ADMP_SOLVER_PCG) {
+        // iterative refinement: FP64 residual r = b - A U (pair coefficients stored in
+        // FP32, products and reciprocal part in FP64), FP32 PCG for the correction
+        std::vector<int> it;
+        std::vector<double> rr;
+        for (int outer = 0; outer < 8; ++outer) {
+            if ((s = matvec<R, double>(ctx, U, ctx->d_res, pol, f1)) != ADMP_OK) return s;
+            hipLaunchKernelGGL(k_lincomb, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, 1.0, ctx->d_b, -1.0,
+                               ctx->d_res, ctx->d_res);
+            if ((s = dots(ctx, ctx->d_res, nullptr, nullptr, pol, h)) != ADMP_OK) return s;
+            double worst = 0;
+            for (int f = 0; f < F; ++f) worst = std::max(worst, h[3 * f + 2]);
+            if (worst < tol) break;
+            // correction: A dU = r, FP32 products, relative reduction 1e-4 (not below tol/2)
+            HIPCHK(hipMemsetAsync(ctx->d_dU, 0, NF * 3 * sizeof(double), st));
+            const double itol = std::max(0.5 * tol, 1e-4 * worst);
+            if ((s = pcg<R, R>(ctx, ctx->d_res, ctx->d_dU, pol, f1, itol, maxiter, it, rr)) != ADMP_OK) return s;
+            hipLaunchKernelGGL(k_vadd, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_dU, U);
+            for (int f = 0; f < F; ++f) info.n_iter[f] += it[f];
         }
     } else {
         // Jacobi sweeps (pme.py:380-422)
         HIPCHK(hipMemcpyAsync(ctx->d_active, act.data(), F * sizeof(int), hipMemcpyHostToDevice, st));
         int i = 0;
         for (i = 0; i < maxiter; ++i) {
-            if ((s = matvec<R>(ctx, U, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
+            if ((s = matvec<R, R>(ctx, U, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
             hipLaunchKernelGGL(k_sub, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_b, ctx->d_Ap);
             if (kind == ADMP_SOLVER_REFERENCE) {
                 if ((s = dots(ctx, ctx->d_Ap, nullptr, nullptr, pol, h)) != ADMP_OK) return s;
@@ -780,12 +1058,19 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
             if (kind == ADMP_SOLVER_STEPS) { info.n_iter[f] = maxiter; info.conv[f] = 1; }
             else if (act[f]) { info.n_iter[f] = maxiter - 1; info.conv[f] = 0; }
         }
-        if (kind == ADMP_SOLVER_STEPS) {
-            if ((s = matvec<R>(ctx, U, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
-            hipLaunchKernelGGL(k_sub, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_b, ctx->d_Ap);
-            if ((s = dots(ctx, ctx->d_Ap, nullptr, nullptr, pol, h)) != ADMP_OK) return s;
-            for (int f = 0; f < F; ++f) info.res[f] = h[3 * f + 2];
+        if (kind == ADMP_SOLVER_REFERENCE) {
+            HIPCHK(hipGetLastError());
+            return ADMP_OK;
         }
+    }
+    // true residual of the final iterate (FP64 products)
+    if ((s = matvec<R, double>(ctx, U, ctx->d_res, pol, f1)) != ADMP_OK) return s;
+    hipLaunchKernelGGL(k_lincomb, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, 1.0, ctx->d_res, -1.0, ctx->d_b,
+                       ctx->d_res);
+    if ((s = dots(ctx, ctx->d_res, nullptr, nullptr, pol, h)) != ADMP_OK) return s;
+    for (int f = 0; f < F; ++f) {
+        info.res[f] = h[3 * f + 2];
+        if (kind != ADMP_SOLVER_STEPS) info.conv[f] = h[3 * f + 2] < tol ? 1 : 0;
     }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
@@ -893,7 +1178,10 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     HIPCHK(hipStreamSynchronize(st));
     std::vector<double> e(F);
     for (int f = 0; f < F; ++f) e[f] = comp[f] + comp[F + f] + comp[2 * F + f] + rec[7 * f];
-    if (energy) HIPCHK(hipMemcpy(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice));
+    if (energy) {
+        HIPCHK(hipMemcpyAsync(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     admp_status ret = ADMP_OK;
     if (stats) {
         memset(stats, 0, sizeof(*stats));
@@ -917,7 +1205,7 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     return ret;
 }
 
-extern "C" admp_status admp_compute(admp_ctx* ctx, const double* pos, const double* box, const double* Q_local,
+static admp_status admp_compute_impl(admp_ctx* ctx, const double* pos, const double* box, const double* Q_local,
                                     const double* pol, const double* thole, const double* mScales,
                                     const double* pScales, const double* dScales, int32_t n_scales, double* U_inout,
                                     const admp_solver_opts* opts, int32_t want_grad, double* energy, double* dpos,
@@ -926,6 +1214,10 @@ extern "C" admp_status admp_compute(admp_ctx* ctx, const double* pos, const doub
     if (!ctx->has_topo) { ctx->err = "admp_set_topology first"; return ADMP_ERR_STATE; }
     if (!ctx->has_nbr) { ctx->err = "no pair list: admp_set_pairs or admp_build_neighbors first"; return ADMP_ERR_STATE; }
     if (ctx->cfg.lpol && (!pol || !thole)) { ctx->err = "pol and thole required"; return ADMP_ERR_ARG; }
+    if (ctx->cfg.lpol) {
+        HIPCHK(hipMemcpyAsync(ctx->d_pol, pol, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        pol = ctx->d_pol;
+    }
     if (n_scales < 1 || n_scales > 8) { ctx->err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
     admp_status s = upload_geometry(ctx, pos, box);
     if (s != ADMP_OK) return s;
@@ -1002,11 +1294,14 @@ static admp_status disp_impl(admp_ctx* ctx, const double* c_list, const PairPara
     HIPCHK(hipStreamSynchronize(st));
     std::vector<double> e(F);
     for (int f = 0; f < F; ++f) e[f] = comp[f] + comp[F + f] + erec[f];
-    if (energy) HIPCHK(hipMemcpy(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice));
+    if (energy) {
+        HIPCHK(hipMemcpyAsync(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     return ADMP_OK;
 }
 
-extern "C" admp_status admp_disp_compute(admp_ctx* ctx, const double* pos, const double* box, const double* c_list,
+static admp_status admp_disp_compute_impl(admp_ctx* ctx, const double* pos, const double* box, const double* c_list,
                                          const double* mScales, int32_t n_scales, int32_t want_grad, double* energy,
                                          double* dpos, double* dbox, double* dc) {
     if (!ctx || !pos || !box || !c_list) return ADMP_ERR_ARG;
@@ -1019,7 +1314,7 @@ extern "C" admp_status admp_disp_compute(admp_ctx* ctx, const double* pos, const
     return disp_impl<double>(ctx, c_list, pp, want_grad, energy, dpos, dbox, dc);
 }
 
-extern "C" admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* box, const double* a,
+static admp_status admp_slater_compute_impl(admp_ctx* ctx, const double* pos, const double* box, const double* a,
                                            const double* b, const double* mScales, int32_t n_scales, int32_t want_grad,
                                            double* energy, double* dpos, double* dbox, double* da, double* db) {
     if (!ctx || !pos || !box || !a || !b) return ADMP_ERR_ARG;
@@ -1060,6 +1355,57 @@ extern "C" admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, con
         if (db) hipLaunchKernelGGL(k_copy_cols, dim3(nblk(NF, 256)), dim3(256), 0, st, NF, 1, 3, 1, ctx->d_gpos_rec, db);
     }
     HIPCHK(hipStreamSynchronize(st));
-    if (energy) HIPCHK(hipMemcpy(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice));
+    if (energy) {
+        HIPCHK(hipMemcpyAsync(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     return ADMP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Public entry points: order the context stream after the caller's stream,
+// run, then order the caller's stream after the context's work.
+// ---------------------------------------------------------------------------
+#define ADMP_WRAP(call)                                   \
+    do {                                                  \
+        if (!ctx) return ADMP_ERR_ARG;                    \
+        admp_status s_ = enter(ctx);                      \
+        if (s_ != ADMP_OK) return s_;                     \
+        admp_status r_ = (call);                          \
+        admp_status l_ = leave(ctx);                      \
+        return r_ != ADMP_OK ? r_ : l_;                   \
+    } while (0)
+
+extern "C" admp_status admp_set_pairs(admp_ctx* ctx, int32_t frame, const void* pairs, int64_t n_pairs,
+                                      int32_t pairs_is_int64) {
+    ADMP_WRAP(admp_set_pairs_impl(ctx, frame, pairs, n_pairs, pairs_is_int64));
+}
+
+extern "C" admp_status admp_build_neighbors(admp_ctx* ctx, const double* pos, const double* box) {
+    ADMP_WRAP(admp_build_neighbors_impl(ctx, pos, box));
+}
+
+extern "C" admp_status admp_get_pairs(admp_ctx* ctx, int32_t frame, int32_t* out, int64_t cap, int64_t* n_out) {
+    ADMP_WRAP(admp_get_pairs_impl(ctx, frame, out, cap, n_out));
+}
+
+extern "C" admp_status admp_compute(admp_ctx* ctx, const double* pos, const double* box, const double* Q_local,
+                                    const double* pol, const double* thole, const double* mScales,
+                                    const double* pScales, const double* dScales, int32_t n_scales, double* U_inout,
+                                    const admp_solver_opts* opts, int32_t want_grad, double* energy, double* dpos,
+                                    double* dbox, double* dQ_local, double* dpol, double* dthole, admp_stats* stats) {
+    ADMP_WRAP(admp_compute_impl(ctx, pos, box, Q_local, pol, thole, mScales, pScales, dScales, n_scales, U_inout, opts,
+                                want_grad, energy, dpos, dbox, dQ_local, dpol, dthole, stats));
+}
+
+extern "C" admp_status admp_disp_compute(admp_ctx* ctx, const double* pos, const double* box, const double* c_list,
+                                         const double* mScales, int32_t n_scales, int32_t want_grad, double* energy,
+                                         double* dpos, double* dbox, double* dc) {
+    ADMP_WRAP(admp_disp_compute_impl(ctx, pos, box, c_list, mScales, n_scales, want_grad, energy, dpos, dbox, dc));
+}
+
+extern "C" admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* box, const double* a,
+                                           const double* b, const double* mScales, int32_t n_scales, int32_t want_grad,
+                                           double* energy, double* dpos, double* dbox, double* da, double* db) {
+    ADMP_WRAP(admp_slater_compute_impl(ctx, pos, box, a, b, mScales, n_scales, want_grad, energy, dpos, dbox, da, db));
 }

@@ -460,10 +460,12 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
 }
 
 // out_I = sum_J c1 V_J + c2 (z.V_J) z   (induced-induced real-space coupling)
-template <class R>
+// RC: stored coefficient type, R: arithmetic type of the products (double for the
+// FP64 residual of the mixed-precision solve)
+template <class RC, class R>
 __global__ void __launch_bounds__(PAIR_BLOCK)
 k_real_matvec(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
-              const int* __restrict__ nbr, const int* __restrict__ ncnt, const R* __restrict__ pcoef,
+              const int* __restrict__ nbr, const int* __restrict__ ncnt, const RC* __restrict__ pcoef,
               const R* __restrict__ v, double* __restrict__ out) {
     const int f = blockIdx.y;
     const int gt = blockIdx.x * PAIR_BLOCK + threadIdx.x;
@@ -476,7 +478,7 @@ k_real_matvec(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __
     double ox = 0, oy = 0, oz = 0;
     const int cnt = active ? ncnt[fo + ii] : 0;
     const int* nl = nbr + (fo + ii) * (size_t)cap;
-    const R* pc = pcoef + (fo + ii) * (size_t)cap * 2;
+    const RC* pc = pcoef + (fo + ii) * (size_t)cap * 2;
     for (int k = sub; k < cnt; k += TPA) {
         const int j = nl[k] & NB_MASK;
         const double4 pj = pos[fo + j];
@@ -486,7 +488,7 @@ k_real_matvec(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __
         const R zx = (R)(dx * rinv), zy = (R)(dy * rinv), zz = (R)(dz * rinv);
         const R* vj = v + (fo + j) * 3;
         const R vx = vj[0], vy = vj[1], vz = vj[2];
-        const R c1 = pc[2 * k], c2 = pc[2 * k + 1];
+        const R c1 = (R)pc[2 * k], c2 = (R)pc[2 * k + 1];
         const R p = c2 * (zx * vx + zy * vy + zz * vz);
         ox += (double)(c1 * vx + p * zx);
         oy += (double)(c1 * vy + p * zy);

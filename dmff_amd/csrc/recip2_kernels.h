@@ -1,0 +1,572 @@
+// Reciprocal space, second generation: atomic-free spreading, real-to-complex
+// FFT passes with the influence function fused into the middle pass, and a
+// multi-lane gather.
+//
+//   spline_prep   per site: grid base m, offset w = m - R, fractional multipoles
+//   grid_sort     sites bucketed by base grid point (counting sort, fixed order)
+//   spread_brick  one wavefront per 8x4x2 brick of grid points; the sites whose
+//                 6^3 support touches the brick are staged in LDS in chunks and
+//                 every lane accumulates its own grid point: each grid value is
+//                 written exactly once (no atomics, no zero-fill)
+//   fft_z_r2c     two real z-lines packed as one complex FFT, split into halves
+//   fft_y         complex passes along y (forward / inverse)
+//   fft_x_infl    forward x FFT, influence function (energy, box term,
+//                 S <- 2 pref C/theta^2 S), inverse x FFT, in one pass
+//   fft_z_c2r     two half spectra -> one complex inverse FFT -> two real lines
+//   gather_ml     eight lanes per site (one per x-offset), shuffled reduction
+#pragma once
+#include "admp_common.h"
+#include "recip_kernels.h"
+
+namespace admp {
+
+// Piece k (= support offset) of the order-6 cardinal B-spline in the local
+// coordinate t = w in [0,1): value, d/dR (= -d/du) and d2/dR2.
+template <class R>
+__device__ __forceinline__ void bs_piece(int k, R t, R& v, R& d1, R& d2) {
+    const R c = (R)(1.0 / 120.0);
+    R p, dp, ddp;
+    switch (k) {
+        case 0: {
+            R t2 = t * t, t3 = t2 * t;
+            p = t3 * t2 * c; dp = t2 * t2 * (R)(1.0 / 24.0); ddp = t3 * (R)(1.0 / 6.0);
+        } break;
+        case 1:
+            p = ((((-(R)5 * t + (R)5) * t + (R)10) * t + (R)10) * t + (R)5) * t + (R)1;
+            dp = (((-(R)25 * t + (R)20) * t + (R)30) * t + (R)20) * t + (R)5;
+            ddp = ((-(R)100 * t + (R)60) * t + (R)60) * t + (R)20;
+            p *= c; dp *= c; ddp *= c;
+            break;
+        case 2:
+            p = (((((R)10 * t - (R)20) * t - (R)20) * t + (R)20) * t + (R)50) * t + (R)26;
+            dp = ((((R)50 * t - (R)80) * t - (R)60) * t + (R)40) * t + (R)50;
+            ddp = (((R)200 * t - (R)240) * t - (R)120) * t + (R)40;
+            p *= c; dp *= c; ddp *= c;
+            break;
+        case 3:
+            p = ((((-(R)10 * t + (R)30) * t) * t - (R)60) * t) * t + (R)66;
+            dp = (((-(R)50 * t + (R)120) * t) * t - (R)120) * t;
+            ddp = ((-(R)200 * t + (R)360) * t) * t - (R)120;
+            p *= c; dp *= c; ddp *= c;
+            break;
+        case 4:
+            p = (((((R)5 * t - (R)20) * t + (R)20) * t + (R)20) * t - (R)50) * t + (R)26;
+            dp = ((((R)25 * t - (R)80) * t + (R)60) * t + (R)40) * t - (R)50;
+            ddp = (((R)100 * t - (R)240) * t + (R)120) * t + (R)40;
+            p *= c; dp *= c; ddp *= c;
+            break;
+        default: {
+            R s = (R)1 - t, s2 = s * s;
+            p = s2 * s2 * s * c; dp = -s2 * s2 * (R)(1.0 / 24.0); ddp = s2 * s * (R)(1.0 / 6.0);
+        } break;
+    }
+    v = p;
+    d1 = -dp;
+    d2 = ddp;
+}
+
+// cell of each site = its base grid point (m mod K), for the counting sort
+__global__ void k_grid_cell(int N, int K0, int K1, int K2, const int4* __restrict__ mbase, int* __restrict__ cell,
+                            int* __restrict__ ccount) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    const int4 m = mbase[a];
+    const int id = (wrapi(m.x, K0) * K1 + wrapi(m.y, K1)) * K2 + wrapi(m.z, K2);
+    cell[a] = id;
+    atomicAdd(&ccount[(size_t)f * K0 * K1 * K2 + id], 1);
+}
+
+// Fractional dipoles only (solver spreads): fq[1..3] = A^T U, others 0.
+template <class R>
+__global__ void k_frac_dipole(int N, const BoxInfo* __restrict__ boxes, const double* __restrict__ U,
+                              R* __restrict__ fq) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    const double* A = boxes[f].A;
+    const double u0 = U[a * 3], u1 = U[a * 3 + 1], u2 = U[a * 3 + 2];
+    R* o = fq + a * 10;
+    o[0] = 0;
+    for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u0 + A[1 * 3 + j] * u1 + A[2 * 3 + j] * u2);
+    for (int k = 4; k < 10; ++k) o[k] = 0;
+}
+
+constexpr int BRX = 2, BRY = 4, BRZ = 8;   // 64 grid points per wavefront
+constexpr int SP_CHUNK = 64;               // sites staged in LDS per round
+
+// LM: 0 charges only, 1 charges + dipoles, 2 full quadrupoles.
+// Staging computes, per candidate site, the start of its support relative to the
+// brick origin (r in [-5, brick-1]) and its 1-D spline tables (value, d/dR,
+// d2/dR2 at the six offsets) in LDS; the accumulation loop then only does table
+// lookups, with no modular arithmetic and no divergent polynomial evaluation.
+template <class R, int LM>
+__global__ void __launch_bounds__(64)
+k_spread_brick(int N, int K0, int K1, int K2, const int4* __restrict__ mbase, const R* __restrict__ wfrac,
+               const R* __restrict__ fq, const int* __restrict__ cstart, const int* __restrict__ catoms,
+               R* __restrict__ rgrid) {
+    constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
+    constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);   // derivative orders needed
+    __shared__ int col_lo[(BRX + 5) * (BRY + 5) * 2];
+    __shared__ int col_off[(BRX + 5) * (BRY + 5) * 2 + 1];
+    __shared__ int s_r[SP_CHUNK][3];
+    __shared__ R s_t[SP_CHUNK][3][NT][6];
+    __shared__ R s_q[SP_CHUNK][NQ];
+    const int f = blockIdx.y;
+    const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY;
+    const int b = blockIdx.x;
+    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+    const int X0 = bx * BRX, Y0 = by * BRY, Z0 = bz * BRZ;
+    const int t = threadIdx.x;
+    const int lx = t / (BRY * BRZ), ly = (t / BRZ) % BRY, lz = t % BRZ;
+    const int px = X0 + lx, py = Y0 + ly, pz = Z0 + lz;
+    const bool own = px < K0 && py < K1 && pz < K2;
+    const size_t G = (size_t)K0 * K1 * K2;
+    const int* cs = cstart + (size_t)f * (G + 1);
+    const int* ca = catoms + (size_t)f * N;
+    const int nx = min(BRX + 5, K0), ny = min(BRY + 5, K1), nzr = min(BRZ + 5, K2);
+    const int ncol = nx * ny;
+    const int zs = wrapi(Z0 - 2, K2);
+    const int zlen1 = min(nzr, K2 - zs), zlen2 = nzr - zlen1;
+    for (int c = t; c < ncol * 2; c += 64) {
+        const int cc = c >> 1, seg = c & 1;
+        const int x = wrapi(X0 - 2 + cc / ny, K0), y = wrapi(Y0 - 2 + cc % ny, K1);
+        const size_t base = ((size_t)x * K1 + y) * K2;
+        int lo = 0, n = 0;
+        if (seg == 0) {
+            lo = cs[base + zs];
+            n = cs[base + zs + zlen1] - lo;
+        } else if (zlen2 > 0) {
+            lo = cs[base];
+            n = cs[base + zlen2] - lo;
+        }
+        col_lo[c] = lo;
+        col_off[c + 1] = n;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        col_off[0] = 0;
+        for (int c = 1; c <= ncol * 2; ++c) { acc += col_off[c]; col_off[c] = acc; }
+    }
+    __syncthreads();
+    const int total = col_off[ncol * 2];
+    R acc = 0;
+    for (int c0 = 0; c0 < total; c0 += SP_CHUNK) {
+        const int nchunk = min(SP_CHUNK, total - c0);
+        if (t < nchunk) {
+            const int gidx = c0 + t;
+            int lo = 0, hi = ncol * 2;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (col_off[mid] <= gidx) lo = mid; else hi = mid;
+            }
+            const int atom = ca[col_lo[lo] + (gidx - col_off[lo])];
+            const size_t a = (size_t)f * N + atom;
+            const int4 m = mbase[a];
+            // support start (grid index of offset 0 is m-3) relative to the brick origin
+            s_r[t][0] = wrapi(m.x - 3 - X0 + 5, K0) - 5;
+            s_r[t][1] = wrapi(m.y - 3 - Y0 + 5, K1) - 5;
+            s_r[t][2] = wrapi(m.z - 3 - Z0 + 5, K2) - 5;
+            for (int d = 0; d < 3; ++d) {
+                const R w = wfrac[a * 3 + d];
+                for (int k = 0; k < 6; ++k) {
+                    R v, d1, d2;
+                    bs_piece<R>(k, w, v, d1, d2);
+                    s_t[t][d][0][k] = v;
+                    if (NT > 1) s_t[t][d][NT > 1 ? 1 : 0][k] = d1;
+                    if (NT > 2) s_t[t][d][NT > 2 ? 2 : 0][k] = d2;
+                }
+            }
+            for (int q = 0; q < NQ; ++q) s_q[t][q] = fq[a * 10 + q];
+        }
+        __syncthreads();
+        if (own) {
+            for (int s = 0; s < nchunk; ++s) {
+                const int k0 = lx - s_r[s][0], k1 = ly - s_r[s][1], k2 = lz - s_r[s][2];
+                if ((unsigned)k0 > 5u || (unsigned)k1 > 5u || (unsigned)k2 > 5u) continue;
+                const R v0 = s_t[s][0][0][k0], v1 = s_t[s][1][0][k1], v2 = s_t[s][2][0][k2];
+                R wgt = s_q[s][0] * v0 * v1 * v2;
+                if (LM >= 1) {
+                    const R a0 = s_t[s][0][NT > 1 ? 1 : 0][k0], a1 = s_t[s][1][NT > 1 ? 1 : 0][k1],
+                            a2 = s_t[s][2][NT > 1 ? 1 : 0][k2];
+                    wgt += s_q[s][NQ > 1 ? 1 : 0] * a0 * v1 * v2 + s_q[s][NQ > 2 ? 2 : 0] * v0 * a1 * v2 +
+                           s_q[s][NQ > 3 ? 3 : 0] * v0 * v1 * a2;
+                    if (LM >= 2) {
+                        const R b0 = s_t[s][0][NT > 2 ? 2 : 0][k0], b1 = s_t[s][1][NT > 2 ? 2 : 0][k1],
+                                b2 = s_t[s][2][NT > 2 ? 2 : 0][k2];
+                        wgt += s_q[s][NQ > 4 ? 4 : 0] * b0 * v1 * v2 + s_q[s][NQ > 5 ? 5 : 0] * v0 * b1 * v2 +
+                               s_q[s][NQ > 6 ? 6 : 0] * v0 * v1 * b2 +
+                               (R)2 * (s_q[s][NQ > 7 ? 7 : 0] * a0 * a1 * v2 + s_q[s][NQ > 8 ? 8 : 0] * a0 * v1 * a2 +
+                                       s_q[s][NQ > 9 ? 9 : 0] * v0 * a1 * a2);
+                    }
+                }
+                acc += wgt;
+            }
+        }
+        __syncthreads();
+    }
+    if (own) rgrid[(size_t)f * G + ((size_t)px * K1 + py) * K2 + pz] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// FFT passes.  Lines are staged in LDS as [element][line]; one Stockham stage per
+// radix with direct f-point butterflies (any factor).  Twiddles tw[t] = e^{-2 pi i t/K}.
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ inline Cplx<R>* stockham(Cplx<R>* in, Cplx<R>* out, int K, int LB, int nl, const FFTPlan& plan,
+                                    const Cplx<R>* __restrict__ tw, int inverse) {
+    int Ns = 1;
+    for (int s = 0; s < plan.nst; ++s) {
+        const int fr = plan.fac[s];
+        const int Kf = K / fr;
+        const int step = K / (Ns * fr);
+        for (int idx = threadIdx.x; idx < nl * K; idx += blockDim.x) {
+            const int li = idx % nl, tt = idx / nl;
+            const int j = tt / fr, k = tt % fr;
+            const int kj = j % Ns;
+            const int inc = ((kj + k * Ns) * step) % K;
+            int accI = 0;
+            R sx = 0, sy = 0;
+            for (int r = 0; r < fr; ++r) {
+                const Cplx<R> x = in[(j + r * Kf) * LB + li];
+                Cplx<R> w = tw[accI];
+                if (inverse) w.y = -w.y;
+                sx += x.x * w.x - x.y * w.y;
+                sy += x.x * w.y + x.y * w.x;
+                accI += inc;
+                if (accI >= K) accI -= K;
+            }
+            out[((j / Ns) * Ns * fr + kj + k * Ns) * LB + li] = Cplx<R>{sx, sy};
+        }
+        __syncthreads();
+        Cplx<R>* tmp = in; in = out; out = tmp;
+        Ns *= fr;
+    }
+    return in;
+}
+
+// real (K0,K1,K2) -> half spectrum (K0,K1,H), H = K2/2+1; LB complex lines (= 2 LB real lines)
+template <class R>
+__global__ void k_fft_z_r2c(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ tw,
+                            const R* __restrict__ rgrid, Cplx<R>* __restrict__ hgrid, int LB) {
+    extern __shared__ unsigned char smem_raw[];
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + (size_t)LB * K2;
+    const int f = blockIdx.y;
+    const int H = K2 / 2 + 1;
+    const int nrl = K0 * K1;                  // real lines
+    const int npairs = (nrl + 1) / 2;
+    const int p0 = blockIdx.x * LB;
+    const int nl = min(LB, npairs - p0);
+    if (nl <= 0) return;
+    const R* rg = rgrid + (size_t)f * K0 * K1 * K2;
+    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
+    for (int idx = threadIdx.x; idx < nl * K2; idx += blockDim.x) {
+        const int li = idx / K2, e = idx % K2;
+        const int la = 2 * (p0 + li), lb = la + 1;
+        const R a = rg[(size_t)la * K2 + e];
+        const R bb = lb < nrl ? rg[(size_t)lb * K2 + e] : (R)0;
+        b0[e * LB + li] = Cplx<R>{a, bb};
+    }
+    __syncthreads();
+    Cplx<R>* res = stockham<R>(b0, b1, K2, LB, nl, plan, tw, 0);
+    for (int idx = threadIdx.x; idx < nl * H; idx += blockDim.x) {
+        const int li = idx / H, k = idx % H;
+        const int la = 2 * (p0 + li), lb = la + 1;
+        const Cplx<R> Z = res[k * LB + li];
+        const Cplx<R> Zc = res[((K2 - k) % K2) * LB + li];
+        // A = (Z + conj Zc)/2 ; B = (Z - conj Zc)/(2i)
+        hg[(size_t)la * H + k] = Cplx<R>{(Z.x + Zc.x) * (R)0.5, (Z.y - Zc.y) * (R)0.5};
+        if (lb < nrl) hg[(size_t)lb * H + k] = Cplx<R>{(Z.y + Zc.y) * (R)0.5, (Zc.x - Z.x) * (R)0.5};
+    }
+}
+
+// half spectrum (K0,K1,H) -> real (K0,K1,K2), inverse, unnormalised
+template <class R>
+__global__ void k_fft_z_c2r(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ tw,
+                            const Cplx<R>* __restrict__ hgrid, R* __restrict__ rgrid, int LB) {
+    extern __shared__ unsigned char smem_raw[];
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + (size_t)LB * K2;
+    const int f = blockIdx.y;
+    const int H = K2 / 2 + 1;
+    const int nrl = K0 * K1;
+    const int npairs = (nrl + 1) / 2;
+    const int p0 = blockIdx.x * LB;
+    const int nl = min(LB, npairs - p0);
+    if (nl <= 0) return;
+    R* rg = rgrid + (size_t)f * K0 * K1 * K2;
+    const Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
+    for (int idx = threadIdx.x; idx < nl * K2; idx += blockDim.x) {
+        const int li = idx / K2, k = idx % K2;
+        const int la = 2 * (p0 + li), lb = la + 1;
+        Cplx<R> A, B;
+        if (k < H) {
+            A = hg[(size_t)la * H + k];
+            B = lb < nrl ? hg[(size_t)lb * H + k] : Cplx<R>{0, 0};
+        } else {
+            Cplx<R> a = hg[(size_t)la * H + (K2 - k)];
+            Cplx<R> bq = lb < nrl ? hg[(size_t)lb * H + (K2 - k)] : Cplx<R>{0, 0};
+            A = Cplx<R>{a.x, -a.y};
+            B = Cplx<R>{bq.x, -bq.y};
+        }
+        // Z = A + i B
+        b0[k * LB + li] = Cplx<R>{A.x - B.y, A.y + B.x};
+    }
+    __syncthreads();
+    Cplx<R>* res = stockham<R>(b0, b1, K2, LB, nl, plan, tw, 1);
+    for (int idx = threadIdx.x; idx < nl * K2; idx += blockDim.x) {
+        const int li = idx / K2, e = idx % K2;
+        const int la = 2 * (p0 + li), lb = la + 1;
+        const Cplx<R> z = res[e * LB + li];
+        rg[(size_t)la * K2 + e] = z.x;
+        if (lb < nrl) rg[(size_t)lb * K2 + e] = z.y;
+    }
+}
+
+// complex lines along y of the half spectrum: lines (x, kz), stride H
+template <class R>
+__global__ void k_fft_y(int K0, int K1, int H, FFTPlan plan, const Cplx<R>* __restrict__ tw,
+                        Cplx<R>* __restrict__ hgrid, int inverse, int LB) {
+    extern __shared__ unsigned char smem_raw[];
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + (size_t)LB * K1;
+    const int f = blockIdx.y;
+    const int nlines = K0 * H;
+    const int l0 = blockIdx.x * LB;
+    const int nl = min(LB, nlines - l0);
+    if (nl <= 0) return;
+    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
+    for (int idx = threadIdx.x; idx < nl * K1; idx += blockDim.x) {
+        const int e = idx / nl, li = idx % nl;
+        const int l = l0 + li;
+        const size_t base = (size_t)(l / H) * K1 * H + (l % H);
+        b0[e * LB + li] = hg[base + (size_t)e * H];
+    }
+    __syncthreads();
+    Cplx<R>* res = stockham<R>(b0, b1, K1, LB, nl, plan, tw, inverse);
+    for (int idx = threadIdx.x; idx < nl * K1; idx += blockDim.x) {
+        const int e = idx / nl, li = idx % nl;
+        const int l = l0 + li;
+        const size_t base = (size_t)(l / H) * K1 * H + (l % H);
+        hg[base + (size_t)e * H] = res[e * LB + li];
+    }
+}
+
+// forward x FFT, influence function, inverse x FFT; lines (y, kz), stride K1*H.
+// Energy/box sums: multiplicity 2 for 0 < kz < K2/2 (the conjugate half), else 1.
+template <class R>
+__global__ void k_fft_x_infl(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ tw,
+                             Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
+                             const double* __restrict__ bmod0, const double* __restrict__ bmod1,
+                             const double* __restrict__ bmod2, double kappa, int kind, double pref,
+                             double* __restrict__ part, int LB) {
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ double sh[16];
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + (size_t)LB * K0;
+    const int f = blockIdx.y;
+    const int H = K2 / 2 + 1;
+    const int nlines = K1 * H;
+    const int l0 = blockIdx.x * LB;
+    const int nl = min(LB, nlines - l0);
+    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
+    const size_t stride = (size_t)K1 * H;
+    double e = 0, w00 = 0, w11 = 0, w22 = 0, w01 = 0, w02 = 0, w12 = 0;
+    if (nl > 0) {
+        for (int idx = threadIdx.x; idx < nl * K0; idx += blockDim.x) {
+            const int el = idx / nl, li = idx % nl;
+            b0[el * LB + li] = hg[(size_t)(l0 + li) + (size_t)el * stride];
+        }
+        __syncthreads();
+        Cplx<R>* res = stockham<R>(b0, b1, K0, LB, nl, plan, tw, 0);
+        const BoxInfo& b = boxes[f];
+        for (int idx = threadIdx.x; idx < nl * K0; idx += blockDim.x) {
+            const int i0 = idx / nl, li = idx % nl;
+            const int l = l0 + li;
+            const int i1 = l / H, i2 = l % H;
+            const int m0 = i0 < (K0 + 1) / 2 ? i0 : i0 - K0;
+            const int m1 = i1 < (K1 + 1) / 2 ? i1 : i1 - K1;
+            const int m2 = i2 < (K2 + 1) / 2 ? i2 : i2 - K2;
+            const double kx = b.kb[0] * m0 + b.kb[1] * m1 + b.kb[2] * m2;
+            const double ky = b.kb[3] * m0 + b.kb[4] * m1 + b.kb[5] * m2;
+            const double kz = b.kb[6] * m0 + b.kb[7] * m1 + b.kb[8] * m2;
+            const double ksq = kx * kx + ky * ky + kz * kz;
+            const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
+            const double ith2 = 1.0 / (th * th);
+            const Cplx<R> S = res[i0 * LB + li];
+            const double s2 = (double)S.x * (double)S.x + (double)S.y * (double)S.y;
+            const double mult = (i2 == 0 || 2 * i2 == K2) ? 1.0 : 2.0;
+            double C = 0, dC = 0;
+            const double V = b.V;
+            if (kind == 0) {
+                if (!(i0 == 0 && i1 == 0 && i2 == 0)) {
+                    C = 2.0 * PI / V / ksq * exp(-ksq / (4.0 * kappa * kappa));
+                    dC = -C * (1.0 / ksq + 1.0 / (4.0 * kappa * kappa));
+                }
+            } else {
+                const double kq = ksq < 1e-16 ? 1e-16 : ksq;
+                const double x2 = kq / (4.0 * kappa * kappa), x = sqrt(x2);
+                const double ex = exp(-x2), ec = erfc(x);
+                double fv, fd, pw;
+                if (kind == 6) {
+                    fv = (1.0 - 2.0 * x2) * ex + 2.0 * x2 * x * SQRT_PI * ec;
+                    fd = -6.0 * x * ex + 6.0 * x2 * SQRT_PI * ec;
+                    pw = kappa * kappa * kappa / 3.0;
+                } else if (kind == 8) {
+                    fv = (3.0 - 2.0 * x2 + 4.0 * x2 * x2) * ex - 4.0 * x2 * x2 * x * SQRT_PI * ec;
+                    fd = (-10.0 * x + 20.0 * x2 * x) * ex - 20.0 * x2 * x2 * SQRT_PI * ec;
+                    pw = kappa * kappa * kappa * kappa * kappa / 45.0;
+                } else {
+                    const double x4 = x2 * x2;
+                    fv = (15.0 - 6.0 * x2 + 4.0 * x4 - 8.0 * x4 * x2) * ex + 8.0 * x4 * x2 * x * SQRT_PI * ec;
+                    fd = (-42.0 * x + 28.0 * x2 * x - 56.0 * x4 * x) * ex + 56.0 * x4 * x2 * SQRT_PI * ec;
+                    pw = pow(kappa, 7.0) / 1260.0;
+                }
+                const double c0 = SQRT_PI * PI / 2.0 / V * pw;
+                C = c0 * fv;
+                dC = ksq < 1e-16 ? 0.0 : c0 * fd / (8.0 * kappa * kappa * x);
+            }
+            const double wgt = pref * s2 * ith2;
+            e += mult * wgt * C;
+            const double wd = -mult * wgt * dC;
+            w00 += wd * kx * kx; w11 += wd * ky * ky; w22 += wd * kz * kz;
+            w01 += wd * kx * ky; w02 += wd * kx * kz; w12 += wd * ky * kz;
+            const double scale = 2.0 * pref * C * ith2;
+            res[i0 * LB + li] = Cplx<R>{(R)(S.x * scale), (R)(S.y * scale)};
+        }
+        __syncthreads();
+        Cplx<R>* other = (res == b0) ? b1 : b0;
+        Cplx<R>* res2 = stockham<R>(res, other, K0, LB, nl, plan, tw, 1);
+        for (int idx = threadIdx.x; idx < nl * K0; idx += blockDim.x) {
+            const int el = idx / nl, li = idx % nl;
+            hg[(size_t)(l0 + li) + (size_t)el * stride] = res2[el * LB + li];
+        }
+    }
+    double vals[7] = {e, w00, w11, w22, w01, w02, w12};
+    for (int k = 0; k < 7; ++k) {
+        double tsum = block_sum(vals[k], sh);
+        if (threadIdx.x == 0) part[((size_t)f * gridDim.x + blockIdx.x) * 7 + k] = tsum;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gather with eight lanes per site (lanes 0..5 own one x-offset each).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void __launch_bounds__(256)
+k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
+            int use_dA, const int4* __restrict__ mbase, const R* __restrict__ wfrac, const R* __restrict__ fq,
+            const double* __restrict__ Qc, const double* __restrict__ U, int qmode, const R* __restrict__ rgrid,
+            int mode, double* __restrict__ out3, double* __restrict__ g_q, double* __restrict__ g_pos,
+            double* __restrict__ vir_atom) {
+    const int f = blockIdx.y;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt >> 3, lane = gt & 7;
+    const bool active = i < N;
+    const int ii = active ? i : 0;
+    const size_t a = (size_t)f * N + ii;
+    const BoxInfo& b = boxes[f];
+    const double* A = use_dA ? b.dA : b.A;
+    const int4 m = mbase[a];
+    R M[3][4][6];
+    bspline6<R>(wfrac[a * 3 + 0], M[0][0], M[0][1], M[0][2], M[0][3]);
+    bspline6<R>(wfrac[a * 3 + 1], M[1][0], M[1][1], M[1][2], M[1][3]);
+    bspline6<R>(wfrac[a * 3 + 2], M[2][0], M[2][1], M[2][2], M[2][3]);
+    const R* gbase = rgrid + (size_t)f * K0 * K1 * K2;
+    const int nd = mode == 1 ? 2 : 4;  // derivative orders needed
+    double fp[4][4][4];
+    for (int c = 0; c < 4; ++c)
+        for (int aa = 0; aa < 4; ++aa)
+            for (int bb = 0; bb < 4; ++bb) fp[c][aa][bb] = 0.0;
+    if (active && lane < 6) {
+        const int k0 = lane;
+        const int g0 = wrapi(m.x + k0 - 3, K0);
+        int gz[6];
+        for (int k2 = 0; k2 < 6; ++k2) gz[k2] = wrapi(m.z + k2 - 3, K2);
+        R u[4][4];
+        for (int aa = 0; aa < 4; ++aa)
+            for (int bb = 0; bb < 4; ++bb) u[aa][bb] = 0;
+        for (int k1 = 0; k1 < 6; ++k1) {
+            const int g1 = wrapi(m.y + k1 - 3, K1);
+            const R* row = gbase + ((size_t)g0 * K1 + g1) * K2;
+            R s[4] = {0, 0, 0, 0};
+            for (int k2 = 0; k2 < 6; ++k2) {
+                const R ph = row[gz[k2]];
+                for (int o = 0; o < nd; ++o) s[o] += ph * M[2][o][k2];
+            }
+            for (int aa = 0; aa < nd; ++aa)
+                for (int bb = 0; aa + bb < nd; ++bb) u[aa][bb] += M[1][aa][k1] * s[bb];
+        }
+        for (int c = 0; c < nd; ++c)
+            for (int aa = 0; c + aa < nd; ++aa)
+                for (int bb = 0; c + aa + bb < nd; ++bb) fp[c][aa][bb] = (double)(M[0][c][k0] * u[aa][bb]);
+    }
+    for (int c = 0; c < nd; ++c)
+        for (int aa = 0; c + aa < nd; ++aa)
+            for (int bb = 0; c + aa + bb < nd; ++bb) fp[c][aa][bb] = group_sum(fp[c][aa][bb], 8);
+    if (!active || lane != 0) return;
+    double p1[3] = {fp[1][0][0], fp[0][1][0], fp[0][0][1]};
+    double dmu[3];
+    for (int c = 0; c < 3; ++c) dmu[c] = A[c * 3 + 0] * p1[0] + A[c * 3 + 1] * p1[1] + A[c * 3 + 2] * p1[2];
+    if (mode == 1) {
+        out3[a * 3 + 0] = dmu[0]; out3[a * 3 + 1] = dmu[1]; out3[a * 3 + 2] = dmu[2];
+        return;
+    }
+    double p2[3][3];
+    p2[0][0] = fp[2][0][0]; p2[1][1] = fp[0][2][0]; p2[2][2] = fp[0][0][2];
+    p2[0][1] = p2[1][0] = fp[1][1][0]; p2[0][2] = p2[2][0] = fp[1][0][1]; p2[1][2] = p2[2][1] = fp[0][1][1];
+    auto p3 = [&](int j, int k, int l) -> double {
+        int o[3] = {0, 0, 0};
+        o[j]++; o[k]++; o[l]++;
+        return fp[o[0]][o[1]][o[2]];
+    };
+    const R* c = fq + a * 10;
+    const double q = (double)c[0];
+    const double fm[3] = {(double)c[1], (double)c[2], (double)c[3]};
+    double fT[3][3];
+    fT[0][0] = c[4]; fT[1][1] = c[5]; fT[2][2] = c[6];
+    fT[0][1] = fT[1][0] = c[7]; fT[0][2] = fT[2][0] = c[8]; fT[1][2] = fT[2][1] = c[9];
+    double g[3];
+    for (int j = 0; j < 3; ++j) {
+        double s = q * p1[j];
+        for (int k = 0; k < 3; ++k) s += fm[k] * p2[k][j];
+        for (int k = 0; k < 3; ++k)
+            for (int l = 0; l < 3; ++l) s += fT[k][l] * p3(k, l, j);
+        g[j] = s;
+    }
+    double Ap2[3][3];
+    for (int x = 0; x < 3; ++x)
+        for (int k = 0; k < 3; ++k) Ap2[x][k] = A[x * 3 + 0] * p2[0][k] + A[x * 3 + 1] * p2[1][k] + A[x * 3 + 2] * p2[2][k];
+    double GT[3][3];
+    for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y)
+            GT[x][y] = (Ap2[x][0] * A[y * 3 + 0] + Ap2[x][1] * A[y * 3 + 1] + Ap2[x][2] * A[y * 3 + 2]) * (1.0 / 3.0);
+    double* gq = g_q + a * 10;
+    gq[0] = fp[0][0][0];
+    gq[1] = dmu[0]; gq[2] = dmu[1]; gq[3] = dmu[2];
+    gq[4] = GT[0][0]; gq[5] = GT[1][1]; gq[6] = GT[2][2]; gq[7] = GT[0][1]; gq[8] = GT[0][2]; gq[9] = GT[1][2];
+    const double4 pp = pos[a];
+    for (int cc = 0; cc < 3; ++cc) g_pos[a * 3 + cc] = A[0 * 3 + cc] * g[0] + A[1 * 3 + cc] * g[1] + A[2 * 3 + cc] * g[2];
+    double mu[3] = {0, 0, 0}, Th[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (qmode & 1) {
+        const double* qc = Qc + a * 10;
+        mu[0] = qc[1]; mu[1] = qc[2]; mu[2] = qc[3];
+        Th[0][0] = qc[4]; Th[1][1] = qc[5]; Th[2][2] = qc[6];
+        Th[0][1] = Th[1][0] = qc[7]; Th[0][2] = Th[2][0] = qc[8]; Th[1][2] = Th[2][1] = qc[9];
+    }
+    if (qmode & 2) { mu[0] += U[a * 3]; mu[1] += U[a * 3 + 1]; mu[2] += U[a * 3 + 2]; }
+    const double r[3] = {pp.x, pp.y, pp.z};
+    double* v = vir_atom + a * 9;
+    for (int j = 0; j < 3; ++j)
+        for (int cc = 0; cc < 3; ++cc) {
+            double TAp = 0;
+            for (int x = 0; x < 3; ++x) TAp += Th[j][x] * Ap2[x][cc];
+            v[j * 3 + cc] = g[j] * r[cc] + mu[j] * p1[cc] + (2.0 / 3.0) * TAp;
+        }
+}
+
+}  // namespace admp

@@ -113,6 +113,59 @@ __global__ void k_spline_prep(int N, const double4* __restrict__ pos, const BoxI
     o[7] = (R)TT[0][1]; o[8] = (R)TT[0][2]; o[9] = (R)TT[1][2];
 }
 
+// Per-site grid base m and offset w only (positions are fixed during a solve).
+__global__ void k_grid_base(int N, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes, int use_dA,
+                            int4* __restrict__ mbase, double* __restrict__ wd) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const BoxInfo& b = boxes[f];
+    const double* A = use_dA ? b.dA : b.A;
+    const size_t a = (size_t)f * N + i;
+    const double4 p = pos[a];
+    int mm[3];
+    for (int j = 0; j < 3; ++j) {
+        const double Rf = A[j * 3 + 0] * p.x + A[j * 3 + 1] * p.y + A[j * 3 + 2] * p.z;
+        mm[j] = (int)ceil(Rf);
+        wd[a * 3 + j] = (double)mm[j] - Rf;
+    }
+    mbase[a] = make_int4(mm[0], mm[1], mm[2], 0);
+}
+
+// Fractional multipoles from Cartesian sources (mode bit 0: Qc, bit 1: add U);
+// also writes the R-precision copy of w.
+template <class R>
+__global__ void k_frac_mult(int N, const BoxInfo* __restrict__ boxes, int use_dA, const double* __restrict__ Qc,
+                            const double* __restrict__ U, int mode, const double* __restrict__ wd,
+                            R* __restrict__ wfrac, R* __restrict__ fq) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const BoxInfo& b = boxes[f];
+    const double* A = use_dA ? b.dA : b.A;
+    const size_t a = (size_t)f * N + i;
+    for (int j = 0; j < 3; ++j) wfrac[a * 3 + j] = (R)wd[a * 3 + j];
+    double q = 0, mu[3] = {0, 0, 0}, Th[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if (mode & 1) {
+        const double* c = Qc + a * 10;
+        q = c[0]; mu[0] = c[1]; mu[1] = c[2]; mu[2] = c[3];
+        Th[0][0] = c[4]; Th[1][1] = c[5]; Th[2][2] = c[6];
+        Th[0][1] = Th[1][0] = c[7]; Th[0][2] = Th[2][0] = c[8]; Th[1][2] = Th[2][1] = c[9];
+    }
+    if (mode & 2) { mu[0] += U[a * 3 + 0]; mu[1] += U[a * 3 + 1]; mu[2] += U[a * 3 + 2]; }
+    double fm[3], TA[3][3];
+    for (int j = 0; j < 3; ++j) fm[j] = A[0 * 3 + j] * mu[0] + A[1 * 3 + j] * mu[1] + A[2 * 3 + j] * mu[2];
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 3; ++k) TA[c][k] = Th[c][0] * A[0 * 3 + k] + Th[c][1] * A[1 * 3 + k] + Th[c][2] * A[2 * 3 + k];
+    R* o = fq + a * 10;
+    o[0] = (R)q; o[1] = (R)fm[0]; o[2] = (R)fm[1]; o[3] = (R)fm[2];
+    const int pr[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    for (int k = 0; k < 6; ++k) {
+        const int j = pr[k][0], l = pr[k][1];
+        o[4 + k] = (R)((A[0 * 3 + j] * TA[0][l] + A[1 * 3 + j] * TA[1][l] + A[2 * 3 + j] * TA[2][l]) * (1.0 / 3.0));
+    }
+}
+
 // Scatter spread with device atomics: one thread per (site, k0, k1), six points
 // along the contiguous axis.  grid is the real part of a complex array.
 template <class R>

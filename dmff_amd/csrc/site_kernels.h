@@ -607,4 +607,66 @@ __global__ void k_vadd(size_t n, const double* __restrict__ in, double* __restri
     if (i < n) out[i] += in[i];
 }
 
+// ----------------------------------------------------------------- device-resident PCG
+struct PcgState {
+    double rz, alpha, beta, res;
+    int done, iter, pad0, pad1;
+};
+
+// after dots(r, z): rz and max|r|; frames already below tol are done
+__global__ void k_pcg_init(int F, const double* __restrict__ dots, double tol, PcgState* __restrict__ st) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    PcgState s;
+    s.rz = dots[f * 3 + 0];
+    s.res = dots[f * 3 + 2];
+    s.alpha = s.beta = 0;
+    s.done = s.res < tol ? 1 : 0;
+    s.iter = 0;
+    s.pad0 = s.pad1 = 0;
+    st[f] = s;
+}
+
+// dots = (p.Ap, -, -): alpha = rz / p.Ap ; x += alpha p ; r -= alpha Ap ; z = Minv r
+__global__ void k_pcg_xr(int N, PcgState* __restrict__ st, const double* __restrict__ dots,
+                         const double* __restrict__ pol, double* __restrict__ x, double* __restrict__ r,
+                         double* __restrict__ z, const double* __restrict__ p, const double* __restrict__ Ap) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || st[f].done) return;
+    const double pAp = dots[f * 3 + 0];
+    const double al = pAp != 0.0 ? st[f].rz / pAp : 0.0;
+    const size_t a = (size_t)f * N + i;
+    const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
+    for (int k = 0; k < 3; ++k) {
+        x[a * 3 + k] += al * p[a * 3 + k];
+        const double rv = r[a * 3 + k] - al * Ap[a * 3 + k];
+        r[a * 3 + k] = rv;
+        z[a * 3 + k] = ap / DIELECTRIC * rv;
+    }
+}
+
+// dots = (r.z, -, max|r|): beta = rz_new / rz ; done when max|r| < tol
+__global__ void k_pcg_beta(int F, const double* __restrict__ dots, double tol, PcgState* __restrict__ st) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F || st[f].done) return;
+    const double rzn = dots[f * 3 + 0];
+    PcgState s = st[f];
+    s.beta = s.rz != 0.0 ? rzn / s.rz : 0.0;
+    s.rz = rzn;
+    s.iter += 1;
+    s.res = dots[f * 3 + 2];
+    if (s.res < tol) s.done = 1;
+    st[f] = s;
+}
+
+__global__ void k_pcg_p(int N, const PcgState* __restrict__ st, const double* __restrict__ z, double* __restrict__ p) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || st[f].done) return;
+    const size_t a = (size_t)f * N + i;
+    const double be = st[f].beta;
+    for (int k = 0; k < 3; ++k) p[a * 3 + k] = z[a * 3 + k] + be * p[a * 3 + k];
+}
+
 }  // namespace admp
