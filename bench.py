@@ -48,6 +48,9 @@ def setup(cfg_id):
     t = lambda x, dt=torch.float64: torch.as_tensor(np.asarray(x), dtype=dt, device=dev)
     D = dict(S=S, rc=rc, kappa=kappa, K=(K1, K2, K3), pos=t(S.positions_nm * 10.0), box=t(box),
              Q=t(S.Q_local), pol=t(S.pol), thole=t(S.thole), map=t(S.map_atomtype, torch.int64), dev=dev)
+    from dmff_amd.admp.pme import TypeReducer
+    D["qtype"] = TypeReducer(S.map_atomtype, device=dev)
+    D["ptype"] = TypeReducer(S.map_poltype, device=dev)
     return D
 
 
@@ -65,10 +68,9 @@ def evaluate(eng, D, tol=1e-6):
     eng.build_neighbors(D["pos"], D["box"])
     E, g, U, info = eng.compute(D["pos"], D["box"], D["Q"], D["pol"], D["thole"], S.mScales, S.pScales, S.dScales,
                                 solver="pcg", tol=tol, maxiter=500, want_grad=True)
-    nt = int(D["map"].max()) + 1
-    dQ = torch.zeros((nt, g["Q"].shape[-1]), dtype=torch.float64, device=D["dev"]).index_add_(0, D["map"], g["Q"][0])
-    dpol = torch.zeros(nt, dtype=torch.float64, device=D["dev"]).index_add_(0, D["map"], g["pol"][0])
-    dth = torch.zeros(nt, dtype=torch.float64, device=D["dev"]).index_add_(0, D["map"], g["thole"][0])
+    dQ = D["qtype"](g["Q"][0])
+    dpol = D["ptype"](g["pol"][0])
+    dth = D["ptype"](g["thole"][0])
     return E, g, U, info, dQ, dpol, dth
 
 

@@ -46,6 +46,26 @@ def setup_ewald_parameters(rc, ethresh, box=None, spacing=None, method="openmm")
     raise ValueError(f"Invalid method: {method}.Valid methods: 'openmm', 'gromacs'")
 
 
+class TypeReducer:
+    """Sums per-atom gradients into per-type gradients, the adjoint of the
+    ``Q_local[map_atomtype]`` expansion (dmff/generators/admp.py:1372-1375).
+
+    A dense one-hot (n_types, N) float64 matrix is built once; the reduction is
+    one GEMM, deterministic and free of atomics (per-type targets are few, so
+    scatter-adds would serialise on them).
+    """
+
+    def __init__(self, type_map, n_types=None, device=None):
+        m = torch.as_tensor(np.asarray(type_map), dtype=torch.int64, device=device)
+        self.n_types = int(m.max()) + 1 if n_types is None else int(n_types)
+        self.W = torch.nn.functional.one_hot(m, self.n_types).to(torch.float64).T.contiguous()
+
+    def __call__(self, per_atom):
+        g = per_atom.reshape(per_atom.shape[0], -1) if per_atom.dim() > 1 else per_atom.reshape(-1, 1)
+        out = self.W @ g
+        return out.reshape((self.n_types,) + tuple(per_atom.shape[1:]))
+
+
 class _ADMPEnergy(torch.autograd.Function):
     """E(pos, box, Q_local, pol, thole) with the gradients computed by the library
     in the same call (value-and-grad), returned on backward."""

@@ -65,6 +65,8 @@ struct admp_ctx {
     void* d_hgrid = nullptr;    // half spectrum, F * max(K0 K1 (K2/2+1)) complex
     double* d_wd = nullptr;     // per-site grid offsets w (double)
     int *d_gcell = nullptr, *d_gccount = nullptr, *d_gcstart = nullptr, *d_gcfill = nullptr, *d_gcatoms = nullptr;
+    int* d_scan_tmp = nullptr;  // tile totals of scan_rows
+    size_t scan_cap = 0;
     const Grid* prepped = nullptr;  // grid whose site bases / sort are current
     double* d_part = nullptr;   // influence partials
     int part_blocks = 0;
@@ -83,6 +85,9 @@ struct admp_ctx {
     PcgState* d_pcg = nullptr;
     int* d_active = nullptr;
     int dots_blocks = 64;
+    double* d_pparts = nullptr;               // fused PCG partials, F x PCG_NPART x nb
+    double *d_swfrac = nullptr, *d_sfq = nullptr;  // solver spline coordinates / fractional dipoles (R-typed)
+    int pcg_nb = 0;
     std::vector<void*> allocs;
     // captured PCG iteration chunks (replayed until the frames converge)
     struct ChunkGraph {
@@ -400,6 +405,10 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_Ap, NF * 3); ALLOC(ctx->d_t1, NF * 3); ALLOC(ctx->d_t2, NF * 3); ALLOC(ctx->d_U, NF * 3);
     ALLOC(ctx->d_dpart, (size_t)ctx->dots_blocks * ctx->F * 3);
     ALLOC(ctx->d_dots, ctx->F * 3);
+    ctx->pcg_nb = (int)nblk(ctx->N, PCG_BLOCK);
+    ALLOC(ctx->d_pparts, (size_t)ctx->F * PCG_NPART * ctx->pcg_nb);
+    ALLOC(ctx->d_swfrac, NF * 3);
+    ALLOC(ctx->d_sfq, NF * 10);
     ALLOC(ctx->d_res, NF * 3);
     ALLOC(ctx->d_dU, NF * 3);
     ALLOC(ctx->d_pcg, ctx->F);
@@ -488,6 +497,22 @@ extern "C" admp_status admp_set_topology(admp_ctx* ctx, const int32_t* axis_type
     if (!rent.empty()) HIPCHK(hipMemcpyAsync(ctx->d_rev_ent, rent.data(), rent.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));  // host vectors are released on return
     ctx->has_topo = true;
+    return ADMP_OK;
+}
+
+// exclusive scan of F rows of n ints into out (n + 1 per row) on stream st
+static admp_status scan_rows(admp_ctx* ctx, const int* in, int* out, int n, int F, hipStream_t st) {
+    const int ntile = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const size_t need = (size_t)ntile * F;
+    if (need > ctx->scan_cap) {
+        dfree(ctx, ctx->d_scan_tmp);
+        HIPCHK(dalloc(ctx, &ctx->d_scan_tmp, need));
+        ctx->scan_cap = need;
+    }
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile, F), dim3(256), 0, st, n, in, out, ctx->d_scan_tmp, ntile);
+    hipLaunchKernelGGL(k_scan_top, dim3(F), dim3(256), 0, st, n, ntile, ctx->d_scan_tmp, out);
+    if (ntile > 1) hipLaunchKernelGGL(k_scan_add, dim3(ntile, F), dim3(256), 0, st, n, ctx->d_scan_tmp, out, ntile);
+    HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
 
@@ -612,10 +637,10 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
     HIPCHK(hipMemsetAsync(ctx->d_cfill, 0, (size_t)cg.ncell * F * sizeof(int), ctx->stream));
     hipLaunchKernelGGL(k_cell_assign, dim3(nblk(N, 256), F), dim3(256), 0, ctx->stream, N, ctx->d_pos, ctx->d_box, cg,
                        ctx->d_cell, ctx->d_ccount);
-    hipLaunchKernelGGL(k_scan, dim3(F), dim3(1024), 0, ctx->stream, cg.ncell, ctx->d_ccount, ctx->d_cstart);
+    if ((s = scan_rows(ctx, ctx->d_ccount, ctx->d_cstart, cg.ncell, F, ctx->stream)) != ADMP_OK) return s;
     hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256), F), dim3(256), 0, ctx->stream, N, cg.ncell, ctx->d_cell,
                        ctx->d_cstart, ctx->d_cfill, ctx->d_catoms);
-    hipLaunchKernelGGL(k_cell_sort, dim3(nblk(cg.ncell, 256), F), dim3(256), 0, ctx->stream, cg.ncell, N, ctx->d_cstart,
+    hipLaunchKernelGGL(k_cell_sort_block, dim3(cg.ncell, F), dim3(256), 0, ctx->stream, cg.ncell, N, ctx->d_cstart,
                        ctx->d_catoms);
     HIPCHK(hipGetLastError());
     if (!ctx->d_nbr) {
@@ -654,7 +679,10 @@ static admp_status admp_get_pairs_impl(admp_ctx* ctx, int32_t frame, int32_t* ou
     // nhalf holds the j > i counts for lists built on the GPU
     int* off;
     HIPCHK(dalloc(ctx, &off, N + 1));
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, N, ctx->d_nhalf + (size_t)frame * N, off);
+    {
+        admp_status s2 = scan_rows(ctx, ctx->d_nhalf + (size_t)frame * N, off, N, 1, ctx->stream);
+        if (s2 != ADMP_OK) { dfree(ctx, off); return s2; }
+    }
     int total = 0;
     HIPCHK(hipMemcpyAsync(&total, off + N, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -687,7 +715,8 @@ static admp_status recip_positions(admp_ctx* ctx, const Grid& g, int use_dA) {
         HIPCHK(hipMemsetAsync(ctx->d_gcfill, 0, G * F * sizeof(int), st));
         hipLaunchKernelGGL(k_grid_cell, dim3(nblk(N, 256), F), dim3(256), 0, st, N, g.K[0], g.K[1], g.K[2],
                            ctx->d_mbase, ctx->d_gcell, ctx->d_gccount);
-        hipLaunchKernelGGL(k_scan, dim3(F), dim3(1024), 0, st, (int)G, ctx->d_gccount, ctx->d_gcstart);
+        admp_status s2 = scan_rows(ctx, ctx->d_gccount, ctx->d_gcstart, (int)G, F, st);
+        if (s2 != ADMP_OK) return s2;
         hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256), F), dim3(256), 0, st, N, (int)G, ctx->d_gcell,
                            ctx->d_gcstart, ctx->d_gcfill, ctx->d_gcatoms);
         hipLaunchKernelGGL(k_cell_sort, dim3(nblk(G, 256), F), dim3(256), 0, st, (int)G, N, ctx->d_gcstart,
@@ -709,18 +738,21 @@ template <class R> static const Cplx<R>* twid(const Grid& g, int d) {
 template <class R>
 static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const double* Qc, const double* U, int qmode,
                              int lmax_eff, int kind, double pref, double kappa, int gather_mode, double* out3,
-                             double* gq, double* gpos, double* vir, bool want_energy) {
+                             double* gq, double* gpos, double* vir, bool want_energy, void* wf_in = nullptr,
+                             void* fq_in = nullptr) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     if (ctx->prepped != &g) {
         admp_status s = recip_positions(ctx, g, use_dA);
         if (s != ADMP_OK) return s;
     }
-    R* wfrac = reinterpret_cast<R*>(ctx->d_wfrac);
-    R* fq = reinterpret_cast<R*>(ctx->d_fq);
-    { ProfScope ps_(ctx, 3);
-    hipLaunchKernelGGL(k_frac_mult<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_box, use_dA, Qc, U, qmode,
-                       ctx->d_wd, wfrac, fq); }
+    R* wfrac = reinterpret_cast<R*>(fq_in ? wf_in : ctx->d_wfrac);
+    R* fq = reinterpret_cast<R*>(fq_in ? fq_in : ctx->d_fq);
+    if (!fq_in) {
+        ProfScope ps_(ctx, 3);
+        hipLaunchKernelGGL(k_frac_mult<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_box, use_dA, Qc, U, qmode,
+                           ctx->d_wd, wfrac, fq);
+    }
     const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
     if (small_grid(g)) {
         // grids below 6 points per axis: scatter with atomics on the complex C2C path
@@ -793,9 +825,14 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     hipLaunchKernelGGL(k_fft_z_c2r<R>, dim3(nblk(npairs, LB), F), dim3(TH), 2 * LB * K2 * sizeof(Cplx<R>), st, K0, K1,
                        K2, g.plan[2], twid<R>(g, 2), hg, rg, LB); }
     { ProfScope ps_(ctx, 7);
-    hipLaunchKernelGGL(k_gather_ml<R>, dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2, ctx->d_pos,
-                       ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, gather_mode, out3, gq, gpos,
-                       vir); }
+    if (gather_mode == 1)
+        hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2,
+                           ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
+                           vir);
+    else
+        hipLaunchKernelGGL((k_gather_ml<R, 0>), dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2,
+                           ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
+                           vir); }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
@@ -856,91 +893,164 @@ struct SolveInfo {
     std::vector<double> res;
 };
 
-// Preconditioned CG on A x = rhs with all scalars on the device; the host looks
-// at the per-frame done flags once per chunk of iterations.  x holds the initial
-// guess on entry.  Returns iterations per frame in it_out and the final
-// recursively updated residual max-norm in res_out.
+// One product q = A p of the fused PCG.  p (and its R copy, fractional dipoles
+// and spline coordinates) were written by k_pcg_dir; p.q partials go to parts[0].
+template <class RC, class R>
+static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
+    const int N = ctx->N, F = ctx->F;
+    hipStream_t st = ctx->stream;
+    const R* vR = sizeof(R) == 8 ? reinterpret_cast<const R*>(ctx->d_p) : reinterpret_cast<const R*>(ctx->d_vR);
+    { ProfScope ps_(ctx, 2);
+    hipLaunchKernelGGL((k_real_matvec<RC, R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
+                       ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
+                       reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1); }
+    double* rec = nullptr;
+    if (ctx->cfg.lpme) {
+        admp_status s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_p, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1,
+                                     ctx->d_t2, nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq);
+        if (s != ADMP_OK) return s;
+        rec = ctx->d_t2;
+    }
+    { ProfScope ps_(ctx, 11);
+    hipLaunchKernelGGL(k_pcg_q, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, f1, pol, ctx->d_p,
+                       ctx->d_t1, rec, ctx->d_Ap, ctx->d_pparts); }
+    HIPCHK(hipGetLastError());
+    return ADMP_OK;
+}
+
+// true residual r = rhs - A x with FP64 products (RC-typed stored pair
+// coefficients), z = M^-1 r, partials into buffer 0; mode 1 starts a solve
+// (p = z), mode 2 only refreshes the residual and convergence state.
+template <class RC, class R>
+static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x, const double* pol, double f1,
+                               double tol, int mode) {
+    const int N = ctx->N, F = ctx->F;
+    hipStream_t st = ctx->stream;
+    admp_status s;
+    if ((s = matvec<RC, double>(ctx, x, ctx->d_res, pol, f1)) != ADMP_OK) return s;
+    ProfScope ps_(ctx, 11);
+    hipLaunchKernelGGL(k_pcg_resid, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, 0, pol, rhs,
+                       ctx->d_res, ctx->d_r, ctx->d_z, ctx->d_pparts);
+    hipLaunchKernelGGL(k_pcg_dir<R>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, 0, mode, tol,
+                       ctx->d_box, ctx->cfg.lpme ? ctx->d_wd : nullptr, ctx->d_pparts, ctx->d_z, ctx->d_p,
+                       sizeof(R) == 8 ? nullptr : reinterpret_cast<R*>(ctx->d_vR), reinterpret_cast<R*>(ctx->d_swfrac),
+                       ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr, ctx->d_pcg);
+    HIPCHK(hipGetLastError());
+    return ADMP_OK;
+}
+
+// Preconditioned CG on A x = rhs (Jacobi preconditioner pol/DIELECTRIC) with
+// all scalars on the device.  Each iteration is pcg_apply + k_pcg_step +
+// k_pcg_dir; chunks of iterations are replayed from captured HIP graphs and the
+// host reads the per-frame state once per chunk.  With R = float the products
+// inside the iteration are FP32 and the recursively updated residual is
+// replaced by the true FP64 residual whenever it has dropped by 1e-2 since the
+// last replacement and before convergence is accepted ("reliable updates",
+// Sleijpen & van der Vorst 1996); the search direction is kept.  x holds the
+// initial guess on entry.
 template <class RC, class R>
 static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double* pol, double f1, double tol,
                        int maxiter, std::vector<int>& it_out, std::vector<double>& res_out) {
     const int N = ctx->N, F = ctx->F;
-    const size_t NF = (size_t)N * F;
     hipStream_t st = ctx->stream;
+    const bool reliable = sizeof(R) == 4;
     admp_status s;
-    if ((s = matvec<RC, R>(ctx, x, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
-    hipLaunchKernelGGL(k_lincomb, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, 1.0, rhs, -1.0, ctx->d_Ap, ctx->d_r);
-    hipLaunchKernelGGL(k_precond, dim3(nblk(N, 256), F), dim3(256), 0, st, N, pol, ctx->d_r, ctx->d_z, ctx->d_p);
-    dots_dev(ctx, ctx->d_r, ctx->d_z, nullptr, pol);
-    hipLaunchKernelGGL(k_pcg_init, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_dots, tol, ctx->d_pcg);
+    if ((s = pcg_restart<RC, R>(ctx, rhs, x, pol, f1, tol, 1)) != ADMP_OK) return s;
     const int chunk = 4;
     std::vector<PcgState> hs(F);
     int done_iters = 0;
-    for (;;) {
+    auto read_state = [&](double& worst, bool& all) -> admp_status {
         HIPCHK(hipMemcpyAsync(hs.data(), ctx->d_pcg, F * sizeof(PcgState), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        bool all = true;
-        for (int f = 0; f < F; ++f) all = all && hs[f].done;
+        all = true;
+        worst = 0;
+        for (int f = 0; f < F; ++f) {
+            all = all && hs[f].done;
+            worst = std::max(worst, hs[f].res);
+        }
+        return ADMP_OK;
+    };
+    double worst = 0;
+    bool all = false;
+    if ((s = read_state(worst, all)) != ADMP_OK) return s;
+    double last_true = worst;
+    for (;;) {
+        if (reliable && done_iters > 0 && (all || worst < 1e-2 * last_true)) {
+            if ((s = pcg_restart<RC, R>(ctx, rhs, x, pol, f1, tol, 2)) != ADMP_OK) return s;
+            if ((s = read_state(worst, all)) != ADMP_OK) return s;
+            last_true = worst;
+        }
         if (all || done_iters >= maxiter) break;
         const int nrun = std::min(chunk, maxiter - done_iters);
         auto iterate = [&]() -> admp_status {
             for (int c = 0; c < nrun; ++c) {
+                const int cur = c & 1;
                 admp_status s2;
-                if ((s2 = matvec<RC, R>(ctx, ctx->d_p, ctx->d_Ap, pol, f1)) != ADMP_OK) return s2;
-                dots_dev(ctx, ctx->d_p, ctx->d_Ap, nullptr, pol);
-                { ProfScope ps_(ctx, 11);
-                hipLaunchKernelGGL(k_pcg_xr, dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->d_pcg, ctx->d_dots, pol, x,
-                                   ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap); }
-                dots_dev(ctx, ctx->d_r, ctx->d_z, nullptr, pol);
-                { ProfScope ps_(ctx, 11);
-                hipLaunchKernelGGL(k_pcg_beta, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_dots, tol, ctx->d_pcg);
-                hipLaunchKernelGGL(k_pcg_p, dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->d_pcg, ctx->d_z, ctx->d_p); }
+                if ((s2 = pcg_apply<RC, R>(ctx, pol, f1)) != ADMP_OK) return s2;
+                ProfScope ps_(ctx, 11);
+                hipLaunchKernelGGL(k_pcg_step, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, cur, tol,
+                                   pol, x, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap, ctx->d_pparts);
+                hipLaunchKernelGGL(k_pcg_dir<R>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb,
+                                   cur ^ 1, 0, tol, ctx->d_box, ctx->cfg.lpme ? ctx->d_wd : nullptr, ctx->d_pparts,
+                                   ctx->d_z, ctx->d_p,
+                                   sizeof(R) == 8 ? nullptr : reinterpret_cast<R*>(ctx->d_vR),
+                                   reinterpret_cast<R*>(ctx->d_swfrac),
+                                   ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr, ctx->d_pcg);
             }
+            HIPCHK(hipGetLastError());
             return ADMP_OK;
         };
         if (!ctx->use_graphs) {
             if ((s = iterate()) != ADMP_OK) return s;
-            done_iters += nrun;
-            continue;
-        }
-        // a chunk of iterations is captured once as a HIP graph and replayed
-        const int type = (sizeof(RC) == 8 ? 1 : 0) + (sizeof(R) == 8 ? 2 : 0);
-        admp_ctx::ChunkGraph* cg = nullptr;
-        for (auto& g : ctx->graphs)
-            if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1)
-                cg = &g;
-        if (!cg) {
-            admp_ctx::ChunkGraph* slot = nullptr;
+        } else {
+            // a chunk of iterations is captured once as a HIP graph and replayed
+            const int type = (sizeof(RC) == 8 ? 1 : 0) + (sizeof(R) == 8 ? 2 : 0);
+            admp_ctx::ChunkGraph* cg = nullptr;
             for (auto& g : ctx->graphs)
-                if (!g.exec) { slot = &g; break; }
-            if (!slot) { clear_graphs(ctx); slot = &ctx->graphs[0]; }
-            const bool prof = ctx->prof;
-            ctx->prof = false;
-            hipGraph_t graph = nullptr;
-            hipError_t be = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-            if (be != hipSuccess) {
+                if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol &&
+                    g.f1 == f1)
+                    cg = &g;
+            if (!cg) {
+                admp_ctx::ChunkGraph* slot = nullptr;
+                for (auto& g : ctx->graphs)
+                    if (!g.exec) { slot = &g; break; }
+                if (!slot) { clear_graphs(ctx); slot = &ctx->graphs[0]; }
+                const bool prof = ctx->prof;
+                ctx->prof = false;
+                hipGraph_t graph = nullptr;
+                hipError_t be = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+                if (be != hipSuccess) {
+                    ctx->prof = prof;
+                    ctx->err = std::string("graph capture begin: ") + hipGetErrorString(be);
+                    return ADMP_ERR_HIP;
+                }
+                admp_status cs = iterate();
+                hipError_t ce = hipStreamEndCapture(st, &graph);
                 ctx->prof = prof;
-                ctx->err = std::string("graph capture begin: ") + hipGetErrorString(be);
-                return ADMP_ERR_HIP;
+                if (cs != ADMP_OK) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return cs;
+                }
+                if (ce != hipSuccess) {
+                    ctx->err = std::string("graph capture: ") + hipGetErrorString(ce);
+                    return ADMP_ERR_HIP;
+                }
+                hipGraphExec_t exec = nullptr;
+                hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) {
+                    ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ie);
+                    return ADMP_ERR_HIP;
+                }
+                slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
+                slot->tol = tol; slot->f1 = f1;
+                cg = slot;
             }
-            admp_status cs = iterate();
-            hipError_t ce = hipStreamEndCapture(st, &graph);
-            ctx->prof = prof;
-            if (cs != ADMP_OK) {
-                if (graph) (void)hipGraphDestroy(graph);
-                return cs;
-            }
-            if (ce != hipSuccess) { ctx->err = std::string("graph capture: ") + hipGetErrorString(ce); return ADMP_ERR_HIP; }
-            hipGraphExec_t exec = nullptr;
-            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) { ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ie); return ADMP_ERR_HIP; }
-            slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
-            slot->tol = tol; slot->f1 = f1;
-            cg = slot;
+            ProfScope ps_(ctx, 12);
+            HIPCHK(hipGraphLaunch(cg->exec, st));
         }
-        { ProfScope ps_(ctx, 12);
-        HIPCHK(hipGraphLaunch(cg->exec, st)); }
         done_iters += nrun;
+        if ((s = read_state(worst, all)) != ADMP_OK) return s;
     }
     it_out.resize(F);
     res_out.resize(F);
@@ -1013,25 +1123,11 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
         if ((s = pcg<double, double>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr)) != ADMP_OK) return s;
         for (int f = 0; f < F; ++f) info.n_iter[f] = it[f];
     } else if (kind == ADMP_SOLVER_PCG) {
-        // iterative refinement: FP64 residual r = b - A U (pair coefficients stored in
-        // FP32, products and reciprocal part in FP64), FP32 PCG for the correction
+        // FP32 products inside the iteration, FP64 true residuals (reliable updates)
         std::vector<int> it;
         std::vector<double> rr;
-        for (int outer = 0; outer < 8; ++outer) {
-            if ((s = matvec<R, double>(ctx, U, ctx->d_res, pol, f1)) != ADMP_OK) return s;
-            hipLaunchKernelGGL(k_lincomb, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, 1.0, ctx->d_b, -1.0,
-                               ctx->d_res, ctx->d_res);
-            if ((s = dots(ctx, ctx->d_res, nullptr, nullptr, pol, h)) != ADMP_OK) return s;
-            double worst = 0;
-            for (int f = 0; f < F; ++f) worst = std::max(worst, h[3 * f + 2]);
-            if (worst < tol) break;
-            // correction: A dU = r, FP32 products, relative reduction 1e-4 (not below tol/2)
-            HIPCHK(hipMemsetAsync(ctx->d_dU, 0, NF * 3 * sizeof(double), st));
-            const double itol = std::max(0.5 * tol, 1e-4 * worst);
-            if ((s = pcg<R, R>(ctx, ctx->d_res, ctx->d_dU, pol, f1, itol, maxiter, it, rr)) != ADMP_OK) return s;
-            hipLaunchKernelGGL(k_vadd, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_dU, U);
-            for (int f = 0; f < F; ++f) info.n_iter[f] += it[f];
-        }
+        if ((s = pcg<R, R>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr)) != ADMP_OK) return s;
+        for (int f = 0; f < F; ++f) info.n_iter[f] = it[f];
     } else {
         // Jacobi sweeps (pme.py:380-422)
         HIPCHK(hipMemcpyAsync(ctx->d_active, act.data(), F * sizeof(int), hipMemcpyHostToDevice, st));

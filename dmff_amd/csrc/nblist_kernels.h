@@ -61,27 +61,84 @@ __global__ void k_cell_assign(int N, const double4* __restrict__ pos, const BoxI
 }
 
 // exclusive scan of ncell counts per frame (single block per frame)
-__global__ void k_scan(int n, const int* __restrict__ in, int* __restrict__ out) {
-    __shared__ int sh[1024];
-    const int f = blockIdx.x;
+// ---------------------------------------------------------------------------
+// Exclusive scan of F rows of n ints: out row f has n + 1 entries, the last one
+// the row total.  Three launches: 1024-element tiles scanned in registers and
+// LDS (k_scan_tiles), the tile totals scanned per row (k_scan_top), tile offsets
+// added back (k_scan_add).
+// ---------------------------------------------------------------------------
+constexpr int SCAN_TILE = 1024;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// block of 256 threads; returns the exclusive prefix of v within the block, total in *tot
+__device__ __forceinline__ int block_excl_scan256(int v, int* ws, int* tot) {
+    const int incl = wave_incl_scan(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    int woff = 0;
+    for (int k = 0; k < w; ++k) woff += ws[k];
+    *tot = ws[0] + ws[1] + ws[2] + ws[3];
+    __syncthreads();
+    return woff + incl - v;
+}
+
+__global__ void __launch_bounds__(256) k_scan_tiles(int n, const int* __restrict__ in, int* __restrict__ out,
+                                                    int* __restrict__ tsum, int ntile) {
+    __shared__ int ws[4];
+    const int f = blockIdx.y, t = blockIdx.x;
     const int* a = in + (size_t)f * n;
     int* o = out + (size_t)f * (n + 1);
-    // chunked: each thread owns a contiguous chunk
-    const int T = blockDim.x;
-    const int chunk = (n + T - 1) / T;
-    const int lo = threadIdx.x * chunk, hi = min(n, lo + chunk);
-    int s = 0;
-    for (int k = lo; k < hi; ++k) s += a[k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < T; ++k) { int t = sh[k]; sh[k] = acc; acc += t; }
-        o[n] = acc;
+    const int base = t * SCAN_TILE + threadIdx.x * 4;
+    int v[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? a[base + k] : 0;
+        s += v[k];
     }
-    __syncthreads();
-    int acc = sh[threadIdx.x];
-    for (int k = lo; k < hi; ++k) { o[k] = acc; acc += a[k]; }
+    int tot;
+    int ex = block_excl_scan256(s, ws, &tot);
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) o[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) tsum[(size_t)f * ntile + t] = tot;
+}
+
+// one block of 256 per row: exclusive scan of the tile totals in place, row total to out[n]
+__global__ void __launch_bounds__(256) k_scan_top(int n, int ntile, int* __restrict__ tsum, int* __restrict__ out) {
+    __shared__ int ws[4];
+    const int f = blockIdx.x;
+    int* ts = tsum + (size_t)f * ntile;
+    int carry = 0;
+    for (int b = 0; b < ntile; b += 256) {
+        const int k = b + threadIdx.x;
+        const int v = k < ntile ? ts[k] : 0;
+        int tot;
+        const int ex = block_excl_scan256(v, ws, &tot);
+        if (k < ntile) ts[k] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) out[(size_t)f * (n + 1) + n] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_scan_add(int n, const int* __restrict__ tsum, int* __restrict__ out,
+                                                  int ntile) {
+    const int f = blockIdx.y, t = blockIdx.x;
+    if (t == 0) return;
+    const int off = tsum[(size_t)f * ntile + t];
+    int* o = out + (size_t)f * (n + 1);
+    for (int k = threadIdx.x; k < SCAN_TILE; k += 256) {
+        const int idx = t * SCAN_TILE + k;
+        if (idx < n) o[idx] += off;
+    }
 }
 
 __global__ void k_cell_fill(int N, int ncell, const int* __restrict__ cell, const int* __restrict__ cstart,
@@ -105,6 +162,37 @@ __global__ void k_cell_sort(int ncell, int N, const int* __restrict__ cstart, in
         int v = a[k], m = k - 1;
         while (m >= lo && a[m] > v) { a[m + 1] = a[m]; --m; }
         a[m + 1] = v;
+    }
+}
+
+// one block of 256 per cell: rank sort of the cell's (distinct) atom indices in LDS.
+// Cells larger than the LDS buffer fall back to an insertion sort by one thread.
+constexpr int CELL_SORT_CAP = 2048;
+
+__global__ void __launch_bounds__(256) k_cell_sort_block(int ncell, int N, const int* __restrict__ cstart,
+                                                         int* __restrict__ catoms) {
+    __shared__ int buf[CELL_SORT_CAP];
+    const int f = blockIdx.y, c = blockIdx.x;
+    int* a = catoms + (size_t)f * N;
+    const int lo = cstart[(size_t)f * (ncell + 1) + c], hi = cstart[(size_t)f * (ncell + 1) + c + 1];
+    const int cnt = hi - lo;
+    if (cnt <= 1) return;
+    if (cnt > CELL_SORT_CAP) {
+        if (threadIdx.x == 0)
+            for (int k = lo + 1; k < hi; ++k) {
+                int v = a[k], m = k - 1;
+                while (m >= lo && a[m] > v) { a[m + 1] = a[m]; --m; }
+                a[m + 1] = v;
+            }
+        return;
+    }
+    for (int k = threadIdx.x; k < cnt; k += 256) buf[k] = a[lo + k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += 256) {
+        const int v = buf[k];
+        int r = 0;
+        for (int m = 0; m < cnt; ++m) r += buf[m] < v;
+        a[lo + r] = v;
     }
 }
 

@@ -447,6 +447,7 @@ __global__ void k_fft_x_infl(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>
         }
     }
     double vals[7] = {e, w00, w11, w22, w01, w02, w12};
+#pragma unroll
     for (int k = 0; k < 7; ++k) {
         double tsum = block_sum(vals[k], sh);
         if (threadIdx.x == 0) part[((size_t)f * gridDim.x + blockIdx.x) * 7 + k] = tsum;
@@ -455,14 +456,19 @@ __global__ void k_fft_x_infl(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>
 
 // ---------------------------------------------------------------------------
 // Gather with eight lanes per site (lanes 0..5 own one x-offset each).
+// MODE 1: dE/dmu only (solver products) into out3.  MODE 0: everything the
+// backward needs: dE/dQ (fractional potential and its first/second derivatives
+// mapped to Cartesian), dE/dpos and the per-atom virial.  Loop bounds are
+// compile-time constants so every array stays in registers.
 // ---------------------------------------------------------------------------
-template <class R>
+template <class R, int MODE>
 __global__ void __launch_bounds__(256)
 k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
             int use_dA, const int4* __restrict__ mbase, const R* __restrict__ wfrac, const R* __restrict__ fq,
             const double* __restrict__ Qc, const double* __restrict__ U, int qmode, const R* __restrict__ rgrid,
-            int mode, double* __restrict__ out3, double* __restrict__ g_q, double* __restrict__ g_pos,
+            double* __restrict__ out3, double* __restrict__ g_q, double* __restrict__ g_pos,
             double* __restrict__ vir_atom) {
+    constexpr int ND = MODE == 1 ? 2 : 4;  // derivative orders needed
     const int f = blockIdx.y;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gt >> 3, lane = gt & 7;
@@ -472,101 +478,168 @@ k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, cons
     const BoxInfo& b = boxes[f];
     const double* A = use_dA ? b.dA : b.A;
     const int4 m = mbase[a];
-    R M[3][4][6];
-    bspline6<R>(wfrac[a * 3 + 0], M[0][0], M[0][1], M[0][2], M[0][3]);
-    bspline6<R>(wfrac[a * 3 + 1], M[1][0], M[1][1], M[1][2], M[1][3]);
-    bspline6<R>(wfrac[a * 3 + 2], M[2][0], M[2][1], M[2][2], M[2][3]);
     const R* gbase = rgrid + (size_t)f * K0 * K1 * K2;
-    const int nd = mode == 1 ? 2 : 4;  // derivative orders needed
-    double fp[4][4][4];
-    for (int c = 0; c < 4; ++c)
-        for (int aa = 0; aa < 4; ++aa)
-            for (int bb = 0; bb < 4; ++bb) fp[c][aa][bb] = 0.0;
+    // fp[c][aa][bb]: d^(c,aa,bb) of the fractional potential, c + aa + bb < ND
+    double fp[ND][ND][ND];
+#pragma unroll
+    for (int c = 0; c < ND; ++c)
+#pragma unroll
+        for (int aa = 0; aa < ND; ++aa)
+#pragma unroll
+            for (int bb = 0; bb < ND; ++bb) fp[c][aa][bb] = 0.0;
     if (active && lane < 6) {
+        R M1[4][6], M2[4][6], M0[4][6];
+        bspline6<R>(wfrac[a * 3 + 1], M1[0], M1[1], M1[2], M1[3]);
+        bspline6<R>(wfrac[a * 3 + 2], M2[0], M2[1], M2[2], M2[3]);
+        bspline6<R>(wfrac[a * 3 + 0], M0[0], M0[1], M0[2], M0[3]);
         const int k0 = lane;
         const int g0 = wrapi(m.x + k0 - 3, K0);
         int gz[6];
+#pragma unroll
         for (int k2 = 0; k2 < 6; ++k2) gz[k2] = wrapi(m.z + k2 - 3, K2);
-        R u[4][4];
-        for (int aa = 0; aa < 4; ++aa)
-            for (int bb = 0; bb < 4; ++bb) u[aa][bb] = 0;
+        R u[ND][ND];
+#pragma unroll
+        for (int aa = 0; aa < ND; ++aa)
+#pragma unroll
+            for (int bb = 0; bb < ND; ++bb) u[aa][bb] = 0;
+#pragma unroll
         for (int k1 = 0; k1 < 6; ++k1) {
             const int g1 = wrapi(m.y + k1 - 3, K1);
             const R* row = gbase + ((size_t)g0 * K1 + g1) * K2;
-            R s[4] = {0, 0, 0, 0};
-            for (int k2 = 0; k2 < 6; ++k2) {
-                const R ph = row[gz[k2]];
-                for (int o = 0; o < nd; ++o) s[o] += ph * M[2][o][k2];
+            R ph[6];
+#pragma unroll
+            for (int k2 = 0; k2 < 6; ++k2) ph[k2] = row[gz[k2]];
+            R sz[ND];
+#pragma unroll
+            for (int o = 0; o < ND; ++o) {
+                sz[o] = 0;
+#pragma unroll
+                for (int k2 = 0; k2 < 6; ++k2) sz[o] += ph[k2] * M2[o][k2];
             }
-            for (int aa = 0; aa < nd; ++aa)
-                for (int bb = 0; aa + bb < nd; ++bb) u[aa][bb] += M[1][aa][k1] * s[bb];
+#pragma unroll
+            for (int aa = 0; aa < ND; ++aa)
+#pragma unroll
+                for (int bb = 0; bb < ND; ++bb)
+                    if (aa + bb < ND) u[aa][bb] += M1[aa][k1] * sz[bb];
         }
-        for (int c = 0; c < nd; ++c)
-            for (int aa = 0; c + aa < nd; ++aa)
-                for (int bb = 0; c + aa + bb < nd; ++bb) fp[c][aa][bb] = (double)(M[0][c][k0] * u[aa][bb]);
+        // select this lane's x row (static indexing: unrolled over k0 candidates)
+        R mx[ND];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) {
+            R v = M0[c][0];
+#pragma unroll
+            for (int t = 1; t < 6; ++t) v = (k0 == t) ? M0[c][t] : v;
+            mx[c] = v;
+        }
+#pragma unroll
+        for (int c = 0; c < ND; ++c)
+#pragma unroll
+            for (int aa = 0; aa < ND; ++aa)
+#pragma unroll
+                for (int bb = 0; bb < ND; ++bb)
+                    if (c + aa + bb < ND) fp[c][aa][bb] = (double)(mx[c] * u[aa][bb]);
     }
-    for (int c = 0; c < nd; ++c)
-        for (int aa = 0; c + aa < nd; ++aa)
-            for (int bb = 0; c + aa + bb < nd; ++bb) fp[c][aa][bb] = group_sum(fp[c][aa][bb], 8);
+#pragma unroll
+    for (int c = 0; c < ND; ++c)
+#pragma unroll
+        for (int aa = 0; aa < ND; ++aa)
+#pragma unroll
+            for (int bb = 0; bb < ND; ++bb)
+                if (c + aa + bb < ND) fp[c][aa][bb] = group_sum(fp[c][aa][bb], 8);
     if (!active || lane != 0) return;
-    double p1[3] = {fp[1][0][0], fp[0][1][0], fp[0][0][1]};
+    const double p1[3] = {fp[1][0][0], fp[0][1][0], fp[0][0][1]};
     double dmu[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) dmu[c] = A[c * 3 + 0] * p1[0] + A[c * 3 + 1] * p1[1] + A[c * 3 + 2] * p1[2];
-    if (mode == 1) {
+    if constexpr (MODE == 1) {
         out3[a * 3 + 0] = dmu[0]; out3[a * 3 + 1] = dmu[1]; out3[a * 3 + 2] = dmu[2];
         return;
-    }
-    double p2[3][3];
-    p2[0][0] = fp[2][0][0]; p2[1][1] = fp[0][2][0]; p2[2][2] = fp[0][0][2];
-    p2[0][1] = p2[1][0] = fp[1][1][0]; p2[0][2] = p2[2][0] = fp[1][0][1]; p2[1][2] = p2[2][1] = fp[0][1][1];
-    auto p3 = [&](int j, int k, int l) -> double {
-        int o[3] = {0, 0, 0};
-        o[j]++; o[k]++; o[l]++;
-        return fp[o[0]][o[1]][o[2]];
-    };
-    const R* c = fq + a * 10;
-    const double q = (double)c[0];
-    const double fm[3] = {(double)c[1], (double)c[2], (double)c[3]};
-    double fT[3][3];
-    fT[0][0] = c[4]; fT[1][1] = c[5]; fT[2][2] = c[6];
-    fT[0][1] = fT[1][0] = c[7]; fT[0][2] = fT[2][0] = c[8]; fT[1][2] = fT[2][1] = c[9];
-    double g[3];
-    for (int j = 0; j < 3; ++j) {
-        double s = q * p1[j];
-        for (int k = 0; k < 3; ++k) s += fm[k] * p2[k][j];
-        for (int k = 0; k < 3; ++k)
-            for (int l = 0; l < 3; ++l) s += fT[k][l] * p3(k, l, j);
-        g[j] = s;
-    }
-    double Ap2[3][3];
-    for (int x = 0; x < 3; ++x)
-        for (int k = 0; k < 3; ++k) Ap2[x][k] = A[x * 3 + 0] * p2[0][k] + A[x * 3 + 1] * p2[1][k] + A[x * 3 + 2] * p2[2][k];
-    double GT[3][3];
-    for (int x = 0; x < 3; ++x)
-        for (int y = 0; y < 3; ++y)
-            GT[x][y] = (Ap2[x][0] * A[y * 3 + 0] + Ap2[x][1] * A[y * 3 + 1] + Ap2[x][2] * A[y * 3 + 2]) * (1.0 / 3.0);
-    double* gq = g_q + a * 10;
-    gq[0] = fp[0][0][0];
-    gq[1] = dmu[0]; gq[2] = dmu[1]; gq[3] = dmu[2];
-    gq[4] = GT[0][0]; gq[5] = GT[1][1]; gq[6] = GT[2][2]; gq[7] = GT[0][1]; gq[8] = GT[0][2]; gq[9] = GT[1][2];
-    const double4 pp = pos[a];
-    for (int cc = 0; cc < 3; ++cc) g_pos[a * 3 + cc] = A[0 * 3 + cc] * g[0] + A[1 * 3 + cc] * g[1] + A[2 * 3 + cc] * g[2];
-    double mu[3] = {0, 0, 0}, Th[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    if (qmode & 1) {
-        const double* qc = Qc + a * 10;
-        mu[0] = qc[1]; mu[1] = qc[2]; mu[2] = qc[3];
-        Th[0][0] = qc[4]; Th[1][1] = qc[5]; Th[2][2] = qc[6];
-        Th[0][1] = Th[1][0] = qc[7]; Th[0][2] = Th[2][0] = qc[8]; Th[1][2] = Th[2][1] = qc[9];
-    }
-    if (qmode & 2) { mu[0] += U[a * 3]; mu[1] += U[a * 3 + 1]; mu[2] += U[a * 3 + 2]; }
-    const double r[3] = {pp.x, pp.y, pp.z};
-    double* v = vir_atom + a * 9;
-    for (int j = 0; j < 3; ++j)
-        for (int cc = 0; cc < 3; ++cc) {
-            double TAp = 0;
-            for (int x = 0; x < 3; ++x) TAp += Th[j][x] * Ap2[x][cc];
-            v[j * 3 + cc] = g[j] * r[cc] + mu[j] * p1[cc] + (2.0 / 3.0) * TAp;
+    } else {
+        double p2[3][3];
+        p2[0][0] = fp[2][0][0]; p2[1][1] = fp[0][2][0]; p2[2][2] = fp[0][0][2];
+        p2[0][1] = p2[1][0] = fp[1][1][0]; p2[0][2] = p2[2][0] = fp[1][0][1]; p2[1][2] = p2[2][1] = fp[0][1][1];
+        // third derivatives d3[j][k][l] (symmetric), from the ten independent entries
+        double d3[3][3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    const int ox = (j == 0) + (k == 0) + (l == 0);
+                    const int oy = (j == 1) + (k == 1) + (l == 1);
+                    const int oz = (j == 2) + (k == 2) + (l == 2);
+                    double v = 0;
+                    if (ox == 3) v = fp[3][0][0];
+                    else if (oy == 3) v = fp[0][3][0];
+                    else if (oz == 3) v = fp[0][0][3];
+                    else if (ox == 2 && oy == 1) v = fp[2][1][0];
+                    else if (ox == 2 && oz == 1) v = fp[2][0][1];
+                    else if (oy == 2 && ox == 1) v = fp[1][2][0];
+                    else if (oy == 2 && oz == 1) v = fp[0][2][1];
+                    else if (oz == 2 && ox == 1) v = fp[1][0][2];
+                    else if (oz == 2 && oy == 1) v = fp[0][1][2];
+                    else v = fp[1][1][1];
+                    d3[j][k][l] = v;
+                }
+        const R* c = fq + a * 10;
+        const double q = (double)c[0];
+        const double fm[3] = {(double)c[1], (double)c[2], (double)c[3]};
+        double fT[3][3];
+        fT[0][0] = c[4]; fT[1][1] = c[5]; fT[2][2] = c[6];
+        fT[0][1] = fT[1][0] = c[7]; fT[0][2] = fT[2][0] = c[8]; fT[1][2] = fT[2][1] = c[9];
+        double g[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double sj = q * p1[j];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sj += fm[k] * p2[k][j];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) sj += fT[k][l] * d3[k][l][j];
+            g[j] = sj;
         }
+        double Ap2[3][3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                Ap2[x][k] = A[x * 3 + 0] * p2[0][k] + A[x * 3 + 1] * p2[1][k] + A[x * 3 + 2] * p2[2][k];
+        double GT[3][3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3; ++y)
+                GT[x][y] = (Ap2[x][0] * A[y * 3 + 0] + Ap2[x][1] * A[y * 3 + 1] + Ap2[x][2] * A[y * 3 + 2]) * (1.0 / 3.0);
+        double* gq = g_q + a * 10;
+        gq[0] = fp[0][0][0];
+        gq[1] = dmu[0]; gq[2] = dmu[1]; gq[3] = dmu[2];
+        gq[4] = GT[0][0]; gq[5] = GT[1][1]; gq[6] = GT[2][2]; gq[7] = GT[0][1]; gq[8] = GT[0][2]; gq[9] = GT[1][2];
+        const double4 pp = pos[a];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            g_pos[a * 3 + cc] = A[0 * 3 + cc] * g[0] + A[1 * 3 + cc] * g[1] + A[2 * 3 + cc] * g[2];
+        double mu[3] = {0, 0, 0}, Th[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (qmode & 1) {
+            const double* qc = Qc + a * 10;
+            mu[0] = qc[1]; mu[1] = qc[2]; mu[2] = qc[3];
+            Th[0][0] = qc[4]; Th[1][1] = qc[5]; Th[2][2] = qc[6];
+            Th[0][1] = Th[1][0] = qc[7]; Th[0][2] = Th[2][0] = qc[8]; Th[1][2] = Th[2][1] = qc[9];
+        }
+        if (qmode & 2) { mu[0] += U[a * 3]; mu[1] += U[a * 3 + 1]; mu[2] += U[a * 3 + 2]; }
+        const double r[3] = {pp.x, pp.y, pp.z};
+        double* v = vir_atom + a * 9;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                double TAp = 0;
+#pragma unroll
+                for (int x = 0; x < 3; ++x) TAp += Th[j][x] * Ap2[x][cc];
+                v[j * 3 + cc] = g[j] * r[cc] + mu[j] * p1[cc] + (2.0 / 3.0) * TAp;
+            }
+    }
 }
 
 }  // namespace admp

@@ -669,4 +669,219 @@ __global__ void k_pcg_p(int N, const PcgState* __restrict__ st, const double* __
     for (int k = 0; k < 3; ++k) p[a * 3 + k] = z[a * 3 + k] + be * p[a * 3 + k];
 }
 
+
+// ---------------------------------------------------------------------------
+// Fused preconditioned CG.  Every per-site kernel runs on blocks of PCG_BLOCK
+// sites and leaves one partial per block; a consumer kernel reduces the nb
+// partials of its frame in every block (a fixed assignment of partials to lanes
+// and a fixed tree, so all blocks see the same bits and the result does not
+// depend on scheduling).  Partials of frame f, each nb long:
+//   [0] p.q   [1] r.z (buffer 0)   [2] r.z (buffer 1)   [3] max|r| (0)   [4] max|r| (1)
+// Buffers alternate with the iteration parity so that beta = rz_new / rz_old
+// can read both.  max|r| runs over sites with pol > 0.001 A^3 (pme.py:393-395).
+// ---------------------------------------------------------------------------
+constexpr int PCG_BLOCK = 256;
+constexpr int PCG_NPART = 5;
+
+__device__ inline double wave_allsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_allmax(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// total (or max) of nb partials; valid in every thread; sh holds >= 16 doubles
+template <bool MAX>
+__device__ inline double parts_total(const double* __restrict__ p, int nb, double* sh) {
+    double v = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) v = MAX ? fmax(v, p[k]) : v + p[k];
+    v = MAX ? wave_allmax(v) : wave_allsum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = sh[0];
+    const int nw = (int)(blockDim.x >> 6);
+    for (int k = 1; k < nw; ++k) t = MAX ? fmax(t, sh[k]) : t + sh[k];
+    __syncthreads();
+    return t;
+}
+
+// block reduction of one value into thread 0
+template <bool MAX>
+__device__ inline double block_red(double v, double* sh) {
+    v = MAX ? wave_allmax(v) : wave_allsum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = sh[0];
+    const int nw = (int)(blockDim.x >> 6);
+    for (int k = 1; k < nw; ++k) t = MAX ? fmax(t, sh[k]) : t + sh[k];
+    __syncthreads();
+    return t;
+}
+
+__device__ inline double inv_diag(double pol) { return (pol < 1e-8 ? 1e-8 : pol) / DIELECTRIC; }
+
+// r = b - Ax, z = M^-1 r, partials r.z and max|r| into buffer `buf`
+__global__ void __launch_bounds__(PCG_BLOCK) k_pcg_resid(int N, int nb, int buf, const double* __restrict__ pol,
+                                                         const double* __restrict__ b, const double* __restrict__ Ax,
+                                                         double* __restrict__ r, double* __restrict__ z,
+                                                         double* __restrict__ parts) {
+    __shared__ double sh[16];
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
+    double rz = 0, mx = 0;
+    if (i < N) {
+        const size_t a = (size_t)f * N + i;
+        const double m = inv_diag(pol[i]);
+        for (int k = 0; k < 3; ++k) {
+            const double rv = b[a * 3 + k] - Ax[a * 3 + k];
+            r[a * 3 + k] = rv;
+            z[a * 3 + k] = m * rv;
+            rz += rv * m * rv;
+            if (pol[i] > 0.001) mx = fmax(mx, fabs(rv));
+        }
+    }
+    rz = block_red<false>(rz, sh);
+    mx = block_red<true>(mx, sh);
+    if (threadIdx.x == 0) {
+        double* P = parts + (size_t)f * PCG_NPART * nb;
+        P[(1 + buf) * nb + blockIdx.x] = rz;
+        P[(3 + buf) * nb + blockIdx.x] = mx;
+    }
+}
+
+// q = real + recip + d p (the diagonal of the polarisation matrix), partial p.q
+__global__ void __launch_bounds__(PCG_BLOCK) k_pcg_q(int N, int nb, double f1, const double* __restrict__ pol,
+                                                     const double* __restrict__ p, const double* __restrict__ real,
+                                                     const double* __restrict__ recip, double* __restrict__ q,
+                                                     double* __restrict__ parts) {
+    __shared__ double sh[16];
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
+    double pq = 0;
+    if (i < N) {
+        const size_t a = (size_t)f * N + i;
+        const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
+        const double d = DIELECTRIC / ap - 2.0 * DIELECTRIC * f1;
+        for (int k = 0; k < 3; ++k) {
+            const double pv = p[a * 3 + k];
+            const double qv = real[a * 3 + k] + (recip ? recip[a * 3 + k] : 0.0) + d * pv;
+            q[a * 3 + k] = qv;
+            pq += pv * qv;
+        }
+    }
+    pq = block_red<false>(pq, sh);
+    if (threadIdx.x == 0) parts[(size_t)f * PCG_NPART * nb + blockIdx.x] = pq;
+}
+
+// x += alpha p, r -= alpha q, z = M^-1 r; partials of the new r into buffer cur^1.
+// A frame whose residual (buffer cur) is already below tol is left unchanged.
+__global__ void __launch_bounds__(PCG_BLOCK) k_pcg_step(int N, int nb, int cur, double tol,
+                                                        const double* __restrict__ pol, double* __restrict__ x,
+                                                        double* __restrict__ r, double* __restrict__ z,
+                                                        const double* __restrict__ p, const double* __restrict__ q,
+                                                        double* __restrict__ parts) {
+    __shared__ double sh[16];
+    const int f = blockIdx.y;
+    double* P = parts + (size_t)f * PCG_NPART * nb;
+    const int nx = cur ^ 1;
+    const double res = parts_total<true>(P + (3 + cur) * nb, nb, sh);
+    if (res < tol) {
+        if (threadIdx.x == 0) {
+            P[(1 + nx) * nb + blockIdx.x] = P[(1 + cur) * nb + blockIdx.x];
+            P[(3 + nx) * nb + blockIdx.x] = P[(3 + cur) * nb + blockIdx.x];
+        }
+        return;
+    }
+    const double rz = parts_total<false>(P + (1 + cur) * nb, nb, sh);
+    const double pq = parts_total<false>(P, nb, sh);
+    const double al = pq != 0.0 ? rz / pq : 0.0;
+    const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
+    double lrz = 0, lmx = 0;
+    if (i < N) {
+        const size_t a = (size_t)f * N + i;
+        const double m = inv_diag(pol[i]);
+        for (int k = 0; k < 3; ++k) {
+            x[a * 3 + k] += al * p[a * 3 + k];
+            const double rv = r[a * 3 + k] - al * q[a * 3 + k];
+            r[a * 3 + k] = rv;
+            z[a * 3 + k] = m * rv;
+            lrz += rv * m * rv;
+            if (pol[i] > 0.001) lmx = fmax(lmx, fabs(rv));
+        }
+    }
+    lrz = block_red<false>(lrz, sh);
+    lmx = block_red<true>(lmx, sh);
+    if (threadIdx.x == 0) {
+        P[(1 + nx) * nb + blockIdx.x] = lrz;
+        P[(3 + nx) * nb + blockIdx.x] = lmx;
+    }
+}
+
+// p = z + beta p (beta = 0 when first), with the R copy of p for the pair kernel,
+// the fractional dipoles A^T p and the spline coordinates for the spread.  Block 0
+// records iteration count, residual and convergence in the frame's PcgState.
+template <class R>
+__global__ void __launch_bounds__(PCG_BLOCK) k_pcg_dir(int N, int nb, int nx, int first, double tol,
+                                                       const BoxInfo* __restrict__ boxes, const double* __restrict__ wd,
+                                                       const double* __restrict__ parts, const double* __restrict__ z,
+                                                       double* __restrict__ p, R* __restrict__ pR,
+                                                       R* __restrict__ wfrac, R* __restrict__ fq,
+                                                       PcgState* __restrict__ st) {
+    __shared__ double sh[16];
+    const int f = blockIdx.y;
+    const double* P = parts + (size_t)f * PCG_NPART * nb;
+    const double res = parts_total<true>(P + (3 + nx) * nb, nb, sh);
+    const double res_prev = first ? 0.0 : parts_total<true>(P + (3 + (nx ^ 1)) * nb, nb, sh);
+    const bool done = res < tol;
+    double beta = 0.0;
+    if (!first) {
+        const double rzn = parts_total<false>(P + (1 + nx) * nb, nb, sh);
+        const double rzo = parts_total<false>(P + (1 + (nx ^ 1)) * nb, nb, sh);
+        beta = rzo != 0.0 ? rzn / rzo : 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        PcgState s = st[f];
+        if (first == 2) {
+            // state refresh after a true-residual replacement
+            s.res = res;
+            s.done = done ? 1 : 0;
+            st[f] = s;
+        }
+    }
+    if (first == 2) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        PcgState s = st[f];
+        if (first) {
+            s.iter = 0;
+        } else if (!(res_prev < tol)) {
+            s.iter += 1;
+        }
+        s.res = res;
+        s.done = done ? 1 : 0;
+        s.beta = beta;
+        st[f] = s;
+    }
+    if (done) return;
+    const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
+    if (i >= N) return;
+    const size_t a = (size_t)f * N + i;
+    double u[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = z[a * 3 + k] + beta * p[a * 3 + k];
+        p[a * 3 + k] = u[k];
+        if (pR) pR[a * 3 + k] = (R)u[k];
+        if (wd) wfrac[a * 3 + k] = (R)wd[a * 3 + k];
+    }
+    if (!fq) return;
+    const double* A = boxes[f].A;
+    R* o = fq + a * 10;
+    o[0] = 0;
+    for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u[0] + A[1 * 3 + j] * u[1] + A[2 * 3 + j] * u[2]);
+    for (int k = 4; k < 10; ++k) o[k] = 0;
+}
 }  // namespace admp
