@@ -38,6 +38,8 @@ struct admp_ctx {
     hipStream_t stream = nullptr;       // the context's own stream (capturable)
     hipStream_t user_stream = nullptr;  // caller's stream: ordered with events on entry and exit
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipStream_t side = nullptr;         // second context stream: real-space products beside the reciprocal chain
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     int N = 0, F = 1, nh = 1;
     int cap = 0;
@@ -65,6 +67,9 @@ struct admp_ctx {
     void* d_hgrid = nullptr;    // half spectrum, F * max(K0 K1 (K2/2+1)) complex
     double* d_wd = nullptr;     // per-site grid offsets w (double)
     int *d_gcell = nullptr, *d_gccount = nullptr, *d_gcstart = nullptr, *d_gcfill = nullptr, *d_gcatoms = nullptr;
+    int *d_bcount = nullptr, *d_bstart = nullptr;  // per-brick candidate counts / offsets
+    int2* d_blist = nullptr;                        // per-brick candidate lists
+    long blist_cap = 0;
     int* d_scan_tmp = nullptr;  // tile totals of scan_rows
     size_t scan_cap = 0;
     const Grid* prepped = nullptr;  // grid whose site bases / sort are current
@@ -134,16 +139,17 @@ struct ProfScope {
     admp_ctx* ctx;
     int key;
     hipEvent_t a = nullptr;
-    ProfScope(admp_ctx* c, int k) : ctx(c), key(k) {
+    hipStream_t sp;
+    ProfScope(admp_ctx* c, int k, hipStream_t s = nullptr) : ctx(c), key(k), sp(s ? s : c->stream) {
         if (ctx->prof) {
             a = ev_get(ctx);
-            (void)hipEventRecord(a, ctx->stream);
+            (void)hipEventRecord(a, sp);
         }
     }
     ~ProfScope() {
         if (ctx->prof) {
             hipEvent_t b = ev_get(ctx);
-            (void)hipEventRecord(b, ctx->stream);
+            (void)hipEventRecord(b, sp);
             ctx->ev_pending.push_back({key, {a, b}});
         }
     }
@@ -305,6 +311,10 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     clear_graphs(ctx);
     for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
     if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -342,6 +352,9 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     const bool mixed = c.precision == ADMP_MIXED;
     admp_status st;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "stream/event creation failed";
@@ -391,6 +404,15 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         ALLOC(ctx->d_gccount, Gmax * ctx->F);
         ALLOC(ctx->d_gcfill, Gmax * ctx->F);
         ALLOC(ctx->d_gcstart, (Gmax + 1) * ctx->F);
+        size_t nbrmax = 0;
+        for (const Grid* gg : {&ctx->eg, &ctx->dg})
+            if (gg->G)
+                nbrmax = std::max(nbrmax, (size_t)((gg->K[0] + BRX - 1) / BRX) * ((gg->K[1] + BRY - 1) / BRY) *
+                                              ((gg->K[2] + BRZ - 1) / BRZ));
+        ALLOC(ctx->d_bcount, nbrmax * ctx->F);
+        ALLOC(ctx->d_bstart, (nbrmax + 1) * ctx->F);
+        ctx->blist_cap = (long)BRICK_MAXCAND * ctx->N;  // per frame
+        ALLOC(ctx->d_blist, (size_t)ctx->blist_cap * ctx->F);
     }
     ALLOC(ctx->d_wd, NF * 3);
     ALLOC(ctx->d_gcell, NF);
@@ -721,6 +743,16 @@ static admp_status recip_positions(admp_ctx* ctx, const Grid& g, int use_dA) {
                            ctx->d_gcstart, ctx->d_gcfill, ctx->d_gcatoms);
         hipLaunchKernelGGL(k_cell_sort, dim3(nblk(G, 256), F), dim3(256), 0, st, (int)G, N, ctx->d_gcstart,
                            ctx->d_gcatoms);
+        // per-brick candidate lists for the spreads of this evaluation
+        const int nbr = ((g.K[0] + BRX - 1) / BRX) * ((g.K[1] + BRY - 1) / BRY) * ((g.K[2] + BRZ - 1) / BRZ);
+        hipLaunchKernelGGL(k_brick_cands<false>, dim3(nbr, F), dim3(64), 0, st, N, g.K[0], g.K[1], g.K[2],
+                           ctx->d_mbase, ctx->d_gcstart, ctx->d_gcatoms, ctx->d_bcount, (const int*)nullptr,
+                           (int2*)nullptr, 0L);
+        s2 = scan_rows(ctx, ctx->d_bcount, ctx->d_bstart, nbr, F, st);
+        if (s2 != ADMP_OK) return s2;
+        hipLaunchKernelGGL(k_brick_cands<true>, dim3(nbr, F), dim3(64), 0, st, N, g.K[0], g.K[1], g.K[2],
+                           ctx->d_mbase, ctx->d_gcstart, ctx->d_gcatoms, ctx->d_bcount, ctx->d_bstart, ctx->d_blist,
+                           ctx->blist_cap);
     }
     }
     HIPCHK(hipGetLastError());
@@ -795,35 +827,40 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     const int nbricks = ((K0 + BRX - 1) / BRX) * ((K1 + BRY - 1) / BRY) * ((K2 + BRZ - 1) / BRZ);
     { ProfScope ps_(ctx, 4);
     if (lmax_eff >= 2)
-        hipLaunchKernelGGL((k_spread_brick<R, 2>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, ctx->d_mbase, wfrac,
-                           fq, ctx->d_gcstart, ctx->d_gcatoms, rg);
+        hipLaunchKernelGGL((k_spread_list<R, 2>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg);
     else if (lmax_eff == 1)
-        hipLaunchKernelGGL((k_spread_brick<R, 1>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, ctx->d_mbase, wfrac,
-                           fq, ctx->d_gcstart, ctx->d_gcatoms, rg);
+        hipLaunchKernelGGL((k_spread_list<R, 1>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg);
     else
-        hipLaunchKernelGGL((k_spread_brick<R, 0>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, ctx->d_mbase, wfrac,
-                           fq, ctx->d_gcstart, ctx->d_gcatoms, rg);
+        hipLaunchKernelGGL((k_spread_list<R, 0>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg);
     }
-    const int LB = 4, TH = 64;
     const int npairs = (K0 * K1 + 1) / 2;
     const int nly = K0 * H, nlx = K1 * H;
+    const size_t Cs = sizeof(Cplx<R>);
     { ProfScope ps_(ctx, 5);
-    hipLaunchKernelGGL(k_fft_z_r2c<R>, dim3(nblk(npairs, LB), F), dim3(TH), 2 * LB * K2 * sizeof(Cplx<R>), st, K0, K1,
-                       K2, g.plan[2], twid<R>(g, 2), rg, hg, LB);
-    hipLaunchKernelGGL(k_fft_y<R>, dim3(nblk(nly, LB), F), dim3(TH), 2 * LB * K1 * sizeof(Cplx<R>), st, K0, K1, H,
-                       g.plan[1], twid<R>(g, 1), hg, 0, LB); }
-    const int nbx = (int)nblk(nlx, LB);
+    hipLaunchKernelGGL(k_fz_r2c<R>, dim3(nblk(npairs, FL), F), dim3(FTH), (2 * FL + 1) * K2 * Cs, st, K0, K1, K2,
+                       g.plan[2], twid<R>(g, 2), rg, hg);
+    hipLaunchKernelGGL(k_fy<R>, dim3(nblk(nly, FL), F), dim3(FTH), (2 * FL + 1) * K1 * Cs, st, K0, K1, H, g.plan[1],
+                       twid<R>(g, 1), hg, 0); }
+    const int nbx = (int)nblk(nlx, FL);
     { ProfScope ps_(ctx, 6);
-    hipLaunchKernelGGL(k_fft_x_infl<R>, dim3(nbx, F), dim3(TH), 2 * LB * K0 * sizeof(Cplx<R>), st, K0, K1, K2,
-                       g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind, pref,
-                       ctx->d_part, LB); }
+    if (want_energy)
+        hipLaunchKernelGGL((k_fx_infl<R, true>), dim3(nbx, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
+                           g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
+                           pref, ctx->d_part);
+    else
+        hipLaunchKernelGGL((k_fx_infl<R, false>), dim3(nbx, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
+                           g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
+                           pref, ctx->d_part); }
     if (want_energy)
         hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbx, 7, ctx->d_part, ctx->d_rec, 0);
     { ProfScope ps_(ctx, 5);
-    hipLaunchKernelGGL(k_fft_y<R>, dim3(nblk(nly, LB), F), dim3(TH), 2 * LB * K1 * sizeof(Cplx<R>), st, K0, K1, H,
-                       g.plan[1], twid<R>(g, 1), hg, 1, LB);
-    hipLaunchKernelGGL(k_fft_z_c2r<R>, dim3(nblk(npairs, LB), F), dim3(TH), 2 * LB * K2 * sizeof(Cplx<R>), st, K0, K1,
-                       K2, g.plan[2], twid<R>(g, 2), hg, rg, LB); }
+    hipLaunchKernelGGL(k_fy<R>, dim3(nblk(nly, FL), F), dim3(FTH), (2 * FL + 1) * K1 * Cs, st, K0, K1, H, g.plan[1],
+                       twid<R>(g, 1), hg, 1);
+    hipLaunchKernelGGL(k_fz_c2r<R>, dim3(nblk(npairs, FL), F), dim3(FTH), (2 * FL + 1) * K2 * Cs, st, K0, K1, K2,
+                       g.plan[2], twid<R>(g, 2), hg, rg); }
     { ProfScope ps_(ctx, 7);
     if (gather_mode == 1)
         hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2,
@@ -900,17 +937,24 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     const R* vR = sizeof(R) == 8 ? reinterpret_cast<const R*>(ctx->d_p) : reinterpret_cast<const R*>(ctx->d_vR);
-    { ProfScope ps_(ctx, 2);
-    hipLaunchKernelGGL((k_real_matvec<RC, R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
-                       ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
+    // the real-space product runs on the side stream, beside the reciprocal chain
+    HIPCHK(hipEventRecord(ctx->ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    { ProfScope ps_(ctx, 2, ctx->side);
+    hipLaunchKernelGGL((k_real_matvec<RC, R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0,
+                       ctx->side, N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
                        reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1); }
+    HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
     double* rec = nullptr;
+    admp_status s = ADMP_OK;
     if (ctx->cfg.lpme) {
-        admp_status s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_p, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1,
-                                     ctx->d_t2, nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq);
-        if (s != ADMP_OK) return s;
+        s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_p, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1, ctx->d_t2, nullptr,
+                         nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq);
         rec = ctx->d_t2;
     }
+    // join before any return so that a capture never ends with an open fork
+    HIPCHK(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    if (s != ADMP_OK) return s;
     { ProfScope ps_(ctx, 11);
     hipLaunchKernelGGL(k_pcg_q, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, f1, pol, ctx->d_p,
                        ctx->d_t1, rec, ctx->d_Ap, ctx->d_pparts); }

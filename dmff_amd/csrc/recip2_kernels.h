@@ -97,6 +97,164 @@ __global__ void k_frac_dipole(int N, const BoxInfo* __restrict__ boxes, const do
 constexpr int BRX = 2, BRY = 4, BRZ = 8;   // 64 grid points per wavefront
 constexpr int SP_CHUNK = 64;               // sites staged in LDS per round
 
+// ---------------------------------------------------------------------------
+// Per-brick candidate lists.  The candidates of a brick are the sites whose
+// base grid point lies in the brick grown by 2 below and 3 above in each axis
+// (their 6-point support can touch the brick).  They are enumerated column by
+// column ((x, y) pairs of the grown region, z ranges of the cell-sorted sites),
+// so the order is fixed.  Positions do not change during a solve, so the lists
+// are built once per evaluation and every spread of the solve reuses them.
+// Entry: {site, (rx + 5) | (ry + 5) << 4 | (rz + 5) << 8} with r the support
+// start relative to the brick origin.  A site is a candidate of at most
+// 5 x 4 x 3 bricks (per axis ceil((B + 5) / B) + 1 brick origins fit a window of
+// B + 5 cells), so 60 N entries bound a list.
+// ---------------------------------------------------------------------------
+constexpr int BRICK_MAXCAND = 60;
+
+template <bool FILL>
+__global__ void __launch_bounds__(64)
+k_brick_cands(int N, int K0, int K1, int K2, const int4* __restrict__ mbase, const int* __restrict__ cstart,
+              const int* __restrict__ catoms, int* __restrict__ bcount, const int* __restrict__ bstart,
+              int2* __restrict__ blist, long cap) {
+    constexpr int NC = (BRX + 5) * (BRY + 5) * 2;  // column segments (two when z wraps)
+    __shared__ int col_lo[NC];
+    __shared__ int col_off[NC + 1];
+    const int f = blockIdx.y;
+    const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY, nbx = (K0 + BRX - 1) / BRX;
+    const int nbr = nbx * nby * nbz;
+    const int b = blockIdx.x;
+    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+    const int X0 = bx * BRX, Y0 = by * BRY, Z0 = bz * BRZ;
+    const int t = threadIdx.x;
+    const size_t G = (size_t)K0 * K1 * K2;
+    const int* cs = cstart + (size_t)f * (G + 1);
+    const int* ca = catoms + (size_t)f * N;
+    const int nx = min(BRX + 5, K0), ny = min(BRY + 5, K1), nzr = min(BRZ + 5, K2);
+    const int nseg = nx * ny * 2;
+    const int zs = wrapi(Z0 - 2, K2);
+    const int zlen1 = min(nzr, K2 - zs), zlen2 = nzr - zlen1;
+    // each lane owns segments 2 t and 2 t + 1
+    int lo2[2] = {0, 0}, n2[2] = {0, 0};
+    for (int k = 0; k < 2; ++k) {
+        const int c = 2 * t + k;
+        if (c >= nseg) continue;
+        const int cc = c >> 1, seg = c & 1;
+        const int x = wrapi(X0 - 2 + cc / ny, K0), y = wrapi(Y0 - 2 + cc % ny, K1);
+        const size_t base = ((size_t)x * K1 + y) * K2;
+        if (seg == 0) {
+            lo2[k] = cs[base + zs];
+            n2[k] = cs[base + zs + zlen1] - lo2[k];
+        } else if (zlen2 > 0) {
+            lo2[k] = cs[base];
+            n2[k] = cs[base + zlen2] - lo2[k];
+        }
+    }
+    const int mine = n2[0] + n2[1];
+    const int incl = wave_incl_scan(mine);
+    const int ex = incl - mine;
+    const int total = __shfl(incl, 63, 64);
+    if (!FILL) {
+        if (t == 0) bcount[(size_t)f * nbr + b] = total;
+        return;
+    }
+    if (2 * t < NC) { col_lo[2 * t] = lo2[0]; col_off[2 * t] = ex; }
+    if (2 * t + 1 < NC) { col_lo[2 * t + 1] = lo2[1]; col_off[2 * t + 1] = ex + n2[0]; }
+    if (t == 0) col_off[NC] = total;
+    __syncthreads();
+    // each frame owns cap entries of the list buffer
+    const long base = bstart[(size_t)f * (nbr + 1) + b];
+    int2* fl = blist + (size_t)f * cap;
+    for (int g = t; g < total; g += 64) {
+        int lo = 0, hi = NC;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (col_off[mid] <= g) lo = mid; else hi = mid;
+        }
+        const int atom = ca[col_lo[lo] + (g - col_off[lo])];
+        const int4 m = mbase[(size_t)f * N + atom];
+        const int rx = wrapi(m.x - 3 - X0 + 5, K0);
+        const int ry = wrapi(m.y - 3 - Y0 + 5, K1);
+        const int rz = wrapi(m.z - 3 - Z0 + 5, K2);
+        if (base + g < cap) fl[base + g] = make_int2(atom, rx | (ry << 4) | (rz << 8));
+    }
+}
+
+// Spread from the per-brick lists: one wavefront per brick, one grid point per
+// lane, candidates staged 64 at a time (spline tables and sources in LDS).
+template <class R, int LM>
+__global__ void __launch_bounds__(64)
+k_spread_list(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R* __restrict__ fq,
+              const int* __restrict__ bstart, const int2* __restrict__ blist, long cap, R* __restrict__ rgrid) {
+    constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
+    constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
+    __shared__ int s_r[SP_CHUNK][3];
+    __shared__ R s_t[SP_CHUNK][3][NT][6];
+    __shared__ R s_q[SP_CHUNK][NQ];
+    const int f = blockIdx.y;
+    const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY, nbx = (K0 + BRX - 1) / BRX;
+    const int nbr = nbx * nby * nbz;
+    const int b = blockIdx.x;
+    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+    const int t = threadIdx.x;
+    const int lx = t / (BRY * BRZ), ly = (t / BRZ) % BRY, lz = t % BRZ;
+    const int px = bx * BRX + lx, py = by * BRY + ly, pz = bz * BRZ + lz;
+    const bool own = px < K0 && py < K1 && pz < K2;
+    const size_t G = (size_t)K0 * K1 * K2;
+    const int lo = bstart[(size_t)f * (nbr + 1) + b];
+    const int total = (int)min((long)bstart[(size_t)f * (nbr + 1) + b + 1], cap) - lo;
+    R acc = 0;
+    for (int c0 = 0; c0 < total; c0 += SP_CHUNK) {
+        const int nchunk = min(SP_CHUNK, total - c0);
+        if (t < nchunk) {
+            const int2 e = blist[(size_t)f * cap + lo + c0 + t];
+            const size_t a = (size_t)f * N + e.x;
+            s_r[t][0] = (e.y & 15) - 5;
+            s_r[t][1] = ((e.y >> 4) & 15) - 5;
+            s_r[t][2] = ((e.y >> 8) & 15) - 5;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const R w = wfrac[a * 3 + d];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    R v, d1, d2;
+                    bs_piece<R>(k, w, v, d1, d2);
+                    s_t[t][d][0][k] = v;
+                    if (NT > 1) s_t[t][d][NT > 1 ? 1 : 0][k] = d1;
+                    if (NT > 2) s_t[t][d][NT > 2 ? 2 : 0][k] = d2;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) s_q[t][q] = fq[a * 10 + q];
+        }
+        __syncthreads();
+        if (own) {
+            for (int s2 = 0; s2 < nchunk; ++s2) {
+                const int k0 = lx - s_r[s2][0], k1 = ly - s_r[s2][1], k2 = lz - s_r[s2][2];
+                if ((unsigned)k0 > 5u || (unsigned)k1 > 5u || (unsigned)k2 > 5u) continue;
+                const R v0 = s_t[s2][0][0][k0], v1 = s_t[s2][1][0][k1], v2 = s_t[s2][2][0][k2];
+                R wgt = s_q[s2][0] * v0 * v1 * v2;
+                if (LM >= 1) {
+                    const R a0 = s_t[s2][0][NT > 1 ? 1 : 0][k0], a1 = s_t[s2][1][NT > 1 ? 1 : 0][k1],
+                            a2 = s_t[s2][2][NT > 1 ? 1 : 0][k2];
+                    wgt += s_q[s2][NQ > 1 ? 1 : 0] * a0 * v1 * v2 + s_q[s2][NQ > 2 ? 2 : 0] * v0 * a1 * v2 +
+                           s_q[s2][NQ > 3 ? 3 : 0] * v0 * v1 * a2;
+                    if (LM >= 2) {
+                        const R b0 = s_t[s2][0][NT > 2 ? 2 : 0][k0], b1 = s_t[s2][1][NT > 2 ? 2 : 0][k1],
+                                b2 = s_t[s2][2][NT > 2 ? 2 : 0][k2];
+                        wgt += s_q[s2][NQ > 4 ? 4 : 0] * b0 * v1 * v2 + s_q[s2][NQ > 5 ? 5 : 0] * v0 * b1 * v2 +
+                               s_q[s2][NQ > 6 ? 6 : 0] * v0 * v1 * b2 +
+                               (R)2 * (s_q[s2][NQ > 7 ? 7 : 0] * a0 * a1 * v2 + s_q[s2][NQ > 8 ? 8 : 0] * a0 * v1 * a2 +
+                                       s_q[s2][NQ > 9 ? 9 : 0] * v0 * a1 * a2);
+                    }
+                }
+                acc += wgt;
+            }
+        }
+        __syncthreads();
+    }
+    if (own) rgrid[(size_t)f * G + ((size_t)px * K1 + py) * K2 + pz] = acc;
+}
+
 // LM: 0 charges only, 1 charges + dipoles, 2 full quadrupoles.
 // Staging computes, per candidate site, the start of its support relative to the
 // brick origin (r in [-5, brick-1]) and its 1-D spline tables (value, d/dR,
@@ -327,6 +485,431 @@ __global__ void k_fft_z_c2r(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>*
     }
 }
 
+// ---------------------------------------------------------------------------
+// Line FFT engine.  L lines (compile time) live in LDS as [element][line]; the
+// block's threads split into L lanes (one per line) times T = blockDim / L
+// butterfly slots.  One Stockham autosort stage per factor r of K: butterfly b
+// (0 <= b < K/r) reads x_q = in[b + q K/r], multiplies x_q by the stage twiddle
+// W_{Ns r}^{q (b mod Ns)} and writes the r-point DFT to
+// out[(b div Ns) Ns r + (b mod Ns) + k Ns].  Radices 2, 3, 4, 5 are written
+// out; other factors use a direct sum per output.  tw[t] = e^{-2 pi i t/K}
+// (conjugated for the inverse) is staged in LDS by the caller.
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ Cplx<R> cadd(Cplx<R> a, Cplx<R> b) { return Cplx<R>{a.x + b.x, a.y + b.y}; }
+template <class R>
+__device__ __forceinline__ Cplx<R> csub(Cplx<R> a, Cplx<R> b) { return Cplx<R>{a.x - b.x, a.y - b.y}; }
+// multiply by -i (sg = +1) or +i (sg = -1)
+template <class R>
+__device__ __forceinline__ Cplx<R> mul_mi(Cplx<R> a, R sg) { return Cplx<R>{sg * a.y, -sg * a.x}; }
+
+template <class R, int L>
+__device__ Cplx<R>* fft_lines(Cplx<R>* in, Cplx<R>* out, int K, const FFTPlan& plan, const Cplx<R>* tw, int inverse) {
+    const int T = (int)blockDim.x / L;
+    const int li = (int)threadIdx.x % L, slot = (int)threadIdx.x / L;
+    const R sg = inverse ? (R)-1 : (R)1;  // sign of the exponent's -i
+    int Ns = 1;
+    for (int s = 0; s < plan.nst; ++s) {
+        const int r = plan.fac[s];
+        const int nb = K / r;
+        const int step = K / (Ns * r);
+        for (int b = slot; b < nb; b += T) {
+            const int jq = b / Ns, kj = b - jq * Ns;
+            const int t1 = kj * step;  // twiddle index of q = 1
+            const int ob = jq * Ns * r + kj;
+            auto twq = [&](int q) -> Cplx<R> {
+                int t = q * t1;
+                t -= (t / K) * K;
+                Cplx<R> w = tw[t];
+                w.y *= sg;
+                return w;
+            };
+            if (r == 2) {
+                const Cplx<R> x0 = in[b * L + li];
+                const Cplx<R> x1 = cmul(in[(b + nb) * L + li], twq(1));
+                out[ob * L + li] = cadd(x0, x1);
+                out[(ob + Ns) * L + li] = csub(x0, x1);
+            } else if (r == 4) {
+                const Cplx<R> x0 = in[b * L + li];
+                const Cplx<R> x1 = cmul(in[(b + nb) * L + li], twq(1));
+                const Cplx<R> x2 = cmul(in[(b + 2 * nb) * L + li], twq(2));
+                const Cplx<R> x3 = cmul(in[(b + 3 * nb) * L + li], twq(3));
+                const Cplx<R> a = cadd(x0, x2), bb = csub(x0, x2), c = cadd(x1, x3), d = mul_mi(csub(x1, x3), sg);
+                out[ob * L + li] = cadd(a, c);
+                out[(ob + Ns) * L + li] = cadd(bb, d);
+                out[(ob + 2 * Ns) * L + li] = csub(a, c);
+                out[(ob + 3 * Ns) * L + li] = csub(bb, d);
+            } else if (r == 3) {
+                const R h = (R)0.86602540378443864676;  // sin(2 pi / 3)
+                const Cplx<R> x0 = in[b * L + li];
+                const Cplx<R> x1 = cmul(in[(b + nb) * L + li], twq(1));
+                const Cplx<R> x2 = cmul(in[(b + 2 * nb) * L + li], twq(2));
+                const Cplx<R> t1c = cadd(x1, x2);
+                const Cplx<R> t2 = Cplx<R>{x0.x - (R)0.5 * t1c.x, x0.y - (R)0.5 * t1c.y};
+                const Cplx<R> d = csub(x1, x2);
+                const Cplx<R> t3 = mul_mi(Cplx<R>{h * d.x, h * d.y}, sg);
+                out[ob * L + li] = cadd(x0, t1c);
+                out[(ob + Ns) * L + li] = cadd(t2, t3);
+                out[(ob + 2 * Ns) * L + li] = csub(t2, t3);
+            } else if (r == 5) {
+                const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+                const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+                const Cplx<R> x0 = in[b * L + li];
+                const Cplx<R> x1 = cmul(in[(b + nb) * L + li], twq(1));
+                const Cplx<R> x2 = cmul(in[(b + 2 * nb) * L + li], twq(2));
+                const Cplx<R> x3 = cmul(in[(b + 3 * nb) * L + li], twq(3));
+                const Cplx<R> x4 = cmul(in[(b + 4 * nb) * L + li], twq(4));
+                const Cplx<R> t1c = cadd(x1, x4), t2 = cadd(x2, x3), t3 = csub(x1, x4), t4 = csub(x2, x3);
+                const Cplx<R> a1 = Cplx<R>{x0.x + c1 * t1c.x + c2 * t2.x, x0.y + c1 * t1c.y + c2 * t2.y};
+                const Cplx<R> a2 = Cplx<R>{x0.x + c2 * t1c.x + c1 * t2.x, x0.y + c2 * t1c.y + c1 * t2.y};
+                const Cplx<R> b1 = mul_mi(Cplx<R>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, sg);
+                const Cplx<R> b2 = mul_mi(Cplx<R>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, sg);
+                out[ob * L + li] = Cplx<R>{x0.x + t1c.x + t2.x, x0.y + t1c.y + t2.y};
+                out[(ob + Ns) * L + li] = cadd(a1, b1);
+                out[(ob + 4 * Ns) * L + li] = csub(a1, b1);
+                out[(ob + 2 * Ns) * L + li] = cadd(a2, b2);
+                out[(ob + 3 * Ns) * L + li] = csub(a2, b2);
+            } else {
+                // direct r-point sum per output: y_k = sum_q x_q W_{Ns r}^{q kj} W_r^{q k}
+                const int kr = K / r;  // W_r = tw[K / r]
+                for (int k = 0; k < r; ++k) {
+                    R sx = 0, sy = 0;
+                    for (int q = 0; q < r; ++q) {
+                        int t = q * t1 + ((q * k) % r) * kr;
+                        t -= (t / K) * K;
+                        Cplx<R> w = tw[t];
+                        w.y *= sg;
+                        const Cplx<R> x = in[(b + q * nb) * L + li];
+                        sx += x.x * w.x - x.y * w.y;
+                        sy += x.x * w.y + x.y * w.x;
+                    }
+                    out[(ob + k * Ns) * L + li] = Cplx<R>{sx, sy};
+                }
+            }
+        }
+        __syncthreads();
+        Cplx<R>* t = in; in = out; out = t;
+        Ns *= r;
+    }
+    return in;
+}
+
+// block shape of the line passes
+constexpr int FL = 4;      // lines per block
+constexpr int FTH = 64;    // threads per block (16 butterfly slots per line)
+
+template <class R>
+__device__ inline void stage_tw(Cplx<R>* dst, const Cplx<R>* __restrict__ src, int K) {
+    for (int k = threadIdx.x; k < K; k += blockDim.x) dst[k] = src[k];
+}
+
+// real rows (K0 K1 of length K2) -> half spectrum (K0, K1, H); FL complex lines = 2 FL rows
+template <class R>
+__global__ void __launch_bounds__(FTH) k_fz_r2c(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ twg,
+                                               const R* __restrict__ rgrid, Cplx<R>* __restrict__ hgrid) {
+    extern __shared__ unsigned char smem_raw[];
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + FL * K2;
+    Cplx<R>* tw = b1 + FL * K2;
+    const int f = blockIdx.y, H = K2 / 2 + 1;
+    const int nrl = K0 * K1, npairs = (nrl + 1) / 2;
+    const int p0 = blockIdx.x * FL;
+    stage_tw(tw, twg, K2);
+    const R* rg = rgrid + (size_t)f * K0 * K1 * K2;
+    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
+#pragma unroll
+    for (int li = 0; li < FL; ++li) {
+        const int la = 2 * (p0 + li), lb = la + 1;
+        for (int e = threadIdx.x; e < K2; e += FTH) {
+            const R a = la < nrl ? rg[(size_t)la * K2 + e] : (R)0;
+            const R bb = lb < nrl ? rg[(size_t)lb * K2 + e] : (R)0;
+            b0[e * FL + li] = Cplx<R>{a, bb};
+        }
+    }
+    __syncthreads();
+    Cplx<R>* res = fft_lines<R, FL>(b0, b1, K2, plan, tw, 0);
+#pragma unroll
+    for (int li = 0; li < FL; ++li) {
+        if (p0 + li >= npairs) break;
+        const int la = 2 * (p0 + li), lb = la + 1;
+        for (int k = threadIdx.x; k < H; k += FTH) {
+            const Cplx<R> Z = res[k * FL + li];
+            const Cplx<R> Zc = res[((K2 - k) % K2) * FL + li];
+            hg[(size_t)la * H + k] = Cplx<R>{(Z.x + Zc.x) * (R)0.5, (Z.y - Zc.y) * (R)0.5};
+            if (lb < nrl) hg[(size_t)lb * H + k] = Cplx<R>{(Z.y + Zc.y) * (R)0.5, (Zc.x - Z.x) * (R)0.5};
+        }
+    }
+}
+
+// half spectrum -> real rows (inverse, unnormalised)
+template <class R>
+__global__ void __launch_bounds__(FTH) k_fz_c2r(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ twg,
+                                               const Cplx<R>* __restrict__ hgrid, R* __restrict__ rgrid) {
+    extern __shared__ unsigned char smem_raw[];
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + FL * K2;
+    Cplx<R>* tw = b1 + FL * K2;
+    const int f = blockIdx.y, H = K2 / 2 + 1;
+    const int nrl = K0 * K1, npairs = (nrl + 1) / 2;
+    const int p0 = blockIdx.x * FL;
+    stage_tw(tw, twg, K2);
+    R* rg = rgrid + (size_t)f * K0 * K1 * K2;
+    const Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
+#pragma unroll
+    for (int li = 0; li < FL; ++li) {
+        const int la = 2 * (p0 + li), lb = la + 1;
+        for (int k = threadIdx.x; k < K2; k += FTH) {
+            Cplx<R> A = {0, 0}, B = {0, 0};
+            const int kk = k < H ? k : K2 - k;
+            if (la < nrl) A = hg[(size_t)la * H + kk];
+            if (lb < nrl) B = hg[(size_t)lb * H + kk];
+            if (k >= H) { A.y = -A.y; B.y = -B.y; }
+            b0[k * FL + li] = Cplx<R>{A.x - B.y, A.y + B.x};  // Z = A + i B
+        }
+    }
+    __syncthreads();
+    Cplx<R>* res = fft_lines<R, FL>(b0, b1, K2, plan, tw, 1);
+#pragma unroll
+    for (int li = 0; li < FL; ++li) {
+        if (p0 + li >= npairs) break;
+        const int la = 2 * (p0 + li), lb = la + 1;
+        for (int e = threadIdx.x; e < K2; e += FTH) {
+            const Cplx<R> z = res[e * FL + li];
+            rg[(size_t)la * K2 + e] = z.x;
+            if (lb < nrl) rg[(size_t)lb * K2 + e] = z.y;
+        }
+    }
+}
+
+// complex lines along y of the half spectrum: lines (x, kz), stride H
+template <class R>
+__global__ void __launch_bounds__(FTH) k_fy(int K0, int K1, int H, FFTPlan plan, const Cplx<R>* __restrict__ twg,
+                                           Cplx<R>* __restrict__ hgrid, int inverse) {
+    extern __shared__ unsigned char smem_raw[];
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + FL * K1;
+    Cplx<R>* tw = b1 + FL * K1;
+    const int f = blockIdx.y;
+    const int nlines = K0 * H;
+    const int l0 = blockIdx.x * FL;
+    stage_tw(tw, twg, K1);
+    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
+    // thread -> (element e, line li) with li fastest: consecutive kz in memory
+    for (int idx = threadIdx.x; idx < FL * K1; idx += FTH) {
+        const int e = idx / FL, li = idx % FL;
+        const int l = l0 + li;
+        Cplx<R> v = {0, 0};
+        if (l < nlines) v = hg[(size_t)(l / H) * K1 * H + (l % H) + (size_t)e * H];
+        b0[e * FL + li] = v;
+    }
+    __syncthreads();
+    Cplx<R>* res = fft_lines<R, FL>(b0, b1, K1, plan, tw, inverse);
+    for (int idx = threadIdx.x; idx < FL * K1; idx += FTH) {
+        const int e = idx / FL, li = idx % FL;
+        const int l = l0 + li;
+        if (l < nlines) hg[(size_t)(l / H) * K1 * H + (l % H) + (size_t)e * H] = res[e * FL + li];
+    }
+}
+
+// forward x FFT, influence function, inverse x FFT; lines (y, kz), stride K1*H.
+// ACC: accumulate energy and box-derivative partials per block (multiplicity 2
+// for 0 < kz < K2/2, the conjugate half).
+template <class R, bool ACC>
+__global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ twg,
+                                                Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
+                                                const double* __restrict__ bmod0, const double* __restrict__ bmod1,
+                                                const double* __restrict__ bmod2, double kappa, int kind, double pref,
+                                                double* __restrict__ part) {
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ double sh[16];
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + FL * K0;
+    Cplx<R>* tw = b1 + FL * K0;
+    const int f = blockIdx.y;
+    const int H = K2 / 2 + 1;
+    const int nlines = K1 * H;
+    const int l0 = blockIdx.x * FL;
+    stage_tw(tw, twg, K0);
+    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
+    const size_t stride = (size_t)K1 * H;
+    for (int idx = threadIdx.x; idx < FL * K0; idx += FTH) {
+        const int e = idx / FL, li = idx % FL;
+        const int l = l0 + li;
+        b0[e * FL + li] = l < nlines ? hg[(size_t)l + (size_t)e * stride] : Cplx<R>{0, 0};
+    }
+    __syncthreads();
+    Cplx<R>* res = fft_lines<R, FL>(b0, b1, K0, plan, tw, 0);
+    const BoxInfo& b = boxes[f];
+    double e = 0, w00 = 0, w11 = 0, w22 = 0, w01 = 0, w02 = 0, w12 = 0;
+    for (int idx = threadIdx.x; idx < FL * K0; idx += FTH) {
+        const int i0 = idx / FL, li = idx % FL;
+        const int l = l0 + li;
+        if (l >= nlines) continue;
+        const int i1 = l / H, i2 = l % H;
+        const int m0 = i0 < (K0 + 1) / 2 ? i0 : i0 - K0;
+        const int m1 = i1 < (K1 + 1) / 2 ? i1 : i1 - K1;
+        const int m2 = i2 < (K2 + 1) / 2 ? i2 : i2 - K2;
+        const double kx = b.kb[0] * m0 + b.kb[1] * m1 + b.kb[2] * m2;
+        const double ky = b.kb[3] * m0 + b.kb[4] * m1 + b.kb[5] * m2;
+        const double kz = b.kb[6] * m0 + b.kb[7] * m1 + b.kb[8] * m2;
+        const double ksq = kx * kx + ky * ky + kz * kz;
+        const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
+        const double ith2 = 1.0 / (th * th);
+        const Cplx<R> S = res[i0 * FL + li];
+        double C = 0, dC = 0;
+        const double V = b.V;
+        if (kind == 0) {
+            if (!(i0 == 0 && i1 == 0 && i2 == 0)) {
+                C = 2.0 * PI / V / ksq * exp(-ksq / (4.0 * kappa * kappa));
+                dC = -C * (1.0 / ksq + 1.0 / (4.0 * kappa * kappa));
+            }
+        } else {
+            const double kq = ksq < 1e-16 ? 1e-16 : ksq;
+            const double x2 = kq / (4.0 * kappa * kappa), x = sqrt(x2);
+            const double ex = exp(-x2), ec = erfc(x);
+            double fv, fd, pw;
+            if (kind == 6) {
+                fv = (1.0 - 2.0 * x2) * ex + 2.0 * x2 * x * SQRT_PI * ec;
+                fd = -6.0 * x * ex + 6.0 * x2 * SQRT_PI * ec;
+                pw = kappa * kappa * kappa / 3.0;
+            } else if (kind == 8) {
+                fv = (3.0 - 2.0 * x2 + 4.0 * x2 * x2) * ex - 4.0 * x2 * x2 * x * SQRT_PI * ec;
+                fd = (-10.0 * x + 20.0 * x2 * x) * ex - 20.0 * x2 * x2 * SQRT_PI * ec;
+                pw = kappa * kappa * kappa * kappa * kappa / 45.0;
+            } else {
+                const double x4 = x2 * x2;
+                fv = (15.0 - 6.0 * x2 + 4.0 * x4 - 8.0 * x4 * x2) * ex + 8.0 * x4 * x2 * x * SQRT_PI * ec;
+                fd = (-42.0 * x + 28.0 * x2 * x - 56.0 * x4 * x) * ex + 56.0 * x4 * x2 * SQRT_PI * ec;
+                pw = pow(kappa, 7.0) / 1260.0;
+            }
+            const double c0 = SQRT_PI * PI / 2.0 / V * pw;
+            C = c0 * fv;
+            dC = ksq < 1e-16 ? 0.0 : c0 * fd / (8.0 * kappa * kappa * x);
+        }
+        if (ACC) {
+            const double s2 = (double)S.x * (double)S.x + (double)S.y * (double)S.y;
+            const double mult = (i2 == 0 || 2 * i2 == K2) ? 1.0 : 2.0;
+            const double wgt = pref * s2 * ith2;
+            e += mult * wgt * C;
+            const double wd = -mult * wgt * dC;
+            w00 += wd * kx * kx; w11 += wd * ky * ky; w22 += wd * kz * kz;
+            w01 += wd * kx * ky; w02 += wd * kx * kz; w12 += wd * ky * kz;
+        }
+        const double scale = 2.0 * pref * C * ith2;
+        res[i0 * FL + li] = Cplx<R>{(R)(S.x * scale), (R)(S.y * scale)};
+    }
+    __syncthreads();
+    Cplx<R>* other = (res == b0) ? b1 : b0;
+    Cplx<R>* res2 = fft_lines<R, FL>(res, other, K0, plan, tw, 1);
+    for (int idx = threadIdx.x; idx < FL * K0; idx += FTH) {
+        const int el = idx / FL, li = idx % FL;
+        const int l = l0 + li;
+        if (l < nlines) hg[(size_t)l + (size_t)el * stride] = res2[el * FL + li];
+    }
+    if (!ACC) return;
+    double vals[7] = {e, w00, w11, w22, w01, w02, w12};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        double tsum = block_sum(vals[k], sh);
+        if (threadIdx.x == 0) part[((size_t)f * gridDim.x + blockIdx.x) * 7 + k] = tsum;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plane kernels: one block per x-plane.  Forward: real (K1, K2) rows -> z R2C
+// (rows packed in pairs as complex lines) -> y C2C on the H columns -> half
+// spectrum plane (K1, H).  Inverse: the reverse.  Everything between the one
+// read and the one write of the plane stays in LDS; twiddles are staged too.
+// LDS: 2 S + K1 + K2 complex, S = max(ceil(K1/2) K2, K1 H).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int plane_S(int K1, int K2) {
+    const int a = ((K1 + 1) / 2) * K2, b = K1 * (K2 / 2 + 1);
+    return a > b ? a : b;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_fft_plane_fwd(int K0, int K1, int K2, FFTPlan pz, FFTPlan py,
+                                                       const Cplx<R>* __restrict__ twz, const Cplx<R>* __restrict__ twy,
+                                                       const R* __restrict__ rgrid, Cplx<R>* __restrict__ hgrid) {
+    extern __shared__ unsigned char smem_raw[];
+    const int H = K2 / 2 + 1, np = (K1 + 1) / 2, S = plane_S(K1, K2);
+    Cplx<R>* A = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* B = A + S;
+    Cplx<R>* tz = B + S;
+    Cplx<R>* ty = tz + K2;
+    const int x = blockIdx.x, f = blockIdx.y;
+    for (int k = threadIdx.x; k < K2; k += blockDim.x) tz[k] = twz[k];
+    for (int k = threadIdx.x; k < K1; k += blockDim.x) ty[k] = twy[k];
+    const R* rp = rgrid + ((size_t)f * K0 + x) * K1 * K2;
+    for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
+        const int li = idx / K2, e = idx - li * K2;
+        const int ya = 2 * li, yb = ya + 1;
+        const R va = rp[(size_t)ya * K2 + e];
+        const R vb = yb < K1 ? rp[(size_t)yb * K2 + e] : (R)0;
+        A[e * np + li] = Cplx<R>{va, vb};
+    }
+    __syncthreads();
+    Cplx<R>* res = stockham<R>(A, B, K2, np, np, pz, tz, 0);
+    Cplx<R>* oth = res == A ? B : A;
+    for (int idx = threadIdx.x; idx < np * H; idx += blockDim.x) {
+        const int li = idx / H, k = idx - li * H;
+        const int ya = 2 * li, yb = ya + 1;
+        const Cplx<R> Z = res[k * np + li];
+        const Cplx<R> Zc = res[((K2 - k) % K2) * np + li];
+        oth[ya * H + k] = Cplx<R>{(Z.x + Zc.x) * (R)0.5, (Z.y - Zc.y) * (R)0.5};
+        if (yb < K1) oth[yb * H + k] = Cplx<R>{(Z.y + Zc.y) * (R)0.5, (Zc.x - Z.x) * (R)0.5};
+    }
+    __syncthreads();
+    Cplx<R>* res2 = stockham<R>(oth, res, K1, H, H, py, ty, 0);
+    Cplx<R>* hp = hgrid + ((size_t)f * K0 + x) * K1 * H;
+    for (int idx = threadIdx.x; idx < K1 * H; idx += blockDim.x) hp[idx] = res2[idx];
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_fft_plane_inv(int K0, int K1, int K2, FFTPlan pz, FFTPlan py,
+                                                       const Cplx<R>* __restrict__ twz, const Cplx<R>* __restrict__ twy,
+                                                       const Cplx<R>* __restrict__ hgrid, R* __restrict__ rgrid) {
+    extern __shared__ unsigned char smem_raw[];
+    const int H = K2 / 2 + 1, np = (K1 + 1) / 2, S = plane_S(K1, K2);
+    Cplx<R>* A = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* B = A + S;
+    Cplx<R>* tz = B + S;
+    Cplx<R>* ty = tz + K2;
+    const int x = blockIdx.x, f = blockIdx.y;
+    for (int k = threadIdx.x; k < K2; k += blockDim.x) tz[k] = twz[k];
+    for (int k = threadIdx.x; k < K1; k += blockDim.x) ty[k] = twy[k];
+    const Cplx<R>* hp = hgrid + ((size_t)f * K0 + x) * K1 * H;
+    for (int idx = threadIdx.x; idx < K1 * H; idx += blockDim.x) A[idx] = hp[idx];
+    __syncthreads();
+    Cplx<R>* res = stockham<R>(A, B, K1, H, H, py, ty, 1);  // plane layout [y][kz]
+    Cplx<R>* oth = res == A ? B : A;
+    for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
+        const int li = idx / K2, k = idx - li * K2;
+        const int ya = 2 * li, yb = ya + 1;
+        Cplx<R> a, bq;
+        if (k < H) {
+            a = res[ya * H + k];
+            bq = yb < K1 ? res[yb * H + k] : Cplx<R>{0, 0};
+        } else {
+            const Cplx<R> ac = res[ya * H + (K2 - k)];
+            const Cplx<R> bc = yb < K1 ? res[yb * H + (K2 - k)] : Cplx<R>{0, 0};
+            a = Cplx<R>{ac.x, -ac.y};
+            bq = Cplx<R>{bc.x, -bc.y};
+        }
+        oth[k * np + li] = Cplx<R>{a.x - bq.y, a.y + bq.x};
+    }
+    __syncthreads();
+    Cplx<R>* res2 = stockham<R>(oth, res, K2, np, np, pz, tz, 1);
+    R* rp = rgrid + ((size_t)f * K0 + x) * K1 * K2;
+    for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
+        const int li = idx / K2, e = idx - li * K2;
+        const int ya = 2 * li, yb = ya + 1;
+        const Cplx<R> z = res2[e * np + li];
+        rp[(size_t)ya * K2 + e] = z.x;
+        if (yb < K1) rp[(size_t)yb * K2 + e] = z.y;
+    }
+}
+
 // complex lines along y of the half spectrum: lines (x, kz), stride H
 template <class R>
 __global__ void k_fft_y(int K0, int K1, int H, FFTPlan plan, const Cplx<R>* __restrict__ tw,
@@ -358,7 +941,7 @@ __global__ void k_fft_y(int K0, int K1, int H, FFTPlan plan, const Cplx<R>* __re
 
 // forward x FFT, influence function, inverse x FFT; lines (y, kz), stride K1*H.
 // Energy/box sums: multiplicity 2 for 0 < kz < K2/2 (the conjugate half), else 1.
-template <class R>
+template <class R, bool ACC>
 __global__ void k_fft_x_infl(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ tw,
                              Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
                              const double* __restrict__ bmod0, const double* __restrict__ bmod1,
@@ -430,11 +1013,13 @@ __global__ void k_fft_x_infl(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>
                 C = c0 * fv;
                 dC = ksq < 1e-16 ? 0.0 : c0 * fd / (8.0 * kappa * kappa * x);
             }
-            const double wgt = pref * s2 * ith2;
-            e += mult * wgt * C;
-            const double wd = -mult * wgt * dC;
-            w00 += wd * kx * kx; w11 += wd * ky * ky; w22 += wd * kz * kz;
-            w01 += wd * kx * ky; w02 += wd * kx * kz; w12 += wd * ky * kz;
+            if (ACC) {
+                const double wgt = pref * s2 * ith2;
+                e += mult * wgt * C;
+                const double wd = -mult * wgt * dC;
+                w00 += wd * kx * kx; w11 += wd * ky * ky; w22 += wd * kz * kz;
+                w01 += wd * kx * ky; w02 += wd * kx * kz; w12 += wd * ky * kz;
+            }
             const double scale = 2.0 * pref * C * ith2;
             res[i0 * LB + li] = Cplx<R>{(R)(S.x * scale), (R)(S.y * scale)};
         }
@@ -446,6 +1031,7 @@ __global__ void k_fft_x_infl(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>
             hg[(size_t)(l0 + li) + (size_t)el * stride] = res2[el * LB + li];
         }
     }
+    if (!ACC) return;
     double vals[7] = {e, w00, w11, w22, w01, w02, w12};
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
