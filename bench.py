@@ -145,6 +145,127 @@ def cpu_baseline(D, n_iter):
                        f"({n_iter}+1) dE/dU passes + E+grad pass = {t_eval:.1f} s")}
 
 
+def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005):
+    """64 perturbed frames of the 12,288-atom system in one batched call (trajectory fitting)."""
+    import numpy as np
+    import torch
+    from dmff_amd.admp.engine import Engine
+    S = D["S"]
+    rng = np.random.default_rng(1002)
+    base = S.positions_nm * 10.0
+    frames = np.stack([base + rng.normal(0.0, sigma_nm * 10.0, base.shape) for _ in range(n_frames)])
+    pos = torch.as_tensor(frames, dtype=torch.float64, device=D["dev"])
+    box = torch.as_tensor(np.stack([S.box_nm * 10.0] * n_frames), dtype=torch.float64, device=D["dev"])
+    eng = Engine(S.n_atoms, 2, True, True, D["kappa"], D["K"], D["rc"], n_frames=n_frames, precision=precision,
+                 device=D["dev"])
+    eng.set_topology(S.axis_types, S.axis_idx, S.bonded)
+    N = S.n_atoms
+
+    def ev():
+        eng.build_neighbors(pos, box)
+        E, g, U, info = eng.compute(pos, box, D["Q"], D["pol"], D["thole"], S.mScales, S.pScales, S.dScales,
+                                    solver="pcg", tol=1e-6, maxiter=500, want_grad=True)
+        # per frame, per type parameter gradients (one GEMM each)
+        dQ = (D["qtype"].W @ g["Q"].transpose(0, 1).reshape(N, -1)).reshape(-1, n_frames, g["Q"].shape[-1])
+        dpol = D["ptype"].W @ g["pol"].T
+        dth = D["ptype"].W @ g["thole"].T
+        return E, g, info, dQ, dpol, dth
+
+    for _ in range(warmup):
+        ev()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ev()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    info = out[2]
+    del eng
+    torch.cuda.empty_cache()
+    return {"frames": n_frames, "ms_per_batch": dt * 1e3, "evals_per_s": n_frames / dt,
+            "solver_iters_max": int(max(info["n_iter"][:min(n_frames, 64)])),
+            "residual_max": float(max(info["residual"][:min(n_frames, 64)]))}
+
+
+def water_dispersion_slater(S, seed):
+    """Per-atom sqrt(C6, C8, C10) (A units) with +-10% jitter; per-type Slater A (kJ/mol), B (1/A)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    C = np.array([[2.0e-3, 1.0e-4, 6.0e-6], [1.0e-4, 3.0e-6, 1.0e-7]])  # kJ/mol nm^p, O and H
+    t = S.map_atomtype
+    Ca = C[t] * rng.uniform(0.9, 1.1, size=(S.n_atoms, 3))
+    Ca = Ca * np.array([1e6, 1e8, 1e10])  # nm^p -> A^p
+    A_t = rng.uniform(1e4, 1e5, size=2)
+    B_t = rng.uniform(30.0, 45.0, size=2) / 10.0  # 1/nm -> 1/A
+    return np.sqrt(Ca), A_t[t], B_t[t]
+
+
+def leg_large(steps=2, warmup=1, precision="mixed"):
+    """32,768 waters (98,304 atoms), rc 10 A: electrostatics + polarisation + dispersion PME (pmax 10) + Slater."""
+    import numpy as np
+    import torch
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    from dmff_amd.admp.water import config_system
+    S, c = config_system(3)
+    dev = "cuda:0"
+    box_np = S.box_nm * 10.0
+    rc = c["rc"] * 10.0
+    kappa, K1, K2, K3 = setup_ewald_parameters(rc, 5e-4, box_np)
+    cl, a, b = water_dispersion_slater(S, 1003)
+    T = lambda x: torch.as_tensor(np.asarray(x), dtype=torch.float64, device=dev)
+    pos, box = T(S.positions_nm * 10.0), T(box_np)
+    Q, pol, th, cl, a, b = T(S.Q_local), T(S.pol), T(S.thole), T(cl), T(a), T(b)
+    eng = Engine(S.n_atoms, 2, True, True, kappa, (K1, K2, K3), rc, precision=precision, disp_kappa=kappa,
+                 dK=(K1, K2, K3), disp_pmax=10, device=dev)
+    eng.set_topology(S.axis_types, S.axis_idx, S.bonded)
+
+    def ev():
+        eng.build_neighbors(pos, box)
+        E, g, U, info = eng.compute(pos, box, Q, pol, th, S.mScales, S.pScales, S.dScales, solver="pcg", tol=1e-6,
+                                    maxiter=500, want_grad=True)
+        Ed, gd = eng.disp_compute(pos, box, cl, S.mScales)
+        Es, gs = eng.slater_compute(pos, box, a, b, S.mScales)
+        return E, Ed, Es, info
+
+    for _ in range(warmup):
+        ev()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        E, Ed, Es, info = ev()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    del eng
+    torch.cuda.empty_cache()
+    return {"n_atoms": S.n_atoms, "grid": [K1, K2, K3], "rc_A": rc, "precision": precision,
+            "ms_per_eval": dt * 1e3, "evals_per_s": 1.0 / dt, "solver_iters": int(info["n_iter"][0]),
+            "E_elec": float(E[0]), "E_disp_backend": float(Ed[0]), "E_slater": float(Es[0])}
+
+
+def leg_serving(stream, precision, slots=4):
+    """Poisson request stream through the continuous-batching server; latency per request."""
+    import torch
+    from dmff_amd.admp.serving import ADMPServer, Request, latency_summary
+    srv = ADMPServer(device="cuda:0", precision=precision, slots=slots, tol=1e-6)
+    futs = []
+    t0 = time.perf_counter()
+    for rid, (t, spec, pos, box) in enumerate(stream):
+        now = time.perf_counter() - t0
+        if t > now:
+            time.sleep(t - now)
+        futs.append(srv.submit(Request(rid, spec, pos, box, arrival=t0 + t)))
+    res = [f.result(timeout=600) for f in futs]
+    span = time.perf_counter() - t0
+    srv.close()
+    del srv
+    torch.cuda.empty_cache()
+    out = latency_summary(res)
+    out.update({"evals_per_s": len(res) / span, "slots": slots, "precision": precision,
+                "warm_frames": int(sum(r.warm for r in res))})
+    return out, res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +274,9 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-json", default=None)
+    ap.add_argument("--legs", default="batch,large,serving",
+                    help="extra measurements beside the headline: batch (64 frames), large (98,304 atoms with "
+                         "dispersion and Slater), serving (request stream, p50/p99); 'none' for the headline only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); the ADMP path here is a "
@@ -167,6 +291,8 @@ def main():
     for prec in ("fp64", "mixed"):
         eng = make_engine(D, prec)
         dt, out, prof = time_path(eng, D, args.steps, args.warmup)
+        if "n_pairs" not in D:
+            D["n_pairs"] = int(eng.get_pairs(0).shape[0])
         E, g, U, info = out[:4]
         res[prec] = dict(dt=dt, E=float(E[0]), g=g, info=info, prof=prof, dQ=out[4])
         log(f"{prec}: {dt * 1e3:.2f} ms/eval, E={float(E[0]):.8f}, iters={info['n_iter'][0]}, "
@@ -212,6 +338,32 @@ def main():
         "kernels_ms_per_eval": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
         "kernel_us_per_launch": {k: v[0] / v[1] * 1e3 for k, v in kprof.items() if v[1]},
     }
+    legs = [x for x in args.legs.split(",") if x and x != "none"]
+    if "batch" in legs:
+        b = {}
+        for prec in ("mixed", "fp64"):
+            b[prec] = leg_batch(D, prec)
+            log(f"batch64 {prec}: {b[prec]['ms_per_batch']:.1f} ms/batch, {b[prec]['evals_per_s']:.1f} evals/s")
+        out["batch64"] = b
+    if "large" in legs:
+        out["large"] = leg_large()
+        log(f"large: {out['large']['ms_per_eval']:.1f} ms/eval ({out['large']['n_atoms']} atoms)")
+    if "serving" in legs:
+        from dmff_amd.admp.serving import make_stream
+        stream = make_stream(32, 20.0, 1004)
+        sv = {}
+        res = {}
+        for prec in ("mixed", "fp64"):
+            sv[prec], res[prec] = leg_serving(stream, prec)
+            log(f"serving {prec}: p50 {sv[prec]['p50_ms']:.1f} ms, p99 {sv[prec]['p99_ms']:.1f} ms, "
+                f"{sv[prec]['evals_per_s']:.1f} evals/s")
+        import torch as _t
+        errs = [float(_t.linalg.norm(a.dpos - f.dpos) / _t.linalg.norm(f.dpos))
+                for a, f in zip(res["mixed"], res["fp64"])]
+        sv["mixed_force_rel_rms_max"] = max(errs)
+        sv["stream"] = ("32 requests, Poisson arrivals at 20/s, water boxes of 64-8192 molecules (log-uniform), "
+                        "30% follow-on frames of a live system, 4 slots, solve to 1e-6")
+        out["serving"] = sv
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(D, f64["info"]["n_iter"][0])
     else:
@@ -220,7 +372,16 @@ def main():
 
 
 def roofline(name, tm, D, prec):
-    """Algorithmic work per launch / measured average launch duration."""
+    """Algorithmic bytes per launch of a kernel family / its measured average launch duration.
+
+    Per-launch algorithmic bytes (w = 4 B in the mixed path, 8 B in FP64; G = K0 K1 K2,
+    Gh = K0 K1 (K2/2 + 1); solver launches, which dominate, spread and gather dipoles only):
+      spread     N (3*8 + 4 w) + G w           positions, fractional charge+dipole, grid written once
+      fft        (2 G w + 12 Gh w) / 4         z R2C, y, y, z C2R passes of one transform pair
+      influence  4 Gh w                        x pass: read + write the half spectrum (fused influence)
+      gather     G w + N 3*8 + N 3*8           grid, positions, field out
+      real_matvec 2P (4 + 2 w) + N 3 (w + 8)   ELL index + two stored coefficients per ordered pair
+    """
     ms, n = tm
     t = ms / n / 1e3
     S = D["S"]
@@ -228,21 +389,23 @@ def roofline(name, tm, D, prec):
     w = 4 if prec == "mixed" else 8
     K = D["K"]
     G = K[0] * K[1] * K[2]
-    if name in ("fft",):
-        # one pass over the grid: read + write G complex values
-        bytes_ = 2 * G * 2 * w
-        return {"kernel": "k_fft_pass", "bound": "hbm", "achieved": bytes_ / t / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": bytes_ / t / 1e9 / HBM_PEAK_GBS, "traffic": None}
-    if name in ("spread",):
-        bytes_ = N * (3 * 8 + 10 * w) + G * w
-        return {"kernel": "k_spread_atomic", "bound": "hbm", "achieved": bytes_ / t / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": bytes_ / t / 1e9 / HBM_PEAK_GBS, "traffic": None}
-    if name in ("gather",):
-        bytes_ = G * w + N * (3 * 8 + 10 * w) + N * 3 * 8
-        return {"kernel": "k_gather", "bound": "hbm", "achieved": bytes_ / t / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": bytes_ / t / 1e9 / HBM_PEAK_GBS, "traffic": None}
-    return {"kernel": name, "bound": "unknown", "achieved": None, "peak": None, "unit": None, "frac": None,
-            "traffic": None, "avg_launch_us": t * 1e6}
+    Gh = K[0] * K[1] * (K[2] // 2 + 1)
+    P = D.get("n_pairs", 0)
+    table = {
+        "spread": ("k_spread_list", N * (3 * 8 + 4 * w) + G * w),
+        "fft": ("k_fz_r2c/k_fy/k_fz_c2r", (2 * G * w + 12 * Gh * w) / 4),
+        "influence": ("k_fx_infl", 4 * Gh * w),
+        "gather": ("k_gather_ml", G * w + N * 3 * 8 + N * 3 * 8),
+        "real_matvec": ("k_real_matvec", 2 * P * (4 + 2 * w) + N * 3 * (w + 8)),
+    }
+    if name not in table or (name == "real_matvec" and not P):
+        return {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
+                "traffic": None, "avg_launch_us": t * 1e6}
+    kname, bytes_ = table[name]
+    ach = bytes_ / t / 1e9
+    return {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": t * 1e6,
+            "algorithmic_bytes_per_launch": bytes_}
 
 
 if __name__ == "__main__":

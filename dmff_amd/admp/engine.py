@@ -64,6 +64,12 @@ class Engine:
     def _stream(self):
         self.lib.admp_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
 
+    def memory_bytes(self):
+        """Device bytes held by the native context."""
+        n = ctypes.c_int64()
+        L.check(self.h, self.lib.admp_memory_bytes(self.h, ctypes.byref(n)), "admp_memory_bytes")
+        return int(n.value)
+
     def profile(self, enable=True, reset=True):
         L.check(self.h, self.lib.admp_profile(self.h, int(enable), int(reset)), "admp_profile")
 

@@ -94,6 +94,7 @@ struct admp_ctx {
     double *d_swfrac = nullptr, *d_sfq = nullptr;  // solver spline coordinates / fractional dipoles (R-typed)
     int pcg_nb = 0;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;  // parallel to allocs
     // captured PCG iteration chunks (replayed until the frames converge)
     struct ChunkGraph {
         hipGraphExec_t exec = nullptr;
@@ -182,9 +183,11 @@ static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1
 
 template <class T> static hipError_t dalloc(admp_ctx* ctx, T** p, size_t n) {
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
     if (e == hipSuccess) {
         ctx->allocs.push_back(q);
+        ctx->alloc_bytes.push_back(bytes);
         *p = reinterpret_cast<T*>(q);
     }
     return e;
@@ -193,7 +196,10 @@ template <class T> static hipError_t dalloc(admp_ctx* ctx, T** p, size_t n) {
 static void dfree(admp_ctx* ctx, void* p) {
     if (!p) return;
     auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), p);
-    if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+    if (it != ctx->allocs.end()) {
+        ctx->alloc_bytes.erase(ctx->alloc_bytes.begin() + (it - ctx->allocs.begin()));
+        ctx->allocs.erase(it);
+    }
     (void)hipFree(p);
 }
 
@@ -1548,4 +1554,12 @@ extern "C" admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, con
                                            const double* b, const double* mScales, int32_t n_scales, int32_t want_grad,
                                            double* energy, double* dpos, double* dbox, double* da, double* db) {
     ADMP_WRAP(admp_slater_compute_impl(ctx, pos, box, a, b, mScales, n_scales, want_grad, energy, dpos, dbox, da, db));
+}
+
+extern "C" admp_status admp_memory_bytes(const admp_ctx* ctx, int64_t* bytes) {
+    if (!ctx || !bytes) return ADMP_ERR_ARG;
+    int64_t t = 0;
+    for (size_t b : ctx->alloc_bytes) t += (int64_t)b;
+    *bytes = t;
+    return ADMP_OK;
 }

@@ -866,7 +866,9 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_pcg_dir(int N, int nb, int nx, in
         s.beta = beta;
         st[f] = s;
     }
-    if (done) return;
+    // p follows every residual update (also the one that reaches tol), so that a
+    // later true-residual replacement resumes from a direction consistent with r
+    if (!first && res_prev < tol) return;
     const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
     if (i >= N) return;
     const size_t a = (size_t)f * N + i;

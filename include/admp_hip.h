@@ -170,6 +170,11 @@ admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* 
 admp_status admp_profile(admp_ctx* ctx, int32_t enable, int32_t reset);
 admp_status admp_profile_get(admp_ctx* ctx, int32_t key, double* total_ms, int64_t* count, const char** name);
 
+/* Instrumentation (no counterpart in the reference): device bytes currently held
+ * by the context (grids, lists, scratch).  Serving uses it to admit requests by
+ * HBM footprint.  Host-only; no GPU work. */
+admp_status admp_memory_bytes(const admp_ctx* ctx, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -299,3 +299,35 @@ def test_batched_frames_equal_single():
                           solver="pcg", tol=1e-8, device=DEV)
         e1 = f1.get_energy(frames[k], box, pairs, Q, pol, th, *args)
         assert abs(float(Eb[k]) - float(e1)) < 1e-9 * abs(float(e1))
+
+
+def test_mixed_batched_frames_converge():
+    """Mixed precision on a batch: frames converge at different iterations and the
+    true-residual replacements resume them; every frame must reach the FP64
+    residual tolerance and agree with the FP64 batch."""
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    s, pos, box = _small_waters()
+    rng = np.random.default_rng(11)
+    F = 4
+    frames = np.stack([pos + rng.normal(0, 0.02 * (k + 1), pos.shape) for k in range(F)])
+    kappa, K1, K2, K3 = setup_ewald_parameters(6.0, 5e-4, box)
+    Q = s.pme.Q_local[s.map_atomtype]
+    pol = s.pme.pol[s.map_poltype]
+    th = s.pme.thole[s.map_poltype]
+    out = {}
+    for prec in ("mixed", "fp64"):
+        eng = Engine(len(pos), 2, True, True, kappa, (K1, K2, K3), 6.0, n_frames=F, precision=prec, device=DEV)
+        eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
+        eng.build_neighbors(frames, np.stack([box] * F))
+        E, g, U, info = eng.compute(frames, np.stack([box] * F), Q, pol, th, s.pme.mScales, s.pme.pScales,
+                                    s.pme.dScales, solver="pcg", tol=1e-6, maxiter=400)
+        print(prec, info["n_iter"], info["residual"], flush=True)
+        assert all(r < 1e-6 for r in info["residual"]), info
+        assert max(info["n_iter"]) < 200
+        out[prec] = (E.cpu().numpy(), g["pos"].cpu().numpy())
+    Em, Gm = out["mixed"]
+    Ed, Gd = out["fp64"]
+    assert np.all(np.abs(Em - Ed) < 2e-5 * np.abs(Ed))
+    for k in range(F):
+        assert np.linalg.norm(Gm[k] - Gd[k]) / np.linalg.norm(Gd[k]) < 1e-4
