@@ -35,7 +35,7 @@ class Config(ctypes.Structure):
 
 class SolverOpts(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("maxiter", ctypes.c_int32), ("steps", ctypes.c_int32),
-                ("tol", ctypes.c_double), ("warm_start", ctypes.c_int32)]
+                ("tol", ctypes.c_double), ("warm_start", ctypes.c_int32), ("fp32_products", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "admp_common.h"
@@ -729,7 +730,10 @@ static admp_status admp_get_pairs_impl(admp_ctx* ctx, int32_t frame, int32_t* ou
 // counting sort of sites by base grid point; it depends only on positions and
 // the box, so it runs once per evaluation and serves every spread of the solve.
 // ---------------------------------------------------------------------------
-static bool small_grid(const Grid& g) { return g.K[0] < 6 || g.K[1] < 6 || g.K[2] < 6; }
+// The brick spread keeps one modular representative of each site's support start
+// relative to the brick, which covers every point of the brick only when each
+// axis has at least brick + 5 points; smaller grids take the scatter path.
+static bool small_grid(const Grid& g) { return g.K[0] < BRX + 5 || g.K[1] < BRY + 5 || g.K[2] < BRZ + 5; }
 
 static admp_status recip_positions(admp_ctx* ctx, const Grid& g, int use_dA) {
     const int N = ctx->N, F = ctx->F;
@@ -942,12 +946,15 @@ template <class RC, class R>
 static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
-    const R* vR = sizeof(R) == 8 ? reinterpret_cast<const R*>(ctx->d_p) : reinterpret_cast<const R*>(ctx->d_vR);
+    // real-space products in FP64 whenever the stored coefficients are FP64 (the
+    // near-field coupling sets the conditioning); R only for the reciprocal chain
+    using RR = typename std::conditional<sizeof(RC) == 8, double, R>::type;
+    const RR* vR = sizeof(RR) == 8 ? reinterpret_cast<const RR*>(ctx->d_p) : reinterpret_cast<const RR*>(ctx->d_vR);
     // the real-space product runs on the side stream, beside the reciprocal chain
     HIPCHK(hipEventRecord(ctx->ev_fork, st));
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
     { ProfScope ps_(ctx, 2, ctx->side);
-    hipLaunchKernelGGL((k_real_matvec<RC, R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0,
+    hipLaunchKernelGGL((k_real_matvec<RC, RR>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0,
                        ctx->side, N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
                        reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1); }
     HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
@@ -968,12 +975,24 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     return ADMP_OK;
 }
 
+// k_pcg_dir with products of type R (p copy for the pair kernel, spline
+// coordinates and fractional dipoles typed R)
+template <class R>
+static void launch_dir(admp_ctx* ctx, int nx, int mode, double tol) {
+    hipLaunchKernelGGL(k_pcg_dir<R>, dim3(ctx->pcg_nb, ctx->F), dim3(PCG_BLOCK), 0, ctx->stream, ctx->N, ctx->pcg_nb,
+                       nx, mode, tol, ctx->d_box, ctx->cfg.lpme ? ctx->d_wd : nullptr, ctx->d_pparts, ctx->d_z,
+                       ctx->d_p, sizeof(R) == 8 ? nullptr : reinterpret_cast<R*>(ctx->d_vR),
+                       reinterpret_cast<R*>(ctx->d_swfrac), ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr,
+                       ctx->d_pcg);
+}
+
 // true residual r = rhs - A x with FP64 products (RC-typed stored pair
 // coefficients), z = M^-1 r, partials into buffer 0; mode 1 starts a solve
-// (p = z), mode 2 only refreshes the residual and convergence state.
-template <class RC, class R>
+// (p = z), mode 2 only refreshes the residual and convergence state.  dbl:
+// the following iterations use FP64 products.
+template <class RC>
 static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x, const double* pol, double f1,
-                               double tol, int mode) {
+                               double tol, int mode, bool dbl) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     admp_status s;
@@ -981,10 +1000,8 @@ static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x
     ProfScope ps_(ctx, 11);
     hipLaunchKernelGGL(k_pcg_resid, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, 0, pol, rhs,
                        ctx->d_res, ctx->d_r, ctx->d_z, ctx->d_pparts);
-    hipLaunchKernelGGL(k_pcg_dir<R>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, 0, mode, tol,
-                       ctx->d_box, ctx->cfg.lpme ? ctx->d_wd : nullptr, ctx->d_pparts, ctx->d_z, ctx->d_p,
-                       sizeof(R) == 8 ? nullptr : reinterpret_cast<R*>(ctx->d_vR), reinterpret_cast<R*>(ctx->d_swfrac),
-                       ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr, ctx->d_pcg);
+    if (dbl) launch_dir<double>(ctx, 0, mode, tol);
+    else launch_dir<float>(ctx, 0, mode, tol);
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
@@ -992,20 +1009,24 @@ static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x
 // Preconditioned CG on A x = rhs (Jacobi preconditioner pol/DIELECTRIC) with
 // all scalars on the device.  Each iteration is pcg_apply + k_pcg_step +
 // k_pcg_dir; chunks of iterations are replayed from captured HIP graphs and the
-// host reads the per-frame state once per chunk.  With R = float the products
-// inside the iteration are FP32 and the recursively updated residual is
-// replaced by the true FP64 residual whenever it has dropped by 1e-2 since the
-// last replacement and before convergence is accepted ("reliable updates",
-// Sleijpen & van der Vorst 1996); the search direction is kept.  x holds the
-// initial guess on entry.
+// host reads the per-frame state once per chunk.  Real-space products use the
+// stored coefficients (FP64 on both paths).  With fp32 (mixed contexts only)
+// the reciprocal products start in FP32: the recursively updated residual is
+// replaced by the true FP64-product residual whenever it has dropped by 1e-2
+// ("reliable updates", Sleijpen & van der Vorst 1996), and once the true
+// residual fell by 1e3 (or after 32 FP32 iterations) the solve restarts its
+// direction and continues with FP64 products.  x holds the initial guess.
 template <class RC, class R>
 static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double* pol, double f1, double tol,
-                       int maxiter, std::vector<int>& it_out, std::vector<double>& res_out) {
+                       int maxiter, std::vector<int>& it_out, std::vector<double>& res_out, bool fp32 = false) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
-    const bool reliable = sizeof(R) == 4;
+    // optional first phase with FP32 reciprocal products (fp32 on a mixed context)
+    bool dbl = sizeof(R) == 8 || !fp32;
+    int fp32_iters = 0;
+    std::vector<int> it_base(F, 0);
     admp_status s;
-    if ((s = pcg_restart<RC, R>(ctx, rhs, x, pol, f1, tol, 1)) != ADMP_OK) return s;
+    if ((s = pcg_restart<RC>(ctx, rhs, x, pol, f1, tol, 1, dbl)) != ADMP_OK) return s;
     const int chunk = 4;
     std::vector<PcgState> hs(F);
     int done_iters = 0;
@@ -1023,38 +1044,49 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
     double worst = 0;
     bool all = false;
     if ((s = read_state(worst, all)) != ADMP_OK) return s;
+    const double r0 = worst;
     double last_true = worst;
     for (;;) {
-        if (reliable && done_iters > 0 && (all || worst < 1e-2 * last_true)) {
-            if ((s = pcg_restart<RC, R>(ctx, rhs, x, pol, f1, tol, 2)) != ADMP_OK) return s;
+        const bool sw = fp32_iters >= 32;
+        if (!dbl && done_iters > 0 && (all || sw || worst < 1e-2 * last_true)) {
+            // replace the recursively updated residual by the true one (FP64 products);
+            // switch to FP64 products after three orders of reduction or 32 FP32 iterations
+            if ((s = pcg_restart<RC>(ctx, rhs, x, pol, f1, tol, 2, false)) != ADMP_OK) return s;
             if ((s = read_state(worst, all)) != ADMP_OK) return s;
             last_true = worst;
+            if (!all && (sw || worst < 1e-3 * r0)) {
+                // restart the direction: an FP32-built p is not conjugate under FP64 products
+                dbl = true;
+                for (int f = 0; f < F; ++f) it_base[f] += hs[f].iter;
+                if ((s = pcg_restart<RC>(ctx, rhs, x, pol, f1, tol, 1, true)) != ADMP_OK) return s;
+                if ((s = read_state(worst, all)) != ADMP_OK) return s;
+            }
         }
         if (all || done_iters >= maxiter) break;
         const int nrun = std::min(chunk, maxiter - done_iters);
+        const bool use_dbl = dbl;
         auto iterate = [&]() -> admp_status {
             for (int c = 0; c < nrun; ++c) {
                 const int cur = c & 1;
                 admp_status s2;
-                if ((s2 = pcg_apply<RC, R>(ctx, pol, f1)) != ADMP_OK) return s2;
+                if (use_dbl) s2 = pcg_apply<RC, double>(ctx, pol, f1);
+                else s2 = pcg_apply<RC, float>(ctx, pol, f1);
+                if (s2 != ADMP_OK) return s2;
                 ProfScope ps_(ctx, 11);
                 hipLaunchKernelGGL(k_pcg_step, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, cur, tol,
                                    pol, x, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap, ctx->d_pparts);
-                hipLaunchKernelGGL(k_pcg_dir<R>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb,
-                                   cur ^ 1, 0, tol, ctx->d_box, ctx->cfg.lpme ? ctx->d_wd : nullptr, ctx->d_pparts,
-                                   ctx->d_z, ctx->d_p,
-                                   sizeof(R) == 8 ? nullptr : reinterpret_cast<R*>(ctx->d_vR),
-                                   reinterpret_cast<R*>(ctx->d_swfrac),
-                                   ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr, ctx->d_pcg);
+                if (use_dbl) launch_dir<double>(ctx, cur ^ 1, 0, tol);
+                else launch_dir<float>(ctx, cur ^ 1, 0, tol);
             }
             HIPCHK(hipGetLastError());
             return ADMP_OK;
         };
+        if (!use_dbl) fp32_iters += nrun;
         if (!ctx->use_graphs) {
             if ((s = iterate()) != ADMP_OK) return s;
         } else {
             // a chunk of iterations is captured once as a HIP graph and replayed
-            const int type = (sizeof(RC) == 8 ? 1 : 0) + (sizeof(R) == 8 ? 2 : 0);
+            const int type = (sizeof(RC) == 8 ? 1 : 0) + (use_dbl ? 2 : 0);
             admp_ctx::ChunkGraph* cg = nullptr;
             for (auto& g : ctx->graphs)
                 if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol &&
@@ -1104,7 +1136,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
     }
     it_out.resize(F);
     res_out.resize(F);
-    for (int f = 0; f < F; ++f) { it_out[f] = hs[f].iter; res_out[f] = hs[f].res; }
+    for (int f = 0; f < F; ++f) { it_out[f] = it_base[f] + hs[f].iter; res_out[f] = hs[f].res; }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
@@ -1160,8 +1192,10 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     else
         HIPCHK(hipMemsetAsync(U, 0, NF * 3 * sizeof(double), st));
     admp_status s;
-    // right-hand side: reciprocal part in FP64 even on the mixed path (FP64 residual)
-    if ((s = solver_rhs<R, double>(ctx, pol, thole, pp)) != ADMP_OK) return s;
+    // right-hand side and the stored real-space dipole coupling in FP64 on both
+    // paths: the near-field coupling is what makes close contacts ill-conditioned,
+    // and FP32 rounding of it changes the solution beyond the residual tolerance
+    if ((s = solver_rhs<double, double>(ctx, pol, thole, pp)) != ADMP_OK) return s;
     info.n_iter.assign(F, 0);
     info.conv.assign(F, 0);
     info.res.assign(F, 0.0);
@@ -1173,17 +1207,20 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
         if ((s = pcg<double, double>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr)) != ADMP_OK) return s;
         for (int f = 0; f < F; ++f) info.n_iter[f] = it[f];
     } else if (kind == ADMP_SOLVER_PCG) {
-        // FP32 products inside the iteration, FP64 true residuals (reliable updates)
+        // FP64 products by default; with fp32_products the first phase runs the
+        // reciprocal products in FP32 with FP64 true residuals (reliable updates)
         std::vector<int> it;
         std::vector<double> rr;
-        if ((s = pcg<R, R>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr)) != ADMP_OK) return s;
+        if ((s = pcg<double, R>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr, o && o->fp32_products)) !=
+            ADMP_OK)
+            return s;
         for (int f = 0; f < F; ++f) info.n_iter[f] = it[f];
     } else {
         // Jacobi sweeps (pme.py:380-422)
         HIPCHK(hipMemcpyAsync(ctx->d_active, act.data(), F * sizeof(int), hipMemcpyHostToDevice, st));
         int i = 0;
         for (i = 0; i < maxiter; ++i) {
-            if ((s = matvec<R, R>(ctx, U, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
+            if ((s = matvec<double, R>(ctx, U, ctx->d_Ap, pol, f1)) != ADMP_OK) return s;
             hipLaunchKernelGGL(k_sub, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_b, ctx->d_Ap);
             if (kind == ADMP_SOLVER_REFERENCE) {
                 if ((s = dots(ctx, ctx->d_Ap, nullptr, nullptr, pol, h)) != ADMP_OK) return s;
@@ -1210,7 +1247,7 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
         }
     }
     // true residual of the final iterate (FP64 products)
-    if ((s = matvec<R, double>(ctx, U, ctx->d_res, pol, f1)) != ADMP_OK) return s;
+    if ((s = matvec<double, double>(ctx, U, ctx->d_res, pol, f1)) != ADMP_OK) return s;
     hipLaunchKernelGGL(k_lincomb, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, 1.0, ctx->d_res, -1.0, ctx->d_b,
                        ctx->d_res);
     if ((s = dots(ctx, ctx->d_res, nullptr, nullptr, pol, h)) != ADMP_OK) return s;

@@ -834,6 +834,25 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_pcg_dir(int N, int nb, int nx, in
                                                        PcgState* __restrict__ st) {
     __shared__ double sh[16];
     const int f = blockIdx.y;
+    if (first == 3) {
+        // refresh the derived arrays of the current p (change of product precision)
+        const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
+        if (i >= N) return;
+        const size_t a = (size_t)f * N + i;
+        double u[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = p[a * 3 + k];
+            if (pR) pR[a * 3 + k] = (R)u[k];
+            if (wd) wfrac[a * 3 + k] = (R)wd[a * 3 + k];
+        }
+        if (!fq) return;
+        const double* A = boxes[f].A;
+        R* o = fq + a * 10;
+        o[0] = 0;
+        for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u[0] + A[1 * 3 + j] * u[1] + A[2 * 3 + j] * u[2]);
+        for (int k = 4; k < 10; ++k) o[k] = 0;
+        return;
+    }
     const double* P = parts + (size_t)f * PCG_NPART * nb;
     const double res = parts_total<true>(P + (3 + nx) * nb, nb, sh);
     const double res_prev = first ? 0.0 : parts_total<true>(P + (3 + (nx ^ 1)) * nb, nb, sh);

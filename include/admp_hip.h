@@ -84,6 +84,10 @@ typedef struct {
     int32_t steps;
     double  tol;               /* max |dE/dU| threshold, kJ/mol/(e*A)         */
     int32_t warm_start;        /* 1: use U_inout as the initial guess         */
+    int32_t fp32_products;     /* PCG on a MIXED context: 1 = reciprocal products in
+                                  FP32 until the true residual fell 1000x (or 32
+                                  iterations), then a restart with FP64 products;
+                                  0 = FP64 products throughout (default)      */
 } admp_solver_opts;
 
 typedef struct {

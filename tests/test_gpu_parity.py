@@ -301,10 +301,12 @@ def test_batched_frames_equal_single():
         assert abs(float(Eb[k]) - float(e1)) < 1e-9 * abs(float(e1))
 
 
-def test_mixed_batched_frames_converge():
-    """Mixed precision on a batch: frames converge at different iterations and the
-    true-residual replacements resume them; every frame must reach the FP64
-    residual tolerance and agree with the FP64 batch."""
+@pytest.mark.parametrize("fp32_products", [False, True])
+def test_mixed_batched_frames_converge(fp32_products):
+    """Mixed precision on a batch: frames converge at different iterations (and,
+    with FP32 reciprocal products, the true-residual replacements and the switch
+    to FP64 products resume them); every frame must reach the FP64 residual
+    tolerance and agree with the FP64 batch."""
     from dmff_amd.admp.engine import Engine
     from dmff_amd.admp.pme import setup_ewald_parameters
     s, pos, box = _small_waters()
@@ -321,7 +323,8 @@ def test_mixed_batched_frames_converge():
         eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
         eng.build_neighbors(frames, np.stack([box] * F))
         E, g, U, info = eng.compute(frames, np.stack([box] * F), Q, pol, th, s.pme.mScales, s.pme.pScales,
-                                    s.pme.dScales, solver="pcg", tol=1e-6, maxiter=400)
+                                    s.pme.dScales, solver="pcg", tol=1e-6, maxiter=400,
+                                    fp32_products=fp32_products and prec == "mixed")
         print(prec, info["n_iter"], info["residual"], flush=True)
         assert all(r < 1e-6 for r in info["residual"]), info
         assert max(info["n_iter"]) < 200
@@ -331,3 +334,30 @@ def test_mixed_batched_frames_converge():
     assert np.all(np.abs(Em - Ed) < 2e-5 * np.abs(Ed))
     for k in range(F):
         assert np.linalg.norm(Gm[k] - Gd[k]) / np.linalg.norm(Gd[k]) < 1e-4
+
+
+@pytest.mark.parametrize("K", [(16, 16, 16), (20, 24, 15), (32, 27, 25), (21, 22, 26), (6, 7, 8)])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_grid_sizes_vs_oracle(K, prec):
+    """Energies and forces on grids whose sizes exercise every FFT radix path
+    (4, 2, 3, 5, 7, 11, 13 and the smallest size the line passes accept),
+    against the oracle on the same grid."""
+    from dmff_amd.admp.engine import Engine
+    s, pos, box = _small_waters()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    kappa = O.setup_ewald_parameters(6.0, 5e-4, box)[0]
+    Ql = s.pme.Q_local[s.map_atomtype]
+    mS = s.pme.mScales
+    p = T(pos).requires_grad_(True)
+    e = O.energy_pme(p, T(box), T(pairs).long(), T(Ql), None, None, None, mS, None, None,
+                     s.axis_types, s.axis_idx, kappa, K, 2, False, True)
+    (gp,) = torch.autograd.grad(e, (p,))
+    eng = Engine(len(pos), 2, False, True, kappa, K, 6.0, precision=prec, device=DEV)
+    eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
+    eng.set_pairs(torch.as_tensor(pairs, device=DEV))
+    E, g, U, info = eng.compute(pos, box, Ql, mScales=mS, want_grad=True)
+    de = abs(float(E[0]) - float(e)) / abs(float(e))
+    df = rel_rms(g["pos"][0].cpu().numpy(), gp.numpy())
+    print(f"K={K} {prec}: E rel {de:.2e} force relrms {df:.2e}", flush=True)
+    etol, ftol = (1e-9, 1e-7) if prec == "fp64" else (2e-5, 1e-4)
+    assert de < etol and df < ftol
