@@ -165,10 +165,10 @@ def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005):
         eng.build_neighbors(pos, box)
         E, g, U, info = eng.compute(pos, box, D["Q"], D["pol"], D["thole"], S.mScales, S.pScales, S.dScales,
                                     solver="pcg", tol=1e-6, maxiter=500, want_grad=True)
-        # per frame, per type parameter gradients (one GEMM each)
-        dQ = (D["qtype"].W @ g["Q"].transpose(0, 1).reshape(N, -1)).reshape(-1, n_frames, g["Q"].shape[-1])
-        dpol = D["ptype"].W @ g["pol"].T
-        dth = D["ptype"].W @ g["thole"].T
+        # per frame, per type parameter gradients
+        dQ = D["qtype"](g["Q"], axis=1)
+        dpol = D["ptype"](g["pol"], axis=1)
+        dth = D["ptype"](g["thole"], axis=1)
         return E, g, info, dQ, dpol, dth
 
     for _ in range(warmup):

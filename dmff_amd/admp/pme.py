@@ -50,20 +50,23 @@ class TypeReducer:
     """Sums per-atom gradients into per-type gradients, the adjoint of the
     ``Q_local[map_atomtype]`` expansion (dmff/generators/admp.py:1372-1375).
 
-    A dense one-hot (n_types, N) float64 matrix is built once; the reduction is
-    one GEMM, deterministic and free of atomics (per-type targets are few, so
-    scatter-adds would serialise on them).
+    The atoms of each type are listed once; a reduction is one gather and one
+    sum per type along the atom axis (deterministic; per-type targets are few,
+    so scatter-adds would serialise on them and a (types x atoms) GEMM maps
+    badly onto library kernels).
     """
 
     def __init__(self, type_map, n_types=None, device=None):
-        m = torch.as_tensor(np.asarray(type_map), dtype=torch.int64, device=device)
+        m = np.asarray(type_map, dtype=np.int64)
         self.n_types = int(m.max()) + 1 if n_types is None else int(n_types)
-        self.W = torch.nn.functional.one_hot(m, self.n_types).to(torch.float64).T.contiguous()
+        self.index = [torch.as_tensor(np.nonzero(m == t)[0], dtype=torch.int64, device=device)
+                      for t in range(self.n_types)]
 
-    def __call__(self, per_atom):
-        g = per_atom.reshape(per_atom.shape[0], -1) if per_atom.dim() > 1 else per_atom.reshape(-1, 1)
-        out = self.W @ g
-        return out.reshape((self.n_types,) + tuple(per_atom.shape[1:]))
+    def __call__(self, per_atom, axis=0):
+        """per_atom: (..., N, ...) with atoms on `axis`; returns the same with types on `axis`."""
+        parts = [per_atom.index_select(axis, ix).sum(axis, keepdim=True) if len(ix) else
+                 torch.zeros_like(per_atom.narrow(axis, 0, 1)) for ix in self.index]
+        return torch.cat(parts, dim=axis)
 
 
 class _ADMPEnergy(torch.autograd.Function):

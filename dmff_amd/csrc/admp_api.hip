@@ -40,6 +40,8 @@ struct admp_ctx {
     hipStream_t user_stream = nullptr;  // caller's stream: ordered with events on entry and exit
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t side = nullptr;         // second context stream: real-space products beside the reciprocal chain
+    PcgState* h_pcg = nullptr;          // pinned host copies of the PCG state (two slots of F)
+    hipEvent_t ev_state[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     int N = 0, F = 1, nh = 1;
@@ -320,6 +322,9 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     for (void* p : ctx->allocs) (void)hipFree(p);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (int k = 0; k < 2; ++k)
+        if (ctx->ev_state[k]) (void)hipEventDestroy(ctx->ev_state[k]);
+    if (ctx->h_pcg) (void)hipHostFree(ctx->h_pcg);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
@@ -361,6 +366,9 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_state[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_state[1], hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_pcg, 2 * sizeof(PcgState) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess) {
@@ -781,7 +789,7 @@ template <class R>
 static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const double* Qc, const double* U, int qmode,
                              int lmax_eff, int kind, double pref, double kappa, int gather_mode, double* out3,
                              double* gq, double* gpos, double* vir, bool want_energy, void* wf_in = nullptr,
-                             void* fq_in = nullptr) {
+                             void* fq_in = nullptr, const PcgState* skip = nullptr) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     if (ctx->prepped != &g) {
@@ -838,48 +846,48 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     { ProfScope ps_(ctx, 4);
     if (lmax_eff >= 2)
         hipLaunchKernelGGL((k_spread_list<R, 2>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
-                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg);
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
     else if (lmax_eff == 1)
         hipLaunchKernelGGL((k_spread_list<R, 1>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
-                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg);
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
     else
         hipLaunchKernelGGL((k_spread_list<R, 0>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
-                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg);
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
     }
     const int npairs = (K0 * K1 + 1) / 2;
     const int nly = K0 * H, nlx = K1 * H;
     const size_t Cs = sizeof(Cplx<R>);
     { ProfScope ps_(ctx, 5);
     hipLaunchKernelGGL(k_fz_r2c<R>, dim3(nblk(npairs, FL), F), dim3(FTH), (2 * FL + 1) * K2 * Cs, st, K0, K1, K2,
-                       g.plan[2], twid<R>(g, 2), rg, hg);
+                       g.plan[2], twid<R>(g, 2), rg, hg, skip);
     hipLaunchKernelGGL(k_fy<R>, dim3(nblk(nly, FL), F), dim3(FTH), (2 * FL + 1) * K1 * Cs, st, K0, K1, H, g.plan[1],
-                       twid<R>(g, 1), hg, 0); }
+                       twid<R>(g, 1), hg, 0, skip); }
     const int nbx = (int)nblk(nlx, FL);
     { ProfScope ps_(ctx, 6);
     if (want_energy)
         hipLaunchKernelGGL((k_fx_infl<R, true>), dim3(nbx, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
                            g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
-                           pref, ctx->d_part);
+                           pref, ctx->d_part, skip);
     else
         hipLaunchKernelGGL((k_fx_infl<R, false>), dim3(nbx, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
                            g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
-                           pref, ctx->d_part); }
+                           pref, ctx->d_part, skip); }
     if (want_energy)
         hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbx, 7, ctx->d_part, ctx->d_rec, 0);
     { ProfScope ps_(ctx, 5);
     hipLaunchKernelGGL(k_fy<R>, dim3(nblk(nly, FL), F), dim3(FTH), (2 * FL + 1) * K1 * Cs, st, K0, K1, H, g.plan[1],
-                       twid<R>(g, 1), hg, 1);
+                       twid<R>(g, 1), hg, 1, skip);
     hipLaunchKernelGGL(k_fz_c2r<R>, dim3(nblk(npairs, FL), F), dim3(FTH), (2 * FL + 1) * K2 * Cs, st, K0, K1, K2,
-                       g.plan[2], twid<R>(g, 2), hg, rg); }
+                       g.plan[2], twid<R>(g, 2), hg, rg, skip); }
     { ProfScope ps_(ctx, 7);
     if (gather_mode == 1)
         hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
-                           vir);
+                           vir, skip);
     else
         hipLaunchKernelGGL((k_gather_ml<R, 0>), dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
-                           vir); }
+                           vir, skip); }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
@@ -902,7 +910,7 @@ static admp_status matvec(admp_ctx* ctx, const double* v, double* out, const dou
     { ProfScope ps_(ctx, 2);
     hipLaunchKernelGGL((k_real_matvec<RC, R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
                        ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
-                       reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1); }
+                       reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1, (const PcgState*)nullptr); }
     double* rec = nullptr;
     if (ctx->cfg.lpme) {
         admp_status s = recip_run<R>(ctx, ctx->eg, 0, nullptr, v, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1, ctx->d_t2,
@@ -956,13 +964,13 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     { ProfScope ps_(ctx, 2, ctx->side);
     hipLaunchKernelGGL((k_real_matvec<RC, RR>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0,
                        ctx->side, N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
-                       reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1); }
+                       reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1, ctx->d_pcg); }
     HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
     double* rec = nullptr;
     admp_status s = ADMP_OK;
     if (ctx->cfg.lpme) {
         s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_p, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1, ctx->d_t2, nullptr,
-                         nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq);
+                         nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg);
         rec = ctx->d_t2;
     }
     // join before any return so that a capture never ends with an open fork
@@ -970,7 +978,7 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     if (s != ADMP_OK) return s;
     { ProfScope ps_(ctx, 11);
     hipLaunchKernelGGL(k_pcg_q, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, f1, pol, ctx->d_p,
-                       ctx->d_t1, rec, ctx->d_Ap, ctx->d_pparts); }
+                       ctx->d_t1, rec, ctx->d_Ap, ctx->d_pparts, ctx->d_pcg); }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
@@ -1027,7 +1035,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
     std::vector<int> it_base(F, 0);
     admp_status s;
     if ((s = pcg_restart<RC>(ctx, rhs, x, pol, f1, tol, 1, dbl)) != ADMP_OK) return s;
-    const int chunk = 4;
+    const int chunk = 4;  // FP32-phase chunk (host reads the state after each)
     std::vector<PcgState> hs(F);
     int done_iters = 0;
     auto read_state = [&](double& worst, bool& all) -> admp_status {
@@ -1039,6 +1047,71 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
             all = all && hs[f].done;
             worst = std::max(worst, hs[f].res);
         }
+        return ADMP_OK;
+    };
+    // launch nrun iterations (FP64 or FP32 reciprocal products), eagerly or as a
+    // captured HIP graph replayed on later calls
+    auto launch_chunk = [&](int nrun, bool use_dbl) -> admp_status {
+        auto iterate = [&]() -> admp_status {
+            for (int c = 0; c < nrun; ++c) {
+                const int cur = c & 1;
+                admp_status s2;
+                if (use_dbl) s2 = pcg_apply<RC, double>(ctx, pol, f1);
+                else s2 = pcg_apply<RC, float>(ctx, pol, f1);
+                if (s2 != ADMP_OK) return s2;
+                ProfScope ps_(ctx, 11);
+                hipLaunchKernelGGL(k_pcg_step, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, cur, tol,
+                                   pol, x, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap, ctx->d_pparts);
+                if (use_dbl) launch_dir<double>(ctx, cur ^ 1, 0, tol);
+                else launch_dir<float>(ctx, cur ^ 1, 0, tol);
+            }
+            HIPCHK(hipGetLastError());
+            return ADMP_OK;
+        };
+        if (!ctx->use_graphs) return iterate();
+        const int type = (sizeof(RC) == 8 ? 1 : 0) + (use_dbl ? 2 : 0);
+        admp_ctx::ChunkGraph* cg = nullptr;
+        for (auto& g : ctx->graphs)
+            if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1)
+                cg = &g;
+        if (!cg) {
+            admp_ctx::ChunkGraph* slot = nullptr;
+            for (auto& g : ctx->graphs)
+                if (!g.exec) { slot = &g; break; }
+            if (!slot) { clear_graphs(ctx); slot = &ctx->graphs[0]; }
+            const bool prof = ctx->prof;
+            ctx->prof = false;
+            hipGraph_t graph = nullptr;
+            hipError_t be = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (be != hipSuccess) {
+                ctx->prof = prof;
+                ctx->err = std::string("graph capture begin: ") + hipGetErrorString(be);
+                return ADMP_ERR_HIP;
+            }
+            admp_status cs = iterate();
+            hipError_t ce = hipStreamEndCapture(st, &graph);
+            ctx->prof = prof;
+            if (cs != ADMP_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return cs;
+            }
+            if (ce != hipSuccess) {
+                ctx->err = std::string("graph capture: ") + hipGetErrorString(ce);
+                return ADMP_ERR_HIP;
+            }
+            hipGraphExec_t exec = nullptr;
+            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ie);
+                return ADMP_ERR_HIP;
+            }
+            slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
+            slot->tol = tol; slot->f1 = f1;
+            cg = slot;
+        }
+        ProfScope ps_(ctx, 12);
+        HIPCHK(hipGraphLaunch(cg->exec, st));
         return ADMP_OK;
     };
     double worst = 0;
@@ -1063,76 +1136,51 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
             }
         }
         if (all || done_iters >= maxiter) break;
+        if (dbl) break;  // FP64 products from here on: pipelined loop below
         const int nrun = std::min(chunk, maxiter - done_iters);
-        const bool use_dbl = dbl;
-        auto iterate = [&]() -> admp_status {
-            for (int c = 0; c < nrun; ++c) {
-                const int cur = c & 1;
-                admp_status s2;
-                if (use_dbl) s2 = pcg_apply<RC, double>(ctx, pol, f1);
-                else s2 = pcg_apply<RC, float>(ctx, pol, f1);
-                if (s2 != ADMP_OK) return s2;
-                ProfScope ps_(ctx, 11);
-                hipLaunchKernelGGL(k_pcg_step, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, cur, tol,
-                                   pol, x, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap, ctx->d_pparts);
-                if (use_dbl) launch_dir<double>(ctx, cur ^ 1, 0, tol);
-                else launch_dir<float>(ctx, cur ^ 1, 0, tol);
-            }
-            HIPCHK(hipGetLastError());
-            return ADMP_OK;
-        };
-        if (!use_dbl) fp32_iters += nrun;
-        if (!ctx->use_graphs) {
-            if ((s = iterate()) != ADMP_OK) return s;
-        } else {
-            // a chunk of iterations is captured once as a HIP graph and replayed
-            const int type = (sizeof(RC) == 8 ? 1 : 0) + (use_dbl ? 2 : 0);
-            admp_ctx::ChunkGraph* cg = nullptr;
-            for (auto& g : ctx->graphs)
-                if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol &&
-                    g.f1 == f1)
-                    cg = &g;
-            if (!cg) {
-                admp_ctx::ChunkGraph* slot = nullptr;
-                for (auto& g : ctx->graphs)
-                    if (!g.exec) { slot = &g; break; }
-                if (!slot) { clear_graphs(ctx); slot = &ctx->graphs[0]; }
-                const bool prof = ctx->prof;
-                ctx->prof = false;
-                hipGraph_t graph = nullptr;
-                hipError_t be = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-                if (be != hipSuccess) {
-                    ctx->prof = prof;
-                    ctx->err = std::string("graph capture begin: ") + hipGetErrorString(be);
-                    return ADMP_ERR_HIP;
-                }
-                admp_status cs = iterate();
-                hipError_t ce = hipStreamEndCapture(st, &graph);
-                ctx->prof = prof;
-                if (cs != ADMP_OK) {
-                    if (graph) (void)hipGraphDestroy(graph);
-                    return cs;
-                }
-                if (ce != hipSuccess) {
-                    ctx->err = std::string("graph capture: ") + hipGetErrorString(ce);
-                    return ADMP_ERR_HIP;
-                }
-                hipGraphExec_t exec = nullptr;
-                hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                if (ie != hipSuccess) {
-                    ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ie);
-                    return ADMP_ERR_HIP;
-                }
-                slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
-                slot->tol = tol; slot->f1 = f1;
-                cg = slot;
-            }
-            ProfScope ps_(ctx, 12);
-            HIPCHK(hipGraphLaunch(cg->exec, st));
-        }
+        fp32_iters += nrun;
+        if ((s = launch_chunk(nrun, false)) != ADMP_OK) return s;
         done_iters += nrun;
         if ((s = read_state(worst, all)) != ADMP_OK) return s;
+    }
+    if (dbl && !all && done_iters < maxiter) {
+        // The host stays one chunk ahead: chunk k+1 is launched before the state
+        // after chunk k is read (pinned copy + event), so the GPU does not idle at
+        // chunk boundaries.  A chunk launched after convergence costs little: the
+        // product kernels return at once for converged frames.
+        const int ch = 2;
+        int q[2], nq = 0, last_slot = 1;
+        auto post = [&](int sl) -> admp_status {
+            HIPCHK(hipMemcpyAsync(ctx->h_pcg + (size_t)sl * F, ctx->d_pcg, F * sizeof(PcgState),
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipEventRecord(ctx->ev_state[sl], st));
+            return ADMP_OK;
+        };
+        auto launch_next = [&]() -> admp_status {
+            const int nrun = std::min(ch, maxiter - done_iters);
+            admp_status s2;
+            if ((s2 = launch_chunk(nrun, true)) != ADMP_OK) return s2;
+            done_iters += nrun;
+            last_slot ^= 1;
+            if ((s2 = post(last_slot)) != ADMP_OK) return s2;
+            q[nq++] = last_slot;
+            return ADMP_OK;
+        };
+        if ((s = launch_next()) != ADMP_OK) return s;
+        while (nq > 0) {
+            if (done_iters < maxiter && nq < 2)
+                if ((s = launch_next()) != ADMP_OK) return s;
+            const int sl = q[0];
+            q[0] = q[1];
+            --nq;
+            HIPCHK(hipEventSynchronize(ctx->ev_state[sl]));
+            all = true;
+            for (int f = 0; f < F; ++f) {
+                hs[f] = ctx->h_pcg[(size_t)sl * F + f];
+                all = all && hs[f].done;
+            }
+            if (all) break;
+        }
     }
     it_out.resize(F);
     res_out.resize(F);

@@ -31,6 +31,13 @@ struct BoxInfo {
 
 template <class R> struct Cplx { R x, y; };
 
+// per-frame state of the device-resident PCG (written by k_pcg_dir / k_pcg_init)
+struct PcgState {
+    double rz, alpha, beta, res;
+    int done, iter, pad0, pad1;
+};
+
+
 template <class R> __device__ __forceinline__ Cplx<R> cmul(Cplx<R> a, Cplx<R> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }

@@ -466,7 +466,8 @@ template <class RC, class R>
 __global__ void __launch_bounds__(PAIR_BLOCK)
 k_real_matvec(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
               const int* __restrict__ nbr, const int* __restrict__ ncnt, const RC* __restrict__ pcoef,
-              const R* __restrict__ v, double* __restrict__ out) {
+              const R* __restrict__ v, double* __restrict__ out, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
     const int f = blockIdx.y;
     const int gt = blockIdx.x * PAIR_BLOCK + threadIdx.x;
     const int i = gt / TPA, sub = gt % TPA;

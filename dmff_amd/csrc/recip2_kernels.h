@@ -184,7 +184,8 @@ k_brick_cands(int N, int K0, int K1, int K2, const int4* __restrict__ mbase, con
 template <class R, int LM>
 __global__ void __launch_bounds__(64)
 k_spread_list(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R* __restrict__ fq,
-              const int* __restrict__ bstart, const int2* __restrict__ blist, long cap, R* __restrict__ rgrid) {
+              const int* __restrict__ bstart, const int2* __restrict__ blist, long cap, R* __restrict__ rgrid, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
     constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
     constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
     __shared__ int s_r[SP_CHUNK][3];
@@ -606,7 +607,8 @@ __device__ inline void stage_tw(Cplx<R>* dst, const Cplx<R>* __restrict__ src, i
 // real rows (K0 K1 of length K2) -> half spectrum (K0, K1, H); FL complex lines = 2 FL rows
 template <class R>
 __global__ void __launch_bounds__(FTH) k_fz_r2c(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ twg,
-                                               const R* __restrict__ rgrid, Cplx<R>* __restrict__ hgrid) {
+                                               const R* __restrict__ rgrid, Cplx<R>* __restrict__ hgrid, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
     extern __shared__ unsigned char smem_raw[];
     Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
     Cplx<R>* b1 = b0 + FL * K2;
@@ -644,7 +646,8 @@ __global__ void __launch_bounds__(FTH) k_fz_r2c(int K0, int K1, int K2, FFTPlan 
 // half spectrum -> real rows (inverse, unnormalised)
 template <class R>
 __global__ void __launch_bounds__(FTH) k_fz_c2r(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ twg,
-                                               const Cplx<R>* __restrict__ hgrid, R* __restrict__ rgrid) {
+                                               const Cplx<R>* __restrict__ hgrid, R* __restrict__ rgrid, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
     extern __shared__ unsigned char smem_raw[];
     Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
     Cplx<R>* b1 = b0 + FL * K2;
@@ -684,7 +687,8 @@ __global__ void __launch_bounds__(FTH) k_fz_c2r(int K0, int K1, int K2, FFTPlan 
 // complex lines along y of the half spectrum: lines (x, kz), stride H
 template <class R>
 __global__ void __launch_bounds__(FTH) k_fy(int K0, int K1, int H, FFTPlan plan, const Cplx<R>* __restrict__ twg,
-                                           Cplx<R>* __restrict__ hgrid, int inverse) {
+                                           Cplx<R>* __restrict__ hgrid, int inverse, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
     extern __shared__ unsigned char smem_raw[];
     Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
     Cplx<R>* b1 = b0 + FL * K1;
@@ -719,7 +723,8 @@ __global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan
                                                 Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
                                                 const double* __restrict__ bmod0, const double* __restrict__ bmod1,
                                                 const double* __restrict__ bmod2, double kappa, int kind, double pref,
-                                                double* __restrict__ part) {
+                                                double* __restrict__ part, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
     extern __shared__ unsigned char smem_raw[];
     __shared__ double sh[16];
     Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
@@ -1053,7 +1058,8 @@ k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, cons
             int use_dA, const int4* __restrict__ mbase, const R* __restrict__ wfrac, const R* __restrict__ fq,
             const double* __restrict__ Qc, const double* __restrict__ U, int qmode, const R* __restrict__ rgrid,
             double* __restrict__ out3, double* __restrict__ g_q, double* __restrict__ g_pos,
-            double* __restrict__ vir_atom) {
+            double* __restrict__ vir_atom, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
     constexpr int ND = MODE == 1 ? 2 : 4;  // derivative orders needed
     const int f = blockIdx.y;
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;

@@ -608,10 +608,6 @@ __global__ void k_vadd(size_t n, const double* __restrict__ in, double* __restri
 }
 
 // ----------------------------------------------------------------- device-resident PCG
-struct PcgState {
-    double rz, alpha, beta, res;
-    int done, iter, pad0, pad1;
-};
 
 // after dots(r, z): rz and max|r|; frames already below tol are done
 __global__ void k_pcg_init(int F, const double* __restrict__ dots, double tol, PcgState* __restrict__ st) {
@@ -758,7 +754,8 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_pcg_resid(int N, int nb, int buf,
 __global__ void __launch_bounds__(PCG_BLOCK) k_pcg_q(int N, int nb, double f1, const double* __restrict__ pol,
                                                      const double* __restrict__ p, const double* __restrict__ real,
                                                      const double* __restrict__ recip, double* __restrict__ q,
-                                                     double* __restrict__ parts) {
+                                                     double* __restrict__ parts, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
     __shared__ double sh[16];
     const int f = blockIdx.y;
     const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
