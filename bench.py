@@ -188,14 +188,15 @@ def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005):
 
 
 def water_dispersion_slater(S, seed):
-    """Per-atom sqrt(C6, C8, C10) (A units) with +-10% jitter; per-type Slater A (kJ/mol), B (1/A)."""
+    """Per-atom sqrt(C6, C8, C10) (A units) with +-10% jitter; per-type Slater A (sqrt kJ/mol, the pair
+    prefactor is A_i A_j as in pairwise.py:156-169) and B (1/A)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     C = np.array([[2.0e-3, 1.0e-4, 6.0e-6], [1.0e-4, 3.0e-6, 1.0e-7]])  # kJ/mol nm^p, O and H
     t = S.map_atomtype
     Ca = C[t] * rng.uniform(0.9, 1.1, size=(S.n_atoms, 3))
     Ca = Ca * np.array([1e6, 1e8, 1e10])  # nm^p -> A^p
-    A_t = rng.uniform(1e4, 1e5, size=2)
+    A_t = np.sqrt(rng.uniform(1e4, 1e5, size=2))  # pair prefactor A_i A_j in [1e4, 1e5] kJ/mol
     B_t = rng.uniform(30.0, 45.0, size=2) / 10.0  # 1/nm -> 1/A
     return np.sqrt(Ca), A_t[t], B_t[t]
 

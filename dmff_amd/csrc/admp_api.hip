@@ -46,7 +46,9 @@ struct admp_ctx {
     std::string err;
     int N = 0, F = 1, nh = 1;
     int cap = 0;
-    bool has_topo = false, has_nbr = false;
+    bool has_topo = false;
+    std::vector<char> frame_nbr;  // per frame: its pair list is in d_nbr / d_ncnt
+    bool nbr_built = false;       // lists (and d_nhalf) come from admp_build_neighbors
     int use_frames = 0;
     // topology
     int *d_axis_type = nullptr, *d_axis_idx = nullptr, *d_xoff = nullptr, *d_xent = nullptr;
@@ -187,7 +189,11 @@ static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1
 template <class T> static hipError_t dalloc(admp_ctx* ctx, T** p, size_t n) {
     void* q = nullptr;
     const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
+    // stream-ordered allocation on the context stream: no device-wide
+    // synchronisation, so contexts driven from other threads (whose streams may
+    // be capturing HIP graphs) are not disturbed.  Every buffer starts zeroed.
+    hipError_t e = hipMallocAsync(&q, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(q, 0, bytes, ctx->stream);
     if (e == hipSuccess) {
         ctx->allocs.push_back(q);
         ctx->alloc_bytes.push_back(bytes);
@@ -203,7 +209,14 @@ static void dfree(admp_ctx* ctx, void* p) {
         ctx->alloc_bytes.erase(ctx->alloc_bytes.begin() + (it - ctx->allocs.begin()));
         ctx->allocs.erase(it);
     }
-    (void)hipFree(p);
+    (void)hipFreeAsync(p, ctx->stream);
+}
+
+static bool all_frames_have_pairs(const admp_ctx* ctx) {
+    if ((int)ctx->frame_nbr.size() != ctx->F) return false;
+    for (char c : ctx->frame_nbr)
+        if (!c) return false;
+    return true;
 }
 
 static std::vector<int> factorize(int K) {
@@ -319,7 +332,8 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     for (auto& p : ctx->ev_pending) { ctx->ev_pool.push_back(p.second.first); ctx->ev_pool.push_back(p.second.second); }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     clear_graphs(ctx);
-    for (void* p : ctx->allocs) (void)hipFree(p);
+    for (void* p : ctx->allocs) (void)hipFreeAsync(p, ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     for (int k = 0; k < 2; ++k)
@@ -557,12 +571,20 @@ static admp_status ensure_cap(admp_ctx* ctx, int need) {
     if (need <= ctx->cap && ctx->d_nbr) return ADMP_OK;
     clear_graphs(ctx);  // captured launches hold the old buffers and capacity
     int cap = std::max(64, ((int)(need * 1.25) + 63) / 64 * 64);
-    dfree(ctx, ctx->d_nbr);
-    dfree(ctx, ctx->d_pcoef);
-    ctx->d_nbr = nullptr;
-    ctx->d_pcoef = nullptr;
     const size_t NF = (size_t)ctx->N * ctx->F;
+    int* old = ctx->d_nbr;
+    const int old_cap = ctx->cap;
+    ctx->d_nbr = nullptr;
     HIPCHK(dalloc(ctx, &ctx->d_nbr, NF * cap));
+    if (old) {
+        // carry every row over to the new stride (lists of other frames stay valid)
+        HIPCHK(hipMemcpy2DAsync(ctx->d_nbr, (size_t)cap * sizeof(int), old, (size_t)old_cap * sizeof(int),
+                                (size_t)old_cap * sizeof(int), NF, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        dfree(ctx, old);
+    }
+    dfree(ctx, ctx->d_pcoef);
+    ctx->d_pcoef = nullptr;
     double* pc;
     HIPCHK(dalloc(ctx, &pc, NF * cap * 2));
     ctx->d_pcoef = pc;
@@ -608,7 +630,6 @@ static admp_status admp_set_pairs_impl(admp_ctx* ctx, int32_t frame, const void*
     if (mx > ctx->cap || !ctx->d_nbr) {
         admp_status s = ensure_cap(ctx, mx);
         if (s != ADMP_OK) return s;
-        if (ctx->F > 1) ctx->has_nbr = false;  // other frames' lists were dropped
     }
     const int cap = ctx->cap;
     int* nbr = ctx->d_nbr + (size_t)frame * N * cap;
@@ -626,7 +647,9 @@ static admp_status admp_set_pairs_impl(admp_ctx* ctx, int32_t frame, const void*
         hipLaunchKernelGGL(k_row_sort, dim3(N), dim3(256), P * sizeof(unsigned int), ctx->stream, N, cap, nbr, cnt);
     }
     HIPCHK(hipGetLastError());
-    ctx->has_nbr = true;
+    if ((int)ctx->frame_nbr.size() != ctx->F) ctx->frame_nbr.assign(ctx->F, 0);
+    ctx->frame_nbr[frame] = 1;
+    ctx->nbr_built = false;
     return ADMP_OK;
 }
 
@@ -705,13 +728,14 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
         if (mx <= ctx->cap) break;
         if ((s = ensure_cap(ctx, mx)) != ADMP_OK) return s;
     }
-    ctx->has_nbr = true;
+    ctx->frame_nbr.assign(ctx->F, 1);
+    ctx->nbr_built = true;
     return ADMP_OK;
 }
 
 static admp_status admp_get_pairs_impl(admp_ctx* ctx, int32_t frame, int32_t* out, int64_t cap, int64_t* n_out) {
     if (!ctx || frame < 0 || frame >= ctx->F || !n_out) return ADMP_ERR_ARG;
-    if (!ctx->has_nbr) { ctx->err = "no neighbour list"; return ADMP_ERR_STATE; }
+    if (!ctx->nbr_built) { ctx->err = "no neighbour list built by admp_build_neighbors"; return ADMP_ERR_STATE; }
     const int N = ctx->N;
     // nhalf holds the j > i counts for lists built on the GPU
     int* off;
@@ -1085,26 +1109,23 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
             hipError_t be = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
             if (be != hipSuccess) {
                 ctx->prof = prof;
-                ctx->err = std::string("graph capture begin: ") + hipGetErrorString(be);
-                return ADMP_ERR_HIP;
+                (void)hipGetLastError();
+                return iterate();  // capture unavailable: launch eagerly
             }
             admp_status cs = iterate();
             hipError_t ce = hipStreamEndCapture(st, &graph);
             ctx->prof = prof;
-            if (cs != ADMP_OK) {
-                if (graph) (void)hipGraphDestroy(graph);
-                return cs;
-            }
-            if (ce != hipSuccess) {
-                ctx->err = std::string("graph capture: ") + hipGetErrorString(ce);
-                return ADMP_ERR_HIP;
-            }
             hipGraphExec_t exec = nullptr;
-            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
+            hipError_t ie = hipErrorUnknown;
+            if (cs == ADMP_OK && ce == hipSuccess && graph)
+                ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
             if (ie != hipSuccess) {
-                ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ie);
-                return ADMP_ERR_HIP;
+                // the capture was invalidated (or failed): nothing of it ran, so the
+                // chunk is launched eagerly; errors of the eager launch are reported
+                (void)hipGetLastError();
+                ctx->err.clear();
+                return iterate();
             }
             slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
             slot->tol = tol; slot->f1 = f1;
@@ -1148,7 +1169,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         // after chunk k is read (pinned copy + event), so the GPU does not idle at
         // chunk boundaries.  A chunk launched after convergence costs little: the
         // product kernels return at once for converged frames.
-        const int ch = 2;
+        const int ch = 4;
         int q[2], nq = 0, last_slot = 1;
         auto post = [&](int sl) -> admp_status {
             HIPCHK(hipMemcpyAsync(ctx->h_pcg + (size_t)sl * F, ctx->d_pcg, F * sizeof(PcgState),
@@ -1443,7 +1464,10 @@ static admp_status admp_compute_impl(admp_ctx* ctx, const double* pos, const dou
                                     double* dbox, double* dQ_local, double* dpol, double* dthole, admp_stats* stats) {
     if (!ctx || !pos || !box || !Q_local) return ADMP_ERR_ARG;
     if (!ctx->has_topo) { ctx->err = "admp_set_topology first"; return ADMP_ERR_STATE; }
-    if (!ctx->has_nbr) { ctx->err = "no pair list: admp_set_pairs or admp_build_neighbors first"; return ADMP_ERR_STATE; }
+    if (!all_frames_have_pairs(ctx)) {
+        ctx->err = "no pair list for every frame: admp_set_pairs per frame or admp_build_neighbors first";
+        return ADMP_ERR_STATE;
+    }
     if (ctx->cfg.lpol && (!pol || !thole)) { ctx->err = "pol and thole required"; return ADMP_ERR_ARG; }
     if (ctx->cfg.lpol) {
         HIPCHK(hipMemcpyAsync(ctx->d_pol, pol, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1536,7 +1560,7 @@ static admp_status admp_disp_compute_impl(admp_ctx* ctx, const double* pos, cons
                                          const double* mScales, int32_t n_scales, int32_t want_grad, double* energy,
                                          double* dpos, double* dbox, double* dc) {
     if (!ctx || !pos || !box || !c_list) return ADMP_ERR_ARG;
-    if (!ctx->has_nbr) { ctx->err = "no pair list"; return ADMP_ERR_STATE; }
+    if (!all_frames_have_pairs(ctx)) { ctx->err = "no pair list for every frame"; return ADMP_ERR_STATE; }
     if (n_scales < 1 || n_scales > 8) { ctx->err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
     admp_status s = upload_geometry(ctx, pos, box);
     if (s != ADMP_OK) return s;
@@ -1549,7 +1573,7 @@ static admp_status admp_slater_compute_impl(admp_ctx* ctx, const double* pos, co
                                            const double* b, const double* mScales, int32_t n_scales, int32_t want_grad,
                                            double* energy, double* dpos, double* dbox, double* da, double* db) {
     if (!ctx || !pos || !box || !a || !b) return ADMP_ERR_ARG;
-    if (!ctx->has_nbr) { ctx->err = "no pair list"; return ADMP_ERR_STATE; }
+    if (!all_frames_have_pairs(ctx)) { ctx->err = "no pair list for every frame"; return ADMP_ERR_STATE; }
     if (n_scales < 1 || n_scales > 8) { ctx->err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
     admp_status s = upload_geometry(ctx, pos, box);
     if (s != ADMP_OK) return s;

@@ -78,24 +78,8 @@ __global__ void k_grid_cell(int N, int K0, int K1, int K2, const int4* __restric
     atomicAdd(&ccount[(size_t)f * K0 * K1 * K2 + id], 1);
 }
 
-// Fractional dipoles only (solver spreads): fq[1..3] = A^T U, others 0.
-template <class R>
-__global__ void k_frac_dipole(int N, const BoxInfo* __restrict__ boxes, const double* __restrict__ U,
-                              R* __restrict__ fq) {
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const size_t a = (size_t)f * N + i;
-    const double* A = boxes[f].A;
-    const double u0 = U[a * 3], u1 = U[a * 3 + 1], u2 = U[a * 3 + 2];
-    R* o = fq + a * 10;
-    o[0] = 0;
-    for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u0 + A[1 * 3 + j] * u1 + A[2 * 3 + j] * u2);
-    for (int k = 4; k < 10; ++k) o[k] = 0;
-}
-
-constexpr int BRX = 2, BRY = 4, BRZ = 8;   // 64 grid points per wavefront
-constexpr int SP_CHUNK = 64;               // sites staged in LDS per round
+constexpr int BRX = 4, BRY = 4, BRZ = 4;   // 64 grid points per wavefront
+constexpr int SP_CHUNK = 32;               // sites staged in LDS per round
 
 // ---------------------------------------------------------------------------
 // Per-brick candidate lists.  The candidates of a brick are the sites whose
@@ -105,18 +89,22 @@ constexpr int SP_CHUNK = 64;               // sites staged in LDS per round
 // so the order is fixed.  Positions do not change during a solve, so the lists
 // are built once per evaluation and every spread of the solve reuses them.
 // Entry: {site, (rx + 5) | (ry + 5) << 4 | (rz + 5) << 8} with r the support
-// start relative to the brick origin.  A site is a candidate of at most
-// 5 x 4 x 3 bricks (per axis ceil((B + 5) / B) + 1 brick origins fit a window of
-// B + 5 cells), so 60 N entries bound a list.
+// start relative to the brick origin.  Per axis at most ceil((B + 5) / B) + 1
+// brick origins fit a window of B + 5 cells, so a site is a candidate of at
+// most 4 x 4 x 4 bricks and 64 N entries bound a list.
 // ---------------------------------------------------------------------------
-constexpr int BRICK_MAXCAND = 60;
+constexpr int BRICK_MAXCAND = ((BRX + 5 + BRX - 1) / BRX + 1) * ((BRY + 5 + BRY - 1) / BRY + 1) *
+                              ((BRZ + 5 + BRZ - 1) / BRZ + 1);
+constexpr int BRICK_SEG = (BRX + 5) * (BRY + 5) * 2;   // column segments (two when z wraps)
+constexpr int BRICK_SEG_LANE = (BRICK_SEG + 63) / 64;   // segments per lane of the wavefront
 
 template <bool FILL>
 __global__ void __launch_bounds__(64)
 k_brick_cands(int N, int K0, int K1, int K2, const int4* __restrict__ mbase, const int* __restrict__ cstart,
               const int* __restrict__ catoms, int* __restrict__ bcount, const int* __restrict__ bstart,
               int2* __restrict__ blist, long cap) {
-    constexpr int NC = (BRX + 5) * (BRY + 5) * 2;  // column segments (two when z wraps)
+    constexpr int NC = BRICK_SEG;
+    constexpr int SL = BRICK_SEG_LANE;
     __shared__ int col_lo[NC];
     __shared__ int col_off[NC + 1];
     const int f = blockIdx.y;
@@ -133,10 +121,13 @@ k_brick_cands(int N, int K0, int K1, int K2, const int4* __restrict__ mbase, con
     const int nseg = nx * ny * 2;
     const int zs = wrapi(Z0 - 2, K2);
     const int zlen1 = min(nzr, K2 - zs), zlen2 = nzr - zlen1;
-    // each lane owns segments 2 t and 2 t + 1
-    int lo2[2] = {0, 0}, n2[2] = {0, 0};
-    for (int k = 0; k < 2; ++k) {
-        const int c = 2 * t + k;
+    // each lane owns segments SL t .. SL t + SL - 1
+    int lo2[SL], n2[SL];
+#pragma unroll
+    for (int k = 0; k < SL; ++k) {
+        lo2[k] = 0;
+        n2[k] = 0;
+        const int c = SL * t + k;
         if (c >= nseg) continue;
         const int cc = c >> 1, seg = c & 1;
         const int x = wrapi(X0 - 2 + cc / ny, K0), y = wrapi(Y0 - 2 + cc % ny, K1);
@@ -149,16 +140,22 @@ k_brick_cands(int N, int K0, int K1, int K2, const int4* __restrict__ mbase, con
             n2[k] = cs[base + zlen2] - lo2[k];
         }
     }
-    const int mine = n2[0] + n2[1];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < SL; ++k) mine += n2[k];
     const int incl = wave_incl_scan(mine);
-    const int ex = incl - mine;
     const int total = __shfl(incl, 63, 64);
     if (!FILL) {
         if (t == 0) bcount[(size_t)f * nbr + b] = total;
         return;
     }
-    if (2 * t < NC) { col_lo[2 * t] = lo2[0]; col_off[2 * t] = ex; }
-    if (2 * t + 1 < NC) { col_lo[2 * t + 1] = lo2[1]; col_off[2 * t + 1] = ex + n2[0]; }
+    int ex = incl - mine;
+#pragma unroll
+    for (int k = 0; k < SL; ++k) {
+        const int c = SL * t + k;
+        if (c < NC) { col_lo[c] = lo2[k]; col_off[c] = ex; }
+        ex += n2[k];
+    }
     if (t == 0) col_off[NC] = total;
     __syncthreads();
     // each frame owns cap entries of the list buffer
@@ -188,7 +185,7 @@ k_spread_list(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const 
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
     constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
     constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
-    __shared__ int s_r[SP_CHUNK][3];
+    __shared__ int s_r[SP_CHUNK];   // packed support starts (+5), 4 bits per axis
     __shared__ R s_t[SP_CHUNK][3][NT][6];
     __shared__ R s_q[SP_CHUNK][NQ];
     const int f = blockIdx.y;
@@ -209,9 +206,7 @@ k_spread_list(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const 
         if (t < nchunk) {
             const int2 e = blist[(size_t)f * cap + lo + c0 + t];
             const size_t a = (size_t)f * N + e.x;
-            s_r[t][0] = (e.y & 15) - 5;
-            s_r[t][1] = ((e.y >> 4) & 15) - 5;
-            s_r[t][2] = ((e.y >> 8) & 15) - 5;
+            s_r[t] = e.y;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const R w = wfrac[a * 3 + d];
@@ -230,7 +225,8 @@ k_spread_list(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const 
         __syncthreads();
         if (own) {
             for (int s2 = 0; s2 < nchunk; ++s2) {
-                const int k0 = lx - s_r[s2][0], k1 = ly - s_r[s2][1], k2 = lz - s_r[s2][2];
+                const int e = s_r[s2];
+                const int k0 = lx + 5 - (e & 15), k1 = ly + 5 - ((e >> 4) & 15), k2 = lz + 5 - ((e >> 8) & 15);
                 if ((unsigned)k0 > 5u || (unsigned)k1 > 5u || (unsigned)k2 > 5u) continue;
                 const R v0 = s_t[s2][0][0][k0], v1 = s_t[s2][1][0][k1], v2 = s_t[s2][2][0][k2];
                 R wgt = s_q[s2][0] * v0 * v1 * v2;
@@ -254,236 +250,6 @@ k_spread_list(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const 
         __syncthreads();
     }
     if (own) rgrid[(size_t)f * G + ((size_t)px * K1 + py) * K2 + pz] = acc;
-}
-
-// LM: 0 charges only, 1 charges + dipoles, 2 full quadrupoles.
-// Staging computes, per candidate site, the start of its support relative to the
-// brick origin (r in [-5, brick-1]) and its 1-D spline tables (value, d/dR,
-// d2/dR2 at the six offsets) in LDS; the accumulation loop then only does table
-// lookups, with no modular arithmetic and no divergent polynomial evaluation.
-template <class R, int LM>
-__global__ void __launch_bounds__(64)
-k_spread_brick(int N, int K0, int K1, int K2, const int4* __restrict__ mbase, const R* __restrict__ wfrac,
-               const R* __restrict__ fq, const int* __restrict__ cstart, const int* __restrict__ catoms,
-               R* __restrict__ rgrid) {
-    constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
-    constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);   // derivative orders needed
-    __shared__ int col_lo[(BRX + 5) * (BRY + 5) * 2];
-    __shared__ int col_off[(BRX + 5) * (BRY + 5) * 2 + 1];
-    __shared__ int s_r[SP_CHUNK][3];
-    __shared__ R s_t[SP_CHUNK][3][NT][6];
-    __shared__ R s_q[SP_CHUNK][NQ];
-    const int f = blockIdx.y;
-    const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY;
-    const int b = blockIdx.x;
-    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
-    const int X0 = bx * BRX, Y0 = by * BRY, Z0 = bz * BRZ;
-    const int t = threadIdx.x;
-    const int lx = t / (BRY * BRZ), ly = (t / BRZ) % BRY, lz = t % BRZ;
-    const int px = X0 + lx, py = Y0 + ly, pz = Z0 + lz;
-    const bool own = px < K0 && py < K1 && pz < K2;
-    const size_t G = (size_t)K0 * K1 * K2;
-    const int* cs = cstart + (size_t)f * (G + 1);
-    const int* ca = catoms + (size_t)f * N;
-    const int nx = min(BRX + 5, K0), ny = min(BRY + 5, K1), nzr = min(BRZ + 5, K2);
-    const int ncol = nx * ny;
-    const int zs = wrapi(Z0 - 2, K2);
-    const int zlen1 = min(nzr, K2 - zs), zlen2 = nzr - zlen1;
-    for (int c = t; c < ncol * 2; c += 64) {
-        const int cc = c >> 1, seg = c & 1;
-        const int x = wrapi(X0 - 2 + cc / ny, K0), y = wrapi(Y0 - 2 + cc % ny, K1);
-        const size_t base = ((size_t)x * K1 + y) * K2;
-        int lo = 0, n = 0;
-        if (seg == 0) {
-            lo = cs[base + zs];
-            n = cs[base + zs + zlen1] - lo;
-        } else if (zlen2 > 0) {
-            lo = cs[base];
-            n = cs[base + zlen2] - lo;
-        }
-        col_lo[c] = lo;
-        col_off[c + 1] = n;
-    }
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        col_off[0] = 0;
-        for (int c = 1; c <= ncol * 2; ++c) { acc += col_off[c]; col_off[c] = acc; }
-    }
-    __syncthreads();
-    const int total = col_off[ncol * 2];
-    R acc = 0;
-    for (int c0 = 0; c0 < total; c0 += SP_CHUNK) {
-        const int nchunk = min(SP_CHUNK, total - c0);
-        if (t < nchunk) {
-            const int gidx = c0 + t;
-            int lo = 0, hi = ncol * 2;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (col_off[mid] <= gidx) lo = mid; else hi = mid;
-            }
-            const int atom = ca[col_lo[lo] + (gidx - col_off[lo])];
-            const size_t a = (size_t)f * N + atom;
-            const int4 m = mbase[a];
-            // support start (grid index of offset 0 is m-3) relative to the brick origin
-            s_r[t][0] = wrapi(m.x - 3 - X0 + 5, K0) - 5;
-            s_r[t][1] = wrapi(m.y - 3 - Y0 + 5, K1) - 5;
-            s_r[t][2] = wrapi(m.z - 3 - Z0 + 5, K2) - 5;
-            for (int d = 0; d < 3; ++d) {
-                const R w = wfrac[a * 3 + d];
-                for (int k = 0; k < 6; ++k) {
-                    R v, d1, d2;
-                    bs_piece<R>(k, w, v, d1, d2);
-                    s_t[t][d][0][k] = v;
-                    if (NT > 1) s_t[t][d][NT > 1 ? 1 : 0][k] = d1;
-                    if (NT > 2) s_t[t][d][NT > 2 ? 2 : 0][k] = d2;
-                }
-            }
-            for (int q = 0; q < NQ; ++q) s_q[t][q] = fq[a * 10 + q];
-        }
-        __syncthreads();
-        if (own) {
-            for (int s = 0; s < nchunk; ++s) {
-                const int k0 = lx - s_r[s][0], k1 = ly - s_r[s][1], k2 = lz - s_r[s][2];
-                if ((unsigned)k0 > 5u || (unsigned)k1 > 5u || (unsigned)k2 > 5u) continue;
-                const R v0 = s_t[s][0][0][k0], v1 = s_t[s][1][0][k1], v2 = s_t[s][2][0][k2];
-                R wgt = s_q[s][0] * v0 * v1 * v2;
-                if (LM >= 1) {
-                    const R a0 = s_t[s][0][NT > 1 ? 1 : 0][k0], a1 = s_t[s][1][NT > 1 ? 1 : 0][k1],
-                            a2 = s_t[s][2][NT > 1 ? 1 : 0][k2];
-                    wgt += s_q[s][NQ > 1 ? 1 : 0] * a0 * v1 * v2 + s_q[s][NQ > 2 ? 2 : 0] * v0 * a1 * v2 +
-                           s_q[s][NQ > 3 ? 3 : 0] * v0 * v1 * a2;
-                    if (LM >= 2) {
-                        const R b0 = s_t[s][0][NT > 2 ? 2 : 0][k0], b1 = s_t[s][1][NT > 2 ? 2 : 0][k1],
-                                b2 = s_t[s][2][NT > 2 ? 2 : 0][k2];
-                        wgt += s_q[s][NQ > 4 ? 4 : 0] * b0 * v1 * v2 + s_q[s][NQ > 5 ? 5 : 0] * v0 * b1 * v2 +
-                               s_q[s][NQ > 6 ? 6 : 0] * v0 * v1 * b2 +
-                               (R)2 * (s_q[s][NQ > 7 ? 7 : 0] * a0 * a1 * v2 + s_q[s][NQ > 8 ? 8 : 0] * a0 * v1 * a2 +
-                                       s_q[s][NQ > 9 ? 9 : 0] * v0 * a1 * a2);
-                    }
-                }
-                acc += wgt;
-            }
-        }
-        __syncthreads();
-    }
-    if (own) rgrid[(size_t)f * G + ((size_t)px * K1 + py) * K2 + pz] = acc;
-}
-
-// ---------------------------------------------------------------------------
-// FFT passes.  Lines are staged in LDS as [element][line]; one Stockham stage per
-// radix with direct f-point butterflies (any factor).  Twiddles tw[t] = e^{-2 pi i t/K}.
-// ---------------------------------------------------------------------------
-template <class R>
-__device__ inline Cplx<R>* stockham(Cplx<R>* in, Cplx<R>* out, int K, int LB, int nl, const FFTPlan& plan,
-                                    const Cplx<R>* __restrict__ tw, int inverse) {
-    int Ns = 1;
-    for (int s = 0; s < plan.nst; ++s) {
-        const int fr = plan.fac[s];
-        const int Kf = K / fr;
-        const int step = K / (Ns * fr);
-        for (int idx = threadIdx.x; idx < nl * K; idx += blockDim.x) {
-            const int li = idx % nl, tt = idx / nl;
-            const int j = tt / fr, k = tt % fr;
-            const int kj = j % Ns;
-            const int inc = ((kj + k * Ns) * step) % K;
-            int accI = 0;
-            R sx = 0, sy = 0;
-            for (int r = 0; r < fr; ++r) {
-                const Cplx<R> x = in[(j + r * Kf) * LB + li];
-                Cplx<R> w = tw[accI];
-                if (inverse) w.y = -w.y;
-                sx += x.x * w.x - x.y * w.y;
-                sy += x.x * w.y + x.y * w.x;
-                accI += inc;
-                if (accI >= K) accI -= K;
-            }
-            out[((j / Ns) * Ns * fr + kj + k * Ns) * LB + li] = Cplx<R>{sx, sy};
-        }
-        __syncthreads();
-        Cplx<R>* tmp = in; in = out; out = tmp;
-        Ns *= fr;
-    }
-    return in;
-}
-
-// real (K0,K1,K2) -> half spectrum (K0,K1,H), H = K2/2+1; LB complex lines (= 2 LB real lines)
-template <class R>
-__global__ void k_fft_z_r2c(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ tw,
-                            const R* __restrict__ rgrid, Cplx<R>* __restrict__ hgrid, int LB) {
-    extern __shared__ unsigned char smem_raw[];
-    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
-    Cplx<R>* b1 = b0 + (size_t)LB * K2;
-    const int f = blockIdx.y;
-    const int H = K2 / 2 + 1;
-    const int nrl = K0 * K1;                  // real lines
-    const int npairs = (nrl + 1) / 2;
-    const int p0 = blockIdx.x * LB;
-    const int nl = min(LB, npairs - p0);
-    if (nl <= 0) return;
-    const R* rg = rgrid + (size_t)f * K0 * K1 * K2;
-    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
-    for (int idx = threadIdx.x; idx < nl * K2; idx += blockDim.x) {
-        const int li = idx / K2, e = idx % K2;
-        const int la = 2 * (p0 + li), lb = la + 1;
-        const R a = rg[(size_t)la * K2 + e];
-        const R bb = lb < nrl ? rg[(size_t)lb * K2 + e] : (R)0;
-        b0[e * LB + li] = Cplx<R>{a, bb};
-    }
-    __syncthreads();
-    Cplx<R>* res = stockham<R>(b0, b1, K2, LB, nl, plan, tw, 0);
-    for (int idx = threadIdx.x; idx < nl * H; idx += blockDim.x) {
-        const int li = idx / H, k = idx % H;
-        const int la = 2 * (p0 + li), lb = la + 1;
-        const Cplx<R> Z = res[k * LB + li];
-        const Cplx<R> Zc = res[((K2 - k) % K2) * LB + li];
-        // A = (Z + conj Zc)/2 ; B = (Z - conj Zc)/(2i)
-        hg[(size_t)la * H + k] = Cplx<R>{(Z.x + Zc.x) * (R)0.5, (Z.y - Zc.y) * (R)0.5};
-        if (lb < nrl) hg[(size_t)lb * H + k] = Cplx<R>{(Z.y + Zc.y) * (R)0.5, (Zc.x - Z.x) * (R)0.5};
-    }
-}
-
-// half spectrum (K0,K1,H) -> real (K0,K1,K2), inverse, unnormalised
-template <class R>
-__global__ void k_fft_z_c2r(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ tw,
-                            const Cplx<R>* __restrict__ hgrid, R* __restrict__ rgrid, int LB) {
-    extern __shared__ unsigned char smem_raw[];
-    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
-    Cplx<R>* b1 = b0 + (size_t)LB * K2;
-    const int f = blockIdx.y;
-    const int H = K2 / 2 + 1;
-    const int nrl = K0 * K1;
-    const int npairs = (nrl + 1) / 2;
-    const int p0 = blockIdx.x * LB;
-    const int nl = min(LB, npairs - p0);
-    if (nl <= 0) return;
-    R* rg = rgrid + (size_t)f * K0 * K1 * K2;
-    const Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
-    for (int idx = threadIdx.x; idx < nl * K2; idx += blockDim.x) {
-        const int li = idx / K2, k = idx % K2;
-        const int la = 2 * (p0 + li), lb = la + 1;
-        Cplx<R> A, B;
-        if (k < H) {
-            A = hg[(size_t)la * H + k];
-            B = lb < nrl ? hg[(size_t)lb * H + k] : Cplx<R>{0, 0};
-        } else {
-            Cplx<R> a = hg[(size_t)la * H + (K2 - k)];
-            Cplx<R> bq = lb < nrl ? hg[(size_t)lb * H + (K2 - k)] : Cplx<R>{0, 0};
-            A = Cplx<R>{a.x, -a.y};
-            B = Cplx<R>{bq.x, -bq.y};
-        }
-        // Z = A + i B
-        b0[k * LB + li] = Cplx<R>{A.x - B.y, A.y + B.x};
-    }
-    __syncthreads();
-    Cplx<R>* res = stockham<R>(b0, b1, K2, LB, nl, plan, tw, 1);
-    for (int idx = threadIdx.x; idx < nl * K2; idx += blockDim.x) {
-        const int li = idx / K2, e = idx % K2;
-        const int la = 2 * (p0 + li), lb = la + 1;
-        const Cplx<R> z = res[e * LB + li];
-        rg[(size_t)la * K2 + e] = z.x;
-        if (lb < nrl) rg[(size_t)lb * K2 + e] = z.y;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -810,231 +576,6 @@ __global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan
         const int el = idx / FL, li = idx % FL;
         const int l = l0 + li;
         if (l < nlines) hg[(size_t)l + (size_t)el * stride] = res2[el * FL + li];
-    }
-    if (!ACC) return;
-    double vals[7] = {e, w00, w11, w22, w01, w02, w12};
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        double tsum = block_sum(vals[k], sh);
-        if (threadIdx.x == 0) part[((size_t)f * gridDim.x + blockIdx.x) * 7 + k] = tsum;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Plane kernels: one block per x-plane.  Forward: real (K1, K2) rows -> z R2C
-// (rows packed in pairs as complex lines) -> y C2C on the H columns -> half
-// spectrum plane (K1, H).  Inverse: the reverse.  Everything between the one
-// read and the one write of the plane stays in LDS; twiddles are staged too.
-// LDS: 2 S + K1 + K2 complex, S = max(ceil(K1/2) K2, K1 H).
-// ---------------------------------------------------------------------------
-__host__ __device__ inline int plane_S(int K1, int K2) {
-    const int a = ((K1 + 1) / 2) * K2, b = K1 * (K2 / 2 + 1);
-    return a > b ? a : b;
-}
-
-template <class R>
-__global__ void __launch_bounds__(256) k_fft_plane_fwd(int K0, int K1, int K2, FFTPlan pz, FFTPlan py,
-                                                       const Cplx<R>* __restrict__ twz, const Cplx<R>* __restrict__ twy,
-                                                       const R* __restrict__ rgrid, Cplx<R>* __restrict__ hgrid) {
-    extern __shared__ unsigned char smem_raw[];
-    const int H = K2 / 2 + 1, np = (K1 + 1) / 2, S = plane_S(K1, K2);
-    Cplx<R>* A = reinterpret_cast<Cplx<R>*>(smem_raw);
-    Cplx<R>* B = A + S;
-    Cplx<R>* tz = B + S;
-    Cplx<R>* ty = tz + K2;
-    const int x = blockIdx.x, f = blockIdx.y;
-    for (int k = threadIdx.x; k < K2; k += blockDim.x) tz[k] = twz[k];
-    for (int k = threadIdx.x; k < K1; k += blockDim.x) ty[k] = twy[k];
-    const R* rp = rgrid + ((size_t)f * K0 + x) * K1 * K2;
-    for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
-        const int li = idx / K2, e = idx - li * K2;
-        const int ya = 2 * li, yb = ya + 1;
-        const R va = rp[(size_t)ya * K2 + e];
-        const R vb = yb < K1 ? rp[(size_t)yb * K2 + e] : (R)0;
-        A[e * np + li] = Cplx<R>{va, vb};
-    }
-    __syncthreads();
-    Cplx<R>* res = stockham<R>(A, B, K2, np, np, pz, tz, 0);
-    Cplx<R>* oth = res == A ? B : A;
-    for (int idx = threadIdx.x; idx < np * H; idx += blockDim.x) {
-        const int li = idx / H, k = idx - li * H;
-        const int ya = 2 * li, yb = ya + 1;
-        const Cplx<R> Z = res[k * np + li];
-        const Cplx<R> Zc = res[((K2 - k) % K2) * np + li];
-        oth[ya * H + k] = Cplx<R>{(Z.x + Zc.x) * (R)0.5, (Z.y - Zc.y) * (R)0.5};
-        if (yb < K1) oth[yb * H + k] = Cplx<R>{(Z.y + Zc.y) * (R)0.5, (Zc.x - Z.x) * (R)0.5};
-    }
-    __syncthreads();
-    Cplx<R>* res2 = stockham<R>(oth, res, K1, H, H, py, ty, 0);
-    Cplx<R>* hp = hgrid + ((size_t)f * K0 + x) * K1 * H;
-    for (int idx = threadIdx.x; idx < K1 * H; idx += blockDim.x) hp[idx] = res2[idx];
-}
-
-template <class R>
-__global__ void __launch_bounds__(256) k_fft_plane_inv(int K0, int K1, int K2, FFTPlan pz, FFTPlan py,
-                                                       const Cplx<R>* __restrict__ twz, const Cplx<R>* __restrict__ twy,
-                                                       const Cplx<R>* __restrict__ hgrid, R* __restrict__ rgrid) {
-    extern __shared__ unsigned char smem_raw[];
-    const int H = K2 / 2 + 1, np = (K1 + 1) / 2, S = plane_S(K1, K2);
-    Cplx<R>* A = reinterpret_cast<Cplx<R>*>(smem_raw);
-    Cplx<R>* B = A + S;
-    Cplx<R>* tz = B + S;
-    Cplx<R>* ty = tz + K2;
-    const int x = blockIdx.x, f = blockIdx.y;
-    for (int k = threadIdx.x; k < K2; k += blockDim.x) tz[k] = twz[k];
-    for (int k = threadIdx.x; k < K1; k += blockDim.x) ty[k] = twy[k];
-    const Cplx<R>* hp = hgrid + ((size_t)f * K0 + x) * K1 * H;
-    for (int idx = threadIdx.x; idx < K1 * H; idx += blockDim.x) A[idx] = hp[idx];
-    __syncthreads();
-    Cplx<R>* res = stockham<R>(A, B, K1, H, H, py, ty, 1);  // plane layout [y][kz]
-    Cplx<R>* oth = res == A ? B : A;
-    for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
-        const int li = idx / K2, k = idx - li * K2;
-        const int ya = 2 * li, yb = ya + 1;
-        Cplx<R> a, bq;
-        if (k < H) {
-            a = res[ya * H + k];
-            bq = yb < K1 ? res[yb * H + k] : Cplx<R>{0, 0};
-        } else {
-            const Cplx<R> ac = res[ya * H + (K2 - k)];
-            const Cplx<R> bc = yb < K1 ? res[yb * H + (K2 - k)] : Cplx<R>{0, 0};
-            a = Cplx<R>{ac.x, -ac.y};
-            bq = Cplx<R>{bc.x, -bc.y};
-        }
-        oth[k * np + li] = Cplx<R>{a.x - bq.y, a.y + bq.x};
-    }
-    __syncthreads();
-    Cplx<R>* res2 = stockham<R>(oth, res, K2, np, np, pz, tz, 1);
-    R* rp = rgrid + ((size_t)f * K0 + x) * K1 * K2;
-    for (int idx = threadIdx.x; idx < np * K2; idx += blockDim.x) {
-        const int li = idx / K2, e = idx - li * K2;
-        const int ya = 2 * li, yb = ya + 1;
-        const Cplx<R> z = res2[e * np + li];
-        rp[(size_t)ya * K2 + e] = z.x;
-        if (yb < K1) rp[(size_t)yb * K2 + e] = z.y;
-    }
-}
-
-// complex lines along y of the half spectrum: lines (x, kz), stride H
-template <class R>
-__global__ void k_fft_y(int K0, int K1, int H, FFTPlan plan, const Cplx<R>* __restrict__ tw,
-                        Cplx<R>* __restrict__ hgrid, int inverse, int LB) {
-    extern __shared__ unsigned char smem_raw[];
-    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
-    Cplx<R>* b1 = b0 + (size_t)LB * K1;
-    const int f = blockIdx.y;
-    const int nlines = K0 * H;
-    const int l0 = blockIdx.x * LB;
-    const int nl = min(LB, nlines - l0);
-    if (nl <= 0) return;
-    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
-    for (int idx = threadIdx.x; idx < nl * K1; idx += blockDim.x) {
-        const int e = idx / nl, li = idx % nl;
-        const int l = l0 + li;
-        const size_t base = (size_t)(l / H) * K1 * H + (l % H);
-        b0[e * LB + li] = hg[base + (size_t)e * H];
-    }
-    __syncthreads();
-    Cplx<R>* res = stockham<R>(b0, b1, K1, LB, nl, plan, tw, inverse);
-    for (int idx = threadIdx.x; idx < nl * K1; idx += blockDim.x) {
-        const int e = idx / nl, li = idx % nl;
-        const int l = l0 + li;
-        const size_t base = (size_t)(l / H) * K1 * H + (l % H);
-        hg[base + (size_t)e * H] = res[e * LB + li];
-    }
-}
-
-// forward x FFT, influence function, inverse x FFT; lines (y, kz), stride K1*H.
-// Energy/box sums: multiplicity 2 for 0 < kz < K2/2 (the conjugate half), else 1.
-template <class R, bool ACC>
-__global__ void k_fft_x_infl(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ tw,
-                             Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
-                             const double* __restrict__ bmod0, const double* __restrict__ bmod1,
-                             const double* __restrict__ bmod2, double kappa, int kind, double pref,
-                             double* __restrict__ part, int LB) {
-    extern __shared__ unsigned char smem_raw[];
-    __shared__ double sh[16];
-    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
-    Cplx<R>* b1 = b0 + (size_t)LB * K0;
-    const int f = blockIdx.y;
-    const int H = K2 / 2 + 1;
-    const int nlines = K1 * H;
-    const int l0 = blockIdx.x * LB;
-    const int nl = min(LB, nlines - l0);
-    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
-    const size_t stride = (size_t)K1 * H;
-    double e = 0, w00 = 0, w11 = 0, w22 = 0, w01 = 0, w02 = 0, w12 = 0;
-    if (nl > 0) {
-        for (int idx = threadIdx.x; idx < nl * K0; idx += blockDim.x) {
-            const int el = idx / nl, li = idx % nl;
-            b0[el * LB + li] = hg[(size_t)(l0 + li) + (size_t)el * stride];
-        }
-        __syncthreads();
-        Cplx<R>* res = stockham<R>(b0, b1, K0, LB, nl, plan, tw, 0);
-        const BoxInfo& b = boxes[f];
-        for (int idx = threadIdx.x; idx < nl * K0; idx += blockDim.x) {
-            const int i0 = idx / nl, li = idx % nl;
-            const int l = l0 + li;
-            const int i1 = l / H, i2 = l % H;
-            const int m0 = i0 < (K0 + 1) / 2 ? i0 : i0 - K0;
-            const int m1 = i1 < (K1 + 1) / 2 ? i1 : i1 - K1;
-            const int m2 = i2 < (K2 + 1) / 2 ? i2 : i2 - K2;
-            const double kx = b.kb[0] * m0 + b.kb[1] * m1 + b.kb[2] * m2;
-            const double ky = b.kb[3] * m0 + b.kb[4] * m1 + b.kb[5] * m2;
-            const double kz = b.kb[6] * m0 + b.kb[7] * m1 + b.kb[8] * m2;
-            const double ksq = kx * kx + ky * ky + kz * kz;
-            const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
-            const double ith2 = 1.0 / (th * th);
-            const Cplx<R> S = res[i0 * LB + li];
-            const double s2 = (double)S.x * (double)S.x + (double)S.y * (double)S.y;
-            const double mult = (i2 == 0 || 2 * i2 == K2) ? 1.0 : 2.0;
-            double C = 0, dC = 0;
-            const double V = b.V;
-            if (kind == 0) {
-                if (!(i0 == 0 && i1 == 0 && i2 == 0)) {
-                    C = 2.0 * PI / V / ksq * exp(-ksq / (4.0 * kappa * kappa));
-                    dC = -C * (1.0 / ksq + 1.0 / (4.0 * kappa * kappa));
-                }
-            } else {
-                const double kq = ksq < 1e-16 ? 1e-16 : ksq;
-                const double x2 = kq / (4.0 * kappa * kappa), x = sqrt(x2);
-                const double ex = exp(-x2), ec = erfc(x);
-                double fv, fd, pw;
-                if (kind == 6) {
-                    fv = (1.0 - 2.0 * x2) * ex + 2.0 * x2 * x * SQRT_PI * ec;
-                    fd = -6.0 * x * ex + 6.0 * x2 * SQRT_PI * ec;
-                    pw = kappa * kappa * kappa / 3.0;
-                } else if (kind == 8) {
-                    fv = (3.0 - 2.0 * x2 + 4.0 * x2 * x2) * ex - 4.0 * x2 * x2 * x * SQRT_PI * ec;
-                    fd = (-10.0 * x + 20.0 * x2 * x) * ex - 20.0 * x2 * x2 * SQRT_PI * ec;
-                    pw = kappa * kappa * kappa * kappa * kappa / 45.0;
-                } else {
-                    const double x4 = x2 * x2;
-                    fv = (15.0 - 6.0 * x2 + 4.0 * x4 - 8.0 * x4 * x2) * ex + 8.0 * x4 * x2 * x * SQRT_PI * ec;
-                    fd = (-42.0 * x + 28.0 * x2 * x - 56.0 * x4 * x) * ex + 56.0 * x4 * x2 * SQRT_PI * ec;
-                    pw = pow(kappa, 7.0) / 1260.0;
-                }
-                const double c0 = SQRT_PI * PI / 2.0 / V * pw;
-                C = c0 * fv;
-                dC = ksq < 1e-16 ? 0.0 : c0 * fd / (8.0 * kappa * kappa * x);
-            }
-            if (ACC) {
-                const double wgt = pref * s2 * ith2;
-                e += mult * wgt * C;
-                const double wd = -mult * wgt * dC;
-                w00 += wd * kx * kx; w11 += wd * ky * ky; w22 += wd * kz * kz;
-                w01 += wd * kx * ky; w02 += wd * kx * kz; w12 += wd * ky * kz;
-            }
-            const double scale = 2.0 * pref * C * ith2;
-            res[i0 * LB + li] = Cplx<R>{(R)(S.x * scale), (R)(S.y * scale)};
-        }
-        __syncthreads();
-        Cplx<R>* other = (res == b0) ? b1 : b0;
-        Cplx<R>* res2 = stockham<R>(res, other, K0, LB, nl, plan, tw, 1);
-        for (int idx = threadIdx.x; idx < nl * K0; idx += blockDim.x) {
-            const int el = idx / nl, li = idx % nl;
-            hg[(size_t)(l0 + li) + (size_t)el * stride] = res2[el * LB + li];
-        }
     }
     if (!ACC) return;
     double vals[7] = {e, w00, w11, w22, w01, w02, w12};

@@ -433,30 +433,6 @@ __global__ void k_dots_final(int F, int nb, const double* __restrict__ part, dou
     out[f * 3 + 0] = a; out[f * 3 + 1] = b; out[f * 3 + 2] = m;
 }
 
-// PCG update step for frame f using coefficients in coef[f*4..]: see admp_api.hip
-//   mode 0: x += alpha p ; r -= alpha Ap ; z = Minv r          (alpha = coef[0])
-//   mode 1: p = z + beta p                                       (beta = coef[1])
-__global__ void k_pcg_update(int N, int mode, const double* __restrict__ coef, const int* __restrict__ active,
-                             const double* __restrict__ pol, double* __restrict__ x, double* __restrict__ r,
-                             double* __restrict__ z, double* __restrict__ p, const double* __restrict__ Ap) {
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || !active[f]) return;
-    const size_t a = (size_t)f * N + i;
-    const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
-    if (mode == 0) {
-        const double al = coef[f * 4 + 0];
-        for (int k = 0; k < 3; ++k) {
-            x[a * 3 + k] += al * p[a * 3 + k];
-            r[a * 3 + k] -= al * Ap[a * 3 + k];
-            z[a * 3 + k] = ap / DIELECTRIC * r[a * 3 + k];
-        }
-    } else {
-        const double be = coef[f * 4 + 1];
-        for (int k = 0; k < 3; ++k) p[a * 3 + k] = z[a * 3 + k] + be * p[a * 3 + k];
-    }
-}
-
 // generic elementwise: out = a*x + b*y (+ c*z)
 __global__ void k_lincomb(size_t n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
                           double* __restrict__ out) {
@@ -533,21 +509,6 @@ __global__ void k_add_self_b(int N, double f1, const double* __restrict__ Qc, do
     for (int k = 0; k < 3; ++k) b[a * 3 + k] += 2.0 * DIELECTRIC * f1 * Qc[a * 10 + 1 + k];
 }
 
-// z = Minv r, p = z   (Minv = pol/DIELECTRIC)
-__global__ void k_precond(int N, const double* __restrict__ pol, const double* __restrict__ r, double* __restrict__ z,
-                          double* __restrict__ p) {
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const size_t a = (size_t)f * N + i;
-    const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
-    for (int k = 0; k < 3; ++k) {
-        const double v = ap / DIELECTRIC * r[a * 3 + k];
-        z[a * 3 + k] = v;
-        if (p) p[a * 3 + k] = v;
-    }
-}
-
 // energy[f] = sum of the per-frame components in red[f*16 + 0..3] (+ rec[f*7])
 __global__ void k_energy_sum(int F, const double* __restrict__ red, const double* __restrict__ rec, double* __restrict__ e) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -608,62 +569,6 @@ __global__ void k_vadd(size_t n, const double* __restrict__ in, double* __restri
 }
 
 // ----------------------------------------------------------------- device-resident PCG
-
-// after dots(r, z): rz and max|r|; frames already below tol are done
-__global__ void k_pcg_init(int F, const double* __restrict__ dots, double tol, PcgState* __restrict__ st) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
-    PcgState s;
-    s.rz = dots[f * 3 + 0];
-    s.res = dots[f * 3 + 2];
-    s.alpha = s.beta = 0;
-    s.done = s.res < tol ? 1 : 0;
-    s.iter = 0;
-    s.pad0 = s.pad1 = 0;
-    st[f] = s;
-}
-
-// dots = (p.Ap, -, -): alpha = rz / p.Ap ; x += alpha p ; r -= alpha Ap ; z = Minv r
-__global__ void k_pcg_xr(int N, PcgState* __restrict__ st, const double* __restrict__ dots,
-                         const double* __restrict__ pol, double* __restrict__ x, double* __restrict__ r,
-                         double* __restrict__ z, const double* __restrict__ p, const double* __restrict__ Ap) {
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || st[f].done) return;
-    const double pAp = dots[f * 3 + 0];
-    const double al = pAp != 0.0 ? st[f].rz / pAp : 0.0;
-    const size_t a = (size_t)f * N + i;
-    const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
-    for (int k = 0; k < 3; ++k) {
-        x[a * 3 + k] += al * p[a * 3 + k];
-        const double rv = r[a * 3 + k] - al * Ap[a * 3 + k];
-        r[a * 3 + k] = rv;
-        z[a * 3 + k] = ap / DIELECTRIC * rv;
-    }
-}
-
-// dots = (r.z, -, max|r|): beta = rz_new / rz ; done when max|r| < tol
-__global__ void k_pcg_beta(int F, const double* __restrict__ dots, double tol, PcgState* __restrict__ st) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F || st[f].done) return;
-    const double rzn = dots[f * 3 + 0];
-    PcgState s = st[f];
-    s.beta = s.rz != 0.0 ? rzn / s.rz : 0.0;
-    s.rz = rzn;
-    s.iter += 1;
-    s.res = dots[f * 3 + 2];
-    if (s.res < tol) s.done = 1;
-    st[f] = s;
-}
-
-__global__ void k_pcg_p(int N, const PcgState* __restrict__ st, const double* __restrict__ z, double* __restrict__ p) {
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || st[f].done) return;
-    const size_t a = (size_t)f * N + i;
-    const double be = st[f].beta;
-    for (int k = 0; k < 3; ++k) p[a * 3 + k] = z[a * 3 + k] + be * p[a * 3 + k];
-}
 
 
 // ---------------------------------------------------------------------------

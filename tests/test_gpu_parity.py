@@ -361,3 +361,27 @@ def test_grid_sizes_vs_oracle(K, prec):
     print(f"K={K} {prec}: E rel {de:.2e} force relrms {df:.2e}", flush=True)
     etol, ftol = (1e-9, 1e-7) if prec == "fp64" else (2e-5, 1e-4)
     assert de < etol and df < ftol
+
+
+def test_per_frame_pairs_survive_capacity_growth():
+    """Pair lists set frame by frame, the second one longer (capacity grows):
+    the first frame's list must survive the reallocation (dmff/utils.py:65-84
+    semantics per frame), so each frame equals a single-frame evaluation."""
+    from dmff_amd.admp.engine import Engine
+    s, pos, box = _small_waters()
+    kappa, K1, K2, K3 = O.setup_ewald_parameters(6.0, 5e-4, box)
+    Ql = s.pme.Q_local[s.map_atomtype]
+    mS = s.pme.mScales
+    lists = [reference_pairs(pos, box, 4.0, s.bonded), reference_pairs(pos, box, 6.0, s.bonded)]
+    eng = Engine(len(pos), 2, False, True, kappa, (K1, K2, K3), 6.0, n_frames=2, device=DEV)
+    eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
+    for k in range(2):
+        eng.set_pairs(torch.as_tensor(lists[k], device=DEV), frame=k)
+    E, g, U, info = eng.compute(np.stack([pos, pos]), np.stack([box, box]), Ql, mScales=mS, want_grad=True)
+    for k in range(2):
+        e1 = Engine(len(pos), 2, False, True, kappa, (K1, K2, K3), 6.0, device=DEV)
+        e1.set_topology(s.axis_types, s.axis_idx, s.bonded)
+        e1.set_pairs(torch.as_tensor(lists[k], device=DEV))
+        Ek, gk, _, _ = e1.compute(pos, box, Ql, mScales=mS, want_grad=True)
+        assert abs(float(E[k]) - float(Ek[0])) <= 1e-12 * abs(float(Ek[0])), (k, float(E[k]), float(Ek[0]))
+        assert rel_rms(g["pos"][k].cpu().numpy(), gk["pos"][0].cpu().numpy()) < 1e-12
