@@ -278,6 +278,9 @@ def main():
     ap.add_argument("--legs", default="batch,large,serving",
                     help="extra measurements beside the headline: batch (64 frames), large (98,304 atoms with "
                          "dispersion and Slater), serving (request stream, p50/p99); 'none' for the headline only")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 256 MiB device-to-device tensor copy five times (the reference "
+                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); the ADMP path here is a "
@@ -285,6 +288,14 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    if args.calibrate_copy:
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: 5 x {src.numel() * 8} bytes read and written")
+        del src, dst
     D = setup(args.config)
     S = D["S"]
     log(f"config {args.config}: {S.n_atoms} atoms, kappa {D['kappa']:.6f}, K {D['K']}")
