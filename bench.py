@@ -145,7 +145,7 @@ def cpu_baseline(D, n_iter):
                        f"({n_iter}+1) dE/dU passes + E+grad pass = {t_eval:.1f} s")}
 
 
-def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005):
+def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005, profile=False):
     """64 perturbed frames of the 12,288-atom system in one batched call (trajectory fitting)."""
     import numpy as np
     import torch
@@ -160,8 +160,10 @@ def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005):
                  device=D["dev"])
     eng.set_topology(S.axis_types, S.axis_idx, S.bonded)
     N = S.n_atoms
+    holder = [eng]
 
     def ev():
+        eng = holder[0]
         eng.build_neighbors(pos, box)
         E, g, U, info = eng.compute(pos, box, D["Q"], D["pol"], D["thole"], S.mScales, S.pScales, S.dScales,
                                     solver="pcg", tol=1e-6, maxiter=500, want_grad=True)
@@ -180,11 +182,40 @@ def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     info = out[2]
+    holder[0] = None
     del eng
     torch.cuda.empty_cache()
-    return {"frames": n_frames, "ms_per_batch": dt * 1e3, "evals_per_s": n_frames / dt,
-            "solver_iters_max": int(max(info["n_iter"][:min(n_frames, 64)])),
-            "residual_max": float(max(info["residual"][:min(n_frames, 64)]))}
+    res = {"frames": n_frames, "ms_per_batch": dt * 1e3, "evals_per_s": n_frames / dt,
+           "solver_iters_max": int(max(info["n_iter"][:min(n_frames, 64)])),
+           "residual_max": float(max(info["residual"][:min(n_frames, 64)]))}
+    if profile:
+        # per-kernel launch durations of one batched evaluation (graphs off, HIP events
+        # around each launch on the stream it runs on); each launch covers all frames
+        os.environ["ADMP_NO_GRAPH"] = "1"
+        try:
+            eng = Engine(S.n_atoms, 2, True, True, D["kappa"], D["K"], D["rc"], n_frames=n_frames,
+                         precision=precision, device=D["dev"])
+        finally:
+            os.environ.pop("ADMP_NO_GRAPH", None)
+        eng.set_topology(S.axis_types, S.axis_idx, S.bonded)
+        holder[0] = eng
+        ev()
+        torch.cuda.synchronize()
+        eng.profile(True, True)
+        ev()
+        torch.cuda.synchronize()
+        prof = eng.profile_get()
+        eng.profile(False, True)
+        P = int(eng.get_pairs(0).shape[0])
+        holder[0] = None
+        del eng
+        torch.cuda.empty_cache()
+        Db = dict(D)
+        Db["n_pairs"] = P
+        res["kernel_us_per_launch"] = {k: v[0] / v[1] * 1e3 for k, v in prof.items() if v[1]}
+        res["roofline"] = {k: roofline(k, prof[k], Db, precision, frames=n_frames)
+                           for k in ("spread", "fft", "influence", "gather", "real_matvec") if prof.get(k, (0, 0))[1]}
+    return res
 
 
 def water_dispersion_slater(S, seed):
@@ -354,7 +385,7 @@ def main():
     if "batch" in legs:
         b = {}
         for prec in ("mixed", "fp64"):
-            b[prec] = leg_batch(D, prec)
+            b[prec] = leg_batch(D, prec, profile=(prec == "mixed"))
             log(f"batch64 {prec}: {b[prec]['ms_per_batch']:.1f} ms/batch, {b[prec]['evals_per_s']:.1f} evals/s")
         out["batch64"] = b
     if "large" in legs:
@@ -383,10 +414,10 @@ def main():
     print(json.dumps(out))
 
 
-def roofline(name, tm, D, prec):
+def roofline(name, tm, D, prec, frames=1):
     """Algorithmic bytes per launch of a kernel family / its measured average launch duration.
 
-    Per-launch algorithmic bytes (w = 4 B in the mixed path, 8 B in FP64; G = K0 K1 K2,
+    Per-launch algorithmic bytes per frame (w = 8 B: solver products are FP64 on both paths; G = K0 K1 K2,
     Gh = K0 K1 (K2/2 + 1); solver launches, which dominate, spread and gather dipoles only):
       spread     N (3*8 + 4 w) + G w           positions, fractional charge+dipole, grid written once
       fft        (2 G w + 12 Gh w) / 4         z R2C, y, y, z C2R passes of one transform pair
@@ -398,7 +429,8 @@ def roofline(name, tm, D, prec):
     t = ms / n / 1e3
     S = D["S"]
     N = S.n_atoms
-    w = 4 if prec == "mixed" else 8
+    # the solver launches, which dominate every family, run FP64 products on both paths
+    w = 8
     K = D["K"]
     G = K[0] * K[1] * K[2]
     Gh = K[0] * K[1] * (K[2] // 2 + 1)
@@ -408,12 +440,13 @@ def roofline(name, tm, D, prec):
         "fft": ("k_fz_r2c/k_fy/k_fz_c2r", (2 * G * w + 12 * Gh * w) / 4),
         "influence": ("k_fx_infl", 4 * Gh * w),
         "gather": ("k_gather_ml", G * w + N * 3 * 8 + N * 3 * 8),
-        "real_matvec": ("k_real_matvec", 2 * P * (4 + 2 * w) + N * 3 * (w + 8)),
+        "real_matvec": ("k_real_matvec", 2 * P * (4 + 2 * 8) + N * 3 * (8 + 8)),
     }
     if name not in table or (name == "real_matvec" and not P):
         return {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
                 "traffic": None, "avg_launch_us": t * 1e6}
     kname, bytes_ = table[name]
+    bytes_ *= frames
     ach = bytes_ / t / 1e9
     return {"kernel": kname, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": t * 1e6,

@@ -74,6 +74,7 @@ struct admp_ctx {
     int *d_gcell = nullptr, *d_gccount = nullptr, *d_gcstart = nullptr, *d_gcfill = nullptr, *d_gcatoms = nullptr;
     int *d_bcount = nullptr, *d_bstart = nullptr;  // per-brick candidate counts / offsets
     int2* d_blist = nullptr;                        // per-brick candidate lists
+    double* d_infl = nullptr;                       // tabulated electrostatic influence function (eg)
     long blist_cap = 0;
     int* d_scan_tmp = nullptr;  // tile totals of scan_rows
     size_t scan_cap = 0;
@@ -440,6 +441,8 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
                                               ((gg->K[2] + BRZ - 1) / BRZ));
         ALLOC(ctx->d_bcount, nbrmax * ctx->F);
         ALLOC(ctx->d_bstart, (nbrmax + 1) * ctx->F);
+        if (ctx->eg.G)
+            ALLOC(ctx->d_infl, (size_t)ctx->eg.K[0] * ctx->eg.K[1] * (ctx->eg.K[2] / 2 + 1) * ctx->F);
         ctx->blist_cap = (long)BRICK_MAXCAND * ctx->N;  // per frame
         ALLOC(ctx->d_blist, (size_t)ctx->blist_cap * ctx->F);
     }
@@ -795,6 +798,11 @@ static admp_status recip_positions(admp_ctx* ctx, const Grid& g, int use_dA) {
         hipLaunchKernelGGL(k_brick_cands<true>, dim3(nbr, F), dim3(64), 0, st, N, g.K[0], g.K[1], g.K[2],
                            ctx->d_mbase, ctx->d_gcstart, ctx->d_gcatoms, ctx->d_bcount, ctx->d_bstart, ctx->d_blist,
                            ctx->blist_cap);
+        if (&g == &ctx->eg && ctx->d_infl && !use_dA) {
+            const size_t nt = (size_t)g.K[0] * g.K[1] * (g.K[2] / 2 + 1);
+            hipLaunchKernelGGL(k_infl_table, dim3(nblk(nt, 256), F), dim3(256), 0, st, g.K[0], g.K[1], g.K[2],
+                               ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], ctx->cfg.kappa, DIELECTRIC, ctx->d_infl);
+        }
     }
     }
     HIPCHK(hipGetLastError());
@@ -869,13 +877,13 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     const int nbricks = ((K0 + BRX - 1) / BRX) * ((K1 + BRY - 1) / BRY) * ((K2 + BRZ - 1) / BRZ);
     { ProfScope ps_(ctx, 4);
     if (lmax_eff >= 2)
-        hipLaunchKernelGGL((k_spread_list<R, 2>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
+        hipLaunchKernelGGL((k_spread_list<R, 2>), dim3(xcd_blocks(nbricks), F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
                            ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
     else if (lmax_eff == 1)
-        hipLaunchKernelGGL((k_spread_list<R, 1>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
+        hipLaunchKernelGGL((k_spread_list<R, 1>), dim3(xcd_blocks(nbricks), F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
                            ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
     else
-        hipLaunchKernelGGL((k_spread_list<R, 0>), dim3(nbricks, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
+        hipLaunchKernelGGL((k_spread_list<R, 0>), dim3(xcd_blocks(nbricks), F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
                            ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
     }
     const int npairs = (K0 * K1 + 1) / 2;
@@ -891,11 +899,14 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     if (want_energy)
         hipLaunchKernelGGL((k_fx_infl<R, true>), dim3(nbx, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
                            g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
-                           pref, ctx->d_part, skip);
+                           pref, ctx->d_part, (const double*)nullptr, skip);
     else
         hipLaunchKernelGGL((k_fx_infl<R, false>), dim3(nbx, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
                            g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
-                           pref, ctx->d_part, skip); }
+                           pref, ctx->d_part,
+                           (&g == &ctx->eg && kind == 0 && !use_dA && pref == DIELECTRIC && kappa == ctx->cfg.kappa)
+                               ? ctx->d_infl : (const double*)nullptr,
+                           skip); }
     if (want_energy)
         hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbx, 7, ctx->d_part, ctx->d_rec, 0);
     { ProfScope ps_(ctx, 5);
@@ -905,13 +916,13 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
                        g.plan[2], twid<R>(g, 2), hg, rg, skip); }
     { ProfScope ps_(ctx, 7);
     if (gather_mode == 1)
-        hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2,
+        hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
-                           vir, skip);
+                           vir, ctx->d_gcatoms, skip);
     else
-        hipLaunchKernelGGL((k_gather_ml<R, 0>), dim3(nblk((size_t)N * 8, 256), F), dim3(256), 0, st, N, K0, K1, K2,
+        hipLaunchKernelGGL((k_gather_ml<R, 0>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
-                           vir, skip); }
+                           vir, ctx->d_gcatoms, skip); }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }

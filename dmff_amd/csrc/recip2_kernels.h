@@ -78,6 +78,15 @@ __global__ void k_grid_cell(int N, int K0, int K1, int K2, const int4* __restric
     atomicAdd(&ccount[(size_t)f * K0 * K1 * K2 + id], 1);
 }
 
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with its own L2.  Work items that share data (neighbouring bricks, sites of one
+// grid region) are therefore renumbered so that each XCD receives one contiguous
+// range: physical block p runs logical item (p % 8) * ceil(n / 8) + p / 8.  The
+// grid is launched with 8 * ceil(n / 8) blocks; logical items >= n return.
+constexpr int N_XCD = 8;
+__host__ __device__ inline int xcd_blocks(int n) { return ((n + N_XCD - 1) / N_XCD) * N_XCD; }
+__device__ inline int xcd_item(int p, int n) { return (p % N_XCD) * ((n + N_XCD - 1) / N_XCD) + p / N_XCD; }
+
 constexpr int BRX = 4, BRY = 4, BRZ = 4;   // 64 grid points per wavefront
 constexpr int SP_CHUNK = 32;               // sites staged in LDS per round
 
@@ -191,7 +200,8 @@ k_spread_list(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const 
     const int f = blockIdx.y;
     const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY, nbx = (K0 + BRX - 1) / BRX;
     const int nbr = nbx * nby * nbz;
-    const int b = blockIdx.x;
+    const int b = xcd_item(blockIdx.x, nbr);
+    if (b >= nbr) return;
     const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
     const int t = threadIdx.x;
     const int lx = t / (BRY * BRZ), ly = (t / BRZ) % BRY, lz = t % BRZ;
@@ -489,7 +499,8 @@ __global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan
                                                 Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
                                                 const double* __restrict__ bmod0, const double* __restrict__ bmod1,
                                                 const double* __restrict__ bmod2, double kappa, int kind, double pref,
-                                                double* __restrict__ part, const PcgState* __restrict__ skip) {
+                                                double* __restrict__ part, const double* __restrict__ tab,
+                                                const PcgState* __restrict__ skip) {
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
     extern __shared__ unsigned char smem_raw[];
     __shared__ double sh[16];
@@ -527,6 +538,12 @@ __global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan
         const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
         const double ith2 = 1.0 / (th * th);
         const Cplx<R> S = res[i0 * FL + li];
+        if (!ACC && tab) {
+            // tabulated 2 pref C_k / theta_k^2 (electrostatic solver products)
+            const double scale = tab[(size_t)f * K0 * nlines + (size_t)i0 * nlines + l];
+            res[i0 * FL + li] = Cplx<R>{(R)(S.x * scale), (R)(S.y * scale)};
+            continue;
+        }
         double C = 0, dC = 0;
         const double V = b.V;
         if (kind == 0) {
@@ -586,6 +603,33 @@ __global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan
     }
 }
 
+// 2 pref C_k / theta_k^2 for the electrostatic influence function (kind 0,
+// recip.py:474-475; k = 0 excluded) on the half spectrum, laid out as the x pass
+// reads it: [frame][i0][line], line = ky * H + kz.  Depends on the box only.
+__global__ void k_infl_table(int K0, int K1, int K2, const BoxInfo* __restrict__ boxes,
+                             const double* __restrict__ bmod0, const double* __restrict__ bmod1,
+                             const double* __restrict__ bmod2, double kappa, double pref, double* __restrict__ tab) {
+    const int f = blockIdx.y;
+    const int H = K2 / 2 + 1;
+    const int nlines = K1 * H;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)K0 * nlines) return;
+    const int i0 = (int)(idx / nlines), l = (int)(idx % nlines);
+    const int i1 = l / H, i2 = l % H;
+    const BoxInfo& b = boxes[f];
+    const int m0 = i0 < (K0 + 1) / 2 ? i0 : i0 - K0;
+    const int m1 = i1 < (K1 + 1) / 2 ? i1 : i1 - K1;
+    const int m2 = i2 < (K2 + 1) / 2 ? i2 : i2 - K2;
+    const double kx = b.kb[0] * m0 + b.kb[1] * m1 + b.kb[2] * m2;
+    const double ky = b.kb[3] * m0 + b.kb[4] * m1 + b.kb[5] * m2;
+    const double kz = b.kb[6] * m0 + b.kb[7] * m1 + b.kb[8] * m2;
+    const double ksq = kx * kx + ky * ky + kz * kz;
+    const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
+    double C = 0;
+    if (!(i0 == 0 && i1 == 0 && i2 == 0)) C = 2.0 * PI / b.V / ksq * exp(-ksq / (4.0 * kappa * kappa));
+    tab[(size_t)f * K0 * nlines + idx] = 2.0 * pref * C / (th * th);
+}
+
 // ---------------------------------------------------------------------------
 // Gather with eight lanes per site (lanes 0..5 own one x-offset each).
 // MODE 1: dE/dmu only (solver products) into out3.  MODE 0: everything the
@@ -599,14 +643,17 @@ k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, cons
             int use_dA, const int4* __restrict__ mbase, const R* __restrict__ wfrac, const R* __restrict__ fq,
             const double* __restrict__ Qc, const double* __restrict__ U, int qmode, const R* __restrict__ rgrid,
             double* __restrict__ out3, double* __restrict__ g_q, double* __restrict__ g_pos,
-            double* __restrict__ vir_atom, const PcgState* __restrict__ skip) {
+            double* __restrict__ vir_atom, const int* __restrict__ order, const PcgState* __restrict__ skip) {
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
     constexpr int ND = MODE == 1 ? 2 : 4;  // derivative orders needed
     const int f = blockIdx.y;
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt >> 3, lane = gt & 7;
-    const bool active = i < N;
-    const int ii = active ? i : 0;
+    // 32 sites per block, taken in cell-sorted order (order[]) and XCD-aware
+    const int nsb = (N + 31) / 32;
+    const int sb = xcd_item(blockIdx.x, nsb);
+    const int k = sb * 32 + (int)(threadIdx.x >> 3), lane = threadIdx.x & 7;
+    const bool active = sb < nsb && k < N;
+    const int i = active ? (order ? order[(size_t)f * N + k] : k) : 0;
+    const int ii = i;
     const size_t a = (size_t)f * N + ii;
     const BoxInfo& b = boxes[f];
     const double* A = use_dA ? b.dA : b.A;

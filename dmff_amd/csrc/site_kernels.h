@@ -610,6 +610,31 @@ __device__ inline double parts_total(const double* __restrict__ p, int nb, doubl
     return t;
 }
 
+// four totals at once (max, max, sum, sum) of partial arrays a, b, c, d; valid in
+// every thread; sh holds >= 64 doubles
+__device__ inline void parts_total4(const double* __restrict__ a, const double* __restrict__ b,
+                                    const double* __restrict__ c, const double* __restrict__ d, int nb, double* sh,
+                                    double& ma, double& mb, double& sc, double& sd) {
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        v0 = fmax(v0, a[k]);
+        v1 = fmax(v1, b[k]);
+        v2 += c[k];
+        v3 += d[k];
+    }
+    v0 = wave_allmax(v0); v1 = wave_allmax(v1); v2 = wave_allsum(v2); v3 = wave_allsum(v3);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) { sh[w] = v0; sh[16 + w] = v1; sh[32 + w] = v2; sh[48 + w] = v3; }
+    __syncthreads();
+    ma = sh[0]; mb = sh[16]; sc = sh[32]; sd = sh[48];
+    const int nw = (int)(blockDim.x >> 6);
+    for (int k = 1; k < nw; ++k) {
+        ma = fmax(ma, sh[k]); mb = fmax(mb, sh[16 + k]); sc += sh[32 + k]; sd += sh[48 + k];
+    }
+    __syncthreads();
+}
+
 // block reduction of one value into thread 0
 template <bool MAX>
 __device__ inline double block_red(double v, double* sh) {
@@ -734,7 +759,7 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_pcg_dir(int N, int nb, int nx, in
                                                        double* __restrict__ p, R* __restrict__ pR,
                                                        R* __restrict__ wfrac, R* __restrict__ fq,
                                                        PcgState* __restrict__ st) {
-    __shared__ double sh[16];
+    __shared__ double sh4[64];
     const int f = blockIdx.y;
     if (first == 3) {
         // refresh the derived arrays of the current p (change of product precision)
@@ -756,15 +781,12 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_pcg_dir(int N, int nb, int nx, in
         return;
     }
     const double* P = parts + (size_t)f * PCG_NPART * nb;
-    const double res = parts_total<true>(P + (3 + nx) * nb, nb, sh);
-    const double res_prev = first ? 0.0 : parts_total<true>(P + (3 + (nx ^ 1)) * nb, nb, sh);
+    double res, res_prev, rzn, rzo;
+    parts_total4(P + (3 + nx) * nb, P + (3 + (nx ^ 1)) * nb, P + (1 + nx) * nb, P + (1 + (nx ^ 1)) * nb, nb, sh4, res,
+                 res_prev, rzn, rzo);
+    if (first) res_prev = 0.0;
     const bool done = res < tol;
-    double beta = 0.0;
-    if (!first) {
-        const double rzn = parts_total<false>(P + (1 + nx) * nb, nb, sh);
-        const double rzo = parts_total<false>(P + (1 + (nx ^ 1)) * nb, nb, sh);
-        beta = rzo != 0.0 ? rzn / rzo : 0.0;
-    }
+    const double beta = (!first && rzo != 0.0) ? rzn / rzo : 0.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         PcgState s = st[f];
         if (first == 2) {
