@@ -876,15 +876,21 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     const int H = K2 / 2 + 1;
     const int nbricks = ((K0 + BRX - 1) / BRX) * ((K1 + BRY - 1) / BRY) * ((K2 + BRZ - 1) / BRZ);
     { ProfScope ps_(ctx, 4);
-    if (lmax_eff >= 2)
-        hipLaunchKernelGGL((k_spread_list<R, 2>), dim3(xcd_blocks(nbricks), F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
+    // one brick per wavefront while that fills the GPU (latency); four bricks per
+    // wavefront with column accumulation for large launches (instruction count)
+    const bool col = (long)nbricks * F >= 16384;
+    const unsigned nb1 = xcd_blocks(nbricks), nb4 = xcd_blocks((nbricks + 3) / 4);
+#define ADMP_SPREAD(LMv)                                                                                              \
+    if (col)                                                                                                         \
+        hipLaunchKernelGGL((k_spread_col<R, LMv>), dim3(nb4, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,           \
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);                                   \
+    else                                                                                                             \
+        hipLaunchKernelGGL((k_spread_list<R, LMv>), dim3(nb1, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,          \
                            ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
-    else if (lmax_eff == 1)
-        hipLaunchKernelGGL((k_spread_list<R, 1>), dim3(xcd_blocks(nbricks), F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
-                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
-    else
-        hipLaunchKernelGGL((k_spread_list<R, 0>), dim3(xcd_blocks(nbricks), F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,
-                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
+    if (lmax_eff >= 2) { ADMP_SPREAD(2) }
+    else if (lmax_eff == 1) { ADMP_SPREAD(1) }
+    else { ADMP_SPREAD(0) }
+#undef ADMP_SPREAD
     }
     const int npairs = (K0 * K1 + 1) / 2;
     const int nly = K0 * H, nlx = K1 * H;

@@ -263,6 +263,113 @@ k_spread_list(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const 
 }
 
 // ---------------------------------------------------------------------------
+// Column spread: four bricks per wavefront (16 lanes each); a lane owns one
+// (x, y) column of the brick's 4 z points.  Per candidate the lane forms the
+// x-y factors once, as coefficients of the z value, first and second
+// derivative tables, and updates its 4 points from them:
+//   w(x,y,z) = C0 vz + C1 az + C2 bz
+//   C0 = q vx vy + mx ax vy + my vx ay + Txx bx vy + Tyy vx by + 2 Txy ax ay
+//   C1 = mz vx vy + 2 Txz ax vy + 2 Tyz vx ay,   C2 = Tzz vx vy
+// (v: value, a: first, b: second derivative tables per axis).  Candidates of
+// every brick are staged SPC at a time (one per lane); the order of the sum
+// over candidates is fixed, so results are bitwise reproducible.
+// ---------------------------------------------------------------------------
+constexpr int SPC = 16;   // candidates per brick per staging round
+
+template <class R, int LM>
+__global__ void __launch_bounds__(64)
+k_spread_col(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R* __restrict__ fq,
+             const int* __restrict__ bstart, const int2* __restrict__ blist, long cap, R* __restrict__ rgrid,
+             const PcgState* __restrict__ skip) {
+    static_assert(BRX == 4 && BRY == 4 && BRZ == 4, "column spread assumes 4x4x4 bricks");
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
+    constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
+    constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
+    __shared__ int s_r[4][SPC];
+    __shared__ R s_t[4][SPC][3][NT][6];
+    __shared__ R s_q[4][SPC][NQ];
+    const int f = blockIdx.y;
+    const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY, nbx = (K0 + BRX - 1) / BRX;
+    const int nbr = nbx * nby * nbz;
+    const int nq4 = (nbr + 3) / 4;
+    const int qb = xcd_item(blockIdx.x, nq4);
+    if (qb >= nq4) return;
+    const int t = threadIdx.x;
+    const int g = t >> 4, c = t & 15;
+    const int b = 4 * qb + g;                      // this lane's brick
+    const bool live = b < nbr;
+    const int bb = live ? b : 0;
+    const int bz = bb % nbz, by = (bb / nbz) % nby, bx = bb / (nbz * nby);
+    const int lx = c >> 2, ly = c & 3;
+    const int px = bx * BRX + lx, py = by * BRY + ly, pz0 = bz * BRZ;
+    const int lo = live ? bstart[(size_t)f * (nbr + 1) + b] : 0;
+    const int total = live ? (int)min((long)bstart[(size_t)f * (nbr + 1) + b + 1], cap) - lo : 0;
+    // rounds needed by the busiest brick of the wavefront
+    int rounds = (total + SPC - 1) / SPC;
+    for (int o = 16; o < 64; o <<= 1) rounds = max(rounds, __shfl_xor(rounds, o, 64));
+    R acc[4] = {0, 0, 0, 0};
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int c0 = rd * SPC;
+        const int nch = max(0, min(SPC, total - c0));
+        if (c < nch) {
+            const int2 e = blist[(size_t)f * cap + lo + c0 + c];
+            const size_t a = (size_t)f * N + e.x;
+            s_r[g][c] = e.y;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const R w = wfrac[a * 3 + d];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    R v, d1, d2;
+                    bs_piece<R>(k, w, v, d1, d2);
+                    s_t[g][c][d][0][k] = v;
+                    if (NT > 1) s_t[g][c][d][NT > 1 ? 1 : 0][k] = d1;
+                    if (NT > 2) s_t[g][c][d][NT > 2 ? 2 : 0][k] = d2;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) s_q[g][c][q] = fq[a * 10 + q];
+        }
+        __syncthreads();
+        for (int s2 = 0; s2 < nch; ++s2) {
+            const int e = s_r[g][s2];
+            const int k0 = lx + 5 - (e & 15), k1 = ly + 5 - ((e >> 4) & 15);
+            if ((unsigned)k0 > 5u || (unsigned)k1 > 5u) continue;
+            const int kz = 5 - ((e >> 8) & 15);   // z table index of the brick's first point
+            const R vx = s_t[g][s2][0][0][k0], vy = s_t[g][s2][1][0][k1];
+            R C0 = s_q[g][s2][0] * vx * vy, C1 = 0, C2 = 0;
+            if (LM >= 1) {
+                const R ax = s_t[g][s2][0][NT > 1 ? 1 : 0][k0], ay = s_t[g][s2][1][NT > 1 ? 1 : 0][k1];
+                C0 += s_q[g][s2][NQ > 1 ? 1 : 0] * ax * vy + s_q[g][s2][NQ > 2 ? 2 : 0] * vx * ay;
+                C1 = s_q[g][s2][NQ > 3 ? 3 : 0] * vx * vy;
+                if (LM >= 2) {
+                    const R bxv = s_t[g][s2][0][NT > 2 ? 2 : 0][k0], byv = s_t[g][s2][1][NT > 2 ? 2 : 0][k1];
+                    C0 += s_q[g][s2][NQ > 4 ? 4 : 0] * bxv * vy + s_q[g][s2][NQ > 5 ? 5 : 0] * vx * byv +
+                          (R)2 * s_q[g][s2][NQ > 7 ? 7 : 0] * ax * ay;
+                    C1 += (R)2 * (s_q[g][s2][NQ > 8 ? 8 : 0] * ax * vy + s_q[g][s2][NQ > 9 ? 9 : 0] * vx * ay);
+                    C2 = s_q[g][s2][NQ > 6 ? 6 : 0] * vx * vy;
+                }
+            }
+#pragma unroll
+            for (int lz = 0; lz < 4; ++lz) {
+                const int k2 = kz + lz;
+                if ((unsigned)k2 > 5u) continue;
+                R w = C0 * s_t[g][s2][2][0][k2];
+                if (LM >= 1) w += C1 * s_t[g][s2][2][NT > 1 ? 1 : 0][k2];
+                if (LM >= 2) w += C2 * s_t[g][s2][2][NT > 2 ? 2 : 0][k2];
+                acc[lz] += w;
+            }
+        }
+        __syncthreads();
+    }
+    if (!live || px >= K0 || py >= K1) return;
+    R* col = rgrid + (size_t)f * K0 * K1 * K2 + ((size_t)px * K1 + py) * K2;
+#pragma unroll
+    for (int lz = 0; lz < 4; ++lz)
+        if (pz0 + lz < K2) col[pz0 + lz] = acc[lz];
+}
+
+// ---------------------------------------------------------------------------
 // Line FFT engine.  L lines (compile time) live in LDS as [element][line]; the
 // block's threads split into L lanes (one per line) times T = blockDim / L
 // butterfly slots.  One Stockham autosort stage per factor r of K: butterfly b

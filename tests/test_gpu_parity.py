@@ -336,11 +336,12 @@ def test_mixed_batched_frames_converge(fp32_products):
         assert np.linalg.norm(Gm[k] - Gd[k]) / np.linalg.norm(Gd[k]) < 1e-4
 
 
-@pytest.mark.parametrize("K", [(16, 16, 16), (20, 24, 15), (32, 27, 25), (21, 22, 26), (6, 7, 8)])
+@pytest.mark.parametrize("K", [(16, 16, 16), (20, 24, 15), (32, 27, 25), (21, 22, 26), (6, 7, 8), (104, 104, 104)])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_grid_sizes_vs_oracle(K, prec):
     """Energies and forces on grids whose sizes exercise every FFT radix path
-    (4, 2, 3, 5, 7, 11, 13 and the smallest size the line passes accept),
+    (4, 2, 3, 5, 7, 11, 13), the scatter path of grids below 9 points per axis,
+    and (104^3: 17,576 bricks) the column spread used for large launches,
     against the oracle on the same grid."""
     from dmff_amd.admp.engine import Engine
     s, pos, box = _small_waters()
