@@ -213,7 +213,7 @@ def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005, prof
         Db = dict(D)
         Db["n_pairs"] = P
         res["kernel_us_per_launch"] = {k: v[0] / v[1] * 1e3 for k, v in prof.items() if v[1]}
-        res["roofline"] = {k: roofline(k, prof[k], Db, precision, frames=n_frames)
+        res["roofline"] = {k: roofline(k, prof[k], Db, precision, frames=n_frames, prof=prof)
                            for k in ("spread", "fft", "influence", "gather", "real_matvec") if prof.get(k, (0, 0))[1]}
     return res
 
@@ -356,7 +356,7 @@ def main():
     prof = mix["prof"]
     cand = {k: v for k, v in kprof.items() if v[1] and k not in ("solver_vec", "solver_graph")}
     dom = max(cand.items(), key=lambda x: x[1][0])
-    roof = roofline(dom[0], dom[1], D, "mixed")
+    roof = roofline(dom[0], dom[1], D, "mixed", prof=kprof)
     out = {
         "metric": "ADMP E+F+grad evals/s @12,288 atoms (mixed precision); FP64 beside it",
         "value": 1.0 / mix["dt"],
@@ -414,16 +414,21 @@ def main():
     print(json.dumps(out))
 
 
-def roofline(name, tm, D, prec, frames=1):
+def roofline(name, tm, D, prec, frames=1, prof=None):
     """Algorithmic bytes per launch of a kernel family / its measured average launch duration.
 
     Per-launch algorithmic bytes per frame (w = 8 B: solver products are FP64 on both paths; G = K0 K1 K2,
     Gh = K0 K1 (K2/2 + 1); solver launches, which dominate, spread and gather dipoles only):
       spread     N (3*8 + 4 w) + G w           positions, fractional charge+dipole, grid written once
-      fft        (2 G w + 12 Gh w) / 4         z R2C, y, y, z C2R passes of one transform pair
-      influence  4 Gh w                        x pass: read + write the half spectrum (fused influence)
       gather     G w + N 3*8 + N 3*8           grid, positions, field out
       real_matvec 2P (4 + 2 w) + N 3 (w + 8)   ELL index + two stored coefficients per ordered pair
+    FFT, fused plane path (two z launches per transform pair, one plane launch):
+      fft        G w + 2 Gh w                  z R2C or C2R: real grid one way, half spectrum the other
+      influence  4 Gh w + 8 Gh                 plane pass: half spectrum in and out, tabulated influence
+    FFT, line path (four z/y launches per transform pair, one x launch):
+      fft        (2 G w + 12 Gh w) / 4         z R2C, y, y, z C2R passes
+      influence  4 Gh w + 8 Gh                 x pass: half spectrum in and out, tabulated influence
+    The path is read off the launch counts: 2 or 4 fft launches per influence launch.
     """
     ms, n = tm
     t = ms / n / 1e3
@@ -435,10 +440,14 @@ def roofline(name, tm, D, prec, frames=1):
     G = K[0] * K[1] * K[2]
     Gh = K[0] * K[1] * (K[2] // 2 + 1)
     P = D.get("n_pairs", 0)
+    plane = False
+    if prof and prof.get("influence", (0, 0))[1]:
+        plane = prof.get("fft", (0, 0))[1] < 3 * prof["influence"][1]
     table = {
         "spread": ("k_spread_list", N * (3 * 8 + 4 * w) + G * w),
-        "fft": ("k_fz_r2c/k_fy/k_fz_c2r", (2 * G * w + 12 * Gh * w) / 4),
-        "influence": ("k_fx_infl", 4 * Gh * w),
+        "fft": ("k_fz_r2c/k_fz_c2r", G * w + 2 * Gh * w) if plane else
+               ("k_fz_r2c/k_fy/k_fz_c2r", (2 * G * w + 12 * Gh * w) / 4),
+        "influence": ("k_fxy_plane" if plane else "k_fx_infl", 4 * Gh * w + 8 * Gh),
         "gather": ("k_gather_ml", G * w + N * 3 * 8 + N * 3 * 8),
         "real_matvec": ("k_real_matvec", 2 * P * (4 + 2 * 8) + N * 3 * (8 + 8)),
     }

@@ -30,6 +30,7 @@ struct Grid {
     void* tw[3] = {nullptr, nullptr, nullptr};   // R precision
     void* tw64[3] = {nullptr, nullptr, nullptr}; // double twiddles (used by the FP64 grids)
     double* bmod[3] = {nullptr, nullptr, nullptr};
+    bool plane[2] = {false, false};  // fused plane FFT pass usable ([0] float, [1] double arithmetic)
 };
 
 }  // namespace
@@ -75,6 +76,7 @@ struct admp_ctx {
     int *d_bcount = nullptr, *d_bstart = nullptr;  // per-brick candidate counts / offsets
     int2* d_blist = nullptr;                        // per-brick candidate lists
     double* d_infl = nullptr;                       // tabulated electrostatic influence function (eg)
+    int infl_pm = 0;                                // its layout: plane-major (1) or line-major (0)
     long blist_cap = 0;
     int* d_scan_tmp = nullptr;  // tile totals of scan_rows
     size_t scan_cap = 0;
@@ -244,6 +246,9 @@ static std::vector<double> bspline_moduli(int K) {
     return out;
 }
 
+// dynamic LDS of k_fxy_plane: two K0 x K1 planes and the x and y twiddles
+static size_t plane_smem(int K0, int K1, size_t cs) { return ((size_t)2 * K0 * K1 + K0 + K1) * cs; }
+
 static admp_status setup_grid(admp_ctx* ctx, Grid& g, int K0, int K1, int K2, bool mixed) {
     g.K[0] = K0; g.K[1] = K1; g.K[2] = K2;
     g.G = (size_t)K0 * K1 * K2;
@@ -279,6 +284,31 @@ static admp_status setup_grid(admp_ctx* ctx, Grid& g, int K0, int K1, int K2, bo
         HIPCHK(dalloc(ctx, &g.bmod[d], K));
         HIPCHK(hipMemcpyAsync(g.bmod[d], bm.data(), K * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    // The fused plane pass keeps two K0 x K1 complex planes and both twiddle
+    // tables in LDS; it is used where that fits the device's per-block limit.
+    const char* env = std::getenv("ADMP_FFT_PLANE");
+    const bool allow = !(env && env[0] == '0') && K0 >= BRX + 5 && K1 >= BRY + 5 && K2 >= BRZ + 5;
+    int dev = 0, lim = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    for (int w = 0; w < 2; ++w) {
+        const size_t cs = w ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
+        const size_t sm = plane_smem(K0, K1, cs);
+        g.plane[w] = allow && sm <= (size_t)lim;
+        if (!g.plane[w]) continue;
+        hipError_t e1, e2;
+        if (w) {
+            e1 = hipFuncSetAttribute((const void*)k_fxy_plane<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+            e2 = hipFuncSetAttribute((const void*)k_fxy_plane<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        } else {
+            e1 = hipFuncSetAttribute((const void*)k_fxy_plane<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+            e2 = hipFuncSetAttribute((const void*)k_fxy_plane<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+        }
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            (void)hipGetLastError();
+            g.plane[w] = false;
+        }
     }
     return ADMP_OK;
 }
@@ -800,8 +830,11 @@ static admp_status recip_positions(admp_ctx* ctx, const Grid& g, int use_dA) {
                            ctx->blist_cap);
         if (&g == &ctx->eg && ctx->d_infl && !use_dA) {
             const size_t nt = (size_t)g.K[0] * g.K[1] * (g.K[2] / 2 + 1);
+            // laid out for the FP64 solver products' FFT path
+            ctx->infl_pm = g.plane[1] ? 1 : 0;
             hipLaunchKernelGGL(k_infl_table, dim3(nblk(nt, 256), F), dim3(256), 0, st, g.K[0], g.K[1], g.K[2],
-                               ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], ctx->cfg.kappa, DIELECTRIC, ctx->d_infl);
+                               ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], ctx->cfg.kappa, DIELECTRIC, ctx->infl_pm,
+                               ctx->d_infl);
         }
     }
     }
@@ -895,31 +928,66 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     const int npairs = (K0 * K1 + 1) / 2;
     const int nly = K0 * H, nlx = K1 * H;
     const size_t Cs = sizeof(Cplx<R>);
-    { ProfScope ps_(ctx, 5);
-    hipLaunchKernelGGL(k_fz_r2c<R>, dim3(nblk(npairs, FL), F), dim3(FTH), (2 * FL + 1) * K2 * Cs, st, K0, K1, K2,
-                       g.plan[2], twid<R>(g, 2), rg, hg, skip);
-    hipLaunchKernelGGL(k_fy<R>, dim3(nblk(nly, FL), F), dim3(FTH), (2 * FL + 1) * K1 * Cs, st, K0, K1, H, g.plan[1],
-                       twid<R>(g, 1), hg, 0, skip); }
-    const int nbx = (int)nblk(nlx, FL);
-    { ProfScope ps_(ctx, 6);
+    const int pm = g.plane[sizeof(R) == 8 ? 1 : 0] ? 1 : 0;
+    // z passes: 8 complex lines (16 real rows) per block where LDS allows, else 4
+    const bool z8 = (size_t)(2 * 8 + 1) * K2 * Cs <= 48 * 1024;
+    auto zpass = [&](int inverse) {
+        ProfScope ps_(ctx, 5);
+        if (!inverse) {
+            if (z8)
+                hipLaunchKernelGGL((k_fz_r2c<R, 8>), dim3(nblk(npairs, 8), F), dim3(128), (2 * 8 + 1) * K2 * Cs, st, K0,
+                                   K1, K2, g.plan[2], twid<R>(g, 2), rg, hg, pm, skip);
+            else
+                hipLaunchKernelGGL((k_fz_r2c<R, 4>), dim3(nblk(npairs, 4), F), dim3(64), (2 * 4 + 1) * K2 * Cs, st, K0,
+                                   K1, K2, g.plan[2], twid<R>(g, 2), rg, hg, pm, skip);
+        } else {
+            if (z8)
+                hipLaunchKernelGGL((k_fz_c2r<R, 8>), dim3(nblk(npairs, 8), F), dim3(128), (2 * 8 + 1) * K2 * Cs, st, K0,
+                                   K1, K2, g.plan[2], twid<R>(g, 2), hg, rg, pm, skip);
+            else
+                hipLaunchKernelGGL((k_fz_c2r<R, 4>), dim3(nblk(npairs, 4), F), dim3(64), (2 * 4 + 1) * K2 * Cs, st, K0,
+                                   K1, K2, g.plan[2], twid<R>(g, 2), hg, rg, pm, skip);
+        }
+    };
+    const bool use_tab = &g == &ctx->eg && kind == 0 && !use_dA && pref == DIELECTRIC && kappa == ctx->cfg.kappa &&
+                         ctx->d_infl && pm == ctx->infl_pm;
+    const double* tab = use_tab ? ctx->d_infl : nullptr;
+    zpass(0);
+    int npart;
+    if (pm) {
+        // fused y / x / influence / inverse x / inverse y, one block per (kz, frame)
+        npart = H;
+        const size_t sm = plane_smem(K0, K1, Cs);
+        ProfScope ps_(ctx, 6);
+        if (want_energy)
+            hipLaunchKernelGGL((k_fxy_plane<R, true>), dim3(H, F), dim3(512), sm, st, K0, K1, K2, g.plan[0], g.plan[1],
+                               twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa,
+                               kind, pref, ctx->d_part, (const double*)nullptr, skip);
+        else
+            hipLaunchKernelGGL((k_fxy_plane<R, false>), dim3(H, F), dim3(512), sm, st, K0, K1, K2, g.plan[0], g.plan[1],
+                               twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa,
+                               kind, pref, ctx->d_part, tab, skip);
+    } else {
+        { ProfScope ps_(ctx, 5);
+        hipLaunchKernelGGL(k_fy<R>, dim3(nblk(nly, FL), F), dim3(FTH), (2 * FL + 1) * K1 * Cs, st, K0, K1, H, g.plan[1],
+                           twid<R>(g, 1), hg, 0, skip); }
+        npart = (int)nblk(nlx, FL);
+        { ProfScope ps_(ctx, 6);
+        if (want_energy)
+            hipLaunchKernelGGL((k_fx_infl<R, true>), dim3(npart, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
+                               g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
+                               pref, ctx->d_part, (const double*)nullptr, skip);
+        else
+            hipLaunchKernelGGL((k_fx_infl<R, false>), dim3(npart, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
+                               g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
+                               pref, ctx->d_part, tab, skip); }
+        { ProfScope ps_(ctx, 5);
+        hipLaunchKernelGGL(k_fy<R>, dim3(nblk(nly, FL), F), dim3(FTH), (2 * FL + 1) * K1 * Cs, st, K0, K1, H, g.plan[1],
+                           twid<R>(g, 1), hg, 1, skip); }
+    }
     if (want_energy)
-        hipLaunchKernelGGL((k_fx_infl<R, true>), dim3(nbx, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
-                           g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
-                           pref, ctx->d_part, (const double*)nullptr, skip);
-    else
-        hipLaunchKernelGGL((k_fx_infl<R, false>), dim3(nbx, F), dim3(FTH), (2 * FL + 1) * K0 * Cs, st, K0, K1, K2,
-                           g.plan[0], twid<R>(g, 0), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa, kind,
-                           pref, ctx->d_part,
-                           (&g == &ctx->eg && kind == 0 && !use_dA && pref == DIELECTRIC && kappa == ctx->cfg.kappa)
-                               ? ctx->d_infl : (const double*)nullptr,
-                           skip); }
-    if (want_energy)
-        hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbx, 7, ctx->d_part, ctx->d_rec, 0);
-    { ProfScope ps_(ctx, 5);
-    hipLaunchKernelGGL(k_fy<R>, dim3(nblk(nly, FL), F), dim3(FTH), (2 * FL + 1) * K1 * Cs, st, K0, K1, H, g.plan[1],
-                       twid<R>(g, 1), hg, 1, skip);
-    hipLaunchKernelGGL(k_fz_c2r<R>, dim3(nblk(npairs, FL), F), dim3(FTH), (2 * FL + 1) * K2 * Cs, st, K0, K1, K2,
-                       g.plan[2], twid<R>(g, 2), hg, rg, skip); }
+        hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, npart, 7, ctx->d_part, ctx->d_rec, 0);
+    zpass(1);
     { ProfScope ps_(ctx, 7);
     if (gather_mode == 1)
         hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,

@@ -387,6 +387,92 @@ __device__ __forceinline__ Cplx<R> csub(Cplx<R> a, Cplx<R> b) { return Cplx<R>{a
 template <class R>
 __device__ __forceinline__ Cplx<R> mul_mi(Cplx<R> a, R sg) { return Cplx<R>{sg * a.y, -sg * a.x}; }
 
+// One radix-r butterfly of a Stockham stage over a line whose element e sits at
+// x[e * es] (in and out point at element 0 of the line).  b: butterfly index
+// (0 <= b < nb = K / r), Ns: product of the radices of the earlier stages.
+template <class R>
+__device__ __forceinline__ void fft_bfly(const Cplx<R>* in, Cplx<R>* out, int es, int K, int r, int nb, int Ns, int b,
+                                         const Cplx<R>* tw, R sg) {
+    const int step = K / (Ns * r);
+    const int jq = b / Ns, kj = b - jq * Ns;
+    const int t1 = kj * step;  // twiddle index of q = 1
+    const int ob = jq * Ns * r + kj;
+    auto twq = [&](int q) -> Cplx<R> {
+        int t = q * t1;
+        t -= (t / K) * K;
+        Cplx<R> w = tw[t];
+        w.y *= sg;
+        return w;
+    };
+    auto I = [&](int e) -> Cplx<R> { return in[e * es]; };
+    auto O = [&](int e, Cplx<R> v) { out[e * es] = v; };
+    if (r == 2) {
+        const Cplx<R> x0 = I(b);
+        const Cplx<R> x1 = cmul(I(b + nb), twq(1));
+        O(ob, cadd(x0, x1));
+        O(ob + Ns, csub(x0, x1));
+    } else if (r == 4) {
+        const Cplx<R> x0 = I(b);
+        const Cplx<R> x1 = cmul(I(b + nb), twq(1));
+        const Cplx<R> x2 = cmul(I(b + 2 * nb), twq(2));
+        const Cplx<R> x3 = cmul(I(b + 3 * nb), twq(3));
+        const Cplx<R> a = cadd(x0, x2), bb = csub(x0, x2), c = cadd(x1, x3), d = mul_mi(csub(x1, x3), sg);
+        O(ob, cadd(a, c));
+        O(ob + Ns, cadd(bb, d));
+        O(ob + 2 * Ns, csub(a, c));
+        O(ob + 3 * Ns, csub(bb, d));
+    } else if (r == 3) {
+        const R h = (R)0.86602540378443864676;  // sin(2 pi / 3)
+        const Cplx<R> x0 = I(b);
+        const Cplx<R> x1 = cmul(I(b + nb), twq(1));
+        const Cplx<R> x2 = cmul(I(b + 2 * nb), twq(2));
+        const Cplx<R> t1c = cadd(x1, x2);
+        const Cplx<R> t2 = Cplx<R>{x0.x - (R)0.5 * t1c.x, x0.y - (R)0.5 * t1c.y};
+        const Cplx<R> d = csub(x1, x2);
+        const Cplx<R> t3 = mul_mi(Cplx<R>{h * d.x, h * d.y}, sg);
+        O(ob, cadd(x0, t1c));
+        O(ob + Ns, cadd(t2, t3));
+        O(ob + 2 * Ns, csub(t2, t3));
+    } else if (r == 5) {
+        const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+        const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+        const Cplx<R> x0 = I(b);
+        const Cplx<R> x1 = cmul(I(b + nb), twq(1));
+        const Cplx<R> x2 = cmul(I(b + 2 * nb), twq(2));
+        const Cplx<R> x3 = cmul(I(b + 3 * nb), twq(3));
+        const Cplx<R> x4 = cmul(I(b + 4 * nb), twq(4));
+        const Cplx<R> t1c = cadd(x1, x4), t2 = cadd(x2, x3), t3 = csub(x1, x4), t4 = csub(x2, x3);
+        const Cplx<R> a1 = Cplx<R>{x0.x + c1 * t1c.x + c2 * t2.x, x0.y + c1 * t1c.y + c2 * t2.y};
+        const Cplx<R> a2 = Cplx<R>{x0.x + c2 * t1c.x + c1 * t2.x, x0.y + c2 * t1c.y + c1 * t2.y};
+        const Cplx<R> b1 = mul_mi(Cplx<R>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, sg);
+        const Cplx<R> b2 = mul_mi(Cplx<R>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, sg);
+        O(ob, Cplx<R>{x0.x + t1c.x + t2.x, x0.y + t1c.y + t2.y});
+        O(ob + Ns, cadd(a1, b1));
+        O(ob + 4 * Ns, csub(a1, b1));
+        O(ob + 2 * Ns, cadd(a2, b2));
+        O(ob + 3 * Ns, csub(a2, b2));
+    } else {
+        // direct r-point sum per output: y_k = sum_q x_q W_{Ns r}^{q kj} W_r^{q k}
+        const int kr = K / r;  // W_r = tw[K / r]
+        for (int k = 0; k < r; ++k) {
+            R sx = 0, sy = 0;
+            for (int q = 0; q < r; ++q) {
+                int t = q * t1 + ((q * k) % r) * kr;
+                t -= (t / K) * K;
+                Cplx<R> w = tw[t];
+                w.y *= sg;
+                const Cplx<R> x = I(b + q * nb);
+                sx += x.x * w.x - x.y * w.y;
+                sy += x.x * w.y + x.y * w.x;
+            }
+            O(ob + k * Ns, Cplx<R>{sx, sy});
+        }
+    }
+}
+
+// L lines of length K stored [element][line] (element e of line li at e * L + li);
+// blockDim.x / L butterfly slots per line.  Ping-pongs between in and out and
+// returns the buffer that holds the result.
 template <class R, int L>
 __device__ Cplx<R>* fft_lines(Cplx<R>* in, Cplx<R>* out, int K, const FFTPlan& plan, const Cplx<R>* tw, int inverse) {
     const int T = (int)blockDim.x / L;
@@ -396,80 +482,32 @@ __device__ Cplx<R>* fft_lines(Cplx<R>* in, Cplx<R>* out, int K, const FFTPlan& p
     for (int s = 0; s < plan.nst; ++s) {
         const int r = plan.fac[s];
         const int nb = K / r;
-        const int step = K / (Ns * r);
-        for (int b = slot; b < nb; b += T) {
-            const int jq = b / Ns, kj = b - jq * Ns;
-            const int t1 = kj * step;  // twiddle index of q = 1
-            const int ob = jq * Ns * r + kj;
-            auto twq = [&](int q) -> Cplx<R> {
-                int t = q * t1;
-                t -= (t / K) * K;
-                Cplx<R> w = tw[t];
-                w.y *= sg;
-                return w;
-            };
-            if (r == 2) {
-                const Cplx<R> x0 = in[b * L + li];
-                const Cplx<R> x1 = cmul(in[(b + nb) * L + li], twq(1));
-                out[ob * L + li] = cadd(x0, x1);
-                out[(ob + Ns) * L + li] = csub(x0, x1);
-            } else if (r == 4) {
-                const Cplx<R> x0 = in[b * L + li];
-                const Cplx<R> x1 = cmul(in[(b + nb) * L + li], twq(1));
-                const Cplx<R> x2 = cmul(in[(b + 2 * nb) * L + li], twq(2));
-                const Cplx<R> x3 = cmul(in[(b + 3 * nb) * L + li], twq(3));
-                const Cplx<R> a = cadd(x0, x2), bb = csub(x0, x2), c = cadd(x1, x3), d = mul_mi(csub(x1, x3), sg);
-                out[ob * L + li] = cadd(a, c);
-                out[(ob + Ns) * L + li] = cadd(bb, d);
-                out[(ob + 2 * Ns) * L + li] = csub(a, c);
-                out[(ob + 3 * Ns) * L + li] = csub(bb, d);
-            } else if (r == 3) {
-                const R h = (R)0.86602540378443864676;  // sin(2 pi / 3)
-                const Cplx<R> x0 = in[b * L + li];
-                const Cplx<R> x1 = cmul(in[(b + nb) * L + li], twq(1));
-                const Cplx<R> x2 = cmul(in[(b + 2 * nb) * L + li], twq(2));
-                const Cplx<R> t1c = cadd(x1, x2);
-                const Cplx<R> t2 = Cplx<R>{x0.x - (R)0.5 * t1c.x, x0.y - (R)0.5 * t1c.y};
-                const Cplx<R> d = csub(x1, x2);
-                const Cplx<R> t3 = mul_mi(Cplx<R>{h * d.x, h * d.y}, sg);
-                out[ob * L + li] = cadd(x0, t1c);
-                out[(ob + Ns) * L + li] = cadd(t2, t3);
-                out[(ob + 2 * Ns) * L + li] = csub(t2, t3);
-            } else if (r == 5) {
-                const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
-                const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
-                const Cplx<R> x0 = in[b * L + li];
-                const Cplx<R> x1 = cmul(in[(b + nb) * L + li], twq(1));
-                const Cplx<R> x2 = cmul(in[(b + 2 * nb) * L + li], twq(2));
-                const Cplx<R> x3 = cmul(in[(b + 3 * nb) * L + li], twq(3));
-                const Cplx<R> x4 = cmul(in[(b + 4 * nb) * L + li], twq(4));
-                const Cplx<R> t1c = cadd(x1, x4), t2 = cadd(x2, x3), t3 = csub(x1, x4), t4 = csub(x2, x3);
-                const Cplx<R> a1 = Cplx<R>{x0.x + c1 * t1c.x + c2 * t2.x, x0.y + c1 * t1c.y + c2 * t2.y};
-                const Cplx<R> a2 = Cplx<R>{x0.x + c2 * t1c.x + c1 * t2.x, x0.y + c2 * t1c.y + c1 * t2.y};
-                const Cplx<R> b1 = mul_mi(Cplx<R>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, sg);
-                const Cplx<R> b2 = mul_mi(Cplx<R>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, sg);
-                out[ob * L + li] = Cplx<R>{x0.x + t1c.x + t2.x, x0.y + t1c.y + t2.y};
-                out[(ob + Ns) * L + li] = cadd(a1, b1);
-                out[(ob + 4 * Ns) * L + li] = csub(a1, b1);
-                out[(ob + 2 * Ns) * L + li] = cadd(a2, b2);
-                out[(ob + 3 * Ns) * L + li] = csub(a2, b2);
-            } else {
-                // direct r-point sum per output: y_k = sum_q x_q W_{Ns r}^{q kj} W_r^{q k}
-                const int kr = K / r;  // W_r = tw[K / r]
-                for (int k = 0; k < r; ++k) {
-                    R sx = 0, sy = 0;
-                    for (int q = 0; q < r; ++q) {
-                        int t = q * t1 + ((q * k) % r) * kr;
-                        t -= (t / K) * K;
-                        Cplx<R> w = tw[t];
-                        w.y *= sg;
-                        const Cplx<R> x = in[(b + q * nb) * L + li];
-                        sx += x.x * w.x - x.y * w.y;
-                        sy += x.x * w.y + x.y * w.x;
-                    }
-                    out[(ob + k * Ns) * L + li] = Cplx<R>{sx, sy};
-                }
-            }
+        for (int b = slot; b < nb; b += T) fft_bfly<R>(in + li, out + li, L, K, r, nb, Ns, b, tw, sg);
+        __syncthreads();
+        Cplx<R>* t = in; in = out; out = t;
+        Ns *= r;
+    }
+    return in;
+}
+
+// L lines of length K with element e of line li at li * ls + e * es, the
+// (line, butterfly) work items of each stage spread over the whole block: line
+// index fastest across threads when ls == 1, butterfly index fastest otherwise,
+// so that neighbouring threads touch neighbouring LDS words.
+template <class R>
+__device__ Cplx<R>* fft_strided(Cplx<R>* in, Cplx<R>* out, int K, int L, int es, int ls, const FFTPlan& plan,
+                                const Cplx<R>* tw, int inverse) {
+    const R sg = inverse ? (R)-1 : (R)1;
+    int Ns = 1;
+    for (int s = 0; s < plan.nst; ++s) {
+        const int r = plan.fac[s];
+        const int nb = K / r;
+        const int items = L * nb;
+        for (int t = threadIdx.x; t < items; t += blockDim.x) {
+            int li, b;
+            if (ls == 1) { b = t / L; li = t - b * L; }
+            else { li = t / nb; b = t - li * nb; }
+            fft_bfly<R>(in + li * ls, out + li * ls, es, K, r, nb, Ns, b, tw, sg);
         }
         __syncthreads();
         Cplx<R>* t = in; in = out; out = t;
@@ -478,7 +516,7 @@ __device__ Cplx<R>* fft_lines(Cplx<R>* in, Cplx<R>* out, int K, const FFTPlan& p
     return in;
 }
 
-// block shape of the line passes
+// block shape of the y and x line passes
 constexpr int FL = 4;      // lines per block
 constexpr int FTH = 64;    // threads per block (16 butterfly slots per line)
 
@@ -487,83 +525,210 @@ __device__ inline void stage_tw(Cplx<R>* dst, const Cplx<R>* __restrict__ src, i
     for (int k = threadIdx.x; k < K; k += blockDim.x) dst[k] = src[k];
 }
 
-// real rows (K0 K1 of length K2) -> half spectrum (K0, K1, H); FL complex lines = 2 FL rows
-template <class R>
-__global__ void __launch_bounds__(FTH) k_fz_r2c(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ twg,
-                                               const R* __restrict__ rgrid, Cplx<R>* __restrict__ hgrid, const PcgState* __restrict__ skip) {
+// Half-spectrum layouts.  Line-major: (K0, K1, H), kz fastest, for the separate
+// y and x line passes.  Plane-major: (H, K0, K1), one contiguous (x, y) plane per
+// kz, for the fused plane pass.  H = K2 / 2 + 1.
+__device__ __forceinline__ size_t hs_index(int pm, int row, int kz, int H, int K0K1) {
+    return pm ? (size_t)kz * K0K1 + row : (size_t)row * H + kz;
+}
+
+// real rows (K0 K1 of length K2) -> half spectrum; FLT complex lines = 2 FLT rows
+// per block of 16 FLT threads
+template <class R, int FLT>
+__global__ void __launch_bounds__(16 * FLT) k_fz_r2c(int K0, int K1, int K2, FFTPlan plan,
+                                                     const Cplx<R>* __restrict__ twg, const R* __restrict__ rgrid,
+                                                     Cplx<R>* __restrict__ hgrid, int pm,
+                                                     const PcgState* __restrict__ skip) {
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
     extern __shared__ unsigned char smem_raw[];
     Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
-    Cplx<R>* b1 = b0 + FL * K2;
-    Cplx<R>* tw = b1 + FL * K2;
+    Cplx<R>* b1 = b0 + FLT * K2;
+    Cplx<R>* tw = b1 + FLT * K2;
     const int f = blockIdx.y, H = K2 / 2 + 1;
-    const int nrl = K0 * K1, npairs = (nrl + 1) / 2;
-    const int p0 = blockIdx.x * FL;
+    const int nrl = K0 * K1;
+    const int r0 = 2 * blockIdx.x * FLT;  // first real row of the block
     stage_tw(tw, twg, K2);
-    const R* rg = rgrid + (size_t)f * K0 * K1 * K2;
-    Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
-#pragma unroll
-    for (int li = 0; li < FL; ++li) {
-        const int la = 2 * (p0 + li), lb = la + 1;
-        for (int e = threadIdx.x; e < K2; e += FTH) {
-            const R a = la < nrl ? rg[(size_t)la * K2 + e] : (R)0;
-            const R bb = lb < nrl ? rg[(size_t)lb * K2 + e] : (R)0;
-            b0[e * FL + li] = Cplx<R>{a, bb};
-        }
+    const R* rg = rgrid + (size_t)f * nrl * K2 + (size_t)r0 * K2;
+    Cplx<R>* hg = hgrid + (size_t)f * nrl * H;
+    // 2 FLT consecutive rows are contiguous: row rr -> line rr / 2, real (even) or imaginary (odd) part
+    R* b0r = reinterpret_cast<R*>(b0);
+    for (int idx = threadIdx.x; idx < 2 * FLT * K2; idx += blockDim.x) {
+        const int rr = idx / K2, e = idx - rr * K2;
+        b0r[(e * FLT + (rr >> 1)) * 2 + (rr & 1)] = r0 + rr < nrl ? rg[idx] : (R)0;
     }
     __syncthreads();
-    Cplx<R>* res = fft_lines<R, FL>(b0, b1, K2, plan, tw, 0);
-#pragma unroll
-    for (int li = 0; li < FL; ++li) {
-        if (p0 + li >= npairs) break;
-        const int la = 2 * (p0 + li), lb = la + 1;
-        for (int k = threadIdx.x; k < H; k += FTH) {
-            const Cplx<R> Z = res[k * FL + li];
-            const Cplx<R> Zc = res[((K2 - k) % K2) * FL + li];
-            hg[(size_t)la * H + k] = Cplx<R>{(Z.x + Zc.x) * (R)0.5, (Z.y - Zc.y) * (R)0.5};
-            if (lb < nrl) hg[(size_t)lb * H + k] = Cplx<R>{(Z.y + Zc.y) * (R)0.5, (Zc.x - Z.x) * (R)0.5};
-        }
+    Cplx<R>* res = fft_lines<R, FLT>(b0, b1, K2, plan, tw, 0);
+    // unpack the two real transforms of each complex line: row (even) a = (Z + conj Z~) / 2, (odd) b = (Z - conj Z~) / 2i
+    for (int idx = threadIdx.x; idx < 2 * FLT * H; idx += blockDim.x) {
+        int rr, k;
+        if (pm) { k = idx / (2 * FLT); rr = idx - k * 2 * FLT; }
+        else { rr = idx / H; k = idx - rr * H; }
+        const int row = r0 + rr;
+        if (row >= nrl) continue;
+        const int li = rr >> 1;
+        const Cplx<R> Z = res[k * FLT + li];
+        const Cplx<R> Zc = res[((K2 - k) % K2) * FLT + li];
+        const Cplx<R> v = (rr & 1) ? Cplx<R>{(Z.y + Zc.y) * (R)0.5, (Zc.x - Z.x) * (R)0.5}
+                                   : Cplx<R>{(Z.x + Zc.x) * (R)0.5, (Z.y - Zc.y) * (R)0.5};
+        hg[hs_index(pm, row, k, H, nrl)] = v;
     }
 }
 
 // half spectrum -> real rows (inverse, unnormalised)
-template <class R>
-__global__ void __launch_bounds__(FTH) k_fz_c2r(int K0, int K1, int K2, FFTPlan plan, const Cplx<R>* __restrict__ twg,
-                                               const Cplx<R>* __restrict__ hgrid, R* __restrict__ rgrid, const PcgState* __restrict__ skip) {
+template <class R, int FLT>
+__global__ void __launch_bounds__(16 * FLT) k_fz_c2r(int K0, int K1, int K2, FFTPlan plan,
+                                                     const Cplx<R>* __restrict__ twg, const Cplx<R>* __restrict__ hgrid,
+                                                     R* __restrict__ rgrid, int pm, const PcgState* __restrict__ skip) {
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
     extern __shared__ unsigned char smem_raw[];
     Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
-    Cplx<R>* b1 = b0 + FL * K2;
-    Cplx<R>* tw = b1 + FL * K2;
+    Cplx<R>* b1 = b0 + FLT * K2;
+    Cplx<R>* tw = b1 + FLT * K2;
     const int f = blockIdx.y, H = K2 / 2 + 1;
-    const int nrl = K0 * K1, npairs = (nrl + 1) / 2;
-    const int p0 = blockIdx.x * FL;
+    const int nrl = K0 * K1;
+    const int r0 = 2 * blockIdx.x * FLT;
     stage_tw(tw, twg, K2);
-    R* rg = rgrid + (size_t)f * K0 * K1 * K2;
-    const Cplx<R>* hg = hgrid + (size_t)f * K0 * K1 * H;
-#pragma unroll
-    for (int li = 0; li < FL; ++li) {
-        const int la = 2 * (p0 + li), lb = la + 1;
-        for (int k = threadIdx.x; k < K2; k += FTH) {
-            Cplx<R> A = {0, 0}, B = {0, 0};
-            const int kk = k < H ? k : K2 - k;
-            if (la < nrl) A = hg[(size_t)la * H + kk];
-            if (lb < nrl) B = hg[(size_t)lb * H + kk];
-            if (k >= H) { A.y = -A.y; B.y = -B.y; }
-            b0[k * FL + li] = Cplx<R>{A.x - B.y, A.y + B.x};  // Z = A + i B
-        }
+    R* rg = rgrid + (size_t)f * nrl * K2 + (size_t)r0 * K2;
+    const Cplx<R>* hg = hgrid + (size_t)f * nrl * H;
+    for (int idx = threadIdx.x; idx < FLT * K2; idx += blockDim.x) {
+        int li, k;
+        if (pm) { k = idx / FLT; li = idx - k * FLT; }
+        else { li = idx / K2; k = idx - li * K2; }
+        const int la = r0 + 2 * li, lb = la + 1;
+        const int kk = k < H ? k : K2 - k;
+        Cplx<R> A = {0, 0}, B = {0, 0};
+        if (la < nrl) A = hg[hs_index(pm, la, kk, H, nrl)];
+        if (lb < nrl) B = hg[hs_index(pm, lb, kk, H, nrl)];
+        if (k >= H) { A.y = -A.y; B.y = -B.y; }
+        b0[k * FLT + li] = Cplx<R>{A.x - B.y, A.y + B.x};  // Z = A + i B
     }
     __syncthreads();
-    Cplx<R>* res = fft_lines<R, FL>(b0, b1, K2, plan, tw, 1);
-#pragma unroll
-    for (int li = 0; li < FL; ++li) {
-        if (p0 + li >= npairs) break;
-        const int la = 2 * (p0 + li), lb = la + 1;
-        for (int e = threadIdx.x; e < K2; e += FTH) {
-            const Cplx<R> z = res[e * FL + li];
-            rg[(size_t)la * K2 + e] = z.x;
-            if (lb < nrl) rg[(size_t)lb * K2 + e] = z.y;
+    const Cplx<R>* res = fft_lines<R, FLT>(b0, b1, K2, plan, tw, 1);
+    const R* resr = reinterpret_cast<const R*>(res);
+    for (int idx = threadIdx.x; idx < 2 * FLT * K2; idx += blockDim.x) {
+        const int rr = idx / K2, e = idx - rr * K2;
+        if (r0 + rr < nrl) rg[idx] = resr[(e * FLT + (rr >> 1)) * 2 + (rr & 1)];
+    }
+}
+
+// Influence function at half-spectrum point (i0, i1, i2) for `kind` (0:
+// electrostatic, recip.py:474-475; 6/8/10: dispersion, pme.py:1063-1082):
+// C_k and dC/d(k^2) with the reciprocal vector k; C = 0 at k = 0 for kind 0.
+struct InflPt {
+    double C, dC, kx, ky, kz;
+};
+__device__ inline InflPt infl_point(const BoxInfo& b, int K0, int K1, int K2, int i0, int i1, int i2, double kappa,
+                                    int kind) {
+    InflPt o;
+    const int m0 = i0 < (K0 + 1) / 2 ? i0 : i0 - K0;
+    const int m1 = i1 < (K1 + 1) / 2 ? i1 : i1 - K1;
+    const int m2 = i2 < (K2 + 1) / 2 ? i2 : i2 - K2;
+    o.kx = b.kb[0] * m0 + b.kb[1] * m1 + b.kb[2] * m2;
+    o.ky = b.kb[3] * m0 + b.kb[4] * m1 + b.kb[5] * m2;
+    o.kz = b.kb[6] * m0 + b.kb[7] * m1 + b.kb[8] * m2;
+    const double ksq = o.kx * o.kx + o.ky * o.ky + o.kz * o.kz;
+    const double V = b.V;
+    o.C = 0;
+    o.dC = 0;
+    if (kind == 0) {
+        if (!(i0 == 0 && i1 == 0 && i2 == 0)) {
+            o.C = 2.0 * PI / V / ksq * exp(-ksq / (4.0 * kappa * kappa));
+            o.dC = -o.C * (1.0 / ksq + 1.0 / (4.0 * kappa * kappa));
         }
+    } else {
+        const double kq = ksq < 1e-16 ? 1e-16 : ksq;
+        const double x2 = kq / (4.0 * kappa * kappa), x = sqrt(x2);
+        const double ex = exp(-x2), ec = erfc(x);
+        double fv, fd, pw;
+        if (kind == 6) {
+            fv = (1.0 - 2.0 * x2) * ex + 2.0 * x2 * x * SQRT_PI * ec;
+            fd = -6.0 * x * ex + 6.0 * x2 * SQRT_PI * ec;
+            pw = kappa * kappa * kappa / 3.0;
+        } else if (kind == 8) {
+            fv = (3.0 - 2.0 * x2 + 4.0 * x2 * x2) * ex - 4.0 * x2 * x2 * x * SQRT_PI * ec;
+            fd = (-10.0 * x + 20.0 * x2 * x) * ex - 20.0 * x2 * x2 * SQRT_PI * ec;
+            pw = kappa * kappa * kappa * kappa * kappa / 45.0;
+        } else {
+            const double x4 = x2 * x2;
+            fv = (15.0 - 6.0 * x2 + 4.0 * x4 - 8.0 * x4 * x2) * ex + 8.0 * x4 * x2 * x * SQRT_PI * ec;
+            fd = (-42.0 * x + 28.0 * x2 * x - 56.0 * x4 * x) * ex + 56.0 * x4 * x2 * SQRT_PI * ec;
+            pw = pow(kappa, 7.0) / 1260.0;
+        }
+        const double c0 = SQRT_PI * PI / 2.0 / V * pw;
+        o.C = c0 * fv;
+        o.dC = ksq < 1e-16 ? 0.0 : c0 * fd / (8.0 * kappa * kappa * x);
+    }
+    return o;
+}
+
+// Fused plane pass on the plane-major half spectrum: one block per (kz, frame)
+// holds the K0 x K1 plane in LDS and does forward y and x transforms, the
+// influence function, inverse x and y transforms, then writes the plane back:
+// one read and one write of the half spectrum instead of three of each.
+// ACC: energy and box-derivative partials per block (multiplicity 2 for
+// 0 < kz < K2/2, the conjugate half).  tab (solver products): tabulated
+// 2 pref C_k / theta_k^2 in the same plane-major order.
+template <class R, bool ACC>
+__global__ void __launch_bounds__(512) k_fxy_plane(int K0, int K1, int K2, FFTPlan plan0, FFTPlan plan1,
+                                                    const Cplx<R>* __restrict__ tw0g, const Cplx<R>* __restrict__ tw1g,
+                                                    Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
+                                                    const double* __restrict__ bmod0, const double* __restrict__ bmod1,
+                                                    const double* __restrict__ bmod2, double kappa, int kind,
+                                                    double pref, double* __restrict__ part,
+                                                    const double* __restrict__ tab, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ double sh[16];
+    const int f = blockIdx.y, i2 = blockIdx.x;
+    const int H = K2 / 2 + 1, P = K0 * K1;
+    Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
+    Cplx<R>* b1 = b0 + P;
+    Cplx<R>* tw0 = b1 + P;
+    Cplx<R>* tw1 = tw0 + K0;
+    Cplx<R>* pl = hgrid + ((size_t)f * H + i2) * P;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) b0[i] = pl[i];
+    stage_tw(tw0, tw0g, K0);
+    stage_tw(tw1, tw1g, K1);
+    __syncthreads();
+    // y lines: line x at x * K1, element stride 1; x lines: line y at y, element stride K1
+    Cplx<R>* res = fft_strided<R>(b0, b1, K1, K0, 1, K1, plan1, tw1, 0);
+    res = fft_strided<R>(res, res == b0 ? b1 : b0, K0, K1, K1, 1, plan0, tw0, 0);
+    const BoxInfo& b = boxes[f];
+    const double* tp = tab ? tab + ((size_t)f * H + i2) * P : nullptr;
+    const double mult = (i2 == 0 || 2 * i2 == K2) ? 1.0 : 2.0;
+    double e = 0, w00 = 0, w11 = 0, w22 = 0, w01 = 0, w02 = 0, w12 = 0;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const Cplx<R> S = res[i];
+        double scale;
+        if (!ACC && tp) {
+            scale = tp[i];
+        } else {
+            const int i0 = i / K1, i1 = i - i0 * K1;
+            const InflPt q = infl_point(b, K0, K1, K2, i0, i1, i2, kappa, kind);
+            const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
+            const double ith2 = 1.0 / (th * th);
+            if (ACC) {
+                const double s2 = (double)S.x * (double)S.x + (double)S.y * (double)S.y;
+                const double wgt = pref * s2 * ith2;
+                e += mult * wgt * q.C;
+                const double wd = -mult * wgt * q.dC;
+                w00 += wd * q.kx * q.kx; w11 += wd * q.ky * q.ky; w22 += wd * q.kz * q.kz;
+                w01 += wd * q.kx * q.ky; w02 += wd * q.kx * q.kz; w12 += wd * q.ky * q.kz;
+            }
+            scale = 2.0 * pref * q.C * ith2;
+        }
+        res[i] = Cplx<R>{(R)(S.x * scale), (R)(S.y * scale)};
+    }
+    __syncthreads();
+    res = fft_strided<R>(res, res == b0 ? b1 : b0, K0, K1, K1, 1, plan0, tw0, 1);
+    res = fft_strided<R>(res, res == b0 ? b1 : b0, K1, K0, 1, K1, plan1, tw1, 1);
+    for (int i = threadIdx.x; i < P; i += blockDim.x) pl[i] = res[i];
+    if (!ACC) return;
+    double vals[7] = {e, w00, w11, w22, w01, w02, w12};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        double tsum = block_sum(vals[k], sh);
+        if (threadIdx.x == 0) part[((size_t)f * gridDim.x + blockIdx.x) * 7 + k] = tsum;
     }
 }
 
@@ -635,15 +800,6 @@ __global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan
         const int l = l0 + li;
         if (l >= nlines) continue;
         const int i1 = l / H, i2 = l % H;
-        const int m0 = i0 < (K0 + 1) / 2 ? i0 : i0 - K0;
-        const int m1 = i1 < (K1 + 1) / 2 ? i1 : i1 - K1;
-        const int m2 = i2 < (K2 + 1) / 2 ? i2 : i2 - K2;
-        const double kx = b.kb[0] * m0 + b.kb[1] * m1 + b.kb[2] * m2;
-        const double ky = b.kb[3] * m0 + b.kb[4] * m1 + b.kb[5] * m2;
-        const double kz = b.kb[6] * m0 + b.kb[7] * m1 + b.kb[8] * m2;
-        const double ksq = kx * kx + ky * ky + kz * kz;
-        const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
-        const double ith2 = 1.0 / (th * th);
         const Cplx<R> S = res[i0 * FL + li];
         if (!ACC && tab) {
             // tabulated 2 pref C_k / theta_k^2 (electrostatic solver products)
@@ -651,46 +807,19 @@ __global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan
             res[i0 * FL + li] = Cplx<R>{(R)(S.x * scale), (R)(S.y * scale)};
             continue;
         }
-        double C = 0, dC = 0;
-        const double V = b.V;
-        if (kind == 0) {
-            if (!(i0 == 0 && i1 == 0 && i2 == 0)) {
-                C = 2.0 * PI / V / ksq * exp(-ksq / (4.0 * kappa * kappa));
-                dC = -C * (1.0 / ksq + 1.0 / (4.0 * kappa * kappa));
-            }
-        } else {
-            const double kq = ksq < 1e-16 ? 1e-16 : ksq;
-            const double x2 = kq / (4.0 * kappa * kappa), x = sqrt(x2);
-            const double ex = exp(-x2), ec = erfc(x);
-            double fv, fd, pw;
-            if (kind == 6) {
-                fv = (1.0 - 2.0 * x2) * ex + 2.0 * x2 * x * SQRT_PI * ec;
-                fd = -6.0 * x * ex + 6.0 * x2 * SQRT_PI * ec;
-                pw = kappa * kappa * kappa / 3.0;
-            } else if (kind == 8) {
-                fv = (3.0 - 2.0 * x2 + 4.0 * x2 * x2) * ex - 4.0 * x2 * x2 * x * SQRT_PI * ec;
-                fd = (-10.0 * x + 20.0 * x2 * x) * ex - 20.0 * x2 * x2 * SQRT_PI * ec;
-                pw = kappa * kappa * kappa * kappa * kappa / 45.0;
-            } else {
-                const double x4 = x2 * x2;
-                fv = (15.0 - 6.0 * x2 + 4.0 * x4 - 8.0 * x4 * x2) * ex + 8.0 * x4 * x2 * x * SQRT_PI * ec;
-                fd = (-42.0 * x + 28.0 * x2 * x - 56.0 * x4 * x) * ex + 56.0 * x4 * x2 * SQRT_PI * ec;
-                pw = pow(kappa, 7.0) / 1260.0;
-            }
-            const double c0 = SQRT_PI * PI / 2.0 / V * pw;
-            C = c0 * fv;
-            dC = ksq < 1e-16 ? 0.0 : c0 * fd / (8.0 * kappa * kappa * x);
-        }
+        const InflPt q = infl_point(b, K0, K1, K2, i0, i1, i2, kappa, kind);
+        const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
+        const double ith2 = 1.0 / (th * th);
         if (ACC) {
             const double s2 = (double)S.x * (double)S.x + (double)S.y * (double)S.y;
             const double mult = (i2 == 0 || 2 * i2 == K2) ? 1.0 : 2.0;
             const double wgt = pref * s2 * ith2;
-            e += mult * wgt * C;
-            const double wd = -mult * wgt * dC;
-            w00 += wd * kx * kx; w11 += wd * ky * ky; w22 += wd * kz * kz;
-            w01 += wd * kx * ky; w02 += wd * kx * kz; w12 += wd * ky * kz;
+            e += mult * wgt * q.C;
+            const double wd = -mult * wgt * q.dC;
+            w00 += wd * q.kx * q.kx; w11 += wd * q.ky * q.ky; w22 += wd * q.kz * q.kz;
+            w01 += wd * q.kx * q.ky; w02 += wd * q.kx * q.kz; w12 += wd * q.ky * q.kz;
         }
-        const double scale = 2.0 * pref * C * ith2;
+        const double scale = 2.0 * pref * q.C * ith2;
         res[i0 * FL + li] = Cplx<R>{(R)(S.x * scale), (R)(S.y * scale)};
     }
     __syncthreads();
@@ -711,30 +840,33 @@ __global__ void __launch_bounds__(FTH) k_fx_infl(int K0, int K1, int K2, FFTPlan
 }
 
 // 2 pref C_k / theta_k^2 for the electrostatic influence function (kind 0,
-// recip.py:474-475; k = 0 excluded) on the half spectrum, laid out as the x pass
-// reads it: [frame][i0][line], line = ky * H + kz.  Depends on the box only.
+// recip.py:474-475; k = 0 excluded) on the half spectrum, laid out as the pass
+// that reads it: plane-major [frame][kz][i0][i1] for the plane pass (pm), else
+// [frame][i0][line], line = ky * H + kz, for the x line pass.  Depends on the
+// box only.
 __global__ void k_infl_table(int K0, int K1, int K2, const BoxInfo* __restrict__ boxes,
                              const double* __restrict__ bmod0, const double* __restrict__ bmod1,
-                             const double* __restrict__ bmod2, double kappa, double pref, double* __restrict__ tab) {
+                             const double* __restrict__ bmod2, double kappa, double pref, int pm,
+                             double* __restrict__ tab) {
     const int f = blockIdx.y;
     const int H = K2 / 2 + 1;
-    const int nlines = K1 * H;
+    const size_t P = (size_t)K0 * K1, nt = P * H;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)K0 * nlines) return;
-    const int i0 = (int)(idx / nlines), l = (int)(idx % nlines);
-    const int i1 = l / H, i2 = l % H;
-    const BoxInfo& b = boxes[f];
-    const int m0 = i0 < (K0 + 1) / 2 ? i0 : i0 - K0;
-    const int m1 = i1 < (K1 + 1) / 2 ? i1 : i1 - K1;
-    const int m2 = i2 < (K2 + 1) / 2 ? i2 : i2 - K2;
-    const double kx = b.kb[0] * m0 + b.kb[1] * m1 + b.kb[2] * m2;
-    const double ky = b.kb[3] * m0 + b.kb[4] * m1 + b.kb[5] * m2;
-    const double kz = b.kb[6] * m0 + b.kb[7] * m1 + b.kb[8] * m2;
-    const double ksq = kx * kx + ky * ky + kz * kz;
+    if (idx >= nt) return;
+    int i0, i1, i2;
+    if (pm) {
+        i2 = (int)(idx / P);
+        const int r = (int)(idx - (size_t)i2 * P);
+        i0 = r / K1; i1 = r - i0 * K1;
+    } else {
+        const int nlines = K1 * H;
+        i0 = (int)(idx / nlines);
+        const int l = (int)(idx - (size_t)i0 * nlines);
+        i1 = l / H; i2 = l - i1 * H;
+    }
+    const InflPt q = infl_point(boxes[f], K0, K1, K2, i0, i1, i2, kappa, 0);
     const double th = bmod0[i0] * bmod1[i1] * bmod2[i2];
-    double C = 0;
-    if (!(i0 == 0 && i1 == 0 && i2 == 0)) C = 2.0 * PI / b.V / ksq * exp(-ksq / (4.0 * kappa * kappa));
-    tab[(size_t)f * K0 * nlines + idx] = 2.0 * pref * C / (th * th);
+    tab[(size_t)f * nt + idx] = 2.0 * pref * q.C / (th * th);
 }
 
 // ---------------------------------------------------------------------------
