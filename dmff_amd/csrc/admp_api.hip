@@ -96,9 +96,12 @@ struct admp_ctx {
            *d_t2 = nullptr, *d_U = nullptr;
     double *d_dpart = nullptr, *d_dots = nullptr, *d_coef = nullptr, *d_res = nullptr, *d_dU = nullptr;
     PcgState* d_pcg = nullptr;
+    double* d_s = nullptr;      // CG: s = A p by recurrence
     int* d_active = nullptr;
     int dots_blocks = 64;
-    double* d_pparts = nullptr;               // fused PCG partials, F x PCG_NPART x nb
+    double* d_pparts = nullptr;               // CG partials (CgParts layout)
+    int cg_nw = 0;                            // z.w slots per frame
+    int cg_nwu = 0;                           // z.w slots written by the last product
     double *d_swfrac = nullptr, *d_sfq = nullptr;  // solver spline coordinates / fractional dipoles (R-typed)
     int pcg_nb = 0;
     std::vector<void*> allocs;
@@ -490,7 +493,9 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_dpart, (size_t)ctx->dots_blocks * ctx->F * 3);
     ALLOC(ctx->d_dots, ctx->F * 3);
     ctx->pcg_nb = (int)nblk(ctx->N, PCG_BLOCK);
-    ALLOC(ctx->d_pparts, (size_t)ctx->F * PCG_NPART * ctx->pcg_nb);
+    ctx->cg_nw = std::max(ctx->pcg_nb, (ctx->N + 31) / 32);
+    ALLOC(ctx->d_pparts, (size_t)ctx->F * (ctx->cg_nw + 4 * ctx->pcg_nb));
+    ALLOC(ctx->d_s, NF * 3);
     ALLOC(ctx->d_swfrac, NF * 3);
     ALLOC(ctx->d_sfq, NF * 10);
     ALLOC(ctx->d_res, NF * 3);
@@ -854,7 +859,8 @@ template <class R>
 static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const double* Qc, const double* U, int qmode,
                              int lmax_eff, int kind, double pref, double kappa, int gather_mode, double* out3,
                              double* gq, double* gpos, double* vir, bool want_energy, void* wf_in = nullptr,
-                             void* fq_in = nullptr, const PcgState* skip = nullptr) {
+                             void* fq_in = nullptr, const PcgState* skip = nullptr, const CgEpi* epi = nullptr,
+                             hipEvent_t before_gather = nullptr) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     if (ctx->prepped != &g) {
@@ -869,6 +875,10 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
                            ctx->d_wd, wfrac, fq);
     }
     const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
+    if (small_grid(g) && gather_mode == 2) {
+        ctx->err = "fused CG product epilogue needs the brick spread path";
+        return ADMP_ERR_ARG;
+    }
     if (small_grid(g)) {
         // grids below 6 points per axis: scatter with atomics on the complex C2C path
         Cplx<R>* grid = reinterpret_cast<Cplx<R>*>(ctx->d_grid);
@@ -988,15 +998,21 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     if (want_energy)
         hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, npart, 7, ctx->d_part, ctx->d_rec, 0);
     zpass(1);
+    if (before_gather) HIPCHK(hipStreamWaitEvent(st, before_gather, 0));
     { ProfScope ps_(ctx, 7);
-    if (gather_mode == 1)
+    const CgEpi none{};
+    if (gather_mode == 2)
+        hipLaunchKernelGGL((k_gather_ml<R, 2>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,
+                           ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
+                           vir, ctx->d_gcatoms, skip, *epi);
+    else if (gather_mode == 1)
         hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
-                           vir, ctx->d_gcatoms, skip);
+                           vir, ctx->d_gcatoms, skip, none);
     else
         hipLaunchKernelGGL((k_gather_ml<R, 0>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
-                           vir, ctx->d_gcatoms, skip); }
+                           vir, ctx->d_gcatoms, skip, none); }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
@@ -1017,7 +1033,8 @@ static admp_status matvec(admp_ctx* ctx, const double* v, double* out, const dou
         vR = reinterpret_cast<const R*>(ctx->d_vR);
     }
     { ProfScope ps_(ctx, 2);
-    hipLaunchKernelGGL((k_real_matvec<RC, R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
+    const int nbf = (int)nblk((size_t)N * TPA, PAIR_BLOCK);
+    hipLaunchKernelGGL((k_real_matvec<RC, R>), dim3(xcd_blocks(nbf * F)), dim3(PAIR_BLOCK), 0, st, N, nbf, F,
                        ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
                        reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1, (const PcgState*)nullptr); }
     double* rec = nullptr;
@@ -1057,8 +1074,12 @@ struct SolveInfo {
     std::vector<double> res;
 };
 
-// One product q = A p of the fused PCG.  p (and its R copy, fractional dipoles
-// and spline coordinates) were written by k_pcg_dir; p.q partials go to parts[0].
+static CgParts cg_parts(const admp_ctx* ctx) { return CgParts{ctx->d_pparts, ctx->pcg_nb, ctx->cg_nw}; }
+
+// One product w = A z of the CG iteration.  z (and its R copy, fractional
+// dipoles and spline coordinates) were written by k_cg_start / k_cg_update.
+// w and its z.w partials come from the gather's CG epilogue (brick path, the
+// real-space product joined before the gather) or from k_cg_w.
 template <class RC, class R>
 static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     const int N = ctx->N, F = ctx->F;
@@ -1066,47 +1087,54 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     // real-space products in FP64 whenever the stored coefficients are FP64 (the
     // near-field coupling sets the conditioning); R only for the reciprocal chain
     using RR = typename std::conditional<sizeof(RC) == 8, double, R>::type;
-    const RR* vR = sizeof(RR) == 8 ? reinterpret_cast<const RR*>(ctx->d_p) : reinterpret_cast<const RR*>(ctx->d_vR);
+    const RR* vR = sizeof(RR) == 8 ? reinterpret_cast<const RR*>(ctx->d_z) : reinterpret_cast<const RR*>(ctx->d_vR);
     // the real-space product runs on the side stream, beside the reciprocal chain
     HIPCHK(hipEventRecord(ctx->ev_fork, st));
     HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
     { ProfScope ps_(ctx, 2, ctx->side);
-    hipLaunchKernelGGL((k_real_matvec<RC, RR>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0,
-                       ctx->side, N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
+    const int nbf = (int)nblk((size_t)N * TPA, PAIR_BLOCK);
+    hipLaunchKernelGGL((k_real_matvec<RC, RR>), dim3(xcd_blocks(nbf * F)), dim3(PAIR_BLOCK), 0, ctx->side, N, nbf, F,
+                       ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
                        reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1, ctx->d_pcg); }
     HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
-    double* rec = nullptr;
+    const bool fused = ctx->cfg.lpme && !small_grid(ctx->eg);
     admp_status s = ADMP_OK;
-    if (ctx->cfg.lpme) {
-        s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_p, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1, ctx->d_t2, nullptr,
-                         nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg);
-        rec = ctx->d_t2;
+    if (fused) {
+        const CgEpi epi{ctx->d_t1, pol, f1, cg_parts(ctx)};
+        s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_z, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 2, ctx->d_Ap,
+                         nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg, &epi, ctx->ev_join);
+        ctx->cg_nwu = (N + 31) / 32;
+    } else if (ctx->cfg.lpme) {
+        s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_z, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1, ctx->d_t2,
+                         nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg);
     }
     // join before any return so that a capture never ends with an open fork
     HIPCHK(hipStreamWaitEvent(st, ctx->ev_join, 0));
     if (s != ADMP_OK) return s;
-    { ProfScope ps_(ctx, 11);
-    hipLaunchKernelGGL(k_pcg_q, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, f1, pol, ctx->d_p,
-                       ctx->d_t1, rec, ctx->d_Ap, ctx->d_pparts, ctx->d_pcg); }
+    if (!fused) {
+        ProfScope ps_(ctx, 11);
+        hipLaunchKernelGGL(k_cg_w, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, f1, pol, ctx->d_z, ctx->d_t1,
+                           ctx->cfg.lpme ? ctx->d_t2 : nullptr, ctx->d_Ap, cg_parts(ctx), ctx->d_pcg);
+        ctx->cg_nwu = ctx->pcg_nb;
+    }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
 
-// k_pcg_dir with products of type R (p copy for the pair kernel, spline
-// coordinates and fractional dipoles typed R)
+// k_cg_update of parity cur with products of type R (z copy for the pair kernel
+// and fractional dipoles typed R)
 template <class R>
-static void launch_dir(admp_ctx* ctx, int nx, int mode, double tol) {
-    hipLaunchKernelGGL(k_pcg_dir<R>, dim3(ctx->pcg_nb, ctx->F), dim3(PCG_BLOCK), 0, ctx->stream, ctx->N, ctx->pcg_nb,
-                       nx, mode, tol, ctx->d_box, ctx->cfg.lpme ? ctx->d_wd : nullptr, ctx->d_pparts, ctx->d_z,
-                       ctx->d_p, sizeof(R) == 8 ? nullptr : reinterpret_cast<R*>(ctx->d_vR),
-                       reinterpret_cast<R*>(ctx->d_swfrac), ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr,
-                       ctx->d_pcg);
+static void launch_update(admp_ctx* ctx, int cur, double* x, double tol, const double* pol) {
+    hipLaunchKernelGGL(k_cg_update<R>, dim3(ctx->pcg_nb, ctx->F), dim3(PCG_BLOCK), 0, ctx->stream, ctx->N,
+                       ctx->cg_nwu, cur, tol, ctx->d_box, pol, x, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_s, ctx->d_Ap,
+                       sizeof(R) == 8 ? nullptr : reinterpret_cast<R*>(ctx->d_vR),
+                       ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr, cg_parts(ctx), ctx->d_pcg);
 }
 
 // true residual r = rhs - A x with FP64 products (RC-typed stored pair
-// coefficients), z = M^-1 r, partials into buffer 0; mode 1 starts a solve
-// (p = z), mode 2 only refreshes the residual and convergence state.  dbl:
-// the following iterations use FP64 products.
+// coefficients), z = M^-1 r and the product inputs of z; mode 1 starts a solve,
+// mode 2 only replaces the residual (direction kept).  dbl: the following
+// iterations use FP64 products.  The next update has parity 0.
 template <class RC>
 static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x, const double* pol, double f1,
                                double tol, int mode, bool dbl) {
@@ -1115,17 +1143,26 @@ static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x
     admp_status s;
     if ((s = matvec<RC, double>(ctx, x, ctx->d_res, pol, f1)) != ADMP_OK) return s;
     ProfScope ps_(ctx, 11);
-    hipLaunchKernelGGL(k_pcg_resid, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, 0, pol, rhs,
-                       ctx->d_res, ctx->d_r, ctx->d_z, ctx->d_pparts);
-    if (dbl) launch_dir<double>(ctx, 0, mode, tol);
-    else launch_dir<float>(ctx, 0, mode, tol);
+    const double* wd = ctx->cfg.lpme ? ctx->d_wd : nullptr;
+    if (dbl)
+        hipLaunchKernelGGL(k_cg_start<double>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->d_box, wd, pol,
+                           rhs, ctx->d_res, ctx->d_r, ctx->d_z, (double*)nullptr,
+                           reinterpret_cast<double*>(ctx->d_swfrac),
+                           ctx->cfg.lpme ? reinterpret_cast<double*>(ctx->d_sfq) : nullptr, cg_parts(ctx));
+    else
+        hipLaunchKernelGGL(k_cg_start<float>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->d_box, wd, pol,
+                           rhs, ctx->d_res, ctx->d_r, ctx->d_z, reinterpret_cast<float*>(ctx->d_vR),
+                           reinterpret_cast<float*>(ctx->d_swfrac),
+                           ctx->cfg.lpme ? reinterpret_cast<float*>(ctx->d_sfq) : nullptr, cg_parts(ctx));
+    hipLaunchKernelGGL(k_cg_publish, dim3(F), dim3(PCG_BLOCK), 0, st, mode, tol, cg_parts(ctx), ctx->d_pcg);
     HIPCHK(hipGetLastError());
     return ADMP_OK;
 }
 
 // Preconditioned CG on A x = rhs (Jacobi preconditioner pol/DIELECTRIC) with
-// all scalars on the device.  Each iteration is pcg_apply + k_pcg_step +
-// k_pcg_dir; chunks of iterations are replayed from captured HIP graphs and the
+// all scalars on the device, in the Chronopoulos-Gear form: each iteration is
+// pcg_apply (w = A z) + k_cg_update; chunks of iterations are replayed from
+// captured HIP graphs and the
 // host reads the per-frame state once per chunk.  Real-space products use the
 // stored coefficients (FP64 on both paths).  With fp32 (mixed contexts only)
 // the reciprocal products start in FP32: the recursively updated residual is
@@ -1163,16 +1200,14 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
     auto launch_chunk = [&](int nrun, bool use_dbl) -> admp_status {
         auto iterate = [&]() -> admp_status {
             for (int c = 0; c < nrun; ++c) {
-                const int cur = c & 1;
+                const int cur = c & 1;  // chunks have even length (but a last one cut by maxiter)
                 admp_status s2;
                 if (use_dbl) s2 = pcg_apply<RC, double>(ctx, pol, f1);
                 else s2 = pcg_apply<RC, float>(ctx, pol, f1);
                 if (s2 != ADMP_OK) return s2;
                 ProfScope ps_(ctx, 11);
-                hipLaunchKernelGGL(k_pcg_step, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->pcg_nb, cur, tol,
-                                   pol, x, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_Ap, ctx->d_pparts);
-                if (use_dbl) launch_dir<double>(ctx, cur ^ 1, 0, tol);
-                else launch_dir<float>(ctx, cur ^ 1, 0, tol);
+                if (use_dbl) launch_update<double>(ctx, cur, x, tol, pol);
+                else launch_update<float>(ctx, cur, x, tol, pol);
             }
             HIPCHK(hipGetLastError());
             return ADMP_OK;

@@ -31,10 +31,35 @@ struct BoxInfo {
 
 template <class R> struct Cplx { R x, y; };
 
-// per-frame state of the device-resident PCG (written by k_pcg_dir / k_pcg_init)
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with its own L2.  Work items that share data (neighbouring bricks, sites of one
+// grid region) are therefore renumbered so that each XCD receives one contiguous
+// range: physical block p runs logical item (p % 8) * ceil(n / 8) + p / 8.  The
+// grid is launched with 8 * ceil(n / 8) blocks; logical items >= n return.
+constexpr int N_XCD = 8;
+__host__ __device__ inline int xcd_blocks(int n) { return ((n + N_XCD - 1) / N_XCD) * N_XCD; }
+__device__ inline int xcd_item(int p, int n) { return (p % N_XCD) * ((n + N_XCD - 1) / N_XCD) + p / N_XCD; }
+
+// Per-frame scalars of the device-resident CG solve.  gam/alp are kept in two
+// slots by iteration parity: the update of parity c reads slot c^1 (written by
+// the previous update) and block 0 writes slot c, so no block reads a value
+// written in the same launch.
 struct PcgState {
-    double rz, alpha, beta, res;
+    double gam[2];  // r.z used by the update of that parity (+inf: restart the direction)
+    double alp[2];  // step length of the update of that parity
+    double res;     // max |r| over polarisable sites
     int done, iter, pad0, pad1;
+};
+
+// Partial sums of the CG solve, per frame: nw slots of z.w, then r.z buffers 0
+// and 1 and max|r| buffers 0 and 1 of nb slots each.
+struct CgParts {
+    double* base;
+    int nb, nw;  // blocks of the per-site kernels; z.w slots per frame
+    __device__ double* frame(int f) const { return base + (size_t)f * (nw + 4 * nb); }
+    __device__ double* zw(int f) const { return frame(f); }
+    __device__ double* rz(int f, int b) const { return frame(f) + nw + b * nb; }
+    __device__ double* mx(int f, int b) const { return frame(f) + nw + (2 + b) * nb; }
 };
 
 

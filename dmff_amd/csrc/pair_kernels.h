@@ -462,14 +462,19 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
 // out_I = sum_J c1 V_J + c2 (z.V_J) z   (induced-induced real-space coupling)
 // RC: stored coefficient type, R: arithmetic type of the products (double for the
 // FP64 residual of the mixed-precision solve)
+// One-dimensional grid over all (frame, atom block) items, renumbered so that
+// each XCD works through a contiguous range of frames and atoms (its L2 then
+// holds the positions and v of few frames).  nbf: atom blocks per frame.
 template <class RC, class R>
 __global__ void __launch_bounds__(PAIR_BLOCK)
-k_real_matvec(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
+k_real_matvec(int N, int nbf, int F, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
               const int* __restrict__ nbr, const int* __restrict__ ncnt, const RC* __restrict__ pcoef,
               const R* __restrict__ v, double* __restrict__ out, const PcgState* __restrict__ skip) {
-    if (skip && skip[blockIdx.y].done) return;  // frame already converged
-    const int f = blockIdx.y;
-    const int gt = blockIdx.x * PAIR_BLOCK + threadIdx.x;
+    const int item = xcd_item(blockIdx.x, nbf * F);
+    if (item >= nbf * F) return;
+    const int f = item / nbf, bxi = item - f * nbf;
+    if (skip && skip[f].done) return;  // frame already converged
+    const int gt = bxi * PAIR_BLOCK + threadIdx.x;
     const int i = gt / TPA, sub = gt % TPA;
     const bool active = i < N;
     const int ii = active ? i : 0;
@@ -480,8 +485,10 @@ k_real_matvec(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __
     const int cnt = active ? ncnt[fo + ii] : 0;
     const int* nl = nbr + (fo + ii) * (size_t)cap;
     const RC* pc = pcoef + (fo + ii) * (size_t)cap * 2;
-    for (int k = sub; k < cnt; k += TPA) {
-        const int j = nl[k] & NB_MASK;
+    // two neighbours per trip, loads issued before either is used; the second of
+    // a trip past the end gets zero coefficients (adds exactly 0), so the sum
+    // runs in the same order as one neighbour per trip
+    auto term = [&](int j, RC c1r, RC c2r) {
         const double4 pj = pos[fo + j];
         double dx = pi_.x - pj.x, dy = pi_.y - pj.y, dz = pi_.z - pj.z, n0, n1, n2;
         pbc_shift(bx, dx, dy, dz, n0, n1, n2);
@@ -489,11 +496,21 @@ k_real_matvec(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __
         const R zx = (R)(dx * rinv), zy = (R)(dy * rinv), zz = (R)(dz * rinv);
         const R* vj = v + (fo + j) * 3;
         const R vx = vj[0], vy = vj[1], vz = vj[2];
-        const R c1 = (R)pc[2 * k], c2 = (R)pc[2 * k + 1];
+        const R c1 = (R)c1r, c2 = (R)c2r;
         const R p = c2 * (zx * vx + zy * vy + zz * vz);
         ox += (double)(c1 * vx + p * zx);
         oy += (double)(c1 * vy + p * zy);
         oz += (double)(c1 * vz + p * zz);
+    };
+    for (int k = sub; k < cnt; k += 2 * TPA) {
+        const int k2 = k + TPA;
+        const bool h2 = k2 < cnt;
+        const int j1 = nl[k] & NB_MASK;
+        const int j2 = h2 ? (nl[k2] & NB_MASK) : j1;
+        const RC a1 = pc[2 * k], b1 = pc[2 * k + 1];
+        const RC a2 = h2 ? pc[2 * k2] : (RC)0, b2 = h2 ? pc[2 * k2 + 1] : (RC)0;
+        term(j1, a1, b1);
+        term(j2, a2, b2);
     }
     ox = group_sum(ox, TPA); oy = group_sum(oy, TPA); oz = group_sum(oz, TPA);
     if (active && sub == 0) {

@@ -78,14 +78,6 @@ __global__ void k_grid_cell(int N, int K0, int K1, int K2, const int4* __restric
     atomicAdd(&ccount[(size_t)f * K0 * K1 * K2 + id], 1);
 }
 
-// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
-// with its own L2.  Work items that share data (neighbouring bricks, sites of one
-// grid region) are therefore renumbered so that each XCD receives one contiguous
-// range: physical block p runs logical item (p % 8) * ceil(n / 8) + p / 8.  The
-// grid is launched with 8 * ceil(n / 8) blocks; logical items >= n return.
-constexpr int N_XCD = 8;
-__host__ __device__ inline int xcd_blocks(int n) { return ((n + N_XCD - 1) / N_XCD) * N_XCD; }
-__device__ inline int xcd_item(int p, int n) { return (p % N_XCD) * ((n + N_XCD - 1) / N_XCD) + p / N_XCD; }
 
 constexpr int BRX = 4, BRY = 4, BRZ = 4;   // 64 grid points per wavefront
 constexpr int SP_CHUNK = 32;               // sites staged in LDS per round
@@ -285,9 +277,11 @@ k_spread_col(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
     constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
     constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
-    __shared__ int s_r[4][SPC];
-    __shared__ R s_t[4][SPC][3][NT][6];
-    __shared__ R s_q[4][SPC][NQ];
+    // FP64 tables stage half as many candidates per round: LDS per wavefront sets the occupancy
+    constexpr int SP = sizeof(R) == 8 ? SPC / 2 : SPC;
+    __shared__ int s_r[4][SP];
+    __shared__ R s_t[4][SP][3][NT][6];
+    __shared__ R s_q[4][SP][NQ];
     const int f = blockIdx.y;
     const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY, nbx = (K0 + BRX - 1) / BRX;
     const int nbr = nbx * nby * nbz;
@@ -305,12 +299,12 @@ k_spread_col(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R
     const int lo = live ? bstart[(size_t)f * (nbr + 1) + b] : 0;
     const int total = live ? (int)min((long)bstart[(size_t)f * (nbr + 1) + b + 1], cap) - lo : 0;
     // rounds needed by the busiest brick of the wavefront
-    int rounds = (total + SPC - 1) / SPC;
+    int rounds = (total + SP - 1) / SP;
     for (int o = 16; o < 64; o <<= 1) rounds = max(rounds, __shfl_xor(rounds, o, 64));
     R acc[4] = {0, 0, 0, 0};
     for (int rd = 0; rd < rounds; ++rd) {
-        const int c0 = rd * SPC;
-        const int nch = max(0, min(SPC, total - c0));
+        const int c0 = rd * SP;
+        const int nch = max(0, min(SP, total - c0));
         if (c < nch) {
             const int2 e = blist[(size_t)f * cap + lo + c0 + c];
             const size_t a = (size_t)f * N + e.x;
@@ -876,15 +870,26 @@ __global__ void k_infl_table(int K0, int K1, int K2, const BoxInfo* __restrict__
 // mapped to Cartesian), dE/dpos and the per-atom virial.  Loop bounds are
 // compile-time constants so every array stays in registers.
 // ---------------------------------------------------------------------------
+// MODE 2: the CG product epilogue (solver products): w = real + dE/dmu + d z
+// into out3 with z = U (d: the diagonal of the polarisation matrix, real: the
+// real-space product), and one z.w partial per block.
+struct CgEpi {
+    const double* real;
+    const double* pol;
+    double f1;
+    CgParts parts;
+};
+
 template <class R, int MODE>
 __global__ void __launch_bounds__(256)
 k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
             int use_dA, const int4* __restrict__ mbase, const R* __restrict__ wfrac, const R* __restrict__ fq,
             const double* __restrict__ Qc, const double* __restrict__ U, int qmode, const R* __restrict__ rgrid,
             double* __restrict__ out3, double* __restrict__ g_q, double* __restrict__ g_pos,
-            double* __restrict__ vir_atom, const int* __restrict__ order, const PcgState* __restrict__ skip) {
+            double* __restrict__ vir_atom, const int* __restrict__ order, const PcgState* __restrict__ skip,
+            CgEpi epi) {
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
-    constexpr int ND = MODE == 1 ? 2 : 4;  // derivative orders needed
+    constexpr int ND = MODE == 0 ? 4 : 2;  // derivative orders needed
     const int f = blockIdx.y;
     // 32 sites per block, taken in cell-sorted order (order[]) and XCD-aware
     const int nsb = (N + 31) / 32;
@@ -965,6 +970,28 @@ k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, cons
 #pragma unroll
             for (int bb = 0; bb < ND; ++bb)
                 if (c + aa + bb < ND) fp[c][aa][bb] = group_sum(fp[c][aa][bb], 8);
+    if constexpr (MODE == 2) {
+        __shared__ double shz[4];
+        double zw = 0;
+        if (active && lane == 0) {
+            const double q1[3] = {fp[1][0][0], fp[0][1][0], fp[0][0][1]};
+            const double ap = epi.pol[ii] < 1e-8 ? 1e-8 : epi.pol[ii];
+            const double d = DIELECTRIC / ap - 2.0 * DIELECTRIC * epi.f1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double dm = A[c * 3 + 0] * q1[0] + A[c * 3 + 1] * q1[1] + A[c * 3 + 2] * q1[2];
+                const double zv = U[a * 3 + c];
+                const double wv = epi.real[a * 3 + c] + dm + d * zv;
+                out3[a * 3 + c] = wv;
+                zw += zv * wv;
+            }
+        }
+        zw = group_sum(zw, 64);
+        if ((threadIdx.x & 63) == 0) shz[threadIdx.x >> 6] = zw;
+        __syncthreads();
+        if (threadIdx.x == 0 && sb < nsb) epi.parts.zw(f)[sb] = ((shz[0] + shz[1]) + shz[2]) + shz[3];
+        return;
+    }
     if (!active || lane != 0) return;
     const double p1[3] = {fp[1][0][0], fp[0][1][0], fp[0][0][1]};
     double dmu[3];

@@ -572,17 +572,23 @@ __global__ void k_vadd(size_t n, const double* __restrict__ in, double* __restri
 
 
 // ---------------------------------------------------------------------------
-// Fused preconditioned CG.  Every per-site kernel runs on blocks of PCG_BLOCK
-// sites and leaves one partial per block; a consumer kernel reduces the nb
-// partials of its frame in every block (a fixed assignment of partials to lanes
-// and a fixed tree, so all blocks see the same bits and the result does not
-// depend on scheduling).  Partials of frame f, each nb long:
-//   [0] p.q   [1] r.z (buffer 0)   [2] r.z (buffer 1)   [3] max|r| (0)   [4] max|r| (1)
-// Buffers alternate with the iteration parity so that beta = rz_new / rz_old
-// can read both.  max|r| runs over sites with pol > 0.001 A^3 (pme.py:393-395).
+// Device-resident preconditioned CG in the Chronopoulos-Gear form (one global
+// reduction point per iteration; Chronopoulos & Gear 1989):
+//   w = A z,  delta = z.w,  gamma = r.z
+//   beta = gamma / gamma_prev,  alpha = gamma / (delta - beta gamma / alpha_prev)
+//   p = z + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,  z = M^-1 r
+// (s = A p by recurrence; the first step after a start takes beta = 0).  Every
+// per-site kernel runs on blocks of PCG_BLOCK sites and leaves one partial per
+// block; each consumer block reduces the partials of its frame in a fixed
+// order, so all blocks see the same bits.  Partials of frame f (CgParts): nw
+// slots of z.w (one per block of the kernel that forms w: the gather epilogue
+// or k_cg_w), then r.z buffers 0 and 1 and max|r| buffers 0 and 1, nb each.
+// The update of parity c reads buffers c and writes buffers c^1.  A frame is
+// flagged converged by the update that finds max|r| < tol (the product before
+// it is spent); max|r| runs over sites with pol > 0.001 A^3 (pme.py:393-395).
 // ---------------------------------------------------------------------------
 constexpr int PCG_BLOCK = 256;
-constexpr int PCG_NPART = 5;
+
 
 __device__ inline double wave_allsum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -593,49 +599,28 @@ __device__ inline double wave_allmax(double v) {
     return v;
 }
 
-// total (or max) of nb partials; valid in every thread; sh holds >= 16 doubles
-template <bool MAX>
-__device__ inline double parts_total(const double* __restrict__ p, int nb, double* sh) {
-    double v = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) v = MAX ? fmax(v, p[k]) : v + p[k];
-    v = MAX ? wave_allmax(v) : wave_allsum(v);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    __syncthreads();
-    if (l == 0) sh[w] = v;
-    __syncthreads();
-    double t = sh[0];
-    const int nw = (int)(blockDim.x >> 6);
-    for (int k = 1; k < nw; ++k) t = MAX ? fmax(t, sh[k]) : t + sh[k];
-    __syncthreads();
-    return t;
-}
-
-// four totals at once (max, max, sum, sum) of partial arrays a, b, c, d; valid in
-// every thread; sh holds >= 64 doubles
-__device__ inline void parts_total4(const double* __restrict__ a, const double* __restrict__ b,
-                                    const double* __restrict__ c, const double* __restrict__ d, int nb, double* sh,
-                                    double& ma, double& mb, double& sc, double& sd) {
-    double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+// three totals at once (max a, sum b, sum c) of partial arrays; valid in every
+// thread; sh holds >= 48 doubles
+__device__ inline void parts_total3(const double* __restrict__ a, const double* __restrict__ b,
+                                    const double* __restrict__ c, int nb, double* sh, double& ma, double& sb,
+                                    double& sc) {
+    double v0 = 0, v1 = 0, v2 = 0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
         v0 = fmax(v0, a[k]);
-        v1 = fmax(v1, b[k]);
+        v1 += b[k];
         v2 += c[k];
-        v3 += d[k];
     }
-    v0 = wave_allmax(v0); v1 = wave_allmax(v1); v2 = wave_allsum(v2); v3 = wave_allsum(v3);
+    v0 = wave_allmax(v0); v1 = wave_allsum(v1); v2 = wave_allsum(v2);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sh[w] = v0; sh[16 + w] = v1; sh[32 + w] = v2; }
     __syncthreads();
-    if (l == 0) { sh[w] = v0; sh[16 + w] = v1; sh[32 + w] = v2; sh[48 + w] = v3; }
-    __syncthreads();
-    ma = sh[0]; mb = sh[16]; sc = sh[32]; sd = sh[48];
+    ma = sh[0]; sb = sh[16]; sc = sh[32];
     const int nw = (int)(blockDim.x >> 6);
-    for (int k = 1; k < nw; ++k) {
-        ma = fmax(ma, sh[k]); mb = fmax(mb, sh[16 + k]); sc += sh[32 + k]; sd += sh[48 + k];
-    }
+    for (int k = 1; k < nw; ++k) { ma = fmax(ma, sh[k]); sb += sh[16 + k]; sc += sh[32 + k]; }
     __syncthreads();
 }
 
-// block reduction of one value into thread 0
+// block reduction of one value into every thread
 template <bool MAX>
 __device__ inline double block_red(double v, double* sh) {
     v = MAX ? wave_allmax(v) : wave_allsum(v);
@@ -651,11 +636,27 @@ __device__ inline double block_red(double v, double* sh) {
 
 __device__ inline double inv_diag(double pol) { return (pol < 1e-8 ? 1e-8 : pol) / DIELECTRIC; }
 
-// r = b - Ax, z = M^-1 r, partials r.z and max|r| into buffer `buf`
-__global__ void __launch_bounds__(PCG_BLOCK) k_pcg_resid(int N, int nb, int buf, const double* __restrict__ pol,
-                                                         const double* __restrict__ b, const double* __restrict__ Ax,
-                                                         double* __restrict__ r, double* __restrict__ z,
-                                                         double* __restrict__ parts) {
+// the product inputs derived from z: its R copy for the pair kernel and the
+// fractional dipoles A^T z (slots 1..3 of the 10-wide fractional multipoles)
+template <class R>
+__device__ inline void cg_derived(const BoxInfo* __restrict__ boxes, int f, size_t a, const double* u, R* zR, R* fq) {
+    if (zR)
+        for (int k = 0; k < 3; ++k) zR[a * 3 + k] = (R)u[k];
+    if (!fq) return;
+    const double* A = boxes[f].A;
+    R* o = fq + a * 10;
+    for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u[0] + A[1 * 3 + j] * u[1] + A[2 * 3 + j] * u[2]);
+}
+
+// r = b - Ax, z = M^-1 r and the product inputs derived from z (with the spline
+// coordinates wfrac of the solve's precision R); partials into buffers 0
+template <class R>
+__global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __restrict__ boxes,
+                                                        const double* __restrict__ wd, const double* __restrict__ pol,
+                                                        const double* __restrict__ b, const double* __restrict__ Ax,
+                                                        double* __restrict__ r, double* __restrict__ z,
+                                                        R* __restrict__ zR, R* __restrict__ wfrac, R* __restrict__ fq,
+                                                        CgParts parts) {
     __shared__ double sh[16];
     const int f = blockIdx.y;
     const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
@@ -663,170 +664,142 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_pcg_resid(int N, int nb, int buf,
     if (i < N) {
         const size_t a = (size_t)f * N + i;
         const double m = inv_diag(pol[i]);
+        double u[3];
         for (int k = 0; k < 3; ++k) {
             const double rv = b[a * 3 + k] - Ax[a * 3 + k];
             r[a * 3 + k] = rv;
-            z[a * 3 + k] = m * rv;
-            rz += rv * m * rv;
+            u[k] = m * rv;
+            z[a * 3 + k] = u[k];
+            rz += rv * u[k];
             if (pol[i] > 0.001) mx = fmax(mx, fabs(rv));
+            if (wd) wfrac[a * 3 + k] = (R)wd[a * 3 + k];
         }
+        if (fq) {
+            R* o = fq + a * 10;
+            o[0] = 0;
+            for (int k = 4; k < 10; ++k) o[k] = 0;
+        }
+        cg_derived<R>(boxes, f, a, u, zR, fq);
     }
     rz = block_red<false>(rz, sh);
     mx = block_red<true>(mx, sh);
     if (threadIdx.x == 0) {
-        double* P = parts + (size_t)f * PCG_NPART * nb;
-        P[(1 + buf) * nb + blockIdx.x] = rz;
-        P[(3 + buf) * nb + blockIdx.x] = mx;
+        parts.rz(f, 0)[blockIdx.x] = rz;
+        parts.mx(f, 0)[blockIdx.x] = mx;
     }
 }
 
-// q = real + recip + d p (the diagonal of the polarisation matrix), partial p.q
-__global__ void __launch_bounds__(PCG_BLOCK) k_pcg_q(int N, int nb, double f1, const double* __restrict__ pol,
-                                                     const double* __restrict__ p, const double* __restrict__ real,
-                                                     const double* __restrict__ recip, double* __restrict__ q,
-                                                     double* __restrict__ parts, const PcgState* __restrict__ skip) {
+// state after k_cg_start (one block per frame): residual and convergence; mode 1
+// (fresh start) also resets the iteration count and restarts the direction for
+// the next update (of parity 0), mode 2 (residual replaced) keeps both
+__global__ void __launch_bounds__(PCG_BLOCK) k_cg_publish(int mode, double tol, CgParts parts,
+                                                          PcgState* __restrict__ st) {
+    __shared__ double sh[48];
+    const int f = blockIdx.x;
+    double mx, g, unused;
+    parts_total3(parts.mx(f, 0), parts.rz(f, 0), parts.rz(f, 0), parts.nb, sh, mx, g, unused);
+    if (threadIdx.x != 0) return;
+    PcgState s = st[f];
+    if (mode == 1) {
+        s.iter = 0;
+        s.gam[1] = INFINITY;
+        s.alp[1] = 0;
+    }
+    s.res = mx;
+    s.done = mx < tol ? 1 : 0;
+    st[f] = s;
+}
+
+// w = A z = real + recip + d z (d: the diagonal of the polarisation matrix); partial z.w
+__global__ void __launch_bounds__(PCG_BLOCK) k_cg_w(int N, double f1, const double* __restrict__ pol,
+                                                    const double* __restrict__ z, const double* __restrict__ real,
+                                                    const double* __restrict__ recip, double* __restrict__ w,
+                                                    CgParts parts, const PcgState* __restrict__ skip) {
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
     __shared__ double sh[16];
     const int f = blockIdx.y;
     const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
-    double pq = 0;
+    double zw = 0;
     if (i < N) {
         const size_t a = (size_t)f * N + i;
         const double ap = pol[i] < 1e-8 ? 1e-8 : pol[i];
         const double d = DIELECTRIC / ap - 2.0 * DIELECTRIC * f1;
         for (int k = 0; k < 3; ++k) {
-            const double pv = p[a * 3 + k];
-            const double qv = real[a * 3 + k] + (recip ? recip[a * 3 + k] : 0.0) + d * pv;
-            q[a * 3 + k] = qv;
-            pq += pv * qv;
+            const double zv = z[a * 3 + k];
+            const double wv = real[a * 3 + k] + (recip ? recip[a * 3 + k] : 0.0) + d * zv;
+            w[a * 3 + k] = wv;
+            zw += zv * wv;
         }
     }
-    pq = block_red<false>(pq, sh);
-    if (threadIdx.x == 0) parts[(size_t)f * PCG_NPART * nb + blockIdx.x] = pq;
+    zw = block_red<false>(zw, sh);
+    if (threadIdx.x == 0) parts.zw(f)[blockIdx.x] = zw;
 }
 
-// x += alpha p, r -= alpha q, z = M^-1 r; partials of the new r into buffer cur^1.
-// A frame whose residual (buffer cur) is already below tol is left unchanged.
-__global__ void __launch_bounds__(PCG_BLOCK) k_pcg_step(int N, int nb, int cur, double tol,
-                                                        const double* __restrict__ pol, double* __restrict__ x,
-                                                        double* __restrict__ r, double* __restrict__ z,
-                                                        const double* __restrict__ p, const double* __restrict__ q,
-                                                        double* __restrict__ parts) {
-    __shared__ double sh[16];
+// One CG step of parity cur (see above); nwu: z.w partials written by the
+// product.  A frame whose residual (buffer cur) is below tol is flagged
+// converged and left unchanged.
+template <class R>
+__global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur, double tol,
+                                                         const BoxInfo* __restrict__ boxes,
+                                                         const double* __restrict__ pol, double* __restrict__ x,
+                                                         double* __restrict__ r, double* __restrict__ z,
+                                                         double* __restrict__ p, double* __restrict__ s,
+                                                         const double* __restrict__ w, R* __restrict__ zR,
+                                                         R* __restrict__ fq, CgParts parts,
+                                                         PcgState* __restrict__ st) {
+    __shared__ double sh[48];
     const int f = blockIdx.y;
-    double* P = parts + (size_t)f * PCG_NPART * nb;
+    if (st[f].done) return;
     const int nx = cur ^ 1;
-    const double res = parts_total<true>(P + (3 + cur) * nb, nb, sh);
+    double res, gamma, delta, unused;
+    parts_total3(parts.mx(f, cur), parts.rz(f, cur), parts.rz(f, cur), parts.nb, sh, res, gamma, unused);
+    parts_total3(parts.mx(f, cur), parts.zw(f), parts.zw(f), nwu, sh, unused, delta, unused);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (res < tol) {
-        if (threadIdx.x == 0) {
-            P[(1 + nx) * nb + blockIdx.x] = P[(1 + cur) * nb + blockIdx.x];
-            P[(3 + nx) * nb + blockIdx.x] = P[(3 + cur) * nb + blockIdx.x];
+        if (lead) {
+            st[f].res = res;
+            st[f].done = 1;
         }
         return;
     }
-    const double rz = parts_total<false>(P + (1 + cur) * nb, nb, sh);
-    const double pq = parts_total<false>(P, nb, sh);
-    const double al = pq != 0.0 ? rz / pq : 0.0;
+    const double gp = st[f].gam[nx], ap = st[f].alp[nx];
+    const bool restart = !(gp < INFINITY) || gp == 0.0 || ap == 0.0;
+    const double beta = restart ? 0.0 : gamma / gp;
+    const double den = restart ? delta : delta - beta * gamma / ap;
+    const double alpha = den != 0.0 ? gamma / den : 0.0;
     const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
     double lrz = 0, lmx = 0;
     if (i < N) {
         const size_t a = (size_t)f * N + i;
         const double m = inv_diag(pol[i]);
+        double u[3];
         for (int k = 0; k < 3; ++k) {
-            x[a * 3 + k] += al * p[a * 3 + k];
-            const double rv = r[a * 3 + k] - al * q[a * 3 + k];
-            r[a * 3 + k] = rv;
-            z[a * 3 + k] = m * rv;
-            lrz += rv * m * rv;
+            const size_t e = a * 3 + k;
+            const double pv = restart ? z[e] : z[e] + beta * p[e];
+            const double sv = restart ? w[e] : w[e] + beta * s[e];
+            p[e] = pv;
+            s[e] = sv;
+            x[e] += alpha * pv;
+            const double rv = r[e] - alpha * sv;
+            r[e] = rv;
+            u[k] = m * rv;
+            z[e] = u[k];
+            lrz += rv * u[k];
             if (pol[i] > 0.001) lmx = fmax(lmx, fabs(rv));
         }
+        cg_derived<R>(boxes, f, a, u, zR, fq);
     }
     lrz = block_red<false>(lrz, sh);
     lmx = block_red<true>(lmx, sh);
     if (threadIdx.x == 0) {
-        P[(1 + nx) * nb + blockIdx.x] = lrz;
-        P[(3 + nx) * nb + blockIdx.x] = lmx;
+        parts.rz(f, nx)[blockIdx.x] = lrz;
+        parts.mx(f, nx)[blockIdx.x] = lmx;
     }
-}
-
-// p = z + beta p (beta = 0 when first), with the R copy of p for the pair kernel,
-// the fractional dipoles A^T p and the spline coordinates for the spread.  Block 0
-// records iteration count, residual and convergence in the frame's PcgState.
-template <class R>
-__global__ void __launch_bounds__(PCG_BLOCK) k_pcg_dir(int N, int nb, int nx, int first, double tol,
-                                                       const BoxInfo* __restrict__ boxes, const double* __restrict__ wd,
-                                                       const double* __restrict__ parts, const double* __restrict__ z,
-                                                       double* __restrict__ p, R* __restrict__ pR,
-                                                       R* __restrict__ wfrac, R* __restrict__ fq,
-                                                       PcgState* __restrict__ st) {
-    __shared__ double sh4[64];
-    const int f = blockIdx.y;
-    if (first == 3) {
-        // refresh the derived arrays of the current p (change of product precision)
-        const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
-        if (i >= N) return;
-        const size_t a = (size_t)f * N + i;
-        double u[3];
-        for (int k = 0; k < 3; ++k) {
-            u[k] = p[a * 3 + k];
-            if (pR) pR[a * 3 + k] = (R)u[k];
-            if (wd) wfrac[a * 3 + k] = (R)wd[a * 3 + k];
-        }
-        if (!fq) return;
-        const double* A = boxes[f].A;
-        R* o = fq + a * 10;
-        o[0] = 0;
-        for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u[0] + A[1 * 3 + j] * u[1] + A[2 * 3 + j] * u[2]);
-        for (int k = 4; k < 10; ++k) o[k] = 0;
-        return;
+    if (lead) {
+        st[f].gam[cur] = gamma;
+        st[f].alp[cur] = alpha;
+        st[f].res = res;
+        st[f].iter += 1;
     }
-    const double* P = parts + (size_t)f * PCG_NPART * nb;
-    double res, res_prev, rzn, rzo;
-    parts_total4(P + (3 + nx) * nb, P + (3 + (nx ^ 1)) * nb, P + (1 + nx) * nb, P + (1 + (nx ^ 1)) * nb, nb, sh4, res,
-                 res_prev, rzn, rzo);
-    if (first) res_prev = 0.0;
-    const bool done = res < tol;
-    const double beta = (!first && rzo != 0.0) ? rzn / rzo : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        PcgState s = st[f];
-        if (first == 2) {
-            // state refresh after a true-residual replacement
-            s.res = res;
-            s.done = done ? 1 : 0;
-            st[f] = s;
-        }
-    }
-    if (first == 2) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        PcgState s = st[f];
-        if (first) {
-            s.iter = 0;
-        } else if (!(res_prev < tol)) {
-            s.iter += 1;
-        }
-        s.res = res;
-        s.done = done ? 1 : 0;
-        s.beta = beta;
-        st[f] = s;
-    }
-    // p follows every residual update (also the one that reaches tol), so that a
-    // later true-residual replacement resumes from a direction consistent with r
-    if (!first && res_prev < tol) return;
-    const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
-    if (i >= N) return;
-    const size_t a = (size_t)f * N + i;
-    double u[3];
-    for (int k = 0; k < 3; ++k) {
-        u[k] = z[a * 3 + k] + beta * p[a * 3 + k];
-        p[a * 3 + k] = u[k];
-        if (pR) pR[a * 3 + k] = (R)u[k];
-        if (wd) wfrac[a * 3 + k] = (R)wd[a * 3 + k];
-    }
-    if (!fq) return;
-    const double* A = boxes[f].A;
-    R* o = fq + a * 10;
-    o[0] = 0;
-    for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u[0] + A[1 * 3 + j] * u[1] + A[2 * 3 + j] * u[2]);
-    for (int k = 4; k < 10; ++k) o[k] = 0;
 }
 }  // namespace admp
