@@ -61,6 +61,7 @@ struct admp_ctx {
     // neighbours
     int *d_nbr = nullptr, *d_ncnt = nullptr, *d_nhalf = nullptr, *d_maxcnt = nullptr;
     int *d_cell = nullptr, *d_ccount = nullptr, *d_cstart = nullptr, *d_cfill = nullptr, *d_catoms = nullptr;
+    double4* d_spos = nullptr;  // positions in cell order (atom index in .w)
     int ncell_alloc = 0;
     // per-site work (allocated for double, reused for float)
     void *d_site = nullptr, *d_pcoef = nullptr, *d_vR = nullptr;
@@ -509,6 +510,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_nhalf, NF);
     ALLOC(ctx->d_cell, NF);
     ALLOC(ctx->d_catoms, NF);
+    ALLOC(ctx->d_spos, NF);
 #undef ALLOC
     *out = ctx;
     return ADMP_OK;
@@ -740,6 +742,8 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
                        ctx->d_cstart, ctx->d_cfill, ctx->d_catoms);
     hipLaunchKernelGGL(k_cell_sort_block, dim3(cg.ncell, F), dim3(256), 0, ctx->stream, cg.ncell, N, ctx->d_cstart,
                        ctx->d_catoms);
+    hipLaunchKernelGGL(k_cell_pos, dim3(nblk(N, 256), F), dim3(256), 0, ctx->stream, N, ctx->d_pos, ctx->d_catoms,
+                       ctx->d_spos);
     HIPCHK(hipGetLastError());
     if (!ctx->d_nbr) {
         // first estimate from density: 1.5 x expected full-list count + margin
@@ -757,7 +761,7 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
     for (int attempt = 0; attempt < 2; ++attempt) {
         HIPCHK(hipMemsetAsync(ctx->d_maxcnt, 0, sizeof(int), ctx->stream));
         { ProfScope ps_(ctx, 8); hipLaunchKernelGGL(k_nbr_build, dim3(nblk(N, 4), F), dim3(256), 0, ctx->stream, N, ctx->cap, ctx->d_pos,
-                           ctx->d_box, cg, rc * rc, ctx->d_cell, ctx->d_cstart, ctx->d_catoms, ctx->d_xoff,
+                           ctx->d_box, cg, rc * rc, ctx->d_cell, ctx->d_cstart, ctx->d_spos, ctx->d_xoff,
                            ctx->d_xent, ctx->d_nbr, ctx->d_ncnt, ctx->d_nhalf, ctx->d_maxcnt); }
         HIPCHK(hipGetLastError());
         int mx = 0;
@@ -906,7 +910,7 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
         hipLaunchKernelGGL(k_influence<R>, dim3(nbI, F), dim3(256), 0, st, K0, K1, K2, ctx->d_box, use_dA, g.bmod[0],
                            g.bmod[1], g.bmod[2], kappa, kind, pref, grid, ctx->d_part); }
         if (want_energy)
-            hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, nbI, 7, ctx->d_part, ctx->d_rec, 0);
+            hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(RED_BLOCK), 0, st, nbI, 7, ctx->d_part, ctx->d_rec, 0);
         fft(1);
         { ProfScope ps_(ctx, 7);
         hipLaunchKernelGGL(k_gather<R>, dim3(nblk(N, 64), F), dim3(64), 0, st, N, K0, K1, K2, ctx->d_pos, ctx->d_box,
@@ -996,7 +1000,7 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
                            twid<R>(g, 1), hg, 1, skip); }
     }
     if (want_energy)
-        hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(256), 0, st, npart, 7, ctx->d_part, ctx->d_rec, 0);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(7, F), dim3(RED_BLOCK), 0, st, npart, 7, ctx->d_part, ctx->d_rec, 0);
     zpass(1);
     if (before_gather) HIPCHK(hipStreamWaitEvent(st, before_gather, 0));
     { ProfScope ps_(ctx, 7);
@@ -1514,9 +1518,9 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     hipLaunchKernelGGL(k_self, dim3(nblk(N, 256), F), dim3(256), 0, st, N, kappa, lpme, lpol, ctx->d_Qc, U, pol,
                        ctx->d_e_self, ctx->d_e_pol, ctx->d_gq, lpol ? ctx->d_gpol : nullptr);
     // per-frame energy components: real, self, polarisation penalty -> d_red [0..F), [F..2F), [2F..3F)
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(256), 0, st, N, 1, ctx->d_e_atom, ctx->d_red, 0);
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(256), 0, st, N, 1, ctx->d_e_self, ctx->d_red + F, 0);
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(256), 0, st, N, 1, ctx->d_e_pol, ctx->d_red + 2 * F, 0);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_atom, ctx->d_red, 0);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_self, ctx->d_red + F, 0);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_pol, ctx->d_red + 2 * F, 0);
     std::vector<double> comp(3 * F);
     HIPCHK(hipMemcpyAsync(comp.data(), ctx->d_red, 3 * F * sizeof(double), hipMemcpyDeviceToHost, st));
     std::vector<double> rec(7 * F);
@@ -1533,9 +1537,9 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         hipLaunchKernelGGL(k_frames_gather, dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->d_rev_off, ctx->d_rev_ent,
                            ctx->d_contrib, ctx->d_gpos);
         // box: direct terms (real pairs + frames), reciprocal A-terms and C_k terms
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(256), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9a, 0);
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(256), 0, st, N, 9, ctx->d_vir_fr, ctx->d_red9a, 1);
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(256), 0, st, N, 9, ctx->d_vir_rec, ctx->d_red9b, 0);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9a, 0);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_fr, ctx->d_red9a, 1);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_rec, ctx->d_red9b, 0);
         if (dbox)
             hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9a,
                                lpme ? ctx->d_red9b : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr,
@@ -1640,7 +1644,7 @@ static admp_status disp_impl(admp_ctx* ctx, const double* c_list, const PairPara
             // recip grads: dE/dc (charge gradient), positions (into t1 scratch), A-term virial
             hipLaunchKernelGGL(k_disp_collect, dim3(nblk(NF, 256)), dim3(256), 0, st, NF, g, gq, gc);
             hipLaunchKernelGGL(k_vadd, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_t1, ctx->d_gpos);
-            hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(256), 0, st, N, 9, ctx->d_vir_rec, ctx->d_red9c, 1);
+            hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_rec, ctx->d_red9c, 1);
             std::vector<double> r7(7 * F);
             HIPCHK(hipMemcpyAsync(r7.data(), ctx->d_rec, 7 * F * sizeof(double), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -1653,12 +1657,12 @@ static admp_status disp_impl(admp_ctx* ctx, const double* c_list, const PairPara
     } else {
         HIPCHK(hipMemsetAsync(ctx->d_e_self, 0, NF * sizeof(double), st));
     }
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(256), 0, st, N, 1, ctx->d_e_atom, ctx->d_red9a, 0);
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(256), 0, st, N, 1, ctx->d_e_self, ctx->d_red9a + F, 0);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_atom, ctx->d_red9a, 0);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_self, ctx->d_red9a + F, 0);
     std::vector<double> comp(2 * F);
     HIPCHK(hipMemcpyAsync(comp.data(), ctx->d_red9a, 2 * F * sizeof(double), hipMemcpyDeviceToHost, st));
     if (want_grad) {
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(256), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9b, 0);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9b, 0);
         if (dbox)
             hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9b,
                                lpme ? ctx->d_red9c : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr, c.dK1,
@@ -1717,11 +1721,11 @@ static admp_status admp_slater_compute_impl(admp_ctx* ctx, const double* pos, co
                            N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_t1, ctx->d_nbr, ctx->d_ncnt, pp, 0, ctx->d_e_atom,
                            ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec); }
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(256), 0, st, N, 1, ctx->d_e_atom, ctx->d_red9a, 0);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_atom, ctx->d_red9a, 0);
     std::vector<double> e(F);
     HIPCHK(hipMemcpyAsync(e.data(), ctx->d_red9a, F * sizeof(double), hipMemcpyDeviceToHost, st));
     if (want_grad) {
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(256), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9b, 0);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9b, 0);
         if (dbox)
             hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9b,
                                (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, 1, 1, 1, 0, dbox, 0);

@@ -13,7 +13,13 @@
 
 namespace admp {
 
-__device__ __forceinline__ double d2_exact(const BoxInfo& b, double dx, double dy, double dz) {
+// box rows and inverse held in registers by the neighbour search
+struct BoxMat {
+    double B[9], Binv[9];
+};
+
+template <class BX>
+__device__ __forceinline__ double d2_exact(const BX& b, double dx, double dy, double dz) {
 #pragma clang fp contract(off)
     double s[3], d[3];
     for (int c = 0; c < 3; ++c) {
@@ -204,12 +210,28 @@ __device__ __forceinline__ int lookup_nbond(int i, int j, const int* __restrict_
     return 0;
 }
 
-// one wavefront (64 lanes) per site; blockDim = 256 -> 4 sites per block
+// positions in cell order, the atom index in .w (exact as a double), so that a
+// candidate of the neighbour search is one load
+__global__ void k_cell_pos(int N, const double4* __restrict__ pos, const int* __restrict__ catoms,
+                           double4* __restrict__ spos) {
+    const int f = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const size_t fo = (size_t)f * N;
+    const int j = catoms[fo + k];
+    double4 p = pos[fo + j];
+    p.w = (double)j;
+    spos[fo + k] = p;
+}
+
+// one wavefront (64 lanes) per site; blockDim = 256 -> 4 sites per block.  The
+// ranges of the (up to 27) neighbour cells are loaded once, one per lane, and
+// broadcast; candidates come from the cell-ordered positions.
 __global__ void __launch_bounds__(256)
 k_nbr_build(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes, CellGrid cg,
-            double rc2, const int* __restrict__ cell, const int* __restrict__ cstart, const int* __restrict__ catoms,
-            const int* __restrict__ xoff, const int* __restrict__ xent, int* __restrict__ nbr, int* __restrict__ ncnt,
-            int* __restrict__ nhalf, int* __restrict__ maxcnt) {
+            double rc2, const int* __restrict__ cell, const int* __restrict__ cstart,
+            const double4* __restrict__ spos, const int* __restrict__ xoff, const int* __restrict__ xent,
+            int* __restrict__ nbr, int* __restrict__ ncnt, int* __restrict__ nhalf, int* __restrict__ maxcnt) {
     const int f = blockIdx.y;
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -236,35 +258,55 @@ k_nbr_build(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __re
     int count = 0, half = 0;
     int* row = nbr + (fo + i) * (size_t)cap;
     const int* cs = cstart + (size_t)f * (cg.ncell + 1);
-    const int* ca = catoms + fo;
-    for (int a0 = 0; a0 < nl[0]; ++a0)
-        for (int a1 = 0; a1 < nl[1]; ++a1)
-            for (int a2 = 0; a2 < nl[2]; ++a2) {
-                const int cid = (lst[0][a0] * cg.nc[1] + lst[1][a1]) * cg.nc[2] + lst[2][a2];
-                const int lo = cs[cid], hi = cs[cid + 1];
-                for (int base = lo; base < hi; base += 64) {
-                    const int k = base + lane;
-                    int j = -1;
-                    bool hit = false;
-                    if (k < hi) {
-                        j = ca[k];
-                        if (j != i) {
-                            const double4 pj = pos[fo + j];
-                            const double d2 = d2_exact(b, pi_.x - pj.x, pi_.y - pj.y, pi_.z - pj.z);
-                            hit = d2 < rc2;
-                        }
-                    }
-                    const unsigned long long m = __ballot(hit);
-                    if (hit) {
-                        const int pre = __popcll(m & ((1ull << lane) - 1ull));
-                        const int slot = count + pre;
-                        if (slot < cap) row[slot] = j | (lookup_nbond(i, j, xoff, xent) << NB_SHIFT);
-                    }
-                    const unsigned long long mh = __ballot(hit && j > i);
-                    count += __popcll(m);
-                    half += __popcll(mh);
-                }
+    const double4* sp = spos + fo;
+    BoxMat bm;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) { bm.B[q] = b.B[q]; bm.Binv[q] = b.Binv[q]; }
+    // atom i's exclusion words (the same for the whole wavefront): lane e holds entry e
+    const int x0 = xoff[i], nx = xoff[i + 1] - x0;
+    const int myx = lane < nx ? xent[x0 + lane] : 0;
+    // lane l < ncand holds the range of neighbour cell l (a0-major, a2 fastest)
+    const int ncand = nl[0] * nl[1] * nl[2];
+    int mylo = 0, myhi = 0;
+    if (lane < ncand) {
+        const int a2 = lane % nl[2], a1 = (lane / nl[2]) % nl[1], a0 = lane / (nl[2] * nl[1]);
+        const int cid = (lst[0][a0] * cg.nc[1] + lst[1][a1]) * cg.nc[2] + lst[2][a2];
+        mylo = cs[cid];
+        myhi = cs[cid + 1];
+    }
+    for (int q = 0; q < ncand; ++q) {
+        const int lo = __shfl(mylo, q, 64), hi = __shfl(myhi, q, 64);
+        for (int base = lo; base < hi; base += 64) {
+            const int k = base + lane;
+            int j = -1;
+            bool hit = false;
+            if (k < hi) {
+                const double4 pj = sp[k];
+                j = (int)pj.w;
+                const double d2 = d2_exact(bm, pi_.x - pj.x, pi_.y - pj.y, pi_.z - pj.z);
+                hit = j != i && d2 < rc2;
             }
+            const unsigned long long m = __ballot(hit);
+            if (hit) {
+                const int pre = __popcll(m & ((1ull << lane) - 1ull));
+                const int slot = count + pre;
+                int nbd = 0;
+                if (nx <= 64) {
+                    bool found = false;
+                    for (int e = 0; e < nx; ++e) {
+                        const int w = __builtin_amdgcn_readlane(myx, e);
+                        if (!found && (w & NB_MASK) == j) { nbd = w >> NB_SHIFT; found = true; }
+                    }
+                } else {
+                    nbd = lookup_nbond(i, j, xoff, xent);
+                }
+                if (slot < cap) row[slot] = j | (nbd << NB_SHIFT);
+            }
+            const unsigned long long mh = __ballot(hit && j > i);
+            count += __popcll(m);
+            half += __popcll(mh);
+        }
+    }
     if (lane == 0) {
         ncnt[fo + i] = count < cap ? count : cap;
         nhalf[fo + i] = half;

@@ -384,16 +384,26 @@ __device__ __forceinline__ Cplx<R> mul_mi(Cplx<R> a, R sg) { return Cplx<R>{sg *
 // One radix-r butterfly of a Stockham stage over a line whose element e sits at
 // x[e * es] (in and out point at element 0 of the line).  b: butterfly index
 // (0 <= b < nb = K / r), Ns: product of the radices of the earlier stages.
+// floor(a / d) for 0 <= a < 2^22 and inv = 1.0f / d: the float product is within
+// one of the quotient and the remainder corrects it (cheaper than an integer
+// division on CDNA, which has none in hardware)
+__device__ __forceinline__ int qdiv(int a, int d, float inv) {
+    int q = (int)((float)a * inv);
+    const int rem = a - q * d;
+    q += (rem >= d) - (rem < 0);
+    return q;
+}
+
 template <class R>
-__device__ __forceinline__ void fft_bfly(const Cplx<R>* in, Cplx<R>* out, int es, int K, int r, int nb, int Ns, int b,
-                                         const Cplx<R>* tw, R sg) {
+__device__ __forceinline__ void fft_bfly(const Cplx<R>* in, Cplx<R>* out, int es, int K, int r, int nb, int Ns,
+                                         float invNs, int b, const Cplx<R>* tw, R sg) {
     const int step = K / (Ns * r);
-    const int jq = b / Ns, kj = b - jq * Ns;
-    const int t1 = kj * step;  // twiddle index of q = 1
+    const int jq = qdiv(b, Ns, invNs), kj = b - jq * Ns;
+    const int t1 = kj * step;  // twiddle index of q = 1 (< K)
     const int ob = jq * Ns * r + kj;
     auto twq = [&](int q) -> Cplx<R> {
         int t = q * t1;
-        t -= (t / K) * K;
+        while (t >= K) t -= K;
         Cplx<R> w = tw[t];
         w.y *= sg;
         return w;
@@ -452,7 +462,7 @@ __device__ __forceinline__ void fft_bfly(const Cplx<R>* in, Cplx<R>* out, int es
             R sx = 0, sy = 0;
             for (int q = 0; q < r; ++q) {
                 int t = q * t1 + ((q * k) % r) * kr;
-                t -= (t / K) * K;
+                while (t >= K) t -= K;
                 Cplx<R> w = tw[t];
                 w.y *= sg;
                 const Cplx<R> x = I(b + q * nb);
@@ -476,7 +486,8 @@ __device__ Cplx<R>* fft_lines(Cplx<R>* in, Cplx<R>* out, int K, const FFTPlan& p
     for (int s = 0; s < plan.nst; ++s) {
         const int r = plan.fac[s];
         const int nb = K / r;
-        for (int b = slot; b < nb; b += T) fft_bfly<R>(in + li, out + li, L, K, r, nb, Ns, b, tw, sg);
+        const float invNs = 1.0f / (float)Ns;
+        for (int b = slot; b < nb; b += T) fft_bfly<R>(in + li, out + li, L, K, r, nb, Ns, invNs, b, tw, sg);
         __syncthreads();
         Cplx<R>* t = in; in = out; out = t;
         Ns *= r;
@@ -497,11 +508,13 @@ __device__ Cplx<R>* fft_strided(Cplx<R>* in, Cplx<R>* out, int K, int L, int es,
         const int r = plan.fac[s];
         const int nb = K / r;
         const int items = L * nb;
+        const float invNs = 1.0f / (float)Ns;
+        const int dv = ls == 1 ? L : nb;
+        const float invdv = 1.0f / (float)dv;
         for (int t = threadIdx.x; t < items; t += blockDim.x) {
-            int li, b;
-            if (ls == 1) { b = t / L; li = t - b * L; }
-            else { li = t / nb; b = t - li * nb; }
-            fft_bfly<R>(in + li * ls, out + li * ls, es, K, r, nb, Ns, b, tw, sg);
+            const int qt = qdiv(t, dv, invdv), rt = t - qt * dv;
+            const int li = ls == 1 ? rt : qt, b = ls == 1 ? qt : rt;
+            fft_bfly<R>(in + li * ls, out + li * ls, es, K, r, nb, Ns, invNs, b, tw, sg);
         }
         __syncthreads();
         Cplx<R>* t = in; in = out; out = t;

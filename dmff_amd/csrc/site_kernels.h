@@ -362,11 +362,25 @@ __global__ void k_self(int N, double kappa, int lpme, int lpol, const double* __
 
 // ----------------------------------------------------------------- reductions
 // out[f*ncol + c] = sum_i in[(f*N + i)*ncol + c]   (one block per (column, frame), fixed order)
-__global__ void k_reduce_cols(int N, int ncol, const double* __restrict__ in, double* __restrict__ out, int accumulate) {
+// one block (RED_BLOCK threads) per (column, frame); four independent partial
+// sums per thread keep several loads in flight, combined in a fixed order
+constexpr int RED_BLOCK = 1024;
+__global__ void __launch_bounds__(RED_BLOCK) k_reduce_cols(int N, int ncol, const double* __restrict__ in,
+                                                           double* __restrict__ out, int accumulate) {
     __shared__ double sh[16];
     const int c = blockIdx.x, f = blockIdx.y;
-    double s = 0;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) s += in[((size_t)f * N + i) * ncol + c];
+    const double* col = in + (size_t)f * N * ncol + c;
+    const int st = blockDim.x;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int i = threadIdx.x;
+    for (; i + 3 * st < N; i += 4 * st) {
+        s0 += col[(size_t)i * ncol];
+        s1 += col[(size_t)(i + st) * ncol];
+        s2 += col[(size_t)(i + 2 * st) * ncol];
+        s3 += col[(size_t)(i + 3 * st) * ncol];
+    }
+    for (; i < N; i += st) s0 += col[(size_t)i * ncol];
+    double s = (s0 + s1) + (s2 + s3);
     s = block_sum(s, sh);
     if (threadIdx.x == 0) {
         if (accumulate) out[(size_t)f * ncol + c] += s;
