@@ -232,8 +232,14 @@ def water_dispersion_slater(S, seed):
     return np.sqrt(Ca), A_t[t], B_t[t]
 
 
+LARGE_DISABLED = ("the 98,304-atom leg is disabled: its last run ended in a GPU memory fault whose cause is "
+                  "not identified yet (its FP32 final pass was the only user of plane passes above 80,000 bytes "
+                  "of LDS, now taken by the line passes); it stays off until the cause is shown")
+
+
 def leg_large(steps=2, warmup=1, precision="mixed"):
     """32,768 waters (98,304 atoms), rc 10 A: electrostatics + polarisation + dispersion PME (pmax 10) + Slater."""
+    raise RuntimeError(LARGE_DISABLED)
     import numpy as np
     import torch
     from dmff_amd.admp.engine import Engine
@@ -306,9 +312,10 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-json", default=None)
-    ap.add_argument("--legs", default="batch,large,serving",
-                    help="extra measurements beside the headline: batch (64 frames), large (98,304 atoms with "
-                         "dispersion and Slater), serving (request stream, p50/p99); 'none' for the headline only")
+    ap.add_argument("--legs", default="batch,serving",
+                    help="extra measurements beside the headline: batch (64 frames), serving (request stream, "
+                         "p50/p99); 'none' for the headline only.  'large' (98,304 atoms with dispersion and "
+                         "Slater) is disabled, see leg_large")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device-to-device tensor copy five times (the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -392,8 +399,8 @@ def main():
             log(f"batch64 {prec}: {b[prec]['ms_per_batch']:.1f} ms/batch, {b[prec]['evals_per_s']:.1f} evals/s")
         out["batch64"] = b
     if "large" in legs:
-        out["large"] = leg_large()
-        log(f"large: {out['large']['ms_per_eval']:.1f} ms/eval ({out['large']['n_atoms']} atoms)")
+        log(LARGE_DISABLED)
+        out["large"] = {"disabled": LARGE_DISABLED}
     if "serving" in legs:
         from dmff_amd.admp.serving import make_stream
         stream = make_stream(32, 20.0, 1004)

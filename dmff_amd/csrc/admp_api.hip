@@ -78,6 +78,7 @@ struct admp_ctx {
     int2* d_blist = nullptr;                        // per-brick candidate lists
     double* d_infl = nullptr;                       // tabulated electrostatic influence function (eg)
     int infl_pm = 0;                                // its layout: plane-major (1) or line-major (0)
+    bool plane_lds_ok = false;                      // plane kernels may use PLANE_LDS_MAX bytes of LDS
     long blist_cap = 0;
     int* d_scan_tmp = nullptr;  // tile totals of scan_rows
     size_t scan_cap = 0;
@@ -252,6 +253,36 @@ static std::vector<double> bspline_moduli(int K) {
 
 // dynamic LDS of k_fxy_plane: two K0 x K1 planes and the x and y twiddles
 static size_t plane_smem(int K0, int K1, size_t cs) { return ((size_t)2 * K0 * K1 + K0 + K1) * cs; }
+constexpr size_t PLANE_LDS_MAX = 80000;
+
+// The dynamic-LDS ceiling of the plane kernels is a process-wide attribute of
+// each kernel: set it once to PLANE_LDS_MAX (every plane launch stays below it),
+// never per grid, so that contexts with different grids cannot lower it for
+// one another.
+static bool plane_kernels_ready() {
+    static const bool ok = [] {
+        int dev = 0, lim = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            (size_t)lim < PLANE_LDS_MAX) {
+            (void)hipGetLastError();
+            return false;
+        }
+        const int m = (int)PLANE_LDS_MAX;
+        const hipError_t e[4] = {
+            hipFuncSetAttribute((const void*)k_fxy_plane<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, m),
+            hipFuncSetAttribute((const void*)k_fxy_plane<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, m),
+            hipFuncSetAttribute((const void*)k_fxy_plane<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, m),
+            hipFuncSetAttribute((const void*)k_fxy_plane<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, m)};
+        for (hipError_t x : e)
+            if (x != hipSuccess) {
+                (void)hipGetLastError();
+                return false;
+            }
+        return true;
+    }();
+    return ok;
+}
 
 static admp_status setup_grid(admp_ctx* ctx, Grid& g, int K0, int K1, int K2, bool mixed) {
     g.K[0] = K0; g.K[1] = K1; g.K[2] = K2;
@@ -290,29 +321,13 @@ static admp_status setup_grid(admp_ctx* ctx, Grid& g, int K0, int K1, int K2, bo
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     // The fused plane pass keeps two K0 x K1 complex planes and both twiddle
-    // tables in LDS; it is used where that fits the device's per-block limit.
+    // tables in LDS.  It is used up to PLANE_LDS_MAX bytes (two blocks per CU;
+    // e.g. 50^2 in FP64, 70^2 in FP32); larger planes take the line passes.
     const char* env = std::getenv("ADMP_FFT_PLANE");
     const bool allow = !(env && env[0] == '0') && K0 >= BRX + 5 && K1 >= BRY + 5 && K2 >= BRZ + 5;
-    int dev = 0, lim = 0;
-    HIPCHK(hipGetDevice(&dev));
-    HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     for (int w = 0; w < 2; ++w) {
         const size_t cs = w ? sizeof(Cplx<double>) : sizeof(Cplx<float>);
-        const size_t sm = plane_smem(K0, K1, cs);
-        g.plane[w] = allow && sm <= (size_t)lim;
-        if (!g.plane[w]) continue;
-        hipError_t e1, e2;
-        if (w) {
-            e1 = hipFuncSetAttribute((const void*)k_fxy_plane<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-            e2 = hipFuncSetAttribute((const void*)k_fxy_plane<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        } else {
-            e1 = hipFuncSetAttribute((const void*)k_fxy_plane<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-            e2 = hipFuncSetAttribute((const void*)k_fxy_plane<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-        }
-        if (e1 != hipSuccess || e2 != hipSuccess) {
-            (void)hipGetLastError();
-            g.plane[w] = false;
-        }
+        g.plane[w] = allow && plane_smem(K0, K1, cs) <= PLANE_LDS_MAX && ctx->plane_lds_ok;
     }
     return ADMP_OK;
 }
@@ -424,6 +439,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         ctx->err = "stream/event creation failed";
         return fail(ADMP_ERR_HIP);
     }
+    ctx->plane_lds_ok = plane_kernels_ready();
     if (c.lpme) {
         if ((st = setup_grid(ctx, ctx->eg, c.K1, c.K2, c.K3, mixed)) != ADMP_OK) return fail(st);
     }
