@@ -287,19 +287,21 @@ k_nbr_build(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __re
                 hit = j != i && d2 < rc2;
             }
             const unsigned long long m = __ballot(hit);
+            // nbond of (i, j) from i's exclusion words, read lane by lane by the
+            // whole wavefront (nx is the same in every lane)
+            int nbd = 0;
+            if (nx <= 64) {
+                bool found = false;
+                for (int e = 0; e < nx; ++e) {
+                    const int w = __builtin_amdgcn_readlane(myx, e);
+                    if (!found && (w & NB_MASK) == j) { nbd = w >> NB_SHIFT; found = true; }
+                }
+            } else if (hit) {
+                nbd = lookup_nbond(i, j, xoff, xent);
+            }
             if (hit) {
                 const int pre = __popcll(m & ((1ull << lane) - 1ull));
                 const int slot = count + pre;
-                int nbd = 0;
-                if (nx <= 64) {
-                    bool found = false;
-                    for (int e = 0; e < nx; ++e) {
-                        const int w = __builtin_amdgcn_readlane(myx, e);
-                        if (!found && (w & NB_MASK) == j) { nbd = w >> NB_SHIFT; found = true; }
-                    }
-                } else {
-                    nbd = lookup_nbond(i, j, xoff, xent);
-                }
                 if (slot < cap) row[slot] = j | (nbd << NB_SHIFT);
             }
             const unsigned long long mh = __ballot(hit && j > i);

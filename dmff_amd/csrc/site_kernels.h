@@ -767,7 +767,8 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
     const int nx = cur ^ 1;
     double res, gamma, delta, unused;
     parts_total3(parts.mx(f, cur), parts.rz(f, cur), parts.rz(f, cur), parts.nb, sh, res, gamma, unused);
-    parts_total3(parts.mx(f, cur), parts.zw(f), parts.zw(f), nwu, sh, unused, delta, unused);
+    // every array of a call is read over that call's bound: the z.w slots (nwu)
+    parts_total3(parts.zw(f), parts.zw(f), parts.zw(f), nwu, sh, unused, delta, unused);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (res < tol) {
         if (lead) {
