@@ -1,0 +1,199 @@
+"""Subprocess entry: run one case through the CPU-emulated library and print its
+results as one JSON line prefixed "RESULT ".  A guard-page fault ends the
+process with status 86 after the handler has named the kernel, block and thread.
+
+    python -m tests.cpu_emu.runner <libadmp_emu.so> <case> [precision] [grid K1,K2,K3 | waters count]
+
+The cases call the same Python interface as the GPU parity tests; only the
+Engine's device set-up is replaced (CPU tensors, the null stream).
+"""
+import ctypes
+import json
+import sys
+
+import numpy as np
+import torch
+
+F64 = torch.float64
+
+
+def patch_engine(lib_path):
+    import dmff_amd._lib as L
+    from dmff_amd.admp import engine as E
+    L.LIB_PATH = lib_path
+    base_init = E.Engine.__init__
+
+    def init(self, *args, **kwargs):
+        kwargs["device"] = "cpu"
+        torch_cuda = torch.cuda
+        saved = (torch_cuda.device, torch_cuda.init)
+
+        class _NoDevice:
+            def __init__(self, dev):
+                pass
+
+            def __enter__(self):
+                return self
+
+            def __exit__(self, *exc):
+                return False
+
+        torch_cuda.device, torch_cuda.init = _NoDevice, (lambda: None)
+        try:
+            base_init(self, *args, **kwargs)
+        finally:
+            torch_cuda.device, torch_cuda.init = saved
+
+    E.Engine.__init__ = init
+    E.Engine._stream = lambda self: self.lib.admp_set_stream(self.h, ctypes.c_void_p(0))
+
+
+def _lst(x):
+    return np.asarray(x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x, dtype=np.float64).tolist()
+
+
+def small_box(precision):
+    """Polarisable PME on the 13.2 A water box, PCG to 1e-9, with every gradient."""
+    from dmff_amd.admp.pme import ADMPPmeForce
+    from tests.systems import reference_pairs, small_water_box
+    s, pos, box = small_water_box()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    f = ADMPPmeForce(box, s.map_atomtype, s.axis_types, s.axis_idx, 6.0, 5e-4, s.pme.lmax, s.pme.lpol, True, None,
+                     precision=precision, solver="pcg", tol=1e-9, device="cpu")
+    T = lambda x: torch.tensor(np.asarray(x, dtype=np.float64), requires_grad=True)
+    p, b, Q = T(pos), T(box), T(s.pme.Q_local[s.map_atomtype])
+    a, t = T(s.pme.pol[s.map_poltype]), T(s.pme.thole[s.map_poltype])
+    e, aux = f.get_energy(p, b, torch.as_tensor(pairs), Q, a, t, s.pme.mScales, s.pme.pScales, s.pme.dScales,
+                          aux={})
+    gp, gb, gq, ga, gt = torch.autograd.grad(e, (p, b, Q, a, t))
+    return {"E": float(e.detach()), "pos": _lst(gp), "box": _lst(gb), "Q": _lst(gq), "pol": _lst(ga),
+            "thole": _lst(gt), "U": _lst(aux["U_ind"] * 10.0), "n_iter": f.last_info["n_iter"]}
+
+
+def frames(precision):
+    """Three perturbed frames of the small box in one call, each frame's neighbour
+    list built by the library, and each frame again alone."""
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    from tests.systems import small_water_box
+    s, pos, box = small_water_box()
+    rng = np.random.default_rng(3)
+    P = np.stack([pos + rng.normal(0, 0.005, pos.shape) for _ in range(3)])
+    kappa, K1, K2, K3 = setup_ewald_parameters(6.0, 5e-4, box)
+    Q = s.pme.Q_local[s.map_atomtype]
+    pol, th = s.pme.pol[s.map_poltype], s.pme.thole[s.map_poltype]
+
+    def run(Pf):
+        F = len(Pf)
+        eng = Engine(len(pos), 2, True, True, kappa, (K1, K2, K3), 6.0, n_frames=F, precision=precision)
+        eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
+        eng.build_neighbors(Pf, np.stack([box] * F))
+        E, g, U, info = eng.compute(Pf, np.stack([box] * F), Q, pol, th, s.pme.mScales, s.pme.pScales,
+                                    s.pme.dScales, solver="pcg", tol=1e-8, maxiter=500, want_grad=True)
+        return _lst(E), _lst(g["pos"])
+
+    Eb, Gb = run(P)
+    single = [run(P[k:k + 1]) for k in range(3)]
+    return {"E_batch": Eb, "G_batch": Gb, "E_single": [x[0][0] for x in single],
+            "G_single": [x[1][0] for x in single]}
+
+
+def disp_slater(precision):
+    """Dispersion PME (pmax 10) and the Slater term on 60 random sites in a 20 A box."""
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    from tests.systems import reference_pairs
+    rng = np.random.default_rng(9)
+    L, n, rc = 20.0, 60, 9.0
+    pos = rng.uniform(0, L, size=(n, 3))
+    c = rng.uniform(5, 40, size=(n, 3))
+    m = np.array([0.0, 0.0, 1.0, 1.0, 1.0, 1.0])
+    bonded = {(0, 1): 1, (1, 0): 1, (2, 3): 2, (3, 2): 2}
+    pairs = reference_pairs(pos, np.eye(3) * L, rc, bonded)
+    kappa, K1, K2, K3 = setup_ewald_parameters(rc, 5e-4, np.eye(3) * L)
+    eng = Engine(n, 0, False, False, 0.0, (0, 0, 0), rc, precision=precision, disp_kappa=kappa, dK=(K1, K2, K3),
+                 disp_pmax=10)
+    eng.set_topology(None, None, bonded)
+    eng.set_pairs(torch.as_tensor(pairs))
+    Ed, gd = eng.disp_compute(pos, np.eye(3) * L, c, m)
+    A, B = rng.uniform(1e2, 3e2, size=n), rng.uniform(3.0, 4.5, size=n)
+    Es, gs = eng.slater_compute(pos, np.eye(3) * L, A, B, m)
+    out = {"E_disp": float(Ed[0]), "E_slater": float(Es[0])}
+    out.update({"disp_" + k: _lst(v[0]) for k, v in gd.items()})
+    out.update({"slater_" + k: _lst(v[0]) for k, v in gs.items()})
+    return out
+
+
+def neighbors(precision):
+    """Library-built neighbour lists: the small water box at three cutoffs and 300
+    random points in an 11 A box (fewer than three cells a side)."""
+    from dmff_amd.admp.engine import Engine
+    from tests.systems import small_water_box
+    s, pos, box = small_water_box()
+    out = {}
+    for rc in (6.0, 4.0):
+        e = Engine(len(pos), 0, False, False, 0.0, (0, 0, 0), rc)
+        e.set_topology(None, None, s.bonded)
+        e.build_neighbors(pos, box)
+        out[f"water_{rc}"] = e.get_pairs(0).numpy().astype(np.int64).tolist()
+    rng = np.random.default_rng(5)
+    p = rng.uniform(0, 11.0, size=(300, 3))
+    p[1] = p[0] + np.array([4.0, 0, 0])
+    p[3] = p[2] + np.array([0, 3.9999999999, 0])
+    for rc in (4.0, 5.4):
+        e = Engine(len(p), 0, False, False, 0.0, (0, 0, 0), rc)
+        e.set_topology(None, None, {})
+        e.build_neighbors(p, np.eye(3) * 11.0)
+        out[f"random_{rc}"] = e.get_pairs(0).numpy().astype(np.int64).tolist()
+    return out
+
+
+def grid(precision, K):
+    """Non-polarisable PME on the small box on a given grid (FFT radix paths, the
+    scatter path of grids under 9 points a side, line-major and plane passes)."""
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    from tests.systems import reference_pairs, small_water_box
+    s, pos, box = small_water_box()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    kappa = setup_ewald_parameters(6.0, 5e-4, box)[0]
+    eng = Engine(len(pos), 2, False, True, kappa, K, 6.0, precision=precision)
+    eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
+    eng.set_pairs(torch.as_tensor(pairs))
+    E, g, U, info = eng.compute(pos, box, s.pme.Q_local[s.map_atomtype], mScales=s.pme.mScales, want_grad=True)
+    return {"E": float(E[0]), "pos": _lst(g["pos"][0])}
+
+
+def waters(precision, n_waters):
+    """A polarisable water box of n_waters (several solver blocks), neighbour list
+    built by the library, PCG to 1e-6, then a warm-started second evaluation."""
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    from dmff_amd.admp.serving import water_spec
+    spec, pos, box = water_spec("w", n_waters, 7)
+    kappa, K1, K2, K3 = setup_ewald_parameters(spec.rc, 5e-4, np.asarray(box))
+    eng = Engine(spec.n_atoms, 2, True, True, kappa, (K1, K2, K3), spec.rc, precision=precision)
+    eng.set_topology(spec.axis_types, spec.axis_idx, spec.exclusions)
+    eng.build_neighbors(pos, box)
+    args = (spec.Q_local, spec.pol, spec.thole, *spec.scales)
+    E, g, U, info = eng.compute(pos, box, *args, solver="pcg", tol=1e-6, maxiter=500, want_grad=True)
+    E2, g2, U2, info2 = eng.compute(pos, box, *args, U_init=U, solver="pcg", tol=1e-6, maxiter=500,
+                                    want_grad=True)
+    return {"E": float(E[0]), "E_warm": float(E2[0]), "n_iter": info["n_iter"][0],
+            "n_iter_warm": info2["n_iter"][0], "residual": info["residual"][0], "K": [K1, K2, K3],
+            "rc": spec.rc, "n_atoms": spec.n_atoms}
+
+
+CASES = {"small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
+
+if __name__ == "__main__":
+    lib, case = sys.argv[1], sys.argv[2]
+    prec = sys.argv[3] if len(sys.argv) > 3 else "fp64"
+    patch_engine(lib)
+    if case == "grid":
+        out = grid(prec, tuple(int(k) for k in sys.argv[4].split(",")))
+    elif case == "waters":
+        out = waters(prec, int(sys.argv[4]))
+    else:
+        out = CASES[case](prec)
+    print("RESULT " + json.dumps(out), flush=True)

@@ -54,3 +54,21 @@ def reference_pairs(pos, box, rc, bonded, pad=0):
         n = pos.shape[0]
         out = np.concatenate([out, np.tile(np.array([[n, n, 0]]), (pad, 1))], axis=0)
     return out
+
+
+def small_water_box(ff="fullpol_forcefield.xml"):
+    """The waters of the 31 A test box that lie wholly inside a 13.2 A corner,
+    shifted into a 13.2 A periodic box (44 molecules, 132 atoms)."""
+    s = load_system(ff, "waterbox_31ang.pdb", "fullpol_residues.xml")
+    p = s.pos_nm * 10
+    mol = p.reshape(-1, 3, 3)
+    ok = np.all((mol > 1.0) & (mol < 13.0), axis=(1, 2))
+    idx = np.nonzero(np.repeat(ok, 3))[0]
+    remap = {int(a): k for k, a in enumerate(idx)}
+    pos = p[idx] - 0.4
+    s.bonded = {(remap[i], remap[j]): v for (i, j), v in s.bonded.items() if i in remap and j in remap}
+    s.axis_types = s.axis_types[idx]
+    s.axis_idx = np.array([[remap.get(int(a), -1) if a >= 0 else -1 for a in row] for row in s.axis_idx[idx]])
+    s.map_atomtype = s.map_atomtype[idx]
+    s.map_poltype = s.map_poltype[idx]
+    return s, pos, np.eye(3) * 13.2

@@ -1,0 +1,153 @@
+"""The library's kernels run on the CPU under an emulation of the HIP runtime
+(tests/cpu_emu): one fibre per GPU thread, wave operations resolved per
+64-lane wave, and every device buffer placed against a guard page, so that a
+read or write past the end of any buffer faults at the access and the run
+ends with status 86 after naming the kernel, block and thread.
+
+Each case runs in a subprocess through the same Python interface as the GPU
+parity tests and is checked against the oracle to the same tolerances:
+  FP64   energy rel 1e-9, gradient rel RMS 1e-7
+  mixed  energy rel 2e-5, force rel RMS 1e-4
+A clean exit shows that no kernel touched memory outside its buffers on these
+inputs (solver partials, neighbour rows, grids, FFT scratch included).
+"""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import admp_oracle as O
+from tests.systems import reference_pairs, small_water_box
+from tests.test_gpu_parity import compare, oracle_eval, rel_rms
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def T(x):
+    return torch.tensor(np.asarray(x, dtype=np.float64))
+
+
+@pytest.fixture(scope="module")
+def emu_lib(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is needed to build the CPU emulation")
+    from tests.cpu_emu.build import build
+    return build(str(tmp_path_factory.mktemp("emu")))
+
+
+def run(lib, *args, timeout=900):
+    env = dict(os.environ, PYTHONPATH=REPO + os.pathsep + os.environ.get("PYTHONPATH", ""))
+    p = subprocess.run([sys.executable, "-m", "tests.cpu_emu.runner", lib, *[str(a) for a in args]], cwd=REPO,
+                       env=env, capture_output=True, text=True, timeout=timeout)
+    tail = (p.stdout[-2000:] + p.stderr[-3000:])
+    assert p.returncode == 0, f"{args}: exit status {p.returncode} (86 = guard-page fault)\n{tail}"
+    lines = [l for l in p.stdout.splitlines() if l.startswith("RESULT ")]
+    assert lines, tail
+    return json.loads(lines[-1][len("RESULT "):])
+
+
+def sorted_pairs(a):
+    a = np.asarray(a, dtype=np.int64).reshape(-1, 3)
+    return a[np.lexsort((a[:, 1], a[:, 0]))]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_small_box_polarisable_pcg(emu_lib, prec):
+    s, pos, box = small_water_box()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    o, _ = oracle_eval(s, pos, box, pairs, 6.0, True, solver="cg", tol=1e-9)
+    g = run(emu_lib, "small_box", prec)
+    if prec == "fp64":
+        ok, msgs = compare(g, o)
+        assert ok and np.abs(np.asarray(g["U"]) - o["U"]).max() < 1e-8, msgs
+    else:
+        ok, msgs = compare(g, o, etol=2e-5, gtol=1e-4, keys=("pos",))
+        assert ok, msgs
+
+
+def test_batched_frames_with_library_neighbour_lists(emu_lib):
+    r = run(emu_lib, "frames", "fp64")
+    for k in range(3):
+        assert abs(r["E_batch"][k] - r["E_single"][k]) < 1e-9 * abs(r["E_single"][k])
+        assert rel_rms(r["G_batch"][k], r["G_single"][k]) < 1e-8
+
+
+def test_dispersion_and_slater(emu_lib):
+    rng = np.random.default_rng(9)
+    L, n, rc = 20.0, 60, 9.0
+    pos = rng.uniform(0, L, size=(n, 3))
+    c = rng.uniform(5, 40, size=(n, 3))
+    m = np.array([0.0, 0.0, 1.0, 1.0, 1.0, 1.0])
+    bonded = {(0, 1): 1, (1, 0): 1, (2, 3): 2, (3, 2): 2}
+    pairs = reference_pairs(pos, np.eye(3) * L, rc, bonded)
+    kappa, K1, K2, K3 = O.setup_ewald_parameters(rc, 5e-4, np.eye(3) * L)
+    p, b, ct = T(pos).requires_grad_(True), T(np.eye(3) * L).requires_grad_(True), T(c).requires_grad_(True)
+    e = O.energy_disp_pme(p, b, pairs, ct, m, kappa, (K1, K2, K3), 10)
+    gp, gb, gc = torch.autograd.grad(e, (p, b, ct))
+    e = e.detach()
+    A, B = rng.uniform(1e2, 3e2, size=n), rng.uniform(3.0, 4.5, size=n)
+    At, Bt = T(A).requires_grad_(True), T(B).requires_grad_(True)
+    p2, b2 = T(pos).requires_grad_(True), T(np.eye(3) * L).requires_grad_(True)
+    es = O.slater_sr_energy(p2, b2, pairs, m, At, Bt)
+    gp2, gb2, ga2, gbb2 = torch.autograd.grad(es, (p2, b2, At, Bt))
+    es = es.detach()
+    r = run(emu_lib, "disp_slater", "fp64")
+    assert abs(r["E_disp"] - float(e)) < 1e-9 * abs(float(e))
+    assert rel_rms(r["disp_pos"], gp.numpy()) < 1e-8
+    assert rel_rms(r["disp_box"], gb.numpy()) < 1e-8
+    assert rel_rms(r["disp_c"], gc.numpy()) < 1e-8
+    assert abs(r["E_slater"] - float(es)) < 1e-10 * max(1.0, abs(float(es)))
+    assert rel_rms(r["slater_pos"], gp2.numpy()) < 1e-9
+    assert rel_rms(r["slater_box"], gb2.numpy()) < 1e-9
+    assert rel_rms(r["slater_a"], ga2.numpy()) < 1e-9
+    assert rel_rms(r["slater_b"], gbb2.numpy()) < 1e-9
+
+
+def test_neighbour_lists_bit_exact(emu_lib):
+    r = run(emu_lib, "neighbors", "fp64")
+    s, pos, box = small_water_box()
+    for rc in (6.0, 4.0):
+        ref = sorted_pairs(reference_pairs(pos, box, rc, s.bonded))
+        got = sorted_pairs(r[f"water_{rc}"])
+        assert ref.shape == got.shape and np.array_equal(ref, got), rc
+    rng = np.random.default_rng(5)
+    p = rng.uniform(0, 11.0, size=(300, 3))
+    p[1] = p[0] + np.array([4.0, 0, 0])
+    p[3] = p[2] + np.array([0, 3.9999999999, 0])
+    for rc in (4.0, 5.4):
+        ref = sorted_pairs(reference_pairs(p, np.eye(3) * 11.0, rc, {}))
+        got = sorted_pairs(r[f"random_{rc}"])
+        assert ref.shape == got.shape and np.array_equal(ref, got), rc
+
+
+@pytest.mark.parametrize("K,prec", [((104, 104, 104), "fp64"), ((104, 104, 104), "mixed"), ((6, 7, 8), "fp64"),
+                                    ((20, 24, 15), "fp64"), ((32, 27, 25), "mixed")])
+def test_grid_paths(emu_lib, K, prec):
+    """104^3 in FP64 takes the line-major FFT passes (its plane exceeds the LDS
+    limit) and the column spread; in mixed precision the fused plane pass."""
+    s, pos, box = small_water_box()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    kappa = O.setup_ewald_parameters(6.0, 5e-4, box)[0]
+    p = T(pos).requires_grad_(True)
+    e = O.energy_pme(p, T(box), T(pairs).long(), T(s.pme.Q_local[s.map_atomtype]), None, None, None,
+                     s.pme.mScales, None, None, s.axis_types, s.axis_idx, kappa, K, 2, False, True)
+    (gp,) = torch.autograd.grad(e, (p,))
+    e = e.detach()
+    r = run(emu_lib, "grid", prec, ",".join(str(k) for k in K))
+    etol, ftol = (1e-9, 1e-7) if prec == "fp64" else (2e-5, 1e-4)
+    assert abs(r["E"] - float(e)) < etol * abs(float(e))
+    assert rel_rms(r["pos"], gp.numpy()) < ftol
+
+
+def test_water_box_several_solver_blocks(emu_lib):
+    """1,536 atoms: six solver blocks, the fused gather epilogue and a warm start."""
+    r64 = run(emu_lib, "waters", "fp64", 512)
+    rmx = run(emu_lib, "waters", "mixed", 512)
+    assert r64["residual"] < 1e-6 and r64["n_iter"] > 5
+    assert r64["n_iter_warm"] <= 1 and abs(r64["E_warm"] - r64["E"]) < 1e-9 * abs(r64["E"])
+    assert abs(rmx["E"] - r64["E"]) < 2e-5 * abs(r64["E"])

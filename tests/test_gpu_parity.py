@@ -12,7 +12,7 @@ import pytest
 import torch
 
 from oracle import admp_oracle as O
-from tests.systems import load_system, reference_pairs
+from tests.systems import load_system, reference_pairs, small_water_box as _small_waters
 
 pytestmark = pytest.mark.gpu
 F64 = torch.float64
@@ -132,22 +132,6 @@ def test_fluctuating_dimer_pme_full_precision_pin():
     print(f"fluct: E {float(e.detach()):.12f} ref {en['fluct_E']:.12f}", flush=True)
     assert abs(float(e.detach()) - en["fluct_E"]) < 1e-8
     assert rel_rms(g.cpu().numpy(), np.asarray(en["fluct_grad"])) < 1e-7
-
-
-def _small_waters(ff="fullpol_forcefield.xml"):
-    s = load_system(ff, "waterbox_31ang.pdb", "fullpol_residues.xml")
-    p = s.pos_nm * 10
-    mol = p.reshape(-1, 3, 3)
-    ok = np.all((mol > 1.0) & (mol < 13.0), axis=(1, 2))
-    idx = np.nonzero(np.repeat(ok, 3))[0]
-    remap = {int(a): k for k, a in enumerate(idx)}
-    pos = p[idx] - 0.4
-    s.bonded = {(remap[i], remap[j]): v for (i, j), v in s.bonded.items() if i in remap and j in remap}
-    s.axis_types = s.axis_types[idx]
-    s.axis_idx = np.array([[remap.get(int(a), -1) if a >= 0 else -1 for a in row] for row in s.axis_idx[idx]])
-    s.map_atomtype = s.map_atomtype[idx]
-    s.map_poltype = s.map_poltype[idx]
-    return s, pos, np.eye(3) * 13.2
 
 
 def test_small_box_pme_polar_pcg():
