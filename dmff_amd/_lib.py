@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libadmp_hip.so")
 ADMP_OK = 0
 ADMP_WARN_NOT_CONVERGED = 5
 ADMP_FP64, ADMP_MIXED = 0, 1
+ADMP_PRECOND_PAIRS, ADMP_PRECOND_DIAGONAL = 0, 1
 SOLVER_REFERENCE, SOLVER_STEPS, SOLVER_PCG = 0, 1, 2
 
 SYMBOLS = [
@@ -35,7 +36,8 @@ class Config(ctypes.Structure):
 
 class SolverOpts(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("maxiter", ctypes.c_int32), ("steps", ctypes.c_int32),
-                ("tol", ctypes.c_double), ("warm_start", ctypes.c_int32), ("fp32_products", ctypes.c_int32)]
+                ("tol", ctypes.c_double), ("warm_start", ctypes.c_int32), ("fp32_products", ctypes.c_int32),
+                ("precond", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -98,7 +98,11 @@ struct admp_ctx {
            *d_t2 = nullptr, *d_U = nullptr;
     double *d_dpart = nullptr, *d_dots = nullptr, *d_coef = nullptr, *d_res = nullptr, *d_dU = nullptr;
     PcgState* d_pcg = nullptr;
-    double* d_s = nullptr;      // CG: s = A p by recurrence
+    double* d_s = nullptr;      // CG: s = A p by recurrence (r and s: two buffers each)
+    int* d_pnear = nullptr;     // nearest coupled neighbour slot per site (-1: none)
+    int* d_pcj = nullptr;       // pair-block preconditioner partner (-1: diagonal)
+    double* d_pcd = nullptr;    // its coefficients, 8 per site
+    bool pc_pairs = true;       // pair blocks on (admp_solver_opts.precond)
     int* d_active = nullptr;
     int dots_blocks = 64;
     double* d_pparts = nullptr;               // CG partials (CgParts layout)
@@ -114,6 +118,7 @@ struct admp_ctx {
         int type = -1, nrun = 0;
         const void *rhs = nullptr, *x = nullptr;
         double tol = 0, f1 = 0;
+        bool pc = true;
     };
     ChunkGraph graphs[4];
     bool use_graphs = true;   // ADMP_NO_GRAPH=1 runs the chunk launches eagerly
@@ -253,7 +258,7 @@ static std::vector<double> bspline_moduli(int K) {
 
 // dynamic LDS of k_fxy_plane: two K0 x K1 planes and the x and y twiddles
 static size_t plane_smem(int K0, int K1, size_t cs) { return ((size_t)2 * K0 * K1 + K0 + K1) * cs; }
-constexpr size_t PLANE_LDS_MAX = 80000;
+constexpr size_t PLANE_LDS_MAX = 81920;   // 80 KiB: two blocks fill a CU's 160 KiB
 
 // The dynamic-LDS ceiling of the plane kernels is a process-wide attribute of
 // each kernel: set it once to PLANE_LDS_MAX (every plane launch stays below it),
@@ -322,7 +327,7 @@ static admp_status setup_grid(admp_ctx* ctx, Grid& g, int K0, int K1, int K2, bo
     }
     // The fused plane pass keeps two K0 x K1 complex planes and both twiddle
     // tables in LDS.  It is used up to PLANE_LDS_MAX bytes (two blocks per CU;
-    // e.g. 50^2 in FP64, 70^2 in FP32); larger planes take the line passes.
+    // up to 50^2 in FP64, 71^2 in FP32); larger planes take the line passes.
     const char* env = std::getenv("ADMP_FFT_PLANE");
     const bool allow = !(env && env[0] == '0') && K0 >= BRX + 5 && K1 >= BRY + 5 && K2 >= BRZ + 5;
     for (int w = 0; w < 2; ++w) {
@@ -505,14 +510,15 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_contrib, NF * 12); ALLOC(ctx->d_vir_fr, NF * 9); ALLOC(ctx->d_gQl, NF * 9);
     ALLOC(ctx->d_red, ctx->F * 16);
     ALLOC(ctx->d_red9a, ctx->F * 9); ALLOC(ctx->d_red9b, ctx->F * 9); ALLOC(ctx->d_red9c, ctx->F * 9);
-    ALLOC(ctx->d_b, NF * 3); ALLOC(ctx->d_r, NF * 3); ALLOC(ctx->d_z, NF * 3); ALLOC(ctx->d_p, NF * 3);
+    ALLOC(ctx->d_b, NF * 3); ALLOC(ctx->d_r, NF * 6); ALLOC(ctx->d_z, NF * 3); ALLOC(ctx->d_p, NF * 3);
     ALLOC(ctx->d_Ap, NF * 3); ALLOC(ctx->d_t1, NF * 3); ALLOC(ctx->d_t2, NF * 3); ALLOC(ctx->d_U, NF * 3);
     ALLOC(ctx->d_dpart, (size_t)ctx->dots_blocks * ctx->F * 3);
     ALLOC(ctx->d_dots, ctx->F * 3);
     ctx->pcg_nb = (int)nblk(ctx->N, PCG_BLOCK);
     ctx->cg_nw = std::max(ctx->pcg_nb, (ctx->N + 31) / 32);
     ALLOC(ctx->d_pparts, (size_t)ctx->F * (ctx->cg_nw + 4 * ctx->pcg_nb));
-    ALLOC(ctx->d_s, NF * 3);
+    ALLOC(ctx->d_s, NF * 6);
+    ALLOC(ctx->d_pnear, NF); ALLOC(ctx->d_pcj, NF); ALLOC(ctx->d_pcd, NF * 8);
     ALLOC(ctx->d_swfrac, NF * 3);
     ALLOC(ctx->d_sfq, NF * 10);
     ALLOC(ctx->d_res, NF * 3);
@@ -1141,6 +1147,9 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     return ADMP_OK;
 }
 
+// partners of the pair-block preconditioner, or null for the diagonal one
+static const int* pc_j(const admp_ctx* ctx) { return ctx->pc_pairs ? ctx->d_pcj : nullptr; }
+
 // k_cg_update of parity cur with products of type R (z copy for the pair kernel
 // and fractional dipoles typed R)
 template <class R>
@@ -1148,7 +1157,8 @@ static void launch_update(admp_ctx* ctx, int cur, double* x, double tol, const d
     hipLaunchKernelGGL(k_cg_update<R>, dim3(ctx->pcg_nb, ctx->F), dim3(PCG_BLOCK), 0, ctx->stream, ctx->N,
                        ctx->cg_nwu, cur, tol, ctx->d_box, pol, x, ctx->d_r, ctx->d_z, ctx->d_p, ctx->d_s, ctx->d_Ap,
                        sizeof(R) == 8 ? nullptr : reinterpret_cast<R*>(ctx->d_vR),
-                       ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr, cg_parts(ctx), ctx->d_pcg);
+                       ctx->cfg.lpme ? reinterpret_cast<R*>(ctx->d_sfq) : nullptr, pc_j(ctx), ctx->d_pcd,
+                       cg_parts(ctx), ctx->d_pcg);
 }
 
 // true residual r = rhs - A x with FP64 products (RC-typed stored pair
@@ -1168,12 +1178,14 @@ static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x
         hipLaunchKernelGGL(k_cg_start<double>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->d_box, wd, pol,
                            rhs, ctx->d_res, ctx->d_r, ctx->d_z, (double*)nullptr,
                            reinterpret_cast<double*>(ctx->d_swfrac),
-                           ctx->cfg.lpme ? reinterpret_cast<double*>(ctx->d_sfq) : nullptr, cg_parts(ctx));
+                           ctx->cfg.lpme ? reinterpret_cast<double*>(ctx->d_sfq) : nullptr, pc_j(ctx), ctx->d_pcd,
+                           cg_parts(ctx));
     else
         hipLaunchKernelGGL(k_cg_start<float>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->d_box, wd, pol,
                            rhs, ctx->d_res, ctx->d_r, ctx->d_z, reinterpret_cast<float*>(ctx->d_vR),
                            reinterpret_cast<float*>(ctx->d_swfrac),
-                           ctx->cfg.lpme ? reinterpret_cast<float*>(ctx->d_sfq) : nullptr, cg_parts(ctx));
+                           ctx->cfg.lpme ? reinterpret_cast<float*>(ctx->d_sfq) : nullptr, pc_j(ctx), ctx->d_pcd,
+                           cg_parts(ctx));
     hipLaunchKernelGGL(k_cg_publish, dim3(F), dim3(PCG_BLOCK), 0, st, mode, tol, cg_parts(ctx), ctx->d_pcg);
     HIPCHK(hipGetLastError());
     return ADMP_OK;
@@ -1236,7 +1248,8 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         const int type = (sizeof(RC) == 8 ? 1 : 0) + (use_dbl ? 2 : 0);
         admp_ctx::ChunkGraph* cg = nullptr;
         for (auto& g : ctx->graphs)
-            if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1)
+            if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1 &&
+                g.pc == ctx->pc_pairs)
                 cg = &g;
         if (!cg) {
             admp_ctx::ChunkGraph* slot = nullptr;
@@ -1268,7 +1281,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
                 return iterate();
             }
             slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
-            slot->tol = tol; slot->f1 = f1;
+            slot->tol = tol; slot->f1 = f1; slot->pc = ctx->pc_pairs;
             cg = slot;
         }
         ProfScope ps_(ctx, 12);
@@ -1368,7 +1381,9 @@ static admp_status solver_rhs(admp_ctx* ctx, const double* pol, const double* th
     { ProfScope ps_(ctx, 1);
     hipLaunchKernelGGL((k_real_permfield<R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
                        ctx->cap, ctx->d_pos, ctx->d_box, site, ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_b,
-                       reinterpret_cast<R*>(ctx->d_pcoef)); }
+                       reinterpret_cast<R*>(ctx->d_pcoef), ctx->d_pnear);
+    hipLaunchKernelGGL((k_pc_pairs<R>), dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box,
+                       site, pol, ctx->d_nbr, pp, ctx->d_pnear, ctx->d_pcj, ctx->d_pcd); }
     HIPCHK(hipGetLastError());
     if (c.lpme) {
         admp_status s = recip_run<RB>(ctx, ctx->eg, 0, ctx->d_Qc, nullptr, 1, lmax_eff, 0, DIELECTRIC, kappa, 1,
@@ -1396,6 +1411,7 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     const double tol = o ? o->tol : 1e-6;
     const int maxiter = o ? (kind == ADMP_SOLVER_STEPS ? o->steps : o->maxiter) : 500;
     const bool mixed = sizeof(R) == 4;
+    ctx->pc_pairs = !(o && o->precond == ADMP_PRECOND_DIAGONAL);
     if (o && o->warm_start && U_in)
         HIPCHK(hipMemcpyAsync(U, U_in, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     else

@@ -22,6 +22,7 @@ namespace admp {
 
 constexpr int TPA = 16;           // lanes per atom
 constexpr int PAIR_BLOCK = 256;   // 16 atoms per block
+constexpr double PC_RPAIR = 2.0;  // A: pair-block preconditioner partner range
 
 // site record (R precision), 16 words per atom
 // 0 q | 1-3 mu | 4-9 Theta xx yy zz xy xz yz | 10 alpha^(1/6) | 11 thole | 12 alpha | 13-15 U
@@ -390,7 +391,7 @@ template <class R>
 __global__ void __launch_bounds__(PAIR_BLOCK)
 k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
                  const R* __restrict__ site, const int* __restrict__ nbr, const int* __restrict__ ncnt,
-                 PairParams pp, double* __restrict__ field, R* __restrict__ pcoef) {
+                 PairParams pp, double* __restrict__ field, R* __restrict__ pcoef, int* __restrict__ pnear) {
     const int f = blockIdx.y;
     const int gt = blockIdx.x * PAIR_BLOCK + threadIdx.x;
     const int i = gt / TPA, sub = gt % TPA;
@@ -406,6 +407,10 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
     const int cnt = active ? ncnt[fo + ii] : 0;
     const int* nl = nbr + (fo + ii) * (size_t)cap;
     R* pc = pcoef + (fo + ii) * (size_t)cap * 2;
+    // nearest neighbour with induced-induced coupling (d-scale > 0) inside
+    // PC_RPAIR: candidate partner of the pair-block preconditioner
+    double nr = PC_RPAIR;
+    int nj = -1, nk = -1;
     for (int k = sub; k < cnt; k += TPA) {
         const int w = nl[k];
         const int j = w & NB_MASK, nb = w >> NB_SHIFT;
@@ -420,6 +425,7 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
         const R TJxx = sj[4], TJyy = sj[5], TJzz = sj[6], TJxy = sj[7], TJxz = sj[8], TJyz = sj[9];
         const int sidx = scale_index(nb, pp.nscale);
         const R ps = (R)pp.ps[sidx], dsc = (R)pp.ds[sidx];
+        if (pp.ds[sidx] > 0.0 && (rd < nr || (rd == nr && nj >= 0 && j < nj))) { nr = rd; nj = j; nk = k; }
         const R mJz = mJx * zx + mJy * zy + mJz_ * zz;
         const R tJx = TJxx * zx + TJxy * zy + TJxz * zz;
         const R tJy = TJxy * zx + TJyy * zy + TJyz * zz;
@@ -453,10 +459,79 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
         pc[2 * k + 1] = ud0 - ud1;
     }
     bx_ = group_sum(bx_, TPA); by_ = group_sum(by_, TPA); bz_ = group_sum(bz_, TPA);
+    // smallest (distance, atom index) over the lanes of the atom
+    for (int o = TPA >> 1; o > 0; o >>= 1) {
+        const double r2 = __shfl_xor(nr, o, TPA);
+        const int j2 = __shfl_xor(nj, o, TPA), k2 = __shfl_xor(nk, o, TPA);
+        if (j2 >= 0 && (nj < 0 || r2 < nr || (r2 == nr && j2 < nj))) { nr = r2; nj = j2; nk = k2; }
+    }
     if (active && sub == 0) {
         const size_t a = fo + i;
         field[a * 3 + 0] = bx_; field[a * 3 + 1] = by_; field[a * 3 + 2] = bz_;
+        if (pnear) pnear[a] = nk;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Pair-block preconditioner (set-up once per solve; positions are fixed).
+// Two sites that are each other's nearest coupled neighbour inside PC_RPAIR
+// form a pair; its 6x6 block of the polarisation matrix is approximated by
+//   [[d_i I, B], [B, d_j I]],  d = DIELECTRIC / alpha,
+//   B = c1 I + c2 z z^T  (the Thole-damped dipole-dipole tensor without Ewald
+//   screening, d-scaled, with the Thole width of the induced-induced product),
+// and its inverse is exact in closed form: along z (b = c1 + c2) and across it
+// (b = c1) the block is [[d_i, b], [b, d_j]] with inverse
+// [[d_j, -b], [-b, d_i]] / (d_i d_j - b^2).  Row i keeps
+//   z, a_par = d_j / D_par, b_par = -b_par / D_par, a_perp, b_perp
+// so that (M^-1 r)_i = a_perp r_i + b_perp r_j
+//                      + ((a_par - a_perp) z.r_i + (b_par - b_perp) z.r_j) z.
+// Sites without a partner (or with a nearly singular block) keep the diagonal
+// preconditioner 1/d_i.  The blocks of close contacts may be indefinite; they
+// carry the local modes that slow a diagonal-preconditioned solve most.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void __launch_bounds__(256)
+k_pc_pairs(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
+           const R* __restrict__ site, const double* __restrict__ pol, const int* __restrict__ nbr,
+           PairParams pp, const int* __restrict__ pnear, int* __restrict__ pcj, double* __restrict__ pcd) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const size_t fo = (size_t)f * N, a = fo + i;
+    pcj[a] = -1;
+    const int k = pnear[a];
+    if (k < 0) return;
+    const int w = nbr[a * (size_t)cap + k];
+    const int j = w & NB_MASK, nb = w >> NB_SHIFT;
+    const int kj = pnear[fo + j];
+    if (kj < 0 || (nbr[(fo + j) * (size_t)cap + kj] & NB_MASK) != i) return;   // not mutual
+    const BoxInfo& bx = boxes[f];
+    const double4 pi_ = pos[a], pj = pos[fo + j];
+    double dx = pi_.x - pj.x, dy = pi_.y - pj.y, dz = pi_.z - pj.z, n0, n1, n2;
+    pbc_shift(bx, dx, dy, dz, n0, n1, n2);
+    const double r = sqrt(dx * dx + dy * dy + dz * dz), ri = 1.0 / r;
+    const int sidx = scale_index(nb, pp.nscale);
+    const double dsc = pp.ds[sidx];
+    const R* si = site + a * SITE;
+    const R* sj = site + (fo + j) * SITE;
+    const double th_a = pp.ps[sidx] > 0.0 ? 5.0 : (double)si[11] + (double)sj[11];
+    double dmp = (double)si[10] * (double)sj[10];
+    if (dmp < 1e-8) dmp = 1e-8;
+    double u = r / dmp;
+    if (!(u < 1e8)) u = 1e8;
+    Thole<double> th;
+    thole_f<double>(th_a * u, th);
+    const double P3 = DIELECTRIC * ri * ri * ri;
+    const double c1 = P3 * dsc * th.c, c2 = P3 * dsc * (-2.0 * th.d0 - th.c);
+    const double di = DIELECTRIC / (pol[i] < 1e-8 ? 1e-8 : pol[i]);
+    const double dj = DIELECTRIC / (pol[j] < 1e-8 ? 1e-8 : pol[j]);
+    const double bpar = c1 + c2, bperp = c1, dd = di * dj;
+    const double Dpar = dd - bpar * bpar, Dperp = dd - bperp * bperp;
+    if (fabs(Dpar) < 1e-3 * dd || fabs(Dperp) < 1e-3 * dd) return;   // nearly singular: keep the diagonal
+    double* o = pcd + a * 8;
+    o[0] = dx * ri; o[1] = dy * ri; o[2] = dz * ri; o[3] = 0.0;
+    o[4] = dj / Dpar; o[5] = -bpar / Dpar; o[6] = dj / Dperp; o[7] = -bperp / Dperp;
+    pcj[a] = j;
 }
 
 // out_I = sum_J c1 V_J + c2 (z.V_J) z   (induced-induced real-space coupling)

@@ -662,14 +662,33 @@ __device__ inline void cg_derived(const BoxInfo* __restrict__ boxes, int f, size
     for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u[0] + A[1 * 3 + j] * u[1] + A[2 * 3 + j] * u[2]);
 }
 
+// z = M^-1 r at site a: the pair block of a (partner j >= 0 with its row of
+// coefficients pd, see k_pc_pairs) or the diagonal m = alpha / DIELECTRIC
+__device__ inline void pc_apply(int j, const double* __restrict__ pd, double m, const double* ri,
+                                const double* rj, double* u) {
+    if (j < 0) {
+        for (int k = 0; k < 3; ++k) u[k] = m * ri[k];
+        return;
+    }
+    const double zx = pd[0], zy = pd[1], zz = pd[2];
+    const double pi = zx * ri[0] + zy * ri[1] + zz * ri[2];
+    const double pj = zx * rj[0] + zy * rj[1] + zz * rj[2];
+    const double t = (pd[4] - pd[6]) * pi + (pd[5] - pd[7]) * pj;
+    u[0] = pd[6] * ri[0] + pd[7] * rj[0] + t * zx;
+    u[1] = pd[6] * ri[1] + pd[7] * rj[1] + t * zy;
+    u[2] = pd[6] * ri[2] + pd[7] * rj[2] + t * zz;
+}
+
 // r = b - Ax, z = M^-1 r and the product inputs derived from z (with the spline
-// coordinates wfrac of the solve's precision R); partials into buffers 0
+// coordinates wfrac of the solve's precision R); r into buffer 0 of its two,
+// partials into buffers 0.  pcj/pcd: pair-block preconditioner (null: diagonal).
 template <class R>
 __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __restrict__ boxes,
                                                         const double* __restrict__ wd, const double* __restrict__ pol,
                                                         const double* __restrict__ b, const double* __restrict__ Ax,
                                                         double* __restrict__ r, double* __restrict__ z,
                                                         R* __restrict__ zR, R* __restrict__ wfrac, R* __restrict__ fq,
+                                                        const int* __restrict__ pcj, const double* __restrict__ pcd,
                                                         CgParts parts) {
     __shared__ double sh[16];
     const int f = blockIdx.y;
@@ -677,16 +696,22 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __
     double rz = 0, mx = 0;
     if (i < N) {
         const size_t a = (size_t)f * N + i;
-        const double m = inv_diag(pol[i]);
-        double u[3];
+        double rv[3], rj[3] = {0, 0, 0}, u[3];
         for (int k = 0; k < 3; ++k) {
-            const double rv = b[a * 3 + k] - Ax[a * 3 + k];
-            r[a * 3 + k] = rv;
-            u[k] = m * rv;
-            z[a * 3 + k] = u[k];
-            rz += rv * u[k];
-            if (pol[i] > 0.001) mx = fmax(mx, fabs(rv));
+            rv[k] = b[a * 3 + k] - Ax[a * 3 + k];
+            r[a * 3 + k] = rv[k];
+            if (pol[i] > 0.001) mx = fmax(mx, fabs(rv[k]));
             if (wd) wfrac[a * 3 + k] = (R)wd[a * 3 + k];
+        }
+        const int j = pcj ? pcj[a] : -1;
+        if (j >= 0) {
+            const size_t aj = (size_t)f * N + j;
+            for (int k = 0; k < 3; ++k) rj[k] = b[aj * 3 + k] - Ax[aj * 3 + k];
+        }
+        pc_apply(j, j >= 0 ? pcd + a * 8 : nullptr, inv_diag(pol[i]), rv, rj, u);
+        for (int k = 0; k < 3; ++k) {
+            z[a * 3 + k] = u[k];
+            rz += rv[k] * u[k];
         }
         if (fq) {
             R* o = fq + a * 10;
@@ -759,7 +784,8 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
                                                          double* __restrict__ r, double* __restrict__ z,
                                                          double* __restrict__ p, double* __restrict__ s,
                                                          const double* __restrict__ w, R* __restrict__ zR,
-                                                         R* __restrict__ fq, CgParts parts,
+                                                         R* __restrict__ fq, const int* __restrict__ pcj,
+                                                         const double* __restrict__ pcd, CgParts parts,
                                                          PcgState* __restrict__ st) {
     __shared__ double sh[48];
     const int f = blockIdx.y;
@@ -783,24 +809,41 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
     const double den = restart ? delta : delta - beta * gamma / ap;
     const double alpha = den != 0.0 ? gamma / den : 0.0;
     const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
+    // r and s have two buffers: this update reads buffer cur and writes cur ^ 1,
+    // so a site can form its partner's new residual from values no thread writes
+    const size_t nf3 = (size_t)gridDim.y * N * 3;
+    const double* rc = r + (size_t)cur * nf3;
+    double* rn = r + (size_t)nx * nf3;
+    const double* sc = s + (size_t)cur * nf3;
+    double* sn = s + (size_t)nx * nf3;
     double lrz = 0, lmx = 0;
     if (i < N) {
         const size_t a = (size_t)f * N + i;
-        const double m = inv_diag(pol[i]);
-        double u[3];
+        double u[3], rv[3], rj[3] = {0, 0, 0};
         for (int k = 0; k < 3; ++k) {
             const size_t e = a * 3 + k;
             const double pv = restart ? z[e] : z[e] + beta * p[e];
-            const double sv = restart ? w[e] : w[e] + beta * s[e];
+            const double sv = restart ? w[e] : w[e] + beta * sc[e];
             p[e] = pv;
-            s[e] = sv;
+            sn[e] = sv;
             x[e] += alpha * pv;
-            const double rv = r[e] - alpha * sv;
-            r[e] = rv;
-            u[k] = m * rv;
-            z[e] = u[k];
-            lrz += rv * u[k];
-            if (pol[i] > 0.001) lmx = fmax(lmx, fabs(rv));
+            rv[k] = rc[e] - alpha * sv;
+            rn[e] = rv[k];
+            if (pol[i] > 0.001) lmx = fmax(lmx, fabs(rv[k]));
+        }
+        const int j = pcj ? pcj[a] : -1;
+        if (j >= 0) {
+            const size_t aj = (size_t)f * N + j;
+            for (int k = 0; k < 3; ++k) {
+                const size_t e = aj * 3 + k;
+                const double sv = restart ? w[e] : w[e] + beta * sc[e];
+                rj[k] = rc[e] - alpha * sv;
+            }
+        }
+        pc_apply(j, j >= 0 ? pcd + a * 8 : nullptr, inv_diag(pol[i]), rv, rj, u);
+        for (int k = 0; k < 3; ++k) {
+            z[a * 3 + k] = u[k];
+            lrz += rv[k] * u[k];
         }
         cg_derived<R>(boxes, f, a, u, zR, fq);
     }

@@ -2,7 +2,7 @@
 results as one JSON line prefixed "RESULT ".  A guard-page fault ends the
 process with status 86 after the handler has named the kernel, block and thread.
 
-    python -m tests.cpu_emu.runner <libadmp_emu.so> <case> [precision] [grid K1,K2,K3 | waters count]
+    python -m tests.cpu_emu.runner <libadmp_emu.so> <case> [precision] [grid K1,K2,K3 | waters/precond count]
 
 The cases call the same Python interface as the GPU parity tests; only the
 Engine's device set-up is replaced (CPU tensors, the null stream).
@@ -184,6 +184,25 @@ def waters(precision, n_waters):
             "rc": spec.rc, "n_atoms": spec.n_atoms}
 
 
+def precond(precision, n_waters):
+    """The same solve with the diagonal and the pair-block preconditioner."""
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    from dmff_amd.admp.serving import water_spec
+    spec, pos, box = water_spec("w", n_waters, 7)
+    kappa, K1, K2, K3 = setup_ewald_parameters(spec.rc, 5e-4, np.asarray(box))
+    eng = Engine(spec.n_atoms, 2, True, True, kappa, (K1, K2, K3), spec.rc, precision=precision)
+    eng.set_topology(spec.axis_types, spec.axis_idx, spec.exclusions)
+    eng.build_neighbors(pos, box)
+    out = {}
+    for pc in ("diagonal", "pairs"):
+        E, g, U, info = eng.compute(pos, box, spec.Q_local, spec.pol, spec.thole, *spec.scales, solver="pcg",
+                                    tol=1e-6, maxiter=500, want_grad=True, precond=pc)
+        out[pc] = {"E": float(E[0]), "pos": _lst(g["pos"][0]), "U": _lst(U[0]), "n_iter": info["n_iter"][0],
+                   "residual": info["residual"][0]}
+    return out
+
+
 CASES = {"small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
 
 if __name__ == "__main__":
@@ -194,6 +213,8 @@ if __name__ == "__main__":
         out = grid(prec, tuple(int(k) for k in sys.argv[4].split(",")))
     elif case == "waters":
         out = waters(prec, int(sys.argv[4]))
+    elif case == "precond":
+        out = precond(prec, int(sys.argv[4]))
     else:
         out = CASES[case](prec)
     print("RESULT " + json.dumps(out), flush=True)
