@@ -42,6 +42,10 @@ struct admp_ctx {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t side = nullptr;         // second context stream: real-space products beside the reciprocal chain
     PcgState* h_pcg = nullptr;          // pinned host copies of the PCG state (two slots of F)
+    // neighbour build whose row-count check is still pending (resolved by nbr_settle)
+    bool nbr_pending = false;
+    int* h_maxcnt = nullptr;            // pinned copy of the longest row
+    hipEvent_t ev_nbr = nullptr;
     hipEvent_t ev_state[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
@@ -50,6 +54,7 @@ struct admp_ctx {
     bool has_topo = false;
     std::vector<char> frame_nbr;  // per frame: its pair list is in d_nbr / d_ncnt
     bool nbr_built = false;       // lists (and d_nhalf) come from admp_build_neighbors
+    CellGrid nbr_cg{};            // cell grid of the last list build
     int use_frames = 0;
     // topology
     int *d_axis_type = nullptr, *d_axis_idx = nullptr, *d_xoff = nullptr, *d_xent = nullptr;
@@ -339,6 +344,8 @@ static admp_status setup_grid(admp_ctx* ctx, Grid& g, int K0, int K1, int K2, bo
 
 extern "C" const char* admp_version(void) { return "admp-hip 0.1 (gfx950)"; }
 
+static admp_status nbr_settle(admp_ctx* ctx);
+
 extern "C" const char* admp_last_error(const admp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 extern "C" admp_status admp_set_stream(admp_ctx* ctx, void* stream) {
@@ -394,6 +401,8 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     for (int k = 0; k < 2; ++k)
         if (ctx->ev_state[k]) (void)hipEventDestroy(ctx->ev_state[k]);
     if (ctx->h_pcg) (void)hipHostFree(ctx->h_pcg);
+    if (ctx->h_maxcnt) (void)hipHostFree(ctx->h_maxcnt);
+    if (ctx->ev_nbr) (void)hipEventDestroy(ctx->ev_nbr);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
@@ -438,6 +447,8 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         hipEventCreateWithFlags(&ctx->ev_state[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_state[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_pcg, 2 * sizeof(PcgState) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_maxcnt, sizeof(int), 0) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_nbr, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess) {
@@ -542,6 +553,7 @@ extern "C" admp_status admp_set_topology(admp_ctx* ctx, const int32_t* axis_type
                                          int64_t n_excl, const int32_t* excl_i, const int32_t* excl_j,
                                          const int32_t* excl_nbond) {
     if (!ctx) return ADMP_ERR_ARG;
+    { const admp_status ns = nbr_settle(ctx); if (ns != ADMP_OK) return ns; }
     const int N = ctx->N;
     std::vector<int> at(N, 5), ai(3 * N, -1);
     if (ctx->use_frames) {
@@ -668,6 +680,7 @@ static admp_status upload_geometry(admp_ctx* ctx, const double* pos, const doubl
 static admp_status admp_set_pairs_impl(admp_ctx* ctx, int32_t frame, const void* pairs, int64_t n_pairs,
                                       int32_t pairs_is_int64) {
     if (!ctx || frame < 0 || frame >= ctx->F || (n_pairs > 0 && !pairs)) return ADMP_ERR_ARG;
+    { const admp_status ns = nbr_settle(ctx); if (ns != ADMP_OK) return ns; }
     const int N = ctx->N;
     int* cnt = ctx->d_ncnt + (size_t)frame * N;
     int* bad = ctx->d_maxcnt;
@@ -715,11 +728,51 @@ static admp_status admp_set_pairs_impl(admp_ctx* ctx, int32_t frame, const void*
     return ADMP_OK;
 }
 
+// The list build writes rows up to the capacity and the longest row it found;
+// that count reaches the host through a pinned copy behind an event, so the
+// caller's next work is queued without waiting for the build.
+static admp_status launch_nbr_build(admp_ctx* ctx) {
+    const int N = ctx->N, F = ctx->F;
+    const CellGrid cg = ctx->nbr_cg;
+    const double rc = ctx->cfg.rc;
+    HIPCHK(hipMemsetAsync(ctx->d_maxcnt, 0, sizeof(int), ctx->stream));
+    { ProfScope ps_(ctx, 8);
+    hipLaunchKernelGGL(k_nbr_build, dim3(nblk(N, 4), F), dim3(256), 0, ctx->stream, N, ctx->cap, ctx->d_pos, ctx->d_box,
+                       cg, rc * rc, ctx->d_cell, ctx->d_cstart, ctx->d_spos, ctx->d_xoff, ctx->d_xent, ctx->d_nbr,
+                       ctx->d_ncnt, ctx->d_nhalf, ctx->d_maxcnt); }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->h_maxcnt, ctx->d_maxcnt, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev_nbr, ctx->stream));
+    ctx->nbr_pending = true;
+    return ADMP_OK;
+}
+
+// Resolve a pending list build before anything reads the lists, the geometry or
+// the cell tables it used: when a row overflowed the capacity, grow it and build
+// again from the same geometry (this one waits).  Every entry point that reads
+// or replaces them calls this first.
+static admp_status nbr_settle(admp_ctx* ctx) {
+    if (!ctx->nbr_pending) return ADMP_OK;
+    ctx->nbr_pending = false;
+    HIPCHK(hipEventSynchronize(ctx->ev_nbr));
+    const int mx = *ctx->h_maxcnt;
+    if (mx <= ctx->cap) return ADMP_OK;
+    admp_status s = ensure_cap(ctx, mx);
+    if (s != ADMP_OK) return s;
+    if ((s = launch_nbr_build(ctx)) != ADMP_OK) return s;
+    ctx->nbr_pending = false;
+    HIPCHK(hipEventSynchronize(ctx->ev_nbr));
+    if (*ctx->h_maxcnt > ctx->cap) { ctx->err = "neighbour list capacity"; return ADMP_ERR_HIP; }
+    return ADMP_OK;
+}
+
 static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, const double* box) {
     if (!ctx || !pos || !box) return ADMP_ERR_ARG;
     if (!ctx->has_topo) { ctx->err = "admp_set_topology first"; return ADMP_ERR_STATE; }
     const int N = ctx->N, F = ctx->F;
-    admp_status s = upload_geometry(ctx, pos, box);
+    admp_status s = nbr_settle(ctx);
+    if (s != ADMP_OK) return s;
+    s = upload_geometry(ctx, pos, box);
     if (s != ADMP_OK) return s;
     // cell grid from frame boxes (host copy)
     std::vector<double> hb(9 * F);
@@ -780,18 +833,8 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
         est = std::min(est, N);
         if ((s = ensure_cap(ctx, std::max(est, 1))) != ADMP_OK) return s;
     }
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        HIPCHK(hipMemsetAsync(ctx->d_maxcnt, 0, sizeof(int), ctx->stream));
-        { ProfScope ps_(ctx, 8); hipLaunchKernelGGL(k_nbr_build, dim3(nblk(N, 4), F), dim3(256), 0, ctx->stream, N, ctx->cap, ctx->d_pos,
-                           ctx->d_box, cg, rc * rc, ctx->d_cell, ctx->d_cstart, ctx->d_spos, ctx->d_xoff,
-                           ctx->d_xent, ctx->d_nbr, ctx->d_ncnt, ctx->d_nhalf, ctx->d_maxcnt); }
-        HIPCHK(hipGetLastError());
-        int mx = 0;
-        HIPCHK(hipMemcpyAsync(&mx, ctx->d_maxcnt, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (mx <= ctx->cap) break;
-        if ((s = ensure_cap(ctx, mx)) != ADMP_OK) return s;
-    }
+    ctx->nbr_cg = cg;
+    if ((s = launch_nbr_build(ctx)) != ADMP_OK) return s;
     ctx->frame_nbr.assign(ctx->F, 1);
     ctx->nbr_built = true;
     return ADMP_OK;
@@ -800,6 +843,7 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
 static admp_status admp_get_pairs_impl(admp_ctx* ctx, int32_t frame, int32_t* out, int64_t cap, int64_t* n_out) {
     if (!ctx || frame < 0 || frame >= ctx->F || !n_out) return ADMP_ERR_ARG;
     if (!ctx->nbr_built) { ctx->err = "no neighbour list built by admp_build_neighbors"; return ADMP_ERR_STATE; }
+    { const admp_status ns = nbr_settle(ctx); if (ns != ADMP_OK) return ns; }
     const int N = ctx->N;
     // nhalf holds the j > i counts for lists built on the GPU
     int* off;
@@ -1340,20 +1384,43 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
             q[nq++] = last_slot;
             return ADMP_OK;
         };
+        // A chunk launched after convergence still costs every graph node's launch
+        // (~10 us each), so the next chunk is not launched ahead when the chunk in
+        // flight is predicted to finish the solve: each frame's residual is
+        // extrapolated geometrically from its last two states.  A wrong
+        // prediction costs one host round trip before the next chunk starts.
+        std::vector<PcgState> older = hs;
+        bool have_older = false;
+        auto finishes_in_flight = [&]() -> bool {
+            if (!have_older) return false;
+            for (int f = 0; f < F; ++f) {
+                if (hs[f].done) continue;
+                const int di = hs[f].iter - older[f].iter;
+                if (di <= 0 || !(hs[f].res > 0.0) || !(hs[f].res < older[f].res)) return false;
+                const double rate = std::log(hs[f].res / older[f].res) / di;   // per iteration, < 0
+                const double need = std::log(tol / hs[f].res) / rate;          // iterations still needed
+                if (!(need <= ch - 1.5)) return false;
+            }
+            return true;
+        };
         if ((s = launch_next()) != ADMP_OK) return s;
         while (nq > 0) {
-            if (done_iters < maxiter && nq < 2)
+            if (done_iters < maxiter && nq < 2 && !finishes_in_flight())
                 if ((s = launch_next()) != ADMP_OK) return s;
             const int sl = q[0];
             q[0] = q[1];
             --nq;
             HIPCHK(hipEventSynchronize(ctx->ev_state[sl]));
+            older = hs;
+            have_older = true;
             all = true;
             for (int f = 0; f < F; ++f) {
                 hs[f] = ctx->h_pcg[(size_t)sl * F + f];
                 all = all && hs[f].done;
             }
             if (all) break;
+            if (nq == 0 && done_iters < maxiter)
+                if ((s = launch_next()) != ADMP_OK) return s;   // the prediction missed
         }
     }
     it_out.resize(F);
@@ -1620,6 +1687,7 @@ static admp_status admp_compute_impl(admp_ctx* ctx, const double* pos, const dou
                                     double* dbox, double* dQ_local, double* dpol, double* dthole, admp_stats* stats) {
     if (!ctx || !pos || !box || !Q_local) return ADMP_ERR_ARG;
     if (!ctx->has_topo) { ctx->err = "admp_set_topology first"; return ADMP_ERR_STATE; }
+    { const admp_status ns = nbr_settle(ctx); if (ns != ADMP_OK) return ns; }
     if (!all_frames_have_pairs(ctx)) {
         ctx->err = "no pair list for every frame: admp_set_pairs per frame or admp_build_neighbors first";
         return ADMP_ERR_STATE;
@@ -1716,6 +1784,7 @@ static admp_status admp_disp_compute_impl(admp_ctx* ctx, const double* pos, cons
                                          const double* mScales, int32_t n_scales, int32_t want_grad, double* energy,
                                          double* dpos, double* dbox, double* dc) {
     if (!ctx || !pos || !box || !c_list) return ADMP_ERR_ARG;
+    { const admp_status ns = nbr_settle(ctx); if (ns != ADMP_OK) return ns; }
     if (!all_frames_have_pairs(ctx)) { ctx->err = "no pair list for every frame"; return ADMP_ERR_STATE; }
     if (n_scales < 1 || n_scales > 8) { ctx->err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
     admp_status s = upload_geometry(ctx, pos, box);
@@ -1729,6 +1798,7 @@ static admp_status admp_slater_compute_impl(admp_ctx* ctx, const double* pos, co
                                            const double* b, const double* mScales, int32_t n_scales, int32_t want_grad,
                                            double* energy, double* dpos, double* dbox, double* da, double* db) {
     if (!ctx || !pos || !box || !a || !b) return ADMP_ERR_ARG;
+    { const admp_status ns = nbr_settle(ctx); if (ns != ADMP_OK) return ns; }
     if (!all_frames_have_pairs(ctx)) { ctx->err = "no pair list for every frame"; return ADMP_ERR_STATE; }
     if (n_scales < 1 || n_scales > 8) { ctx->err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
     admp_status s = upload_geometry(ctx, pos, box);

@@ -136,6 +136,11 @@ def neighbors(precision):
         e.set_topology(None, None, s.bonded)
         e.build_neighbors(pos, box)
         out[f"water_{rc}"] = e.get_pairs(0).numpy().astype(np.int64).tolist()
+    # capacity 8 per row: the first build overflows and is rebuilt before the lists are read
+    e = Engine(len(pos), 0, False, False, 0.0, (0, 0, 0), 6.0, max_neighbors=8)
+    e.set_topology(None, None, s.bonded)
+    e.build_neighbors(pos, box)
+    out["water_6.0_cap8"] = e.get_pairs(0).numpy().astype(np.int64).tolist()
     rng = np.random.default_rng(5)
     p = rng.uniform(0, 11.0, size=(300, 3))
     p[1] = p[0] + np.array([4.0, 0, 0])

@@ -115,6 +115,9 @@ def test_neighbour_lists_bit_exact(emu_lib):
         ref = sorted_pairs(reference_pairs(pos, box, rc, s.bonded))
         got = sorted_pairs(r[f"water_{rc}"])
         assert ref.shape == got.shape and np.array_equal(ref, got), rc
+    ref = sorted_pairs(reference_pairs(pos, box, 6.0, s.bonded))
+    got = sorted_pairs(r["water_6.0_cap8"])
+    assert ref.shape == got.shape and np.array_equal(ref, got)
     rng = np.random.default_rng(5)
     p = rng.uniform(0, 11.0, size=(300, 3))
     p[1] = p[0] + np.array([4.0, 0, 0])

@@ -192,6 +192,15 @@ def test_neighbor_list_bit_exact():
         b = got[np.lexsort((got[:, 1], got[:, 0]))]
         print(f"rc={rc}: oracle {len(a)} pairs, gpu {len(b)} pairs", flush=True)
         assert a.shape == b.shape and np.array_equal(a, b)
+    # a row capacity of 8: the first build overflows and is rebuilt before the lists are read
+    ref = reference_pairs(pos, box, 6.0, s.bonded)
+    e = Engine(len(pos), 0, False, False, 0.0, (0, 0, 0), 6.0, max_neighbors=8, device=DEV)
+    e.set_topology(None, None, s.bonded)
+    e.build_neighbors(pos, box)
+    got = e.get_pairs(0).cpu().numpy().astype(np.int64)
+    a = ref[np.lexsort((ref[:, 1], ref[:, 0]))]
+    b = got[np.lexsort((got[:, 1], got[:, 0]))]
+    assert a.shape == b.shape and np.array_equal(a, b)
     # random points with pairs at the cutoff to the last bit, small box (nc < 3)
     rng = np.random.default_rng(5)
     L = 11.0
