@@ -279,11 +279,13 @@ static bool plane_kernels_ready() {
             return false;
         }
         const int m = (int)PLANE_LDS_MAX;
-        const hipError_t e[4] = {
+        const hipError_t e[6] = {
             hipFuncSetAttribute((const void*)k_fxy_plane<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, m),
             hipFuncSetAttribute((const void*)k_fxy_plane<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, m),
             hipFuncSetAttribute((const void*)k_fxy_plane<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, m),
-            hipFuncSetAttribute((const void*)k_fxy_plane<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, m)};
+            hipFuncSetAttribute((const void*)k_fxy_plane<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, m),
+            hipFuncSetAttribute((const void*)k_plane_matvec<double>, hipFuncAttributeMaxDynamicSharedMemorySize, m),
+            hipFuncSetAttribute((const void*)k_plane_matvec<float>, hipFuncAttributeMaxDynamicSharedMemorySize, m)};
         for (hipError_t x : e)
             if (x != hipSuccess) {
                 (void)hipGetLastError();
@@ -922,6 +924,13 @@ template <class R> static const Cplx<R>* twid(const Grid& g, int d) {
     return reinterpret_cast<const Cplx<R>*>(sizeof(R) == 8 ? g.tw64[d] : g.tw[d]);
 }
 
+// The real-space product v -> out (FP64 stored coefficients) launched with the
+// plane pass of a solver product (k_plane_matvec)
+struct MatvecJob {
+    const double* v;
+    double* out;
+};
+
 // Spread the sources (Qc, U, qmode), transform, apply the influence function,
 // transform back and gather.  gather_mode 1: dE/dmu only into out3; 0: full
 // gradients into gq, gpos, vir (dE/dA per site).  Energy/box sums land in d_rec.
@@ -930,7 +939,7 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
                              int lmax_eff, int kind, double pref, double kappa, int gather_mode, double* out3,
                              double* gq, double* gpos, double* vir, bool want_energy, void* wf_in = nullptr,
                              void* fq_in = nullptr, const PcgState* skip = nullptr, const CgEpi* epi = nullptr,
-                             hipEvent_t before_gather = nullptr) {
+                             hipEvent_t before_gather = nullptr, const MatvecJob* mv = nullptr) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     if (ctx->prepped != &g) {
@@ -1009,6 +1018,10 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     const int nly = K0 * H, nlx = K1 * H;
     const size_t Cs = sizeof(Cplx<R>);
     const int pm = g.plane[sizeof(R) == 8 ? 1 : 0] ? 1 : 0;
+    if (mv && (!pm || want_energy)) {
+        ctx->err = "real-space product with the plane pass needs a solver product on the plane path";
+        return ADMP_ERR_STATE;
+    }
     // z passes: 8 complex lines (16 real rows) per block where LDS allows, else 4
     const bool z8 = (size_t)(2 * 8 + 1) * K2 * Cs <= 48 * 1024;
     auto zpass = [&](int inverse) {
@@ -1043,7 +1056,14 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
             hipLaunchKernelGGL((k_fxy_plane<R, true>), dim3(H, F), dim3(512), sm, st, K0, K1, K2, g.plan[0], g.plan[1],
                                twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa,
                                kind, pref, ctx->d_part, (const double*)nullptr, skip);
-        else
+        else if (mv) {
+            const int npb = xcd_blocks(H * F);
+            const int nbf = (int)nblk((size_t)N * TPA, 512);
+            hipLaunchKernelGGL((k_plane_matvec<R>), dim3(npb + xcd_blocks(nbf * F)), dim3(512), sm, st, K0, K1, K2,
+                               g.plan[0], g.plan[1], twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1],
+                               g.bmod[2], kappa, kind, pref, tab, F, npb, N, nbf, ctx->cap, ctx->d_pos, ctx->d_nbr,
+                               ctx->d_ncnt, reinterpret_cast<const double*>(ctx->d_pcoef), mv->v, mv->out, skip);
+        } else
             hipLaunchKernelGGL((k_fxy_plane<R, false>), dim3(H, F), dim3(512), sm, st, K0, K1, K2, g.plan[0], g.plan[1],
                                twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa,
                                kind, pref, ctx->d_part, tab, skip);
@@ -1158,18 +1178,31 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
     // near-field coupling sets the conditioning); R only for the reciprocal chain
     using RR = typename std::conditional<sizeof(RC) == 8, double, R>::type;
     const RR* vR = sizeof(RR) == 8 ? reinterpret_cast<const RR*>(ctx->d_z) : reinterpret_cast<const RR*>(ctx->d_vR);
-    // the real-space product runs on the side stream, beside the reciprocal chain
-    HIPCHK(hipEventRecord(ctx->ev_fork, st));
-    HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-    { ProfScope ps_(ctx, 2, ctx->side);
-    const int nbf = (int)nblk((size_t)N * TPA, PAIR_BLOCK);
-    hipLaunchKernelGGL((k_real_matvec<RC, RR>), dim3(xcd_blocks(nbf * F)), dim3(PAIR_BLOCK), 0, ctx->side, N, nbf, F,
-                       ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
-                       reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1, ctx->d_pcg); }
-    HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
     const bool fused = ctx->cfg.lpme && !small_grid(ctx->eg);
+    // With few frames the plane pass fills only K2/2+1 workgroups per frame: the
+    // real-space product then runs inside that launch (k_plane_matvec) instead of
+    // on the side stream, which saves the fork and the join of every iteration.
+    const int H = ctx->eg.K[2] / 2 + 1;
+    const bool with_plane = fused && sizeof(RC) == 8 && ctx->eg.plane[sizeof(R) == 8 ? 1 : 0] && H * F <= 128;
+    if (!with_plane) {
+        // the real-space product runs on the side stream, beside the reciprocal chain
+        HIPCHK(hipEventRecord(ctx->ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        { ProfScope ps_(ctx, 2, ctx->side);
+        const int nbf = (int)nblk((size_t)N * TPA, PAIR_BLOCK);
+        hipLaunchKernelGGL((k_real_matvec<RC, RR>), dim3(xcd_blocks(nbf * F)), dim3(PAIR_BLOCK), 0, ctx->side, N, nbf,
+                           F, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_nbr, ctx->d_ncnt,
+                           reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1, ctx->d_pcg); }
+        HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
+    }
     admp_status s = ADMP_OK;
-    if (fused) {
+    if (with_plane) {
+        const CgEpi epi{ctx->d_t1, pol, f1, cg_parts(ctx)};
+        const MatvecJob mv{ctx->d_z, ctx->d_t1};
+        s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_z, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 2, ctx->d_Ap,
+                         nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg, &epi, nullptr, &mv);
+        ctx->cg_nwu = (N + 31) / 32;
+    } else if (fused) {
         const CgEpi epi{ctx->d_t1, pol, f1, cg_parts(ctx)};
         s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_z, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 2, ctx->d_Ap,
                          nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg, &epi, ctx->ev_join);
@@ -1179,7 +1212,7 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
                          nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg);
     }
     // join before any return so that a capture never ends with an open fork
-    HIPCHK(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    if (!with_plane) HIPCHK(hipStreamWaitEvent(st, ctx->ev_join, 0));
     if (s != ADMP_OK) return s;
     if (!fused) {
         ProfScope ps_(ctx, 11);

@@ -540,16 +540,17 @@ k_pc_pairs(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __res
 // One-dimensional grid over all (frame, atom block) items, renumbered so that
 // each XCD works through a contiguous range of frames and atoms (its L2 then
 // holds the positions and v of few frames).  nbf: atom blocks per frame.
-template <class RC, class R>
-__global__ void __launch_bounds__(PAIR_BLOCK)
-k_real_matvec(int N, int nbf, int F, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
-              const int* __restrict__ nbr, const int* __restrict__ ncnt, const RC* __restrict__ pcoef,
-              const R* __restrict__ v, double* __restrict__ out, const PcgState* __restrict__ skip) {
-    const int item = xcd_item(blockIdx.x, nbf * F);
-    if (item >= nbf * F) return;
+// item: (frame, atom block of BS threads) pair, numbered frame-major; nbf atom
+// blocks per frame
+template <class RC, class R, int BS>
+__device__ __forceinline__ void real_matvec_body(int item, int N, int nbf, int cap, const double4* __restrict__ pos,
+                                                 const BoxInfo* __restrict__ boxes, const int* __restrict__ nbr,
+                                                 const int* __restrict__ ncnt, const RC* __restrict__ pcoef,
+                                                 const R* __restrict__ v, double* __restrict__ out,
+                                                 const PcgState* __restrict__ skip) {
     const int f = item / nbf, bxi = item - f * nbf;
     if (skip && skip[f].done) return;  // frame already converged
-    const int gt = bxi * PAIR_BLOCK + threadIdx.x;
+    const int gt = bxi * BS + threadIdx.x;
     const int i = gt / TPA, sub = gt % TPA;
     const bool active = i < N;
     const int ii = active ? i : 0;
@@ -592,6 +593,16 @@ k_real_matvec(int N, int nbf, int F, int cap, const double4* __restrict__ pos, c
         const size_t a = fo + i;
         out[a * 3 + 0] = ox; out[a * 3 + 1] = oy; out[a * 3 + 2] = oz;
     }
+}
+
+template <class RC, class R>
+__global__ void __launch_bounds__(PAIR_BLOCK)
+k_real_matvec(int N, int nbf, int F, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
+              const int* __restrict__ nbr, const int* __restrict__ ncnt, const RC* __restrict__ pcoef,
+              const R* __restrict__ v, double* __restrict__ out, const PcgState* __restrict__ skip) {
+    const int item = xcd_item(blockIdx.x, nbf * F);
+    if (item >= nbf * F) return;
+    real_matvec_body<RC, R, PAIR_BLOCK>(item, N, nbf, cap, pos, boxes, nbr, ncnt, pcoef, v, out, skip);
 }
 
 }  // namespace admp

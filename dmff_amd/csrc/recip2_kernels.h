@@ -16,6 +16,7 @@
 //   gather_ml     eight lanes per site (one per x-offset), shuffled reduction
 #pragma once
 #include "admp_common.h"
+#include "pair_kernels.h"
 #include "recip_kernels.h"
 
 namespace admp {
@@ -675,18 +676,17 @@ __device__ inline InflPt infl_point(const BoxInfo& b, int K0, int K1, int K2, in
 // ACC: energy and box-derivative partials per block (multiplicity 2 for
 // 0 < kz < K2/2, the conjugate half).  tab (solver products): tabulated
 // 2 pref C_k / theta_k^2 in the same plane-major order.
+// plane i2 of frame f (one workgroup); nplane: planes per frame (energy partials)
 template <class R, bool ACC>
-__global__ void __launch_bounds__(512) k_fxy_plane(int K0, int K1, int K2, FFTPlan plan0, FFTPlan plan1,
-                                                    const Cplx<R>* __restrict__ tw0g, const Cplx<R>* __restrict__ tw1g,
-                                                    Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
-                                                    const double* __restrict__ bmod0, const double* __restrict__ bmod1,
-                                                    const double* __restrict__ bmod2, double kappa, int kind,
-                                                    double pref, double* __restrict__ part,
-                                                    const double* __restrict__ tab, const PcgState* __restrict__ skip) {
-    if (skip && skip[blockIdx.y].done) return;  // frame already converged
+__device__ __forceinline__ void fxy_plane_body(int f, int i2, int nplane, int K0, int K1, int K2, FFTPlan plan0,
+                                               FFTPlan plan1, const Cplx<R>* __restrict__ tw0g,
+                                               const Cplx<R>* __restrict__ tw1g, Cplx<R>* __restrict__ hgrid,
+                                               const BoxInfo* __restrict__ boxes, const double* __restrict__ bmod0,
+                                               const double* __restrict__ bmod1, const double* __restrict__ bmod2,
+                                               double kappa, int kind, double pref, double* __restrict__ part,
+                                               const double* __restrict__ tab) {
     extern __shared__ unsigned char smem_raw[];
     __shared__ double sh[16];
-    const int f = blockIdx.y, i2 = blockIdx.x;
     const int H = K2 / 2 + 1, P = K0 * K1;
     Cplx<R>* b0 = reinterpret_cast<Cplx<R>*>(smem_raw);
     Cplx<R>* b1 = b0 + P;
@@ -735,8 +735,21 @@ __global__ void __launch_bounds__(512) k_fxy_plane(int K0, int K1, int K2, FFTPl
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
         double tsum = block_sum(vals[k], sh);
-        if (threadIdx.x == 0) part[((size_t)f * gridDim.x + blockIdx.x) * 7 + k] = tsum;
+        if (threadIdx.x == 0) part[((size_t)f * nplane + i2) * 7 + k] = tsum;
     }
+}
+
+template <class R, bool ACC>
+__global__ void __launch_bounds__(512) k_fxy_plane(int K0, int K1, int K2, FFTPlan plan0, FFTPlan plan1,
+                                                    const Cplx<R>* __restrict__ tw0g, const Cplx<R>* __restrict__ tw1g,
+                                                    Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
+                                                    const double* __restrict__ bmod0, const double* __restrict__ bmod1,
+                                                    const double* __restrict__ bmod2, double kappa, int kind,
+                                                    double pref, double* __restrict__ part,
+                                                    const double* __restrict__ tab, const PcgState* __restrict__ skip) {
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
+    fxy_plane_body<R, ACC>(blockIdx.y, blockIdx.x, gridDim.x, K0, K1, K2, plan0, plan1, tw0g, tw1g, hgrid, boxes,
+                           bmod0, bmod1, bmod2, kappa, kind, pref, part, tab);
 }
 
 // complex lines along y of the half spectrum: lines (x, kz), stride H
@@ -1099,6 +1112,39 @@ k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, cons
                 v[j * 3 + cc] = g[j] * r[cc] + mu[j] * p1[cc] + (2.0 / 3.0) * TAp;
             }
     }
+}
+
+// ---------------------------------------------------------------------------
+// The plane pass of a solver product and the real-space product of the same z
+// in one launch.  The plane pass runs one workgroup per kz plane (26 for a 50^3
+// grid), far too few to fill the GPU, and the real-space product needs only z:
+// its workgroups fill the idle CUs, and no second stream (fork and join) is
+// needed.  Blocks [0, nplane_blocks) are planes (plane-major over frames;
+// blocks past H F return), the rest are real-space items of 512 threads (32
+// sites) in XCD order.  Every block is given the plane pass's dynamic LDS.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void __launch_bounds__(512)
+k_plane_matvec(int K0, int K1, int K2, FFTPlan plan0, FFTPlan plan1, const Cplx<R>* __restrict__ tw0g,
+               const Cplx<R>* __restrict__ tw1g, Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
+               const double* __restrict__ bmod0, const double* __restrict__ bmod1, const double* __restrict__ bmod2,
+               double kappa, int kind, double pref, const double* __restrict__ tab, int F, int nplane_blocks,
+               int N, int nbf, int cap, const double4* __restrict__ pos, const int* __restrict__ nbr,
+               const int* __restrict__ ncnt, const double* __restrict__ pcoef, const double* __restrict__ v,
+               double* __restrict__ out, const PcgState* __restrict__ skip) {
+    const int H = K2 / 2 + 1;
+    const int b = blockIdx.x;
+    if (b < nplane_blocks) {
+        if (b >= H * F) return;
+        const int f = b / H, i2 = b - f * H;
+        if (skip && skip[f].done) return;  // frame already converged
+        fxy_plane_body<R, false>(f, i2, H, K0, K1, K2, plan0, plan1, tw0g, tw1g, hgrid, boxes, bmod0, bmod1, bmod2,
+                                 kappa, kind, pref, nullptr, tab);
+        return;
+    }
+    const int item = xcd_item(b - nplane_blocks, nbf * F);
+    if (item >= nbf * F) return;
+    real_matvec_body<double, double, 512>(item, N, nbf, cap, pos, boxes, nbr, ncnt, pcoef, v, out, skip);
 }
 
 }  // namespace admp
