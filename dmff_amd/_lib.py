@@ -20,6 +20,7 @@ SYMBOLS = [
     "admp_version", "admp_ctx_create", "admp_ctx_destroy", "admp_last_error", "admp_set_stream",
     "admp_set_topology", "admp_set_pairs", "admp_build_neighbors", "admp_get_pairs", "admp_compute",
     "admp_disp_compute", "admp_slater_compute", "admp_profile", "admp_profile_get", "admp_memory_bytes",
+    "admp_type_reduce",
 ]
 
 
@@ -80,6 +81,7 @@ def load():
     lib.admp_profile.argtypes = [P, i32, i32]
     lib.admp_profile_get.argtypes = [P, i32, ctypes.POINTER(d), ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_char_p)]
     lib.admp_memory_bytes.argtypes = [P, ctypes.POINTER(i64)]
+    lib.admp_type_reduce.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int32, P, P]
     for name in SYMBOLS:
         getattr(lib, name).restype = ctypes.c_int if name not in ("admp_version", "admp_last_error") else \
             getattr(lib, name).restype

@@ -7,11 +7,13 @@ Energies are torch scalars that carry gradients with respect to positions,
 box, Q_local, pol and tholes through hand-written backward kernels (the
 induced dipoles are held fixed, as the reference's stop_gradient does).
 """
+import ctypes
 import math
 
 import numpy as np
 import torch
 
+from .. import _lib as L
 from .engine import Engine, _dev
 
 F64 = torch.float64
@@ -50,23 +52,35 @@ class TypeReducer:
     """Sums per-atom gradients into per-type gradients, the adjoint of the
     ``Q_local[map_atomtype]`` expansion (dmff/generators/admp.py:1372-1375).
 
-    The atoms of each type are listed once; a reduction is one gather and one
-    sum per type along the atom axis (deterministic; per-type targets are few,
-    so scatter-adds would serialise on them and a (types x atoms) GEMM maps
-    badly onto library kernels).
-    """
+    One library launch (``admp_type_reduce``): a workgroup per (type, component)
+    sums its atoms in a fixed order, so results are deterministic.  Inputs are
+    gradients (no autograd through the reduction)."""
 
     def __init__(self, type_map, n_types=None, device=None):
         m = np.asarray(type_map, dtype=np.int64)
         self.n_types = int(m.max()) + 1 if n_types is None else int(n_types)
-        self.index = [torch.as_tensor(np.nonzero(m == t)[0], dtype=torch.int64, device=device)
-                      for t in range(self.n_types)]
+        self.map = torch.as_tensor(m.astype(np.int32), device=device)
+        self.lib = L.load()
 
     def __call__(self, per_atom, axis=0):
         """per_atom: (..., N, ...) with atoms on `axis`; returns the same with types on `axis`."""
-        parts = [per_atom.index_select(axis, ix).sum(axis, keepdim=True) if len(ix) else
-                 torch.zeros_like(per_atom.narrow(axis, 0, 1)) for ix in self.index]
-        return torch.cat(parts, dim=axis)
+        x = per_atom.detach().to(torch.float64)
+        shape = list(x.shape)
+        axis = axis % len(shape)
+        n = shape[axis]
+        if n != self.map.numel():
+            raise ValueError(f"{n} atoms on axis {axis}, type map has {self.map.numel()}")
+        pre = int(np.prod(shape[:axis])) if axis else 1
+        d = int(np.prod(shape[axis + 1:])) if axis + 1 < len(shape) else 1
+        x = x.contiguous()
+        out = torch.empty([pre, self.n_types, d], dtype=torch.float64, device=x.device)
+        stream = torch.cuda.current_stream(x.device).cuda_stream if x.is_cuda else 0
+        st = self.lib.admp_type_reduce(ctypes.c_void_p(x.data_ptr()), pre, n, d, ctypes.c_void_p(self.map.data_ptr()),
+                                       self.n_types, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream))
+        if st != L.ADMP_OK:
+            raise L.ADMPError(f"admp_type_reduce failed with status {st}")
+        shape[axis] = self.n_types
+        return out.reshape(shape)
 
 
 class _ADMPEnergy(torch.autograd.Function):

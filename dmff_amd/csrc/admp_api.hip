@@ -1281,7 +1281,8 @@ static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x
 // direction and continues with FP64 products.  x holds the initial guess.
 template <class RC, class R>
 static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double* pol, double f1, double tol,
-                       int maxiter, std::vector<int>& it_out, std::vector<double>& res_out, bool fp32 = false) {
+                       int maxiter, std::vector<int>& it_out, std::vector<double>& res_out, bool fp32 = false,
+                       bool cold = false) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     // optional first phase with FP32 reciprocal products (fp32 on a mixed context)
@@ -1367,7 +1368,11 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
     };
     double worst = 0;
     bool all = false;
-    if ((s = read_state(worst, all)) != ADMP_OK) return s;
+    // From x = 0 the start state cannot be converged: with FP64 products the first
+    // chunk is queued at once, without waiting for that state (its residual is
+    // only needed to steer the FP32 phase).
+    const bool read_start = !(dbl && cold && maxiter > 0);
+    if (read_start && (s = read_state(worst, all)) != ADMP_OK) return s;
     const double r0 = worst;
     double last_true = worst;
     for (;;) {
@@ -1423,7 +1428,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         // extrapolated geometrically from its last two states.  A wrong
         // prediction costs one host round trip before the next chunk starts.
         std::vector<PcgState> older = hs;
-        bool have_older = false;
+        bool have_older = false, have_hs = read_start;
         auto finishes_in_flight = [&]() -> bool {
             if (!have_older) return false;
             for (int f = 0; f < F; ++f) {
@@ -1445,7 +1450,8 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
             --nq;
             HIPCHK(hipEventSynchronize(ctx->ev_state[sl]));
             older = hs;
-            have_older = true;
+            have_older = have_hs;
+            have_hs = true;
             all = true;
             for (int f = 0; f < F; ++f) {
                 hs[f] = ctx->h_pcg[(size_t)sl * F + f];
@@ -1512,7 +1518,8 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     const int maxiter = o ? (kind == ADMP_SOLVER_STEPS ? o->steps : o->maxiter) : 500;
     const bool mixed = sizeof(R) == 4;
     ctx->pc_pairs = !(o && o->precond == ADMP_PRECOND_DIAGONAL);
-    if (o && o->warm_start && U_in)
+    const bool warm = o && o->warm_start && U_in;
+    if (warm)
         HIPCHK(hipMemcpyAsync(U, U_in, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     else
         HIPCHK(hipMemsetAsync(U, 0, NF * 3 * sizeof(double), st));
@@ -1529,14 +1536,15 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     if (kind == ADMP_SOLVER_PCG && !mixed) {
         std::vector<int> it;
         std::vector<double> rr;
-        if ((s = pcg<double, double>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr)) != ADMP_OK) return s;
+        if ((s = pcg<double, double>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr, false, !warm)) != ADMP_OK)
+            return s;
         for (int f = 0; f < F; ++f) info.n_iter[f] = it[f];
     } else if (kind == ADMP_SOLVER_PCG) {
         // FP64 products by default; with fp32_products the first phase runs the
         // reciprocal products in FP32 with FP64 true residuals (reliable updates)
         std::vector<int> it;
         std::vector<double> rr;
-        if ((s = pcg<double, R>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr, o && o->fp32_products)) !=
+        if ((s = pcg<double, R>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr, o && o->fp32_products, !warm)) !=
             ADMP_OK)
             return s;
         for (int f = 0; f < F; ++f) info.n_iter[f] = it[f];
@@ -1922,6 +1930,16 @@ extern "C" admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, con
                                            const double* b, const double* mScales, int32_t n_scales, int32_t want_grad,
                                            double* energy, double* dpos, double* dbox, double* da, double* db) {
     ADMP_WRAP(admp_slater_compute_impl(ctx, pos, box, a, b, mScales, n_scales, want_grad, energy, dpos, dbox, da, db));
+}
+
+extern "C" admp_status admp_type_reduce(const double* per_site, int32_t n_frames, int32_t n_sites, int32_t d,
+                                        const int32_t* type_map, int32_t n_types, double* out, void* stream) {
+    if (n_frames < 1 || n_sites < 0 || d < 1 || n_types < 1 || !type_map || !out || (n_sites > 0 && !per_site))
+        return ADMP_ERR_ARG;
+    if ((long)n_types * d > 65535 * 64 || n_frames > 65535) return ADMP_ERR_ARG;
+    hipLaunchKernelGGL(k_type_reduce, dim3(n_types * d, n_frames), dim3(256), 0, (hipStream_t)stream, per_site,
+                       n_sites, d, type_map, out, n_types);
+    return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }
 
 extern "C" admp_status admp_memory_bytes(const admp_ctx* ctx, int64_t* bytes) {

@@ -860,4 +860,18 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
         st[f].iter += 1;
     }
 }
+
+// per-type sums of a per-site array: block (t * d + c, f); sites in a fixed
+// strided order per thread, then a fixed-order block reduction
+__global__ void __launch_bounds__(256) k_type_reduce(const double* __restrict__ in, int N, int d,
+                                                     const int* __restrict__ tmap, double* __restrict__ out,
+                                                     int n_types) {
+    __shared__ double sh[16];
+    const int f = blockIdx.y, t = blockIdx.x / d, c = blockIdx.x - t * d;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x)
+        if (tmap[i] == t) acc += in[((size_t)f * N + i) * d + c];
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[((size_t)f * n_types + t) * d + c] = acc;
+}
 }  // namespace admp

@@ -179,6 +179,16 @@ admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* 
 admp_status admp_profile(admp_ctx* ctx, int32_t enable, int32_t reset);
 admp_status admp_profile_get(admp_ctx* ctx, int32_t key, double* total_ms, int64_t* count, const char** name);
 
+/* Per-type parameter gradients: out[f][t][c] = sum over sites i with
+ * type_map[i] == t of per_site[f][i][c], summed in a fixed order (results do not
+ * depend on scheduling).  The adjoint of the per-type -> per-site expansion
+ * Q_local[map_atomtype] (dmff/generators/admp.py:1372-1375), which jax.grad
+ * forms in the reference.  per_site (F,N,d), type_map (N,), out (F,n_types,d):
+ * device pointers; stream: a hipStream_t (NULL = the legacy default stream).
+ * Needs no context. */
+admp_status admp_type_reduce(const double* per_site, int32_t n_frames, int32_t n_sites, int32_t d,
+                             const int32_t* type_map, int32_t n_types, double* out, void* stream);
+
 /* Instrumentation (no counterpart in the reference): device bytes currently held
  * by the context (grids, lists, scratch).  Serving uses it to admit requests by
  * HBM footprint.  Host-only; no GPU work. */

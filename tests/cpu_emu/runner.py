@@ -208,7 +208,19 @@ def precond(precision, n_waters):
     return out
 
 
-CASES = {"small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
+def type_reduce(precision):
+    """Per-type sums of per-site arrays through the library (atoms on axis 0 and 1)."""
+    from dmff_amd.admp.pme import TypeReducer
+    rng = np.random.default_rng(11)
+    tmap = rng.integers(0, 3, size=1000)
+    a = rng.normal(size=(1000, 9))
+    b = rng.normal(size=(2, 1000))
+    tr = TypeReducer(tmap)
+    return {"map": tmap.tolist(), "a": a.tolist(), "b": b.tolist(), "ra": _lst(tr(torch.as_tensor(a))),
+            "rb": _lst(tr(torch.as_tensor(b), axis=1))}
+
+
+CASES = {"type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
 
 if __name__ == "__main__":
     lib, case = sys.argv[1], sys.argv[2]

@@ -166,3 +166,12 @@ def test_pair_preconditioner_same_solution_fewer_iterations(emu_lib):
     assert abs(p["E"] - d["E"]) < 1e-12 * abs(d["E"])
     assert rel_rms(p["pos"], d["pos"]) < 1e-8
     assert np.abs(np.asarray(p["U"]) - np.asarray(d["U"])).max() < 1e-8
+
+
+def test_type_reduce(emu_lib):
+    r = run(emu_lib, "type_reduce", "fp64")
+    m, a, b = np.asarray(r["map"]), np.asarray(r["a"]), np.asarray(r["b"])
+    ra = np.stack([a[m == t].sum(0) for t in range(3)])
+    rb = np.stack([b[:, m == t].sum(1) for t in range(3)], axis=1)
+    assert np.allclose(r["ra"], ra, rtol=1e-12, atol=1e-12)
+    assert np.allclose(r["rb"], rb, rtol=1e-12, atol=1e-12)

@@ -496,3 +496,16 @@ def test_backward_central_differences_12288_atoms():
         fd = (vals[0] - vals[1]) / (2 * h)
         an = float(gQ[t, k]) if kind == "Q" else float((gP if kind == "pol" else gT)[t])
         check(an, fd, (kind, t, k))
+
+
+def test_type_reduce_deterministic():
+    from dmff_amd.admp.pme import TypeReducer
+    rng = np.random.default_rng(12)
+    tmap = rng.integers(0, 4, size=12288)
+    a = torch.as_tensor(rng.normal(size=(12288, 9)), device=DEV)
+    tr = TypeReducer(tmap, device=DEV)
+    r1, r2 = tr(a), tr(a)
+    assert torch.equal(r1, r2)
+    an = a.cpu().numpy()
+    ref = np.stack([an[tmap == t].sum(0) for t in range(4)])
+    assert np.allclose(r1.cpu().numpy(), ref, rtol=1e-12, atol=1e-12)
