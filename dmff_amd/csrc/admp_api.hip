@@ -1000,7 +1000,12 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     { ProfScope ps_(ctx, 4);
     // one brick per wavefront while that fills the GPU (latency); four bricks per
     // wavefront with column accumulation for large launches (instruction count)
-    const bool col = (long)nbricks * F >= 16384;
+    // ADMP_SPREAD=list|col forces one variant (measurements)
+    static const int spread_force = [] {
+        const char* e = std::getenv("ADMP_SPREAD");
+        return !e ? 0 : (std::strcmp(e, "col") == 0 ? 2 : (std::strcmp(e, "list") == 0 ? 1 : 0));
+    }();
+    const bool col = spread_force ? spread_force == 2 : (long)nbricks * F >= 16384;
     const unsigned nb1 = xcd_blocks(nbricks), nb4 = xcd_blocks((nbricks + 3) / 4);
 #define ADMP_SPREAD(LMv)                                                                                              \
     if (col)                                                                                                         \
@@ -1937,7 +1942,7 @@ extern "C" admp_status admp_type_reduce(const double* per_site, int32_t n_frames
     if (n_frames < 1 || n_sites < 0 || d < 1 || n_types < 1 || !type_map || !out || (n_sites > 0 && !per_site))
         return ADMP_ERR_ARG;
     if ((long)n_types * d > 65535 * 64 || n_frames > 65535) return ADMP_ERR_ARG;
-    hipLaunchKernelGGL(k_type_reduce, dim3(n_types * d, n_frames), dim3(256), 0, (hipStream_t)stream, per_site,
+    hipLaunchKernelGGL(k_type_reduce, dim3(n_types * d, n_frames), dim3(TR_BLOCK), 0, (hipStream_t)stream, per_site,
                        n_sites, d, type_map, out, n_types);
     return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
 }

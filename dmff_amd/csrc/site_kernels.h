@@ -861,17 +861,25 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
     }
 }
 
-// per-type sums of a per-site array: block (t * d + c, f); sites in a fixed
-// strided order per thread, then a fixed-order block reduction
-__global__ void __launch_bounds__(256) k_type_reduce(const double* __restrict__ in, int N, int d,
-                                                     const int* __restrict__ tmap, double* __restrict__ out,
-                                                     int n_types) {
+// per-type sums of a per-site array: block (t * d + c, f) of 1024 threads; each
+// thread keeps four accumulators over four strided site sequences (four loads
+// in flight), then a fixed-order block reduction: the summation order is fixed
+constexpr int TR_BLOCK = 1024;
+__global__ void __launch_bounds__(TR_BLOCK) k_type_reduce(const double* __restrict__ in, int N, int d,
+                                                          const int* __restrict__ tmap, double* __restrict__ out,
+                                                          int n_types) {
     __shared__ double sh[16];
     const int f = blockIdx.y, t = blockIdx.x / d, c = blockIdx.x - t * d;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < N; i += blockDim.x)
-        if (tmap[i] == t) acc += in[((size_t)f * N + i) * d + c];
-    acc = block_sum(acc, sh);
+    const double* src = in + (size_t)f * N * d + c;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i0 = threadIdx.x; i0 < N; i0 += 4 * TR_BLOCK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * TR_BLOCK;
+            if (i < N && tmap[i] == t) a[u] += src[(size_t)i * d];
+        }
+    }
+    const double acc = block_sum((a[0] + a[1]) + (a[2] + a[3]), sh);
     if (threadIdx.x == 0) out[((size_t)f * n_types + t) * d + c] = acc;
 }
 }  // namespace admp
