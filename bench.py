@@ -447,7 +447,7 @@ def roofline(name, tm, D, prec, frames=1, prof=None):
     if prof and prof.get("influence", (0, 0))[1]:
         plane = prof.get("fft", (0, 0))[1] < 3 * prof["influence"][1]
     table = {
-        "spread": ("k_spread_list", N * (3 * 8 + 4 * w) + G * w),
+        "spread": ("k_spread_cw", N * (3 * 8 + 4 * w) + G * w),
         "fft": ("k_fz_r2c/k_fz_c2r", G * w + 2 * Gh * w) if plane else
                ("k_fz_r2c/k_fy/k_fz_c2r", (2 * G * w + 12 * Gh * w) / 4),
         "influence": ("k_fxy_plane" if plane else "k_fx_infl", 4 * Gh * w + 8 * Gh),

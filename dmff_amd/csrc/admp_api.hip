@@ -998,18 +998,27 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     const int H = K2 / 2 + 1;
     const int nbricks = ((K0 + BRX - 1) / BRX) * ((K1 + BRY - 1) / BRY) * ((K2 + BRZ - 1) / BRZ);
     { ProfScope ps_(ctx, 4);
-    // one brick per wavefront while that fills the GPU (latency); four bricks per
-    // wavefront with column accumulation for large launches (instruction count)
-    // ADMP_SPREAD=list|col forces one variant (measurements)
+    // ADMP_SPREAD=list|col|cw forces one variant (measurements)
     static const int spread_force = [] {
         const char* e = std::getenv("ADMP_SPREAD");
-        return !e ? 0 : (std::strcmp(e, "col") == 0 ? 2 : (std::strcmp(e, "list") == 0 ? 1 : 0));
+        if (!e) return 0;
+        if (std::strcmp(e, "list") == 0) return 1;
+        if (std::strcmp(e, "col") == 0) return 2;
+        if (std::strcmp(e, "cw") == 0) return 3;
+        return 0;
     }();
-    const bool col = spread_force ? spread_force == 2 : (long)nbricks * F >= 16384;
+    // default: one brick per wavefront with column accumulation (k_spread_cw); at
+    // 12,288 sites it takes 19 us per solver spread where one grid point per lane
+    // (list) takes 27 and four bricks per wavefront (col) 45, and it is also the
+    // fastest for the 64-frame batch
+    const int variant = spread_force ? spread_force : 3;
     const unsigned nb1 = xcd_blocks(nbricks), nb4 = xcd_blocks((nbricks + 3) / 4);
 #define ADMP_SPREAD(LMv)                                                                                              \
-    if (col)                                                                                                         \
+    if (variant == 2)                                                                                                \
         hipLaunchKernelGGL((k_spread_col<R, LMv>), dim3(nb4, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,           \
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);                                   \
+    else if (variant == 3)                                                                                           \
+        hipLaunchKernelGGL((k_spread_cw<R, LMv>), dim3(nb1, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,            \
                            ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);                                   \
     else                                                                                                             \
         hipLaunchKernelGGL((k_spread_list<R, LMv>), dim3(nb1, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,          \

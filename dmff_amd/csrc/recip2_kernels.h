@@ -365,6 +365,108 @@ k_spread_col(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R
 }
 
 // ---------------------------------------------------------------------------
+// Column spread with one brick per wavefront: lane = (quarter g, column c); the
+// 16 columns of the brick each hold 4 z points, and the four lanes of a column
+// take every fourth candidate of each staged round (g, g+4, ...), forming the
+// x-y factors of a candidate once per column as in k_spread_col.  The four
+// quarter sums of a point are combined by a fixed shuffle tree, so the result is
+// bitwise reproducible.  Per candidate a lane reads 4 x-y and up to 12 z table
+// values for 4 points, where the one-point-per-lane spread reads 9 for one.
+// ---------------------------------------------------------------------------
+template <class R, int LM>
+__global__ void __launch_bounds__(64)
+k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R* __restrict__ fq,
+            const int* __restrict__ bstart, const int2* __restrict__ blist, long cap, R* __restrict__ rgrid,
+            const PcgState* __restrict__ skip) {
+    static_assert(BRX == 4 && BRY == 4 && BRZ == 4, "column spread assumes 4x4x4 bricks");
+    if (skip && skip[blockIdx.y].done) return;  // frame already converged
+    constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
+    constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
+    __shared__ int s_r[SP_CHUNK];
+    __shared__ R s_t[SP_CHUNK][3][NT][6];
+    __shared__ R s_q[SP_CHUNK][NQ];
+    const int f = blockIdx.y;
+    const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY, nbx = (K0 + BRX - 1) / BRX;
+    const int nbr = nbx * nby * nbz;
+    const int b = xcd_item(blockIdx.x, nbr);
+    if (b >= nbr) return;
+    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+    const int t = threadIdx.x;
+    const int g = t >> 4, c = t & 15;
+    const int lx = c >> 2, ly = c & 3;
+    const int px = bx * BRX + lx, py = by * BRY + ly, pz0 = bz * BRZ;
+    const int lo = bstart[(size_t)f * (nbr + 1) + b];
+    const int total = (int)min((long)bstart[(size_t)f * (nbr + 1) + b + 1], cap) - lo;
+    R acc[4] = {0, 0, 0, 0};
+    for (int c0 = 0; c0 < total; c0 += SP_CHUNK) {
+        const int nchunk = min(SP_CHUNK, total - c0);
+        if (t < nchunk) {
+            const int2 e = blist[(size_t)f * cap + lo + c0 + t];
+            const size_t a = (size_t)f * N + e.x;
+            s_r[t] = e.y;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const R w = wfrac[a * 3 + d];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    R v, d1, d2;
+                    bs_piece<R>(k, w, v, d1, d2);
+                    s_t[t][d][0][k] = v;
+                    if (NT > 1) s_t[t][d][NT > 1 ? 1 : 0][k] = d1;
+                    if (NT > 2) s_t[t][d][NT > 2 ? 2 : 0][k] = d2;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) s_q[t][q] = fq[a * 10 + q];
+        }
+        __syncthreads();
+        for (int s2 = g; s2 < nchunk; s2 += 4) {
+            const int e = s_r[s2];
+            const int k0 = lx + 5 - (e & 15), k1 = ly + 5 - ((e >> 4) & 15);
+            if ((unsigned)k0 > 5u || (unsigned)k1 > 5u) continue;
+            const int kz = 5 - ((e >> 8) & 15);   // z table index of the brick's first point
+            const R vx = s_t[s2][0][0][k0], vy = s_t[s2][1][0][k1];
+            R C0 = s_q[s2][0] * vx * vy, C1 = 0, C2 = 0;
+            if (LM >= 1) {
+                const R ax = s_t[s2][0][NT > 1 ? 1 : 0][k0], ay = s_t[s2][1][NT > 1 ? 1 : 0][k1];
+                C0 += s_q[s2][NQ > 1 ? 1 : 0] * ax * vy + s_q[s2][NQ > 2 ? 2 : 0] * vx * ay;
+                C1 = s_q[s2][NQ > 3 ? 3 : 0] * vx * vy;
+                if (LM >= 2) {
+                    const R bxv = s_t[s2][0][NT > 2 ? 2 : 0][k0], byv = s_t[s2][1][NT > 2 ? 2 : 0][k1];
+                    C0 += s_q[s2][NQ > 4 ? 4 : 0] * bxv * vy + s_q[s2][NQ > 5 ? 5 : 0] * vx * byv +
+                          (R)2 * s_q[s2][NQ > 7 ? 7 : 0] * ax * ay;
+                    C1 += (R)2 * (s_q[s2][NQ > 8 ? 8 : 0] * ax * vy + s_q[s2][NQ > 9 ? 9 : 0] * vx * ay);
+                    C2 = s_q[s2][NQ > 6 ? 6 : 0] * vx * vy;
+                }
+            }
+#pragma unroll
+            for (int lz = 0; lz < 4; ++lz) {
+                const int k2 = kz + lz;
+                if ((unsigned)k2 > 5u) continue;
+                R w = C0 * s_t[s2][2][0][k2];
+                if (LM >= 1) w += C1 * s_t[s2][2][NT > 1 ? 1 : 0][k2];
+                if (LM >= 2) w += C2 * s_t[s2][2][NT > 2 ? 2 : 0][k2];
+                acc[lz] += w;
+            }
+        }
+        __syncthreads();
+    }
+    // quarter sums of each point: (g0 + g1) + (g2 + g3) in every lane
+#pragma unroll
+    for (int lz = 0; lz < 4; ++lz) {
+        acc[lz] += __shfl_xor(acc[lz], 16, 64);
+        acc[lz] += __shfl_xor(acc[lz], 32, 64);
+    }
+    // lane group g writes z point g of its column
+    R mine = acc[0];
+#pragma unroll
+    for (int lz = 1; lz < 4; ++lz)
+        if (g == lz) mine = acc[lz];
+    if (px >= K0 || py >= K1 || pz0 + g >= K2) return;
+    rgrid[(size_t)f * K0 * K1 * K2 + ((size_t)px * K1 + py) * K2 + pz0 + g] = mine;
+}
+
+// ---------------------------------------------------------------------------
 // Line FFT engine.  L lines (compile time) live in LDS as [element][line]; the
 // block's threads split into L lanes (one per line) times T = blockDim / L
 // butterfly slots.  One Stockham autosort stage per factor r of K: butterfly b

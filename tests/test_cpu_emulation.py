@@ -40,8 +40,9 @@ def emu_lib(tmp_path_factory):
     return build(str(tmp_path_factory.mktemp("emu")))
 
 
-def run(lib, *args, timeout=900):
+def run(lib, *args, timeout=900, extra_env=None):
     env = dict(os.environ, PYTHONPATH=REPO + os.pathsep + os.environ.get("PYTHONPATH", ""))
+    env.update(extra_env or {})
     p = subprocess.run([sys.executable, "-m", "tests.cpu_emu.runner", lib, *[str(a) for a in args]], cwd=REPO,
                        env=env, capture_output=True, text=True, timeout=timeout)
     tail = (p.stdout[-2000:] + p.stderr[-3000:])
@@ -132,7 +133,7 @@ def test_neighbour_lists_bit_exact(emu_lib):
                                     ((20, 24, 15), "fp64"), ((32, 27, 25), "mixed")])
 def test_grid_paths(emu_lib, K, prec):
     """104^3 in FP64 takes the line-major FFT passes (its plane exceeds the LDS
-    limit) and the column spread; in mixed precision the fused plane pass."""
+    limit); in mixed precision the fused plane pass."""
     s, pos, box = small_water_box()
     pairs = reference_pairs(pos, box, 6.0, s.bonded)
     kappa = O.setup_ewald_parameters(6.0, 5e-4, box)[0]
@@ -175,3 +176,15 @@ def test_type_reduce(emu_lib):
     rb = np.stack([b[:, m == t].sum(1) for t in range(3)], axis=1)
     assert np.allclose(r["ra"], ra, rtol=1e-12, atol=1e-12)
     assert np.allclose(r["rb"], rb, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("variant", ["list", "col"])
+def test_small_box_other_spread_variants(emu_lib, variant):
+    """The one-point-per-lane and four-bricks-per-wavefront spreads (not the
+    default) against the oracle on the polarisable small box."""
+    s, pos, box = small_water_box()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    o, _ = oracle_eval(s, pos, box, pairs, 6.0, True, solver="cg", tol=1e-9)
+    g = run(emu_lib, "small_box", "fp64", extra_env={"ADMP_SPREAD": variant})
+    ok, msgs = compare(g, o)
+    assert ok, msgs

@@ -381,8 +381,7 @@ def test_mixed_batched_frames_converge(fp32_products):
 def test_grid_sizes_vs_oracle(K, prec):
     """Energies and forces on grids whose sizes exercise every FFT radix path
     (4, 2, 3, 5, 7, 11, 13), the scatter path of grids below 9 points per axis,
-    and (104^3: 17,576 bricks) the column spread used for large launches,
-    against the oracle on the same grid."""
+    and (104^3) the line-major FFT passes, against the oracle on the same grid."""
     from dmff_amd.admp.engine import Engine
     s, pos, box = _small_waters()
     pairs = reference_pairs(pos, box, 6.0, s.bonded)
