@@ -431,7 +431,9 @@ def roofline(name, tm, D, prec, frames=1, prof=None):
     FFT, line path (four z/y launches per transform pair, one x launch):
       fft        (2 G w + 12 Gh w) / 4         z R2C, y, y, z C2R passes
       influence  4 Gh w + 8 Gh                 x pass: half spectrum in and out, tabulated influence
-    The path is read off the launch counts: 2 or 4 fft launches per influence launch.
+    plane_matvec (solver products with few frames: the plane pass and the real-space product in one launch)
+                 4 Gh w + 8 Gh + 2P (4 + 2 w) + N 3 (w + 8)
+    The path is read off the launch counts: 2 or 4 fft launches per plane (influence or plane_matvec) launch.
     """
     ms, n = tm
     t = ms / n / 1e3
@@ -444,8 +446,9 @@ def roofline(name, tm, D, prec, frames=1, prof=None):
     Gh = K[0] * K[1] * (K[2] // 2 + 1)
     P = D.get("n_pairs", 0)
     plane = False
-    if prof and prof.get("influence", (0, 0))[1]:
-        plane = prof.get("fft", (0, 0))[1] < 3 * prof["influence"][1]
+    n_plane = (prof.get("influence", (0, 0))[1] + prof.get("plane_matvec", (0, 0))[1]) if prof else 0
+    if n_plane:
+        plane = prof.get("fft", (0, 0))[1] < 3 * n_plane
     table = {
         "spread": ("k_spread_cw", N * (3 * 8 + 4 * w) + G * w),
         "fft": ("k_fz_r2c/k_fz_c2r", G * w + 2 * Gh * w) if plane else
@@ -453,8 +456,9 @@ def roofline(name, tm, D, prec, frames=1, prof=None):
         "influence": ("k_fxy_plane" if plane else "k_fx_infl", 4 * Gh * w + 8 * Gh),
         "gather": ("k_gather_ml", G * w + N * 3 * 8 + N * 3 * 8),
         "real_matvec": ("k_real_matvec", 2 * P * (4 + 2 * 8) + N * 3 * (8 + 8)),
+        "plane_matvec": ("k_plane_matvec", 4 * Gh * w + 8 * Gh + 2 * P * (4 + 2 * 8) + N * 3 * (8 + 8)),
     }
-    if name not in table or (name == "real_matvec" and not P):
+    if name not in table or (name in ("real_matvec", "plane_matvec") and not P):
         return {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
                 "traffic": None, "avg_launch_us": t * 1e6}
     kname, bytes_ = table[name]

@@ -138,7 +138,7 @@ struct admp_ctx {
 
 static const char* PROF_NAMES[16] = {"real_final", "real_permfield", "real_matvec", "spline_prep", "spread",
                                      "fft", "influence", "gather", "nbr_build", "frames", "pair_short",
-                                     "solver_vec", "solver_graph", "", "", ""};
+                                     "solver_vec", "solver_graph", "plane_matvec", "", ""};
 
 static void clear_graphs(admp_ctx* ctx) {
     for (auto& g : ctx->graphs) {
@@ -1065,7 +1065,7 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
         // fused y / x / influence / inverse x / inverse y, one block per (kz, frame)
         npart = H;
         const size_t sm = plane_smem(K0, K1, Cs);
-        ProfScope ps_(ctx, 6);
+        ProfScope ps_(ctx, mv ? 13 : 6);
         if (want_energy)
             hipLaunchKernelGGL((k_fxy_plane<R, true>), dim3(H, F), dim3(512), sm, st, K0, K1, K2, g.plan[0], g.plan[1],
                                twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa,

@@ -175,7 +175,7 @@ admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* 
  * admp_profile_get synchronises the stream and returns the accumulated milliseconds
  * and launch count of family `key` (0 real_final, 1 real_permfield, 2 real_matvec,
  * 3 spline_prep, 4 spread, 5 fft, 6 influence, 7 gather, 8 nbr_build, 9 frames,
- * 10 pair_short) and its name. */
+ * 10 pair_short, 11 solver_vec, 12 solver_graph, 13 plane_matvec) and its name. */
 admp_status admp_profile(admp_ctx* ctx, int32_t enable, int32_t reset);
 admp_status admp_profile_get(admp_ctx* ctx, int32_t key, double* total_ms, int64_t* count, const char** name);
 
