@@ -360,6 +360,7 @@ def main():
     cand = {k: v for k, v in kprof.items() if v[1] and k not in ("solver_vec", "solver_graph")}
     dom = max(cand.items(), key=lambda x: x[1][0])
     roof = roofline(dom[0], dom[1], D, "mixed", prof=kprof)
+    roof["traffic"], roof["traffic_source"] = measured_traffic(roof["kernel"])
     out = {
         "metric": "ADMP E+F+grad evals/s @12,288 atoms (mixed precision); FP64 beside it",
         "value": 1.0 / mix["dt"],
@@ -415,6 +416,28 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+COUNTER_SUMMARY = os.path.join("profiles", "admp_12288_hbm_counters.md")
+
+
+def measured_traffic(kname):
+    """HBM bytes per launch of kernel `kname` at this workload, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors
+    read off a 256 MiB copy), from the committed summary of two rocprofv3 --pmc runs of this script
+    (`--legs none --steps 1 --warmup 0 --calibrate-copy`, one counter each).  (None, None) without a row for it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), COUNTER_SUMMARY)
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None, None
+    for line in lines:
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0].startswith("`" + kname + "<"):
+            try:
+                return int(cells[4]) * 1024, COUNTER_SUMMARY + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+            except ValueError:
+                return None, None
+    return None, None
 
 
 def roofline(name, tm, D, prec, frames=1, prof=None):
