@@ -629,6 +629,11 @@ extern "C" admp_status admp_set_topology(admp_ctx* ctx, const int32_t* axis_type
 
 // exclusive scan of F rows of n ints into out (n + 1 per row) on stream st
 static admp_status scan_rows(admp_ctx* ctx, const int* in, int* out, int n, int F, hipStream_t st) {
+    if (n <= 4 * 4096) {
+        hipLaunchKernelGGL(k_scan_small, dim3(F), dim3(1024), 0, st, n, in, out);
+        HIPCHK(hipGetLastError());
+        return ADMP_OK;
+    }
     const int ntile = (n + SCAN_TILE - 1) / SCAN_TILE;
     const size_t need = (size_t)ntile * F;
     if (need > ctx->scan_cap) {

@@ -18,6 +18,21 @@ struct BoxMat {
     double B[9], Binv[9];
 };
 
+// d2_exact for a box with zero off-diagonal entries (B and Binv): the terms
+// d_k * 0 of the general form add exact zeros, so the value is the same bit for
+// bit (a zero's sign aside, which the square removes)
+template <class BX>
+__device__ __forceinline__ double d2_ortho(const BX& b, double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    const double t0 = dx * b.Binv[0], t1 = dy * b.Binv[4], t2 = dz * b.Binv[8];
+    const double s0 = t0 - floor(t0 + 0.5), s1 = t1 - floor(t1 + 0.5), s2 = t2 - floor(t2 + 0.5);
+    const double d0 = s0 * b.B[0], d1 = s1 * b.B[4], d2 = s2 * b.B[8];
+    double r = d0 * d0;
+    r = r + d1 * d1;
+    r = r + d2 * d2;
+    return r;
+}
+
 template <class BX>
 __device__ __forceinline__ double d2_exact(const BX& b, double dx, double dy, double dz) {
 #pragma clang fp contract(off)
@@ -147,6 +162,41 @@ __global__ void __launch_bounds__(256) k_scan_add(int n, const int* __restrict__
     }
 }
 
+// one block of 1024 per row for short rows (one launch instead of the three
+// above): tiles of 4096 scanned in turn with a running carry
+__global__ void __launch_bounds__(1024) k_scan_small(int n, const int* __restrict__ in, int* __restrict__ out) {
+    __shared__ int ws[16];
+    const int f = blockIdx.x;
+    const int* a = in + (size_t)f * n;
+    int* o = out + (size_t)f * (n + 1);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int carry = 0;
+    for (int base0 = 0; base0 < n; base0 += 4 * 1024) {
+        const int base = base0 + threadIdx.x * 4;
+        int v[4], sum = 0;
+        for (int k = 0; k < 4; ++k) {
+            v[k] = base + k < n ? a[base + k] : 0;
+            sum += v[k];
+        }
+        const int incl = wave_incl_scan(sum);
+        if (lane == 63) ws[w] = incl;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int k = 0; k < 16; ++k) {
+            if (k < w) woff += ws[k];
+            tot += ws[k];
+        }
+        __syncthreads();
+        int ex = carry + woff + incl - sum;
+        for (int k = 0; k < 4; ++k) {
+            if (base + k < n) o[base + k] = ex;
+            ex += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0) o[n] = carry;
+}
+
 __global__ void k_cell_fill(int N, int ncell, const int* __restrict__ cell, const int* __restrict__ cstart,
                             int* __restrict__ cfill, int* __restrict__ catoms) {
     const int f = blockIdx.y;
@@ -262,6 +312,10 @@ k_nbr_build(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __re
     BoxMat bm;
 #pragma unroll
     for (int q = 0; q < 9; ++q) { bm.B[q] = b.B[q]; bm.Binv[q] = b.Binv[q]; }
+    bool ortho = true;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+        if (q % 4 != 0) ortho = ortho && bm.B[q] == 0.0 && bm.Binv[q] == 0.0;
     // atom i's exclusion words (the same for the whole wavefront): lane e holds entry e
     const int x0 = xoff[i], nx = xoff[i + 1] - x0;
     const int myx = lane < nx ? xent[x0 + lane] : 0;
@@ -283,7 +337,8 @@ k_nbr_build(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __re
             if (k < hi) {
                 const double4 pj = sp[k];
                 j = (int)pj.w;
-                const double d2 = d2_exact(bm, pi_.x - pj.x, pi_.y - pj.y, pi_.z - pj.z);
+                const double d2 = ortho ? d2_ortho(bm, pi_.x - pj.x, pi_.y - pj.y, pi_.z - pj.z)
+                                        : d2_exact(bm, pi_.x - pj.x, pi_.y - pj.y, pi_.z - pj.z);
                 hit = j != i && d2 < rc2;
             }
             const unsigned long long m = __ballot(hit);
@@ -312,7 +367,9 @@ k_nbr_build(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __re
     if (lane == 0) {
         ncnt[fo + i] = count < cap ? count : cap;
         nhalf[fo + i] = half;
-        atomicMax(maxcnt, count);
+        // only overflowing rows report (the host needs the longest row only to
+        // grow the capacity): no same-address atomic per site in the common case
+        if (count > cap) atomicMax(maxcnt, count);
     }
 }
 
