@@ -45,6 +45,7 @@ struct admp_ctx {
     // neighbour build whose row-count check is still pending (resolved by nbr_settle)
     bool nbr_pending = false;
     int* h_maxcnt = nullptr;            // pinned copy of the longest row
+    double* h_comp = nullptr;           // pinned energy components (3 F real/self/pol, 7 F reciprocal)
     hipEvent_t ev_nbr = nullptr;
     hipEvent_t ev_state[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -404,6 +405,7 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
         if (ctx->ev_state[k]) (void)hipEventDestroy(ctx->ev_state[k]);
     if (ctx->h_pcg) (void)hipHostFree(ctx->h_pcg);
     if (ctx->h_maxcnt) (void)hipHostFree(ctx->h_maxcnt);
+    if (ctx->h_comp) (void)hipHostFree(ctx->h_comp);
     if (ctx->ev_nbr) (void)hipEventDestroy(ctx->ev_nbr);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -450,6 +452,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         hipEventCreateWithFlags(&ctx->ev_state[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_pcg, 2 * sizeof(PcgState) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_maxcnt, sizeof(int), 0) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_comp, 10 * sizeof(double) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_nbr, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
@@ -1698,10 +1701,9 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_atom, ctx->d_red, 0);
     hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_self, ctx->d_red + F, 0);
     hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_pol, ctx->d_red + 2 * F, 0);
-    std::vector<double> comp(3 * F);
-    HIPCHK(hipMemcpyAsync(comp.data(), ctx->d_red, 3 * F * sizeof(double), hipMemcpyDeviceToHost, st));
-    std::vector<double> rec(7 * F);
-    HIPCHK(hipMemcpyAsync(rec.data(), ctx->d_rec, 7 * F * sizeof(double), hipMemcpyDeviceToHost, st));
+    // total energy per frame on the device (real + self + polarisation + reciprocal)
+    if (energy)
+        hipLaunchKernelGGL(k_energy_total, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_red, ctx->d_rec, energy);
     if (want_grad) {
         // dE/dQ_global total -> frames backward
         hipLaunchKernelGGL(k_add3, dim3(nblk(NF * 10, 256)), dim3(256), 0, st, NF * 10, ctx->d_gq, ctx->d_gq_rec,
@@ -1721,20 +1723,22 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
             hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9a,
                                lpme ? ctx->d_red9b : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr,
                                c.K1, c.K2, c.K3, 0, dbox, 0);
-        if (dpos) HIPCHK(hipMemcpyAsync(dpos, ctx->d_gpos, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        if (dQ_local)
-            HIPCHK(hipMemcpyAsync(dQ_local, ctx->d_gQl, NF * ctx->nh * sizeof(double), hipMemcpyDeviceToDevice, st));
-        if (dpol) HIPCHK(hipMemcpyAsync(dpol, ctx->d_gpol, NF * sizeof(double), hipMemcpyDeviceToDevice, st));
-        if (dthole) HIPCHK(hipMemcpyAsync(dthole, ctx->d_gth, NF * sizeof(double), hipMemcpyDeviceToDevice, st));
+        // the four gradient outputs in one launch
+        CopyJobs cj{};
+        cj.src[0] = ctx->d_gpos; cj.dst[0] = dpos; cj.n[0] = dpos ? NF * 3 : 0;
+        cj.src[1] = ctx->d_gQl; cj.dst[1] = dQ_local; cj.n[1] = dQ_local ? NF * ctx->nh : 0;
+        cj.src[2] = ctx->d_gpol; cj.dst[2] = dpol; cj.n[2] = dpol ? NF : 0;
+        cj.src[3] = ctx->d_gth; cj.dst[3] = dthole; cj.n[3] = dthole ? NF : 0;
+        const size_t nmax = std::max(std::max(cj.n[0], cj.n[1]), std::max(cj.n[2], cj.n[3]));
+        if (nmax) hipLaunchKernelGGL(k_copy4, dim3(nblk(nmax, 256), 4), dim3(256), 0, st, cj);
         HIPCHK(hipGetLastError());
     }
+    // energy components for the stats: one pinned copy after everything is queued
+    HIPCHK(hipMemcpyAsync(ctx->h_comp, ctx->d_red, 3 * F * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ctx->h_comp + 3 * F, ctx->d_rec, 7 * F * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    std::vector<double> e(F);
-    for (int f = 0; f < F; ++f) e[f] = comp[f] + comp[F + f] + comp[2 * F + f] + rec[7 * f];
-    if (energy) {
-        HIPCHK(hipMemcpyAsync(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-    }
+    const double* comp = ctx->h_comp;
+    const double* rec = ctx->h_comp + 3 * F;
     admp_status ret = ADMP_OK;
     if (stats) {
         memset(stats, 0, sizeof(*stats));

@@ -882,4 +882,24 @@ __global__ void __launch_bounds__(TR_BLOCK) k_type_reduce(const double* __restri
     const double acc = block_sum((a[0] + a[1]) + (a[2] + a[3]), sh);
     if (threadIdx.x == 0) out[((size_t)f * n_types + t) * d + c] = acc;
 }
+
+// energy of frame f: ((real + self) + polarisation) + reciprocal, from the
+// per-frame sums red[f], red[F + f], red[2F + f] and rec[7 f]
+__global__ void k_energy_total(int F, const double* __restrict__ red, const double* __restrict__ rec,
+                               double* __restrict__ e) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < F) e[f] = red[f] + red[F + f] + red[2 * F + f] + rec[7 * f];
+}
+
+// up to four device-to-device copies in one launch (grid.y selects the copy)
+struct CopyJobs {
+    const double* src[4];
+    double* dst[4];
+    size_t n[4];
+};
+__global__ void __launch_bounds__(256) k_copy4(CopyJobs j) {
+    const int c = blockIdx.y;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < j.n[c]) j.dst[c][i] = j.src[c][i];
+}
 }  // namespace admp
