@@ -1080,13 +1080,13 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
                                kind, pref, ctx->d_part, (const double*)nullptr, skip);
         else if (mv) {
             const int npb = xcd_blocks(H * F);
-            const int nbf = (int)nblk((size_t)N * TPA, 512);
-            hipLaunchKernelGGL((k_plane_matvec<R>), dim3(npb + xcd_blocks(nbf * F)), dim3(512), sm, st, K0, K1, K2,
-                               g.plan[0], g.plan[1], twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1],
+            const int nbf = (int)nblk((size_t)N * TPA, PLANE_THREADS);
+            hipLaunchKernelGGL((k_plane_matvec<R>), dim3(npb + xcd_blocks(nbf * F)), dim3(PLANE_THREADS), sm, st, K0, K1,
+                               K2, g.plan[0], g.plan[1], twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1],
                                g.bmod[2], kappa, kind, pref, tab, F, npb, N, nbf, ctx->cap, ctx->d_pos, ctx->d_nbr,
                                ctx->d_ncnt, reinterpret_cast<const double*>(ctx->d_pcoef), mv->v, mv->out, skip);
         } else
-            hipLaunchKernelGGL((k_fxy_plane<R, false>), dim3(H, F), dim3(512), sm, st, K0, K1, K2, g.plan[0], g.plan[1],
+            hipLaunchKernelGGL((k_fxy_plane<R, false>), dim3(H, F), dim3(PLANE_THREADS), sm, st, K0, K1, K2, g.plan[0], g.plan[1],
                                twid<R>(g, 0), twid<R>(g, 1), hg, ctx->d_box, g.bmod[0], g.bmod[1], g.bmod[2], kappa,
                                kind, pref, ctx->d_part, tab, skip);
     } else {

@@ -778,6 +778,10 @@ __device__ inline InflPt infl_point(const BoxInfo& b, int K0, int K1, int K2, in
 // ACC: energy and box-derivative partials per block (multiplicity 2 for
 // 0 < kz < K2/2, the conjugate half).  tab (solver products): tabulated
 // 2 pref C_k / theta_k^2 in the same plane-major order.
+// threads of a solver plane-pass workgroup (one per kz plane); the variant that
+// also sums energy and box derivatives keeps 512 (it spills at 1024)
+constexpr int PLANE_THREADS = 1024;
+
 // plane i2 of frame f (one workgroup); nplane: planes per frame (energy partials)
 template <class R, bool ACC>
 __device__ __forceinline__ void fxy_plane_body(int f, int i2, int nplane, int K0, int K1, int K2, FFTPlan plan0,
@@ -842,7 +846,7 @@ __device__ __forceinline__ void fxy_plane_body(int f, int i2, int nplane, int K0
 }
 
 template <class R, bool ACC>
-__global__ void __launch_bounds__(512) k_fxy_plane(int K0, int K1, int K2, FFTPlan plan0, FFTPlan plan1,
+__global__ void __launch_bounds__(ACC ? 512 : PLANE_THREADS) k_fxy_plane(int K0, int K1, int K2, FFTPlan plan0, FFTPlan plan1,
                                                     const Cplx<R>* __restrict__ tw0g, const Cplx<R>* __restrict__ tw1g,
                                                     Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
                                                     const double* __restrict__ bmod0, const double* __restrict__ bmod1,
@@ -1223,10 +1227,10 @@ k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, cons
 // its workgroups fill the idle CUs, and no second stream (fork and join) is
 // needed.  Blocks [0, nplane_blocks) are planes (plane-major over frames;
 // blocks past H F return), the rest are real-space items of 512 threads (32
-// sites) in XCD order.  Every block is given the plane pass's dynamic LDS.
+// sites per 512 threads) in XCD order.  Every block is given the plane pass's dynamic LDS.
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(PLANE_THREADS)
 k_plane_matvec(int K0, int K1, int K2, FFTPlan plan0, FFTPlan plan1, const Cplx<R>* __restrict__ tw0g,
                const Cplx<R>* __restrict__ tw1g, Cplx<R>* __restrict__ hgrid, const BoxInfo* __restrict__ boxes,
                const double* __restrict__ bmod0, const double* __restrict__ bmod1, const double* __restrict__ bmod2,
@@ -1246,7 +1250,7 @@ k_plane_matvec(int K0, int K1, int K2, FFTPlan plan0, FFTPlan plan1, const Cplx<
     }
     const int item = xcd_item(b - nplane_blocks, nbf * F);
     if (item >= nbf * F) return;
-    real_matvec_body<double, double, 512>(item, N, nbf, cap, pos, boxes, nbr, ncnt, pcoef, v, out, skip);
+    real_matvec_body<double, double, PLANE_THREADS>(item, N, nbf, cap, pos, boxes, nbr, ncnt, pcoef, v, out, skip);
 }
 
 }  // namespace admp
