@@ -109,6 +109,7 @@ struct admp_ctx {
     int* d_pcj = nullptr;       // pair-block preconditioner partner (-1: diagonal)
     double* d_pcd = nullptr;    // its coefficients, 8 per site
     bool pc_pairs = true;       // pair blocks on (admp_solver_opts.precond)
+    CgCheck spread_chk{};       // convergence test of the next solver product (set by pcg_apply)
     int* d_active = nullptr;
     int dots_blocks = 64;
     double* d_pparts = nullptr;               // CG partials (CgParts layout)
@@ -1027,7 +1028,7 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
                            ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);                                   \
     else if (variant == 3)                                                                                           \
         hipLaunchKernelGGL((k_spread_cw<R, LMv>), dim3(nb1, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,            \
-                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);                                   \
+                           ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip, ctx->spread_chk);                  \
     else                                                                                                             \
         hipLaunchKernelGGL((k_spread_list<R, LMv>), dim3(nb1, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,          \
                            ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);
@@ -1193,7 +1194,7 @@ static CgParts cg_parts(const admp_ctx* ctx) { return CgParts{ctx->d_pparts, ctx
 // w and its z.w partials come from the gather's CG epilogue (brick path, the
 // real-space product joined before the gather) or from k_cg_w.
 template <class RC, class R>
-static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
+static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1, int cur, double tol) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     // real-space products in FP64 whenever the stored coefficients are FP64 (the
@@ -1217,6 +1218,8 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
                            reinterpret_cast<const RC*>(ctx->d_pcoef), vR, ctx->d_t1, ctx->d_pcg); }
         HIPCHK(hipEventRecord(ctx->ev_join, ctx->side));
     }
+    // the product's first kernel tests the residual the next update would test
+    ctx->spread_chk = CgCheck{cg_parts(ctx), ctx->d_pcg, tol, cur, 1};
     admp_status s = ADMP_OK;
     if (with_plane) {
         const CgEpi epi{ctx->d_t1, pol, f1, cg_parts(ctx)};
@@ -1234,6 +1237,7 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1) {
                          nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg);
     }
     // join before any return so that a capture never ends with an open fork
+    ctx->spread_chk.on = 0;
     if (!with_plane) HIPCHK(hipStreamWaitEvent(st, ctx->ev_join, 0));
     if (s != ADMP_OK) return s;
     if (!fused) {
@@ -1334,8 +1338,8 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
             for (int c = 0; c < nrun; ++c) {
                 const int cur = c & 1;  // chunks have even length (but a last one cut by maxiter)
                 admp_status s2;
-                if (use_dbl) s2 = pcg_apply<RC, double>(ctx, pol, f1);
-                else s2 = pcg_apply<RC, float>(ctx, pol, f1);
+                if (use_dbl) s2 = pcg_apply<RC, double>(ctx, pol, f1, cur, tol);
+                else s2 = pcg_apply<RC, float>(ctx, pol, f1, cur, tol);
                 if (s2 != ADMP_OK) return s2;
                 ProfScope ps_(ctx, 11);
                 if (use_dbl) launch_update<double>(ctx, cur, x, tol, pol);

@@ -62,6 +62,17 @@ struct CgParts {
     __device__ double* mx(int f, int b) const { return frame(f) + nw + (2 + b) * nb; }
 };
 
+// Convergence test at the start of a CG product: the max|r| partials of parity
+// cur (written by the previous update, read by the next one) are reduced by the
+// product's first kernel; when they are below tol the frame is flagged
+// converged and the product (whose result that update would discard) is skipped.
+struct CgCheck {
+    CgParts parts;
+    PcgState* st;
+    double tol;
+    int cur, on;
+};
+
 
 template <class R> __device__ __forceinline__ Cplx<R> cmul(Cplx<R> a, Cplx<R> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};

@@ -377,9 +377,24 @@ template <class R, int LM>
 __global__ void __launch_bounds__(64)
 k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R* __restrict__ fq,
             const int* __restrict__ bstart, const int2* __restrict__ blist, long cap, R* __restrict__ rgrid,
-            const PcgState* __restrict__ skip) {
+            const PcgState* __restrict__ skip, CgCheck chk) {
     static_assert(BRX == 4 && BRY == 4 && BRZ == 4, "column spread assumes 4x4x4 bricks");
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
+    if (chk.on) {
+        // every block reduces the same partials in the same order: one decision
+        const int f = blockIdx.y;
+        const double* mxp = chk.parts.mx(f, chk.cur);
+        double m = 0.0;
+        for (int k = threadIdx.x; k < chk.parts.nb; k += 64) m = fmax(m, mxp[k]);
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        if (m < chk.tol) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                chk.st[f].res = m;
+                chk.st[f].done = 1;
+            }
+            return;
+        }
+    }
     constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
     constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
     __shared__ int s_r[SP_CHUNK];
