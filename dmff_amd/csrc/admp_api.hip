@@ -1270,22 +1270,24 @@ static void launch_update(admp_ctx* ctx, int cur, double* x, double tol, const d
 // iterations use FP64 products.  The next update has parity 0.
 template <class RC>
 static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x, const double* pol, double f1,
-                               double tol, int mode, bool dbl) {
+                               double tol, int mode, bool dbl, bool x_zero = false) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     admp_status s;
-    if ((s = matvec<RC, double>(ctx, x, ctx->d_res, pol, f1)) != ADMP_OK) return s;
+    // from x = 0 the residual is the right-hand side: no product
+    const double* Ax = x_zero ? nullptr : ctx->d_res;
+    if (!x_zero && (s = matvec<RC, double>(ctx, x, ctx->d_res, pol, f1)) != ADMP_OK) return s;
     ProfScope ps_(ctx, 11);
     const double* wd = ctx->cfg.lpme ? ctx->d_wd : nullptr;
     if (dbl)
         hipLaunchKernelGGL(k_cg_start<double>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->d_box, wd, pol,
-                           rhs, ctx->d_res, ctx->d_r, ctx->d_z, (double*)nullptr,
+                           rhs, Ax, ctx->d_r, ctx->d_z, (double*)nullptr,
                            reinterpret_cast<double*>(ctx->d_swfrac),
                            ctx->cfg.lpme ? reinterpret_cast<double*>(ctx->d_sfq) : nullptr, pc_j(ctx), ctx->d_pcd,
                            cg_parts(ctx));
     else
         hipLaunchKernelGGL(k_cg_start<float>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->d_box, wd, pol,
-                           rhs, ctx->d_res, ctx->d_r, ctx->d_z, reinterpret_cast<float*>(ctx->d_vR),
+                           rhs, Ax, ctx->d_r, ctx->d_z, reinterpret_cast<float*>(ctx->d_vR),
                            reinterpret_cast<float*>(ctx->d_swfrac),
                            ctx->cfg.lpme ? reinterpret_cast<float*>(ctx->d_sfq) : nullptr, pc_j(ctx), ctx->d_pcd,
                            cg_parts(ctx));
@@ -1316,7 +1318,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
     int fp32_iters = 0;
     std::vector<int> it_base(F, 0);
     admp_status s;
-    if ((s = pcg_restart<RC>(ctx, rhs, x, pol, f1, tol, 1, dbl)) != ADMP_OK) return s;
+    if ((s = pcg_restart<RC>(ctx, rhs, x, pol, f1, tol, 1, dbl, cold)) != ADMP_OK) return s;
     const int chunk = 4;  // FP32-phase chunk (host reads the state after each)
     std::vector<PcgState> hs(F);
     int done_iters = 0;

@@ -679,7 +679,7 @@ __device__ inline void pc_apply(int j, const double* __restrict__ pd, double m, 
     u[2] = pd[6] * ri[2] + pd[7] * rj[2] + t * zz;
 }
 
-// r = b - Ax, z = M^-1 r and the product inputs derived from z (with the spline
+// r = b - Ax (Ax null: x = 0, r = b), z = M^-1 r and the product inputs derived from z (with the spline
 // coordinates wfrac of the solve's precision R); r into buffer 0 of its two,
 // partials into buffers 0.  pcj/pcd: pair-block preconditioner (null: diagonal).
 template <class R>
@@ -698,7 +698,7 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __
         const size_t a = (size_t)f * N + i;
         double rv[3], rj[3] = {0, 0, 0}, u[3];
         for (int k = 0; k < 3; ++k) {
-            rv[k] = b[a * 3 + k] - Ax[a * 3 + k];
+            rv[k] = Ax ? b[a * 3 + k] - Ax[a * 3 + k] : b[a * 3 + k];
             r[a * 3 + k] = rv[k];
             if (pol[i] > 0.001) mx = fmax(mx, fabs(rv[k]));
             if (wd) wfrac[a * 3 + k] = (R)wd[a * 3 + k];
@@ -706,7 +706,7 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __
         const int j = pcj ? pcj[a] : -1;
         if (j >= 0) {
             const size_t aj = (size_t)f * N + j;
-            for (int k = 0; k < 3; ++k) rj[k] = b[aj * 3 + k] - Ax[aj * 3 + k];
+            for (int k = 0; k < 3; ++k) rj[k] = Ax ? b[aj * 3 + k] - Ax[aj * 3 + k] : b[aj * 3 + k];
         }
         pc_apply(j, j >= 0 ? pcd + a * 8 : nullptr, inv_diag(pol[i]), rv, rj, u);
         for (int k = 0; k < 3; ++k) {
