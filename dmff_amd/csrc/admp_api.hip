@@ -1523,8 +1523,8 @@ static admp_status solver_rhs(admp_ctx* ctx, const double* pol, const double* th
         admp_status s = recip_run<RB>(ctx, ctx->eg, 0, ctx->d_Qc, nullptr, 1, lmax_eff, 0, DIELECTRIC, kappa, 1,
                                       ctx->d_t2, nullptr, nullptr, nullptr, false);
         if (s != ADMP_OK) return s;
-        hipLaunchKernelGGL(k_sub, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_t2, ctx->d_b);
-        hipLaunchKernelGGL(k_add_self_b, dim3(nblk(N, 256), F), dim3(256), 0, st, N, f1, ctx->d_Qc, ctx->d_b);
+        hipLaunchKernelGGL(k_rhs_finish, dim3(nblk(N, 256), F), dim3(256), 0, st, N, f1, ctx->d_Qc, ctx->d_t2,
+                           ctx->d_b);
     }
     HIPCHK(hipGetLastError());
     return ADMP_OK;
@@ -1704,9 +1704,8 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     hipLaunchKernelGGL(k_self, dim3(nblk(N, 256), F), dim3(256), 0, st, N, kappa, lpme, lpol, ctx->d_Qc, U, pol,
                        ctx->d_e_self, ctx->d_e_pol, ctx->d_gq, lpol ? ctx->d_gpol : nullptr);
     // per-frame energy components: real, self, polarisation penalty -> d_red [0..F), [F..2F), [2F..3F)
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_atom, ctx->d_red, 0);
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_self, ctx->d_red + F, 0);
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_pol, ctx->d_red + 2 * F, 0);
+    hipLaunchKernelGGL(k_reduce3, dim3(3, F), dim3(RED_BLOCK), 0, st, N,
+                       Reduce3{{ctx->d_e_atom, ctx->d_e_self, ctx->d_e_pol}}, ctx->d_red);
     // total energy per frame on the device (real + self + polarisation + reciprocal)
     if (energy)
         hipLaunchKernelGGL(k_energy_total, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_red, ctx->d_rec, energy);

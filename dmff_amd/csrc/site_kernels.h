@@ -388,6 +388,30 @@ __global__ void __launch_bounds__(RED_BLOCK) k_reduce_cols(int N, int ncol, cons
     }
 }
 
+// three single-column reductions in one launch: block (k, f) sums in[k] of frame
+// f (N values) in the order of k_reduce_cols into out[k * F + f]
+struct Reduce3 {
+    const double* in[3];
+};
+__global__ void __launch_bounds__(RED_BLOCK) k_reduce3(int N, Reduce3 r, double* __restrict__ out) {
+    __shared__ double sh[16];
+    const int k = blockIdx.x, f = blockIdx.y;
+    const double* col = r.in[k] + (size_t)f * N;
+    const int st = blockDim.x;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int i = threadIdx.x;
+    for (; i + 3 * st < N; i += 4 * st) {
+        s0 += col[i];
+        s1 += col[i + st];
+        s2 += col[i + 2 * st];
+        s3 += col[i + 3 * st];
+    }
+    for (; i < N; i += st) s0 += col[i];
+    double s = (s0 + s1) + (s2 + s3);
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[(size_t)k * gridDim.y + f] = s;
+}
+
 // ----------------------------------------------------------------- solver algebra (FP64)
 // y = sum of terms: out = real + recip + diag * v   where diag_i = D/max(pol,1e-8) - 2 D f1
 __global__ void k_apply_diag(int N, double f1, const double* __restrict__ pol, const double* __restrict__ v,
@@ -515,12 +539,20 @@ __global__ void k_box_grad(int F, const BoxInfo* __restrict__ boxes, const doubl
 
 
 // b += 2 D f1 mu_perm (self-energy part of -dE/dU at U = 0)
-__global__ void k_add_self_b(int N, double f1, const double* __restrict__ Qc, double* __restrict__ b) {
+// b = (b - recip) + 2 D f1 mu per component: the reciprocal field of the
+// permanent multipoles taken off and the self term added in one pass
+__global__ void k_rhs_finish(int N, double f1, const double* __restrict__ Qc, const double* __restrict__ recip,
+                             double* __restrict__ b) {
     const int f = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const size_t a = (size_t)f * N + i;
-    for (int k = 0; k < 3; ++k) b[a * 3 + k] += 2.0 * DIELECTRIC * f1 * Qc[a * 10 + 1 + k];
+    for (int k = 0; k < 3; ++k) {
+        double v = b[a * 3 + k];
+        v -= recip[a * 3 + k];
+        v += 2.0 * DIELECTRIC * f1 * Qc[a * 10 + 1 + k];
+        b[a * 3 + k] = v;
+    }
 }
 
 // energy[f] = sum of the per-frame components in red[f*16 + 0..3] (+ rec[f*7])
