@@ -69,8 +69,9 @@ def evaluate(eng, D, tol=1e-6):
     E, g, U, info = eng.compute(D["pos"], D["box"], D["Q"], D["pol"], D["thole"], S.mScales, S.pScales, S.dScales,
                                 solver="pcg", tol=tol, maxiter=500, want_grad=True)
     dQ = D["qtype"](g["Q"][0])
-    dpol = D["ptype"](g["pol"][0])
-    dth = D["ptype"](g["thole"][0])
+    # pol and thole share the polarisation-type map: one reduction of both columns
+    dpt = D["ptype"](torch.stack((g["pol"][0], g["thole"][0]), dim=1))
+    dpol, dth = dpt[:, 0], dpt[:, 1]
     return E, g, U, info, dQ, dpol, dth
 
 
