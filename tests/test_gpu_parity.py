@@ -294,6 +294,36 @@ def test_batched_frames_equal_single():
         assert abs(float(Eb[k]) - float(e1)) < 1e-9 * abs(float(e1))
 
 
+def test_batched_frames_side_stream_product():
+    """14 frames of the small box: 14 x 10 kz planes exceed the 128 that ride in the
+    plane launch, so the real-space product runs on the second stream (the path of
+    the 64-frame batch); every frame equals its single-frame evaluation."""
+    from dmff_amd.admp.engine import Engine
+    s, pos, box = _small_waters()
+    rng = np.random.default_rng(21)
+    F = 14
+    P = np.stack([pos + rng.normal(0, 0.01, pos.shape) for _ in range(F)])
+    kappa, K1, K2, K3 = O.setup_ewald_parameters(6.0, 5e-4, box)
+    assert (K3 // 2 + 1) * F > 128
+    args = (s.pme.Q_local[s.map_atomtype], s.pme.pol[s.map_poltype], s.pme.thole[s.map_poltype],
+            s.pme.mScales, s.pme.pScales, s.pme.dScales)
+
+    def run(Pf):
+        eng = Engine(len(pos), 2, True, True, kappa, (K1, K2, K3), 6.0, n_frames=len(Pf), device=DEV)
+        eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
+        eng.build_neighbors(Pf, np.stack([box] * len(Pf)))
+        E, g, U, info = eng.compute(Pf, np.stack([box] * len(Pf)), *args, solver="pcg", tol=1e-9, maxiter=500,
+                                    want_grad=True)
+        assert all(info["converged"][: len(Pf)])
+        return E.cpu().numpy(), g["pos"].cpu().numpy()
+
+    Eb, Gb = run(P)
+    for k in (0, 7, 13):
+        E1, G1 = run(P[k:k + 1])
+        assert abs(Eb[k] - E1[0]) < 1e-9 * abs(E1[0]), (k, Eb[k], E1[0])
+        assert rel_rms(Gb[k], G1[0]) < 1e-7
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_pair_preconditioner_same_solution_fewer_iterations(prec):
     """4096 waters (the 12,288-site system): the pair-block preconditioner and the
