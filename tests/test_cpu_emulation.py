@@ -71,8 +71,11 @@ def test_small_box_polarisable_pcg(emu_lib, prec):
         assert ok, msgs
 
 
-def test_batched_frames_with_library_neighbour_lists(emu_lib):
-    r = run(emu_lib, "frames", "fp64")
+@pytest.mark.parametrize("n_frames", [3, 14])
+def test_batched_frames_with_library_neighbour_lists(emu_lib, n_frames):
+    """3 frames: the real-space product rides in the plane launch; 14 frames (140
+    planes): it runs on the second stream."""
+    r = run(emu_lib, "frames", "fp64", n_frames)
     for k in range(3):
         assert abs(r["E_batch"][k] - r["E_single"][k]) < 1e-9 * abs(r["E_single"][k])
         assert rel_rms(r["G_batch"][k], r["G_single"][k]) < 1e-8
