@@ -890,12 +890,13 @@ static admp_status recip_positions(admp_ctx* ctx, const Grid& g, int use_dA) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     { ProfScope ps_(ctx, 3);
+    // the sort's counters are zeroed by k_grid_base (no separate fills)
+    const bool sort = !small_grid(g);
     hipLaunchKernelGGL(k_grid_base, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_pos, ctx->d_box, use_dA,
-                       ctx->d_mbase, ctx->d_wd);
-    if (!small_grid(g)) {
+                       ctx->d_mbase, ctx->d_wd, sort ? ctx->d_gccount : nullptr, sort ? ctx->d_gcfill : nullptr,
+                       sort ? g.G * F : (size_t)0);
+    if (sort) {
         const size_t G = g.G;
-        HIPCHK(hipMemsetAsync(ctx->d_gccount, 0, G * F * sizeof(int), st));
-        HIPCHK(hipMemsetAsync(ctx->d_gcfill, 0, G * F * sizeof(int), st));
         hipLaunchKernelGGL(k_grid_cell, dim3(nblk(N, 256), F), dim3(256), 0, st, N, g.K[0], g.K[1], g.K[2],
                            ctx->d_mbase, ctx->d_gcell, ctx->d_gccount);
         admp_status s2 = scan_rows(ctx, ctx->d_gccount, ctx->d_gcstart, (int)G, F, st);

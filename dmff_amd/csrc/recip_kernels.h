@@ -114,10 +114,20 @@ __global__ void k_spline_prep(int N, const double4* __restrict__ pos, const BoxI
 }
 
 // Per-site grid base m and offset w only (positions are fixed during a solve).
+// Also zeroes n_zero ints of zero_a and zero_b (the counters of the site sort
+// that follows; null when there is none), grid-stride over all threads.
 __global__ void k_grid_base(int N, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes, int use_dA,
-                            int4* __restrict__ mbase, double* __restrict__ wd) {
+                            int4* __restrict__ mbase, double* __restrict__ wd, int* __restrict__ zero_a,
+                            int* __restrict__ zero_b, size_t n_zero) {
     const int f = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (zero_a) {
+        const size_t nthreads = (size_t)gridDim.x * blockDim.x * gridDim.y;
+        for (size_t k = (size_t)f * gridDim.x * blockDim.x + i; k < n_zero; k += nthreads) {
+            zero_a[k] = 0;
+            zero_b[k] = 0;
+        }
+    }
     if (i >= N) return;
     const BoxInfo& b = boxes[f];
     const double* A = use_dA ? b.dA : b.A;
