@@ -10,6 +10,12 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libadmp_hip.so")
 
+# True only for a build of the library whose "device" memory is host memory (the
+# CPU emulation of the HIP runtime used by the test suite); the HIP library
+# itself needs device pointers, and entry points that take torch tensors
+# refuse CPU tensors while this is False.
+HOST_MEMORY = False
+
 ADMP_OK = 0
 ADMP_WARN_NOT_CONVERGED = 5
 ADMP_FP64, ADMP_MIXED = 0, 1
@@ -87,6 +93,28 @@ def load():
             getattr(lib, name).restype
     _lib = lib
     return lib
+
+
+def check_scales(mScales, pScales=None, dScales=None):
+    """Topological scale vectors as the library takes them: equal lengths in 1..8
+    (index nbond-1, with nbond = 0 wrapping to the last entry as pme.py:1111-1125
+    does); raises ValueError otherwise."""
+    import numpy as np
+    out = []
+    n = None
+    for name, s in (("mScales", mScales), ("pScales", pScales), ("dScales", dScales)):
+        if s is None:
+            out.append(None)
+            continue
+        a = np.ascontiguousarray(np.asarray(s.detach().cpu() if hasattr(s, "detach") else s, dtype=np.float64)).reshape(-1)
+        if n is None:
+            n = a.shape[0]
+        if a.shape[0] != n:
+            raise ValueError(f"{name} has {a.shape[0]} entries, mScales has {n}: scale vectors must have equal lengths")
+        out.append(a)
+    if n is None or not 1 <= n <= 8:
+        raise ValueError(f"scale vectors need 1..8 entries, got {n}")
+    return out
 
 
 def check(ctx, status, what):

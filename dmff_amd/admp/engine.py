@@ -149,9 +149,9 @@ class Engine:
         lpol = bool(self.cfg.lpol)
         polt = _dev(pol, dev).reshape(N) if lpol else None
         tht = _dev(thole, dev).reshape(N) if lpol else None
-        ns = len(np.asarray(mScales))
-        sc = [np.ascontiguousarray(np.asarray(s if s is not None else np.ones(ns), dtype=np.float64))
-              for s in (mScales, pScales, dScales)]
+        sc = L.check_scales(mScales, pScales, dScales)
+        ns = sc[0].shape[0]
+        sc = [s if s is not None else np.ones(ns) for s in sc]
         U = torch.zeros((F, N, 3), dtype=F64, device=dev)
         warm = 0
         if U_init is not None and lpol:
@@ -193,7 +193,7 @@ class Engine:
         pos = _dev(pos, dev).reshape(F, N, 3)
         box = _dev(box, dev).reshape(F, 3, 3)
         c = _dev(c_list, dev).reshape(N, 3)
-        ms = np.ascontiguousarray(np.asarray(mScales, dtype=np.float64))
+        (ms,) = L.check_scales(mScales)[:1]
         E = torch.empty(F, dtype=F64, device=dev)
         g = {}
         if want_grad:
@@ -214,7 +214,7 @@ class Engine:
         box = _dev(box, dev).reshape(F, 3, 3)
         a = _dev(a, dev).reshape(N)
         b = _dev(b, dev).reshape(N)
-        ms = np.ascontiguousarray(np.asarray(mScales, dtype=np.float64))
+        (ms,) = L.check_scales(mScales)[:1]
         E = torch.empty(F, dtype=F64, device=dev)
         g = {}
         if want_grad:

@@ -59,23 +59,40 @@ class TypeReducer:
     def __init__(self, type_map, n_types=None, device=None):
         m = np.asarray(type_map, dtype=np.int64)
         self.n_types = int(m.max()) + 1 if n_types is None else int(n_types)
-        self.map = torch.as_tensor(m.astype(np.int32), device=device)
+        self._map_host = torch.as_tensor(m.astype(np.int32))
+        self._maps = {}
+        if device is not None:
+            self._map_on(torch.device(device))
         self.lib = L.load()
 
+    @property
+    def map(self):
+        return self._map_host
+
+    def _map_on(self, dev):
+        key = str(dev)
+        if key not in self._maps:
+            self._maps[key] = self._map_host.to(dev)
+        return self._maps[key]
+
     def __call__(self, per_atom, axis=0):
-        """per_atom: (..., N, ...) with atoms on `axis`; returns the same with types on `axis`."""
+        """per_atom: (..., N, ...) with atoms on `axis`; returns the same with types on `axis`.
+        per_atom must live in device memory (a CUDA tensor): the reduction is a GPU launch."""
         x = per_atom.detach().to(torch.float64)
         shape = list(x.shape)
         axis = axis % len(shape)
         n = shape[axis]
-        if n != self.map.numel():
-            raise ValueError(f"{n} atoms on axis {axis}, type map has {self.map.numel()}")
+        if n != self._map_host.numel():
+            raise ValueError(f"{n} atoms on axis {axis}, type map has {self._map_host.numel()}")
+        if not x.is_cuda and not L.HOST_MEMORY:
+            raise ValueError("TypeReducer: per_atom must be a CUDA tensor (the reduction runs on the GPU)")
+        tmap = self._map_on(x.device)
         pre = int(np.prod(shape[:axis])) if axis else 1
         d = int(np.prod(shape[axis + 1:])) if axis + 1 < len(shape) else 1
         x = x.contiguous()
         out = torch.empty([pre, self.n_types, d], dtype=torch.float64, device=x.device)
         stream = torch.cuda.current_stream(x.device).cuda_stream if x.is_cuda else 0
-        st = self.lib.admp_type_reduce(ctypes.c_void_p(x.data_ptr()), pre, n, d, ctypes.c_void_p(self.map.data_ptr()),
+        st = self.lib.admp_type_reduce(ctypes.c_void_p(x.data_ptr()), pre, n, d, ctypes.c_void_p(tmap.data_ptr()),
                                        self.n_types, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream))
         if st != L.ADMP_OK:
             raise L.ADMPError(f"admp_type_reduce failed with status {st}")

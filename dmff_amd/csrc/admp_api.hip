@@ -45,6 +45,8 @@ struct admp_ctx {
     // neighbour build whose row-count check is still pending (resolved by nbr_settle)
     bool nbr_pending = false;
     int* h_maxcnt = nullptr;            // pinned copy of the longest row
+    int* h_geom = nullptr;              // pinned copy of the geometry flags (GEOM_BAD_*)
+    int* d_geom = nullptr;              // geometry flags of the last upload_geometry
     double* h_comp = nullptr;           // pinned energy components (3 F real/self/pol, 7 F reciprocal)
     hipEvent_t ev_nbr = nullptr;
     hipEvent_t ev_state[2] = {nullptr, nullptr};
@@ -406,6 +408,7 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
         if (ctx->ev_state[k]) (void)hipEventDestroy(ctx->ev_state[k]);
     if (ctx->h_pcg) (void)hipHostFree(ctx->h_pcg);
     if (ctx->h_maxcnt) (void)hipHostFree(ctx->h_maxcnt);
+    if (ctx->h_geom) (void)hipHostFree(ctx->h_geom);
     if (ctx->h_comp) (void)hipHostFree(ctx->h_comp);
     if (ctx->ev_nbr) (void)hipEventDestroy(ctx->ev_nbr);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
@@ -453,6 +456,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         hipEventCreateWithFlags(&ctx->ev_state[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_pcg, 2 * sizeof(PcgState) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_maxcnt, sizeof(int), 0) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_geom, sizeof(int), 0) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_comp, 10 * sizeof(double) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_nbr, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
@@ -545,6 +549,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_coef, ctx->F * 4);
     ALLOC(ctx->d_active, ctx->F);
     ALLOC(ctx->d_maxcnt, 1);
+    ALLOC(ctx->d_geom, 1);
     ALLOC(ctx->d_ncnt, NF);
     ALLOC(ctx->d_nhalf, NF);
     ALLOC(ctx->d_cell, NF);
@@ -677,15 +682,35 @@ static admp_status ensure_cap(admp_ctx* ctx, int need) {
     return ADMP_OK;
 }
 
+// Positions and boxes of every frame into the context.  Non-finite or
+// out-of-range positions and unusable boxes are replaced on the device (so no
+// kernel derives an index from them) and flagged in d_geom; the entry point
+// queues geom_fetch() before its final synchronisation and returns
+// ADMP_ERR_ARG from geom_check() after it.
 static admp_status upload_geometry(admp_ctx* ctx, const double* pos, const double* box) {
     const size_t NF = (size_t)ctx->N * ctx->F;
     ctx->prepped = nullptr;  // site grid bases / sort follow the positions
-    hipLaunchKernelGGL(k_pack_pos, dim3(nblk(NF, 256)), dim3(256), 0, ctx->stream, NF, pos, ctx->d_pos);
+    HIPCHK(hipMemsetAsync(ctx->d_geom, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_pack_pos, dim3(nblk(NF, 256)), dim3(256), 0, ctx->stream, NF, pos, ctx->d_pos, ctx->d_geom);
     const admp_config& c = ctx->cfg;
     hipLaunchKernelGGL(k_box_setup, dim3(nblk(ctx->F, 64)), dim3(64), 0, ctx->stream, ctx->F, box, c.lpme ? c.K1 : 0,
-                       c.lpme ? c.K2 : 0, c.lpme ? c.K3 : 0, c.dK1, c.dK2, c.dK3, ctx->d_box);
+                       c.lpme ? c.K2 : 0, c.lpme ? c.K3 : 0, c.dK1, c.dK2, c.dK3, ctx->d_box, ctx->d_geom);
     HIPCHK(hipGetLastError());
     return ADMP_OK;
+}
+
+static admp_status geom_fetch(admp_ctx* ctx) {
+    HIPCHK(hipMemcpyAsync(ctx->h_geom, ctx->d_geom, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    return ADMP_OK;
+}
+
+// after the stream has passed geom_fetch()
+static admp_status geom_check(admp_ctx* ctx) {
+    const int g = *ctx->h_geom;
+    if (!g) return ADMP_OK;
+    ctx->err = (g & GEOM_BAD_BOX) ? "box not finite or degenerate (each height must be >= 0.5 A)"
+                                  : "positions not finite or beyond 1e5 A";
+    return ADMP_ERR_ARG;
 }
 
 static admp_status admp_set_pairs_impl(admp_ctx* ctx, int32_t frame, const void* pairs, int64_t n_pairs,
@@ -766,6 +791,7 @@ static admp_status nbr_settle(admp_ctx* ctx) {
     if (!ctx->nbr_pending) return ADMP_OK;
     ctx->nbr_pending = false;
     HIPCHK(hipEventSynchronize(ctx->ev_nbr));
+    { const admp_status g = geom_check(ctx); if (g != ADMP_OK) return g; }
     const int mx = *ctx->h_maxcnt;
     if (mx <= ctx->cap) return ADMP_OK;
     admp_status s = ensure_cap(ctx, mx);
@@ -789,6 +815,11 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
     std::vector<double> hb(9 * F);
     HIPCHK(hipMemcpyAsync(hb.data(), box, 9 * F * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int f = 0; f < F; ++f)
+        if (!box_usable(hb.data() + 9 * f)) {
+            ctx->err = "box not finite or degenerate (each height must be >= 0.5 A)";
+            return ADMP_ERR_ARG;
+        }
     const double rc = ctx->cfg.rc;
     CellGrid cg;
     cg.nc[0] = cg.nc[1] = cg.nc[2] = 1 << 20;
@@ -845,6 +876,7 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
         if ((s = ensure_cap(ctx, std::max(est, 1))) != ADMP_OK) return s;
     }
     ctx->nbr_cg = cg;
+    if ((s = geom_fetch(ctx)) != ADMP_OK) return s;  // read back with the build's longest row
     if ((s = launch_nbr_build(ctx)) != ADMP_OK) return s;
     ctx->frame_nbr.assign(ctx->F, 1);
     ctx->nbr_built = true;
@@ -1742,7 +1774,9 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     // energy components for the stats: one pinned copy after everything is queued
     HIPCHK(hipMemcpyAsync(ctx->h_comp, ctx->d_red, 3 * F * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(ctx->h_comp + 3 * F, ctx->d_rec, 7 * F * sizeof(double), hipMemcpyDeviceToHost, st));
+    if ((s = geom_fetch(ctx)) != ADMP_OK) return s;
     HIPCHK(hipStreamSynchronize(st));
+    if ((s = geom_check(ctx)) != ADMP_OK) return s;
     const double* comp = ctx->h_comp;
     const double* rec = ctx->h_comp + 3 * F;
     admp_status ret = ADMP_OK;
@@ -1858,7 +1892,9 @@ static admp_status disp_impl(admp_ctx* ctx, const double* c_list, const PairPara
         if (dpos) HIPCHK(hipMemcpyAsync(dpos, ctx->d_gpos, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
         if (dc) HIPCHK(hipMemcpyAsync(dc, gc, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     }
+    if (geom_fetch(ctx) != ADMP_OK) return ADMP_ERR_HIP;
     HIPCHK(hipStreamSynchronize(st));
+    { const admp_status g = geom_check(ctx); if (g != ADMP_OK) return g; }
     std::vector<double> e(F);
     for (int f = 0; f < F; ++f) e[f] = comp[f] + comp[F + f] + erec[f];
     if (energy) {
@@ -1923,7 +1959,9 @@ static admp_status admp_slater_compute_impl(admp_ctx* ctx, const double* pos, co
         if (da) hipLaunchKernelGGL(k_copy_cols, dim3(nblk(NF, 256)), dim3(256), 0, st, NF, 1, 3, 0, ctx->d_gpos_rec, da);
         if (db) hipLaunchKernelGGL(k_copy_cols, dim3(nblk(NF, 256)), dim3(256), 0, st, NF, 1, 3, 1, ctx->d_gpos_rec, db);
     }
+    if ((s = geom_fetch(ctx)) != ADMP_OK) return s;
     HIPCHK(hipStreamSynchronize(st));
+    if ((s = geom_check(ctx)) != ADMP_OK) return s;
     if (energy) {
         HIPCHK(hipMemcpyAsync(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -10,11 +10,43 @@ namespace admp {
 // 10-digit sqrt(3) of the reference's harmonic conversions (dmff/admp/multipole.py:12)
 constexpr double RT3_LIT = 1.73205080757;
 
+// A box is usable when its entries are finite and each of its three heights
+// (|det| / |a_e x a_f|, the slab width along reciprocal direction d) is at
+// least BOX_MIN_HEIGHT A: then every fractional coordinate of a position within
+// POS_MAX_ABS A stays far inside int range when it becomes a cell or grid index.
+#define BOX_MIN_HEIGHT 0.5
+#define POS_MAX_ABS 1.0e5
+#define GEOM_BAD_POS 1
+#define GEOM_BAD_BOX 2
+
+__device__ __host__ inline bool box_usable(const double* B) {
+    for (int k = 0; k < 9; ++k)
+        if (!(fabs(B[k]) < 1e300)) return false;  // NaN and infinities fail
+    const double c0[3] = {B[4] * B[8] - B[5] * B[7], B[5] * B[6] - B[3] * B[8], B[3] * B[7] - B[4] * B[6]};
+    const double c1[3] = {B[7] * B[2] - B[8] * B[1], B[8] * B[0] - B[6] * B[2], B[6] * B[1] - B[7] * B[0]};
+    const double c2[3] = {B[1] * B[5] - B[2] * B[4], B[2] * B[3] - B[0] * B[5], B[0] * B[4] - B[1] * B[3]};
+    const double det = fabs(B[0] * c0[0] + B[1] * c0[1] + B[2] * c0[2]);
+    const double n0 = sqrt(c0[0] * c0[0] + c0[1] * c0[1] + c0[2] * c0[2]);
+    const double n1 = sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]);
+    const double n2 = sqrt(c2[0] * c2[0] + c2[1] * c2[1] + c2[2] * c2[2]);
+    // height along d = det / |a_e x a_f|; compare without dividing by a zero norm
+    return det >= BOX_MIN_HEIGHT * n0 && det >= BOX_MIN_HEIGHT * n1 && det >= BOX_MIN_HEIGHT * n2 && det > 0;
+}
+
+// Boxes of every frame: inverse, grid maps and reciprocal rows.  An unusable
+// box (box_usable) is replaced by a 100 A cube so that nothing downstream
+// derives an index from it, and GEOM_BAD_BOX is set in *flags; the entry point
+// reports the error once its work has drained.
 __global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K1, int K2, int dK0, int dK1,
-                            int dK2, BoxInfo* __restrict__ out) {
+                            int dK2, BoxInfo* __restrict__ out, int* __restrict__ flags) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
-    const double* B = box + (size_t)f * 9;
+    double B[9];
+    for (int k = 0; k < 9; ++k) B[k] = box[(size_t)f * 9 + k];
+    if (!box_usable(B)) {
+        for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0) ? 100.0 : 0.0;
+        atomicOr(flags, GEOM_BAD_BOX);
+    }
     BoxInfo b;
     for (int k = 0; k < 9; ++k) b.B[k] = B[k];
     const double det = B[0] * (B[4] * B[8] - B[5] * B[7]) - B[1] * (B[3] * B[8] - B[5] * B[6]) +
@@ -42,9 +74,20 @@ __global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K
     out[f] = b;
 }
 
-__global__ void k_pack_pos(size_t n, const double* __restrict__ p, double4* __restrict__ o) {
+// Positions to double4.  A coordinate that is not finite or exceeds POS_MAX_ABS
+// is replaced by 0 and GEOM_BAD_POS is set in *flags (reported by the entry point).
+__global__ void k_pack_pos(size_t n, const double* __restrict__ p, double4* __restrict__ o, int* __restrict__ flags) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) o[i] = make_double4(p[i * 3], p[i * 3 + 1], p[i * 3 + 2], 0.0);
+    if (i >= n) return;
+    double x = p[i * 3], y = p[i * 3 + 1], z = p[i * 3 + 2];
+    const bool ok = fabs(x) <= POS_MAX_ABS && fabs(y) <= POS_MAX_ABS && fabs(z) <= POS_MAX_ABS;
+    if (!ok) {
+        x = fabs(x) <= POS_MAX_ABS ? x : 0.0;
+        y = fabs(y) <= POS_MAX_ABS ? y : 0.0;
+        z = fabs(z) <= POS_MAX_ABS ? z : 0.0;
+        atomicOr(flags, GEOM_BAD_POS);
+    }
+    o[i] = make_double4(x, y, z, 0.0);
 }
 
 // ----------------------------------------------------------------- local frames

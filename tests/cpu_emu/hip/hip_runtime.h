@@ -75,6 +75,7 @@ inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 template <class T> inline T atomicAdd(T* p, T v) { T o = *p; *p = o + v; return o; }
 inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
 inline int atomicExch(int* p, int v) { int o = *p; *p = v; return o; }
+inline int atomicOr(int* p, int v) { int o = *p; *p = o | v; return o; }
 template <class A, class B> inline auto min(A a, B b) -> typename std::common_type<A, B>::type { return a < b ? a : b; }
 template <class A, class B> inline auto max(A a, B b) -> typename std::common_type<A, B>::type { return a > b ? a : b; }
 

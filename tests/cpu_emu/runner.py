@@ -21,6 +21,7 @@ def patch_engine(lib_path):
     import dmff_amd._lib as L
     from dmff_amd.admp import engine as E
     L.LIB_PATH = lib_path
+    L.HOST_MEMORY = True  # the emulated library's device memory is host memory
     base_init = E.Engine.__init__
 
     def init(self, *args, **kwargs):
@@ -222,7 +223,58 @@ def type_reduce(precision):
             "rb": _lst(tr(torch.as_tensor(b), axis=1))}
 
 
-CASES = {"type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
+def bad_geometry(precision):
+    """Non-finite positions and degenerate boxes: every entry point returns an
+    argument error (no kernel derives an index from them) and the context still
+    evaluates the next good call correctly."""
+    import dmff_amd._lib as L
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    from tests.systems import reference_pairs, small_water_box
+    s, pos, box = small_water_box()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    kappa, K1, K2, K3 = setup_ewald_parameters(6.0, 5e-4, box)
+    eng = Engine(len(pos), 2, True, True, kappa, (K1, K2, K3), 6.0, precision=precision)
+    eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
+    eng.set_pairs(torch.as_tensor(pairs))
+    args = (s.pme.Q_local[s.map_atomtype], s.pme.pol[s.map_poltype], s.pme.thole[s.map_poltype],
+            s.pme.mScales, s.pme.pScales, s.pme.dScales)
+    out = {}
+
+    def attempt(name, fn):
+        try:
+            fn()
+            out[name] = "ok"
+        except L.ADMPError as e:
+            out[name] = str(e)
+
+    bad_nan = pos.copy()
+    bad_nan[5, 1] = np.nan
+    bad_far = pos.copy()
+    bad_far[7, 2] = 3.0e6
+    flat = box.copy()
+    flat[2] = flat[0] + flat[1] * 1e-9
+    nanbox = box.copy()
+    nanbox[1, 1] = np.inf
+    kw = dict(solver="pcg", tol=1e-8, maxiter=200, want_grad=True)
+    attempt("compute_nan_pos", lambda: eng.compute(bad_nan, box, *args, **kw))
+    attempt("compute_far_pos", lambda: eng.compute(bad_far, box, *args, **kw))
+    attempt("compute_flat_box", lambda: eng.compute(pos, flat, *args, **kw))
+    attempt("compute_inf_box", lambda: eng.compute(pos, nanbox, *args, **kw))
+    m = np.array([0.0, 0.0, 1.0, 1.0, 1.0, 1.0])
+    attempt("slater_nan_pos", lambda: eng.slater_compute(bad_nan, box, np.ones(len(pos)), np.ones(len(pos)), m))
+    attempt("build_flat_box", lambda: eng.build_neighbors(pos, flat))
+    # a neighbour build from bad positions reports at the next call that reads its lists
+    eng.build_neighbors(bad_nan, box)
+    attempt("after_bad_build", lambda: eng.get_pairs(0))
+    eng.set_pairs(torch.as_tensor(pairs))
+    E, g, U, info = eng.compute(pos, box, *args, **kw)
+    out["E_good"] = float(E[0])
+    out["pos_good"] = _lst(g["pos"][0])
+    return out
+
+
+CASES = {"bad_geometry": bad_geometry, "type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
 
 if __name__ == "__main__":
     lib, case = sys.argv[1], sys.argv[2]

@@ -191,3 +191,18 @@ def test_small_box_other_spread_variants(emu_lib, variant):
     g = run(emu_lib, "small_box", "fp64", extra_env={"ADMP_SPREAD": variant})
     ok, msgs = compare(g, o)
     assert ok, msgs
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_bad_geometry_is_an_argument_error(emu_lib, prec):
+    r = run(emu_lib, "bad_geometry", prec)
+    for k in ("compute_nan_pos", "compute_far_pos", "slater_nan_pos", "after_bad_build"):
+        assert "status 1" in r[k] and "positions" in r[k], (k, r[k])
+    for k in ("compute_flat_box", "compute_inf_box", "build_flat_box"):
+        assert "status 1" in r[k] and "box" in r[k], (k, r[k])
+    s, pos, box = small_water_box()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    o, _ = oracle_eval(s, pos, box, pairs, 6.0, True, solver="cg", tol=1e-9)
+    tol_e, tol_g = (1e-9, 1e-6) if prec == "fp64" else (2e-5, 1e-4)
+    assert abs(r["E_good"] - o["E"]) <= tol_e * abs(o["E"])
+    assert rel_rms(np.asarray(r["pos_good"]), o["pos"]) <= tol_g
