@@ -254,17 +254,17 @@ WATER_DENSITY_NM3 = 33.43
 
 
 def water_spec(key, n_waters, seed, rc=8.0):
-    """A polarisable water box of n_waters at liquid density; the cutoff is
-    capped below half the box length so the minimum-image pair set stays
-    well defined for the smallest boxes."""
+    """A polarisable water box of n_waters at liquid density with cutoff rc.
+    Boxes shorter than 2 rc keep rc: the pair set is then the reference's
+    single minimum image, each pair once at its rounded-fractional image
+    (dmff/admp/spatial.py:13-32), and kappa follows rc as in pme.py:460-466."""
     from .water import make_water_system
     L_nm = (n_waters / WATER_DENSITY_NM3) ** (1.0 / 3.0)
     n_side = int(math.ceil(n_waters ** (1.0 / 3.0) - 1e-9))
     S = make_water_system(n_waters, L_nm, seed, polarizable=True, n_side=n_side)
     box = S.box_nm * 10.0
-    rc_eff = min(rc, 0.5 * L_nm * 10.0 - 0.05)
     spec = SystemSpec(key, S.n_atoms, S.axis_types, S.axis_idx, S.bonded, S.Q_local, S.pol, S.thole,
-                      S.mScales, S.pScales, S.dScales, rc_eff)
+                      S.mScales, S.pScales, S.dScales, rc)
     return spec, S.positions_nm * 10.0, box
 
 

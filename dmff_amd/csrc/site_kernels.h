@@ -39,6 +39,7 @@ __device__ __host__ inline bool box_usable(const double* B) {
 // reports the error once its work has drained.
 __global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K1, int K2, int dK0, int dK1,
                             int dK2, BoxInfo* __restrict__ out, int* __restrict__ flags) {
+#pragma clang fp contract(off)
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     double B[9];
@@ -52,6 +53,9 @@ __global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K
     const double det = B[0] * (B[4] * B[8] - B[5] * B[7]) - B[1] * (B[3] * B[8] - B[5] * B[6]) +
                        B[2] * (B[3] * B[7] - B[4] * B[6]);
     const double id = 1.0 / det;
+    // inverse: elementwise for a diagonal box (what an LU inverse gives there),
+    // cofactors over the determinant otherwise; the oracle's brute-force pair
+    // search forms the same values
     b.Binv[0] = (B[4] * B[8] - B[5] * B[7]) * id;
     b.Binv[1] = (B[2] * B[7] - B[1] * B[8]) * id;
     b.Binv[2] = (B[1] * B[5] - B[2] * B[4]) * id;
@@ -61,6 +65,13 @@ __global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K
     b.Binv[6] = (B[3] * B[7] - B[4] * B[6]) * id;
     b.Binv[7] = (B[1] * B[6] - B[0] * B[7]) * id;
     b.Binv[8] = (B[0] * B[4] - B[1] * B[3]) * id;
+    b.ortho = (B[1] == 0 && B[2] == 0 && B[3] == 0 && B[5] == 0 && B[6] == 0 && B[7] == 0) ? 1 : 0;
+    if (b.ortho) {
+        for (int k = 0; k < 9; ++k) b.Binv[k] = 0.0;
+        b.Binv[0] = 1.0 / B[0];
+        b.Binv[4] = 1.0 / B[4];
+        b.Binv[8] = 1.0 / B[8];
+    }
     b.V = det;
     const int K[3] = {K0, K1, K2}, dK[3] = {dK0, dK1, dK2};
     for (int j = 0; j < 3; ++j)
@@ -69,7 +80,6 @@ __global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K
             b.dA[j * 3 + c] = dK[j] * b.Binv[c * 3 + j];
             b.kb[c * 3 + j] = 2.0 * PI * b.Binv[c * 3 + j];
         }
-    b.ortho = (B[1] == 0 && B[2] == 0 && B[3] == 0 && B[5] == 0 && B[6] == 0 && B[7] == 0) ? 1 : 0;
     b.pad = 0;
     out[f] = b;
 }

@@ -914,6 +914,25 @@ def slater_sr_energy(positions, box, pairs, mScales, a_list, b_list):
 # Neighbour list: exact FP64 brute force with a fixed operation order
 # (pair-set semantics of dmff/common/nblist.py:27-55: i<j, min-image d < rc)
 # ---------------------------------------------------------------------------
+def box_inverse(B):
+    """Inverse of a box (rows = lattice vectors) as a fixed formula: elementwise
+    reciprocals for a diagonal box (what an LU inverse such as jnp.linalg.inv,
+    dmff/admp/spatial.py:28, returns there) and the cofactor matrix times
+    1/det otherwise.  An LU inverse of a general box may differ from this in
+    the last bit; the pair set can then differ only for pairs within a few ulp
+    of rc, so the brute force fixes this formula to be comparable bit for bit."""
+    B = np.asarray(B, dtype=np.float64)
+    if B[0, 1] == 0 and B[0, 2] == 0 and B[1, 0] == 0 and B[1, 2] == 0 and B[2, 0] == 0 and B[2, 1] == 0:
+        return np.diag([1.0 / B[0, 0], 1.0 / B[1, 1], 1.0 / B[2, 2]])
+    b = B.reshape(-1)
+    det = b[0] * (b[4] * b[8] - b[5] * b[7]) - b[1] * (b[3] * b[8] - b[5] * b[6]) + b[2] * (b[3] * b[7] - b[4] * b[6])
+    idt = 1.0 / det
+    inv = [(b[4] * b[8] - b[5] * b[7]) * idt, (b[2] * b[7] - b[1] * b[8]) * idt, (b[1] * b[5] - b[2] * b[4]) * idt,
+           (b[5] * b[6] - b[3] * b[8]) * idt, (b[0] * b[8] - b[2] * b[6]) * idt, (b[2] * b[3] - b[0] * b[5]) * idt,
+           (b[3] * b[7] - b[4] * b[6]) * idt, (b[1] * b[6] - b[0] * b[7]) * idt, (b[0] * b[4] - b[1] * b[3]) * idt]
+    return np.asarray(inv, dtype=np.float64).reshape(3, 3)
+
+
 def brute_force_pairs(positions, box, rc):
     """All i<j with minimum-image distance < rc.  The squared distance is formed
     with single-rounded IEEE operations in the order
@@ -923,7 +942,7 @@ def brute_force_pairs(positions, box, rc):
     must agree bit for bit.  Returns an (Np,2) int64 array sorted by (i, j)."""
     pos = np.asarray(positions, dtype=np.float64)
     B = np.asarray(box, dtype=np.float64)
-    Bi = np.linalg.inv(B)
+    Bi = box_inverse(B)
     n = pos.shape[0]
     rc2 = float(rc) * float(rc)
     out = []

@@ -153,6 +153,12 @@ def neighbors(precision):
         e.set_topology(None, None, {})
         e.build_neighbors(p, np.eye(3) * 11.0)
         out[f"random_{rc}"] = e.get_pairs(0).numpy().astype(np.int64).tolist()
+    from tests.systems import neighbour_cases
+    for name, p, b, rc in neighbour_cases():
+        e = Engine(len(p), 0, False, False, 0.0, (0, 0, 0), rc)
+        e.set_topology(None, None, {})
+        e.build_neighbors(p, b)
+        out[name] = e.get_pairs(0).numpy().astype(np.int64).tolist()
     return out
 
 

@@ -72,3 +72,26 @@ def small_water_box(ff="fullpol_forcefield.xml"):
     s.map_atomtype = s.map_atomtype[idx]
     s.map_poltype = s.map_poltype[idx]
     return s, pos, np.eye(3) * 13.2
+
+
+TRICLINIC_BOX = np.array([[20.0, 0.0, 0.0], [5.0, 19.0, 0.0], [-4.0, 3.0, 18.0]])
+
+
+def neighbour_cases():
+    """Extra neighbour-list cases beyond the water boxes: (name, positions, box, rc).
+      small_rc8   192 random points in a 12.4 A cube at rc 8 A (rc > L/2: the
+                  reference's single minimum image, one cell a side)
+      tric_*      400 points in a triclinic box (rows a, b, c), rc 6 A and
+                  rc 9.4 A (above half the smallest height, 17.1 A), with a pair at
+                  rc to 1e-10 A."""
+    rng = np.random.default_rng(17)
+    out = []
+    p = rng.uniform(0, 12.4, size=(192, 3))
+    out.append(("small_rc8", p, np.eye(3) * 12.4, 8.0))
+    s = rng.uniform(0, 1, size=(400, 3))
+    p = s @ TRICLINIC_BOX
+    d = np.array([1.0, 2.0, -2.0]) / 3.0
+    p[1] = p[0] + d * 5.9999999999
+    for rc in (6.0, 9.4):
+        out.append((f"tric_{rc}", p, TRICLINIC_BOX.copy(), rc))
+    return out

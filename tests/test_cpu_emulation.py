@@ -130,6 +130,11 @@ def test_neighbour_lists_bit_exact(emu_lib):
         ref = sorted_pairs(reference_pairs(p, np.eye(3) * 11.0, rc, {}))
         got = sorted_pairs(r[f"random_{rc}"])
         assert ref.shape == got.shape and np.array_equal(ref, got), rc
+    from tests.systems import neighbour_cases
+    for name, p, b, rc in neighbour_cases():
+        ref = sorted_pairs(reference_pairs(p, b, rc, {}))
+        got = sorted_pairs(r[name])
+        assert len(ref) > 0 and ref.shape == got.shape and np.array_equal(ref, got), name
 
 
 @pytest.mark.parametrize("K,prec", [((104, 104, 104), "fp64"), ((104, 104, 104), "mixed"), ((6, 7, 8), "fp64"),

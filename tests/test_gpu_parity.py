@@ -216,6 +216,18 @@ def test_neighbor_list_bit_exact():
         a = ref[np.lexsort((ref[:, 1], ref[:, 0]))]
         b = got[np.lexsort((got[:, 1], got[:, 0]))]
         assert np.array_equal(a, b), (len(a), len(b))
+    # rc above half the box (single minimum image) and triclinic boxes
+    from tests.systems import neighbour_cases
+    for name, p, bx, rc in neighbour_cases():
+        ref = reference_pairs(p, bx, rc, {})
+        e = Engine(len(p), 0, False, False, 0.0, (0, 0, 0), rc, device=DEV)
+        e.set_topology(None, None, {})
+        e.build_neighbors(p, bx)
+        got = e.get_pairs(0).cpu().numpy().astype(np.int64)
+        a = ref[np.lexsort((ref[:, 1], ref[:, 0]))]
+        b = got[np.lexsort((got[:, 1], got[:, 0]))]
+        print(f"{name}: oracle {len(a)} pairs, gpu {len(b)} pairs", flush=True)
+        assert len(a) > 0 and a.shape == b.shape and np.array_equal(a, b), name
 
 
 def test_mixed_precision_error_bounds():
