@@ -26,8 +26,12 @@ SYMBOLS = [
     "admp_version", "admp_ctx_create", "admp_ctx_destroy", "admp_last_error", "admp_set_stream",
     "admp_set_topology", "admp_set_pairs", "admp_build_neighbors", "admp_get_pairs", "admp_compute",
     "admp_disp_compute", "admp_slater_compute", "admp_profile", "admp_profile_get", "admp_memory_bytes",
-    "admp_type_reduce",
+    "admp_type_reduce", "admp_pairwise_compute",
 ]
+
+# admp_sr_kind and each kind's per-site parameter count
+SR_SLATER, SR_TT_QQ_C6, SR_TT_QQ, SR_SLATER_DAMP = 1, 2, 3, 4
+SR_NPRM = {SR_SLATER: 2, SR_TT_QQ_C6: 4, SR_TT_QQ: 2, SR_SLATER_DAMP: 4}
 
 
 class Config(ctypes.Structure):
@@ -88,6 +92,7 @@ def load():
     lib.admp_profile_get.argtypes = [P, i32, ctypes.POINTER(d), ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_char_p)]
     lib.admp_memory_bytes.argtypes = [P, ctypes.POINTER(i64)]
     lib.admp_type_reduce.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int32, P, P]
+    lib.admp_pairwise_compute.argtypes = [P, i32, P, P, P, i32, P, i32, i32, P, P, P, P]
     for name in SYMBOLS:
         getattr(lib, name).restype = ctypes.c_int if name not in ("admp_version", "admp_last_error") else \
             getattr(lib, name).restype

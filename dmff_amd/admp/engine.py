@@ -227,3 +227,27 @@ class Engine:
                                           _ptr(g.get("pos")), _ptr(g.get("box")), _ptr(g.get("a")), _ptr(g.get("b")))
         L.check(self.h, st, "admp_slater_compute")
         return E, g
+
+    def pairwise_compute(self, kind, pos, box, prm, mScales, want_grad=True):
+        """generate_pairwise_interaction kernels (dmff/admp/pairwise.py:98-169) over the
+        context's pair lists.  kind: _lib.SR_*; prm (N, n_prm) per-site parameters in
+        the kernel's order.  Returns E (F,) and {pos, box, prm} gradients."""
+        F, N = self.n_frames, self.n_atoms
+        dev = self.device
+        npr = L.SR_NPRM[int(kind)]
+        pos = _dev(pos, dev).reshape(F, N, 3)
+        box = _dev(box, dev).reshape(F, 3, 3)
+        prm = _dev(prm, dev).reshape(N, npr)
+        (ms,) = L.check_scales(mScales)[:1]
+        E = torch.empty(F, dtype=F64, device=dev)
+        g = {}
+        if want_grad:
+            g = {"pos": torch.empty((F, N, 3), dtype=F64, device=dev),
+                 "box": torch.empty((F, 3, 3), dtype=F64, device=dev),
+                 "prm": torch.empty((F, N, npr), dtype=F64, device=dev)}
+        self._stream()
+        st = self.lib.admp_pairwise_compute(self.h, int(kind), _ptr(pos), _ptr(box), _ptr(prm), npr,
+                                            ms.ctypes.data_as(ctypes.c_void_p), len(ms), int(want_grad), _ptr(E),
+                                            _ptr(g.get("pos")), _ptr(g.get("box")), _ptr(g.get("prm")))
+        L.check(self.h, st, "admp_pairwise_compute")
+        return E, g

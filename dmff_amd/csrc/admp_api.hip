@@ -1846,7 +1846,7 @@ static admp_status disp_impl(admp_ctx* ctx, const double* c_list, const PairPara
     p2.kappa = kappa;
     { ProfScope ps_(ctx, 10); hipLaunchKernelGGL((k_pair_short<R, 0>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
                        ctx->cap, ctx->d_pos, ctx->d_box, c_list, ctx->d_nbr, ctx->d_ncnt, p2, c.disp_pmax, ctx->d_e_atom,
-                       ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec /* (F,N,3) dE/dc */); }
+                       ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec /* (F,N,3) dE/dc */, 3); }
     double* gc = ctx->d_gpos_rec;
     HIPCHK(hipMemsetAsync(ctx->d_red9b, 0, F * 9 * sizeof(double), st));
     std::vector<double> erec(F, 0.0);
@@ -1918,6 +1918,66 @@ static admp_status admp_disp_compute_impl(admp_ctx* ctx, const double* pos, cons
     return disp_impl<double>(ctx, c_list, pp, want_grad, energy, dpos, dbox, dc);
 }
 
+template <class R, int KIND>
+static void launch_pair_short(admp_ctx* ctx, const double* prm, const PairParams& pp) {
+    const int N = ctx->N, F = ctx->F;
+    ProfScope ps_(ctx, 10);
+    hipLaunchKernelGGL((k_pair_short<R, KIND>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0,
+                       ctx->stream, N, ctx->cap, ctx->d_pos, ctx->d_box, prm, ctx->d_nbr, ctx->d_ncnt, pp, 0,
+                       ctx->d_e_atom, ctx->d_gpos, ctx->d_vir_atom, ctx->d_gq, sr_nprm(KIND));
+}
+
+// generate_pairwise_interaction(kernel) (pairwise.py:47-95) on the context's
+// lists: per-site energies, forces, virial and parameter gradients from one
+// k_pair_short launch, then fixed-order reductions.
+static admp_status pairwise_impl(admp_ctx* ctx, int kind, const double* prm, const PairParams& pp, int want_grad,
+                                 double* energy, double* dpos, double* dbox, double* dprm) {
+    const int N = ctx->N, F = ctx->F;
+    const size_t NF = (size_t)N * F;
+    hipStream_t st = ctx->stream;
+    const bool f32 = ctx->cfg.precision == ADMP_MIXED;
+    switch (kind) {
+        case SR_SLATER: f32 ? launch_pair_short<float, SR_SLATER>(ctx, prm, pp) : launch_pair_short<double, SR_SLATER>(ctx, prm, pp); break;
+        case SR_TT_QQ_C6: f32 ? launch_pair_short<float, SR_TT_QQ_C6>(ctx, prm, pp) : launch_pair_short<double, SR_TT_QQ_C6>(ctx, prm, pp); break;
+        case SR_TT_QQ: f32 ? launch_pair_short<float, SR_TT_QQ>(ctx, prm, pp) : launch_pair_short<double, SR_TT_QQ>(ctx, prm, pp); break;
+        case SR_SLATER_DAMP: f32 ? launch_pair_short<float, SR_SLATER_DAMP>(ctx, prm, pp) : launch_pair_short<double, SR_SLATER_DAMP>(ctx, prm, pp); break;
+        default: ctx->err = "unknown short-range kind"; return ADMP_ERR_ARG;
+    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_atom, ctx->d_red9a, 0);
+    if (energy) HIPCHK(hipMemcpyAsync(energy, ctx->d_red9a, F * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (want_grad) {
+        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9b, 0);
+        if (dbox)
+            hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9b,
+                               (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, 1, 1, 1, 0, dbox, 0);
+        if (dpos) HIPCHK(hipMemcpyAsync(dpos, ctx->d_gpos, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (dprm)
+            HIPCHK(hipMemcpyAsync(dprm, ctx->d_gq, NF * sr_nprm(kind) * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHK(hipGetLastError());
+    admp_status s;
+    if ((s = geom_fetch(ctx)) != ADMP_OK) return s;
+    HIPCHK(hipStreamSynchronize(st));
+    return geom_check(ctx);
+}
+
+static admp_status admp_pairwise_compute_impl(admp_ctx* ctx, int32_t kind, const double* pos, const double* box,
+                                             const double* prm, int32_t n_prm, const double* mScales,
+                                             int32_t n_scales, int32_t want_grad, double* energy, double* dpos,
+                                             double* dbox, double* dprm) {
+    if (!ctx || !pos || !box || !prm) return ADMP_ERR_ARG;
+    if (kind < 1 || kind > SR_KIND_MAX) { ctx->err = "unknown short-range kind"; return ADMP_ERR_ARG; }
+    if (n_prm != sr_nprm(kind)) { ctx->err = "n_prm does not match the kind's parameter count"; return ADMP_ERR_ARG; }
+    { const admp_status ns = nbr_settle(ctx); if (ns != ADMP_OK) return ns; }
+    if (!all_frames_have_pairs(ctx)) { ctx->err = "no pair list for every frame"; return ADMP_ERR_STATE; }
+    if (n_scales < 1 || n_scales > 8) { ctx->err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
+    admp_status s = upload_geometry(ctx, pos, box);
+    if (s != ADMP_OK) return s;
+    PairParams pp = make_pp(mScales, nullptr, nullptr, n_scales, 0.0);
+    return pairwise_impl(ctx, kind, prm, pp, want_grad, energy, dpos, dbox, dprm);
+}
+
 static admp_status admp_slater_compute_impl(admp_ctx* ctx, const double* pos, const double* box, const double* a,
                                            const double* b, const double* mScales, int32_t n_scales, int32_t want_grad,
                                            double* energy, double* dpos, double* dbox, double* da, double* db) {
@@ -1931,40 +1991,15 @@ static admp_status admp_slater_compute_impl(admp_ctx* ctx, const double* pos, co
     const size_t NF = (size_t)N * F;
     hipStream_t st = ctx->stream;
     PairParams pp = make_pp(mScales, nullptr, nullptr, n_scales, 0.0);
-    // interleave (A, B) per site into t1 (N x 2 <= NF x 3)
-    std::vector<double> ha(N), hb(N), ab(2 * N);
-    HIPCHK(hipMemcpyAsync(ha.data(), a, N * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(hb.data(), b, N * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    for (int i = 0; i < N; ++i) { ab[2 * i] = ha[i]; ab[2 * i + 1] = hb[i]; }
-    HIPCHK(hipMemcpyAsync(ctx->d_t1, ab.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, st));
-    if (ctx->cfg.precision == ADMP_MIXED)
-        { ProfScope ps_(ctx, 10); hipLaunchKernelGGL((k_pair_short<float, 1>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st,
-                           N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_t1, ctx->d_nbr, ctx->d_ncnt, pp, 0, ctx->d_e_atom,
-                           ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec); }
-    else
-        { ProfScope ps_(ctx, 10); hipLaunchKernelGGL((k_pair_short<double, 1>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st,
-                           N, ctx->cap, ctx->d_pos, ctx->d_box, ctx->d_t1, ctx->d_nbr, ctx->d_ncnt, pp, 0, ctx->d_e_atom,
-                           ctx->d_gpos, ctx->d_vir_atom, ctx->d_gpos_rec); }
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1, F), dim3(RED_BLOCK), 0, st, N, 1, ctx->d_e_atom, ctx->d_red9a, 0);
-    std::vector<double> e(F);
-    HIPCHK(hipMemcpyAsync(e.data(), ctx->d_red9a, F * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (want_grad) {
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9b, 0);
-        if (dbox)
-            hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9b,
-                               (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, 1, 1, 1, 0, dbox, 0);
-        if (dpos) HIPCHK(hipMemcpyAsync(dpos, ctx->d_gpos, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        if (da) hipLaunchKernelGGL(k_copy_cols, dim3(nblk(NF, 256)), dim3(256), 0, st, NF, 1, 3, 0, ctx->d_gpos_rec, da);
-        if (db) hipLaunchKernelGGL(k_copy_cols, dim3(nblk(NF, 256)), dim3(256), 0, st, NF, 1, 3, 1, ctx->d_gpos_rec, db);
-    }
-    if ((s = geom_fetch(ctx)) != ADMP_OK) return s;
-    HIPCHK(hipStreamSynchronize(st));
-    if ((s = geom_check(ctx)) != ADMP_OK) return s;
-    if (energy) {
-        HIPCHK(hipMemcpyAsync(energy, e.data(), F * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+    // (A, B) per site interleaved into t1 (N x 2 <= NF x 3)
+    hipLaunchKernelGGL(k_interleave2, dim3(nblk(N, 256)), dim3(256), 0, st, N, a, b, ctx->d_t1);
+    double* g2 = want_grad && (da || db) ? ctx->d_t2 : nullptr;  // (F,N,2) dE/d(A,B)
+    if ((s = pairwise_impl(ctx, SR_SLATER, ctx->d_t1, pp, want_grad, energy, dpos, dbox, g2)) != ADMP_OK) return s;
+    if (g2) {
+        if (da) hipLaunchKernelGGL(k_copy_cols, dim3(nblk(NF, 256)), dim3(256), 0, st, NF, 1, 2, 0, g2, da);
+        if (db) hipLaunchKernelGGL(k_copy_cols, dim3(nblk(NF, 256)), dim3(256), 0, st, NF, 1, 2, 1, g2, db);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
     }
     return ADMP_OK;
 }
@@ -2015,6 +2050,14 @@ extern "C" admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, con
                                            const double* b, const double* mScales, int32_t n_scales, int32_t want_grad,
                                            double* energy, double* dpos, double* dbox, double* da, double* db) {
     ADMP_WRAP(admp_slater_compute_impl(ctx, pos, box, a, b, mScales, n_scales, want_grad, energy, dpos, dbox, da, db));
+}
+
+extern "C" admp_status admp_pairwise_compute(admp_ctx* ctx, int32_t kind, const double* pos, const double* box,
+                                             const double* prm, int32_t n_prm, const double* mScales,
+                                             int32_t n_scales, int32_t want_grad, double* energy, double* dpos,
+                                             double* dbox, double* dprm) {
+    ADMP_WRAP(admp_pairwise_compute_impl(ctx, kind, pos, box, prm, n_prm, mScales, n_scales, want_grad, energy, dpos,
+                                         dbox, dprm));
 }
 
 extern "C" admp_status admp_type_reduce(const double* per_site, int32_t n_frames, int32_t n_sites, int32_t d,

@@ -17,6 +17,7 @@
 // written once, so results are deterministic.
 #pragma once
 #include "admp_common.h"
+#include "sr_kernels.h"
 
 namespace admp {
 
@@ -613,18 +614,18 @@ namespace admp {
 // Short-range scalar pair terms on the same full lists:
 //   MODE 0: dispersion real space, sum_p (m + g_p(kappa^2 r^2) - 1) c_i^p c_j^p / r^p
 //           (disp_pme.py:180-226; g_p of :229-261)
-//   MODE 1: Slater-ISA, m A_i A_j (1/3 (Br)^2 + Br + 1) e^{-Br}, B = sqrt(B_i B_j)
-//           (pairwise.py:156-169)
-// prm: MODE 0 -> c (N,3); MODE 1 -> (A, B) interleaved (N,2).
-// Outputs per site: energy (half of each pair), dE/dr_i, virial (half), and
-// dE/dparams of site i (3 for dispersion, 2 for Slater).
+//   MODE SR_*: the generate_pairwise_interaction kernels of sr_kernels.h
+//           (pairwise.py:98-169)
+// prm: MODE 0 -> c (N,3); SR kinds -> (N, sr_nprm(MODE)) in the kernel's
+// parameter order.  Outputs per site: energy (half of each pair), dE/dr_i,
+// virial (half), and dE/dparams of site i (row stride gstride).
 // ---------------------------------------------------------------------------
 template <class R, int MODE>
 __global__ void __launch_bounds__(PAIR_BLOCK)
 k_pair_short(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
              const double* __restrict__ prm, const int* __restrict__ nbr, const int* __restrict__ ncnt,
              PairParams pp, int pmax, double* __restrict__ e_atom, double* __restrict__ g_pos,
-             double* __restrict__ vir_atom, double* __restrict__ g_prm) {
+             double* __restrict__ vir_atom, double* __restrict__ g_prm, int gstride) {
     const int f = blockIdx.y;
     const int gt = blockIdx.x * PAIR_BLOCK + threadIdx.x;
     const int i = gt / TPA, sub = gt % TPA;
@@ -633,11 +634,12 @@ k_pair_short(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __r
     const BoxInfo& bx = boxes[f];
     const size_t fo = (size_t)f * N;
     const double4 pi_ = pos[fo + ii];
-    const int np = MODE == 0 ? 3 : 2;
-    R pI[3] = {0, 0, 0};
+    constexpr int np = MODE == 0 ? 3 : sr_nprm(MODE);
+    R pI[SR_NPRM_MAX] = {0, 0, 0, 0};
     for (int k = 0; k < np; ++k) pI[k] = (R)prm[(size_t)ii * np + k];
     const R kap2 = (R)(pp.kappa * pp.kappa);
-    double E = 0, gx = 0, gy = 0, gz = 0, gp0 = 0, gp1 = 0, gp2 = 0;
+    double E = 0, gx = 0, gy = 0, gz = 0;
+    double gp[SR_NPRM_MAX] = {0, 0, 0, 0};
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int cnt = active ? ncnt[fo + ii] : 0;
     const int* nl = nbr + (fo + ii) * (size_t)cap;
@@ -650,8 +652,9 @@ k_pair_short(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __r
         const double rd = sqrt(dx * dx + dy * dy + dz * dz);
         const R r = (R)rd, rinv = (R)(1.0 / rd);
         const R m = (R)pp.ms[scale_index(nb, pp.nscale)];
-        R e = 0, dEdr = 0, d0 = 0, d1 = 0, d2 = 0;
-        if (MODE == 0) {
+        R e = 0, dEdr = 0;
+        R dI[SR_NPRM_MAX] = {0, 0, 0, 0};
+        if constexpr (MODE == 0) {
             const R r2 = r * r, x2 = kap2 * r2, ex = r_exp<R>(-x2);
             const R x4 = x2 * x2;
             const R g6 = ((R)1 + x2 + (R)0.5 * x4) * ex;
@@ -666,51 +669,42 @@ k_pair_short(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __r
             R s6 = m + g6 - (R)1;
             e = s6 * pI[0] * cj6 * ir6;
             dEdr = pI[0] * cj6 * (dg6 * dx2dr * ir6 - (R)6 * s6 * ir6 * rinv);
-            d0 = s6 * cj6 * ir6;
+            dI[0] = s6 * cj6 * ir6;
             if (pmax >= 8) {
                 R s8 = m + g8 - (R)1;
                 e += s8 * pI[1] * cj8 * ir8;
                 dEdr += pI[1] * cj8 * (dg8 * dx2dr * ir8 - (R)8 * s8 * ir8 * rinv);
-                d1 = s8 * cj8 * ir8;
+                dI[1] = s8 * cj8 * ir8;
             }
             if (pmax >= 10) {
                 R s10 = m + g10 - (R)1;
                 e += s10 * pI[2] * cj10 * ir10;
                 dEdr += pI[2] * cj10 * (dg10 * dx2dr * ir10 - (R)10 * s10 * ir10 * rinv);
-                d2 = s10 * cj10 * ir10;
+                dI[2] = s10 * cj10 * ir10;
             }
         } else {
-            const R Aj = (R)prm[(size_t)j * 2], Bj = (R)prm[(size_t)j * 2 + 1];
-            const R B = sqrt(pI[1] * Bj);
-            const R br = B * r;
-            const R ex = r_exp<R>(-br);
-            const R P = br * br * (R)(1.0 / 3.0) + br + (R)1;
-            const R dPe = -(br * (R)(1.0 / 3.0)) * ((R)1 + br) * ex;   // d(P e^-x)/dx
-            const R AA = pI[0] * Aj;
-            e = m * AA * P * ex;
-            dEdr = m * AA * B * dPe;
-            d0 = m * Aj * P * ex;
-            // dE/dB_i = m AA r dPe * dB/dB_i,  dB/dB_i = B_j / (2 B)
-            d1 = B > (R)0 ? m * AA * r * dPe * Bj / ((R)2 * B) : (R)0;
+            R pJ[SR_NPRM_MAX] = {0, 0, 0, 0}, dJ[SR_NPRM_MAX];
+            for (int q = 0; q < np; ++q) pJ[q] = (R)prm[(size_t)j * np + q];
+            sr_pair<R, MODE>(r, m, pI, pJ, e, dEdr, dI, dJ);
         }
         const R fx = dEdr * (R)dx * rinv, fy = dEdr * (R)dy * rinv, fz = dEdr * (R)dz * rinv;
         E += (double)e;
         gx += (double)fx; gy += (double)fy; gz += (double)fz;
-        gp0 += (double)d0; gp1 += (double)d1; gp2 += (double)d2;
+        for (int q = 0; q < np; ++q) gp[q] += (double)dI[q];
         const double nn[3] = {n0, n1, n2}, ff[3] = {(double)fx, (double)fy, (double)fz};
         for (int p = 0; p < 3; ++p)
             for (int q = 0; q < 3; ++q) v[p * 3 + q] -= nn[p] * ff[q];
     }
     E = group_sum(E, TPA);
     gx = group_sum(gx, TPA); gy = group_sum(gy, TPA); gz = group_sum(gz, TPA);
-    gp0 = group_sum(gp0, TPA); gp1 = group_sum(gp1, TPA); gp2 = group_sum(gp2, TPA);
+    for (int q = 0; q < np; ++q) gp[q] = group_sum(gp[q], TPA);
     for (int k = 0; k < 9; ++k) v[k] = group_sum(v[k], TPA);
     if (active && sub == 0) {
         const size_t a = fo + i;
         e_atom[a] = 0.5 * E;
         g_pos[a * 3] = gx; g_pos[a * 3 + 1] = gy; g_pos[a * 3 + 2] = gz;
         for (int k = 0; k < 9; ++k) vir_atom[a * 9 + k] = 0.5 * v[k];
-        g_prm[a * 3] = gp0; g_prm[a * 3 + 1] = gp1; g_prm[a * 3 + 2] = gp2;
+        for (int q = 0; q < np; ++q) g_prm[a * gstride + q] = gp[q];
     }
 }
 

@@ -623,6 +623,14 @@ __global__ void k_add3(size_t n, const double* __restrict__ x, const double* __r
 }
 
 // strided copy of one column-block: out[a*w + k] = in[a*win + off + k]
+// o[2i] = a[i], o[2i+1] = b[i]
+__global__ void k_interleave2(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    o[2 * (size_t)i] = a[i];
+    o[2 * (size_t)i + 1] = b[i];
+}
+
 __global__ void k_copy_cols(size_t n, int w, int win, int off, const double* __restrict__ in, double* __restrict__ out) {
     size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;

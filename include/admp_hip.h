@@ -15,6 +15,7 @@
  *   dmff/admp/pme.py:342-423  optimize_Uind        admp_compute (solver options)
  *   dmff/admp/disp_pme.py:13-77 ADMPDispPmeForce.get_energy  admp_disp_compute
  *   dmff/admp/pairwise.py:47-95,156-169 slater_sr_kernel       admp_slater_compute
+ *   dmff/admp/pairwise.py:47-169 generate_pairwise_interaction   admp_pairwise_compute
  *
  * Conventions
  *   - Units are the reference backend's: Angstrom, e, kJ/mol (DIELECTRIC =
@@ -168,6 +169,28 @@ admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* 
                                 const double* a, const double* b, const double* mScales,
                                 int32_t n_scales, int32_t want_grad, double* energy,
                                 double* dpos, double* dbox, double* da, double* db);
+
+/* Short-range pair kernels of generate_pairwise_interaction (dmff/admp/pairwise.py:47-95)
+ * and their per-site parameters, in the kernel's argument order:
+ *   ADMP_SR_SLATER       slater_sr_kernel (pairwise.py:156-169)           (A, B)
+ *   ADMP_SR_TT_QQ_C6     TT_damping_qq_c6_kernel (pairwise.py:98-117)     (A, B, Q, C)
+ *   ADMP_SR_TT_QQ        TT_damping_qq_kernel (pairwise.py:120-128)       (B, Q)
+ *   ADMP_SR_SLATER_DAMP  slater_disp_damping_kernel (pairwise.py:131-153) (B, C6, C8, C10)
+ * Parameters are in the kernels' own units (the generators' conversions,
+ * dmff/generators/admp.py:190-199,508-509,640-643,772-773, happen in the host layer). */
+typedef enum {
+    ADMP_SR_SLATER = 1, ADMP_SR_TT_QQ_C6 = 2, ADMP_SR_TT_QQ = 3, ADMP_SR_SLATER_DAMP = 4
+} admp_sr_kind;
+
+/* pair_int(positions, box, pairs, mScales, *atomic_params) of
+ * generate_pairwise_interaction (pairwise.py:47-95) for every frame, over the
+ * context's pair lists (admp_set_pairs / admp_build_neighbors): energy (F,) and,
+ * with want_grad, dpos (F,N,3), dbox (F,3,3) and dprm (F,N,n_prm), the gradient
+ * with respect to each site's parameters.  prm (N, n_prm) device, n_prm the
+ * kind's parameter count.  Synchronises the stream once. */
+admp_status admp_pairwise_compute(admp_ctx* ctx, int32_t kind, const double* pos, const double* box,
+                                  const double* prm, int32_t n_prm, const double* mScales, int32_t n_scales,
+                                  int32_t want_grad, double* energy, double* dpos, double* dbox, double* dprm);
 
 /* Instrumentation (no counterpart in the reference): per-kernel-family timing with
  * HIP events recorded around the library's launches on the context stream.

@@ -896,18 +896,85 @@ def energy_disp_pme(positions, box, pairs, c_list, mScales, kappa, K, pmax, lpme
 # ---------------------------------------------------------------------------
 # Slater-ISA short range  (dmff/admp/pairwise.py:47-95,156-169)
 # ---------------------------------------------------------------------------
-def slater_sr_energy(positions, box, pairs, mScales, a_list, b_list):
-    """generate_pairwise_interaction(slater_sr_kernel): sum m*Ai*Aj*(1/3 (Br)^2+Br+1) e^-Br."""
-    pairs = _live_pairs(pairs)
-    box_inv = torch.linalg.inv(box)
-    i, j, nb = pairs[:, 0], pairs[:, 1], pairs[:, 2]
-    m = _t(mScales)[nb - 1]
-    dr = torch.linalg.norm(pbc_shift(positions[i] - positions[j], box, box_inv), dim=1)
-    b = torch.sqrt(b_list[i] * b_list[j])
-    a = a_list[i] * a_list[j]
+def generate_pairwise_interaction(pair_int_kernel):
+    """dmff/admp/pairwise.py:47-95: pair_int(positions, box, pairs, mScales,
+    *atomic_params) = sum over live pairs (i < j, pairs.py regularisation) of
+    kernel(dr, m, p1_i, p1_j, p2_i, p2_j, ...), m = mScales[nbond - 1] with the
+    Python wrap for nbond = 0."""
+    def pair_int(positions, box, pairs, mScales, *atomic_params):
+        pairs = _live_pairs(pairs)
+        box_inv = torch.linalg.inv(box)
+        i, j, nb = pairs[:, 0], pairs[:, 1], pairs[:, 2]
+        m = _t(mScales)[nb - 1]
+        dr = torch.linalg.norm(pbc_shift(positions[i] - positions[j], box, box_inv), dim=1)
+        args = []
+        for prm in atomic_params:
+            args += [prm[i], prm[j]]
+        return torch.sum(pair_int_kernel(dr, m, *args))
+    return pair_int
+
+
+def slater_sr_kernel(dr, m, ai, aj, bi, bj):
+    """dmff/admp/pairwise.py:156-169."""
+    b = torch.sqrt(bi * bj)
+    a = ai * aj
     br = b * dr
     P = 1 / 3 * br * br + br + 1
-    return torch.sum(a * P * torch.exp(-br) * m)
+    return a * P * torch.exp(-br) * m
+
+
+def TT_damping_qq_c6_kernel(dr, m, ai, aj, bi, bj, qi, qj, ci, cj):
+    """dmff/admp/pairwise.py:98-117 (r in bohr for the first two terms)."""
+    a = torch.sqrt(ai * aj)
+    b = torch.sqrt(bi * bj)
+    c = ci * cj
+    q = qi * qj
+    r = dr * 1.889726878
+    br = b * r
+    br2 = br * br
+    br3 = br2 * br
+    br4 = br3 * br
+    br5 = br4 * br
+    br6 = br5 * br
+    exp_br = torch.exp(-br)
+    f = 2625.5 * a * exp_br + (-2625.5) * exp_br * (1 + br) * q / r \
+        + exp_br * (1 + br + br2 / 2 + br3 / 6 + br4 / 24 + br5 / 120 + br6 / 720) * c / dr ** 6
+    return f * m
+
+
+def TT_damping_qq_kernel(dr, m, bi, bj, qi, qj):
+    """dmff/admp/pairwise.py:120-128."""
+    b = torch.sqrt(bi * bj)
+    q = qi * qj
+    br = b * dr
+    exp_br = torch.exp(-br)
+    f = -DIELECTRIC * exp_br * (1 + br) * q / dr
+    return f * m
+
+
+def slater_disp_damping_kernel(dr, m, bi, bj, c6i, c6j, c8i, c8j, c10i, c10j):
+    """dmff/admp/pairwise.py:131-153 (Slater-ISA damping, JCTC 12, 3851)."""
+    b = torch.sqrt(bi * bj)
+    c6 = c6i * c6j
+    c8 = c8i * c8j
+    c10 = c10i * c10j
+    br = b * dr
+    br2 = br * br
+    x = br - (2 * br2 + 3 * br) / (br2 + 3 * br + 3)
+    s6 = 1 + x + x ** 2 / 2 + x ** 3 / 6 + x ** 4 / 24 + x ** 5 / 120 + x ** 6 / 720
+    s8 = s6 + x ** 7 / 5040 + x ** 8 / 40320
+    s10 = s8 + x ** 9 / 362880 + x ** 10 / 3628800
+    exp_x = torch.exp(-x)
+    return (exp_x * s6 * c6 / dr ** 6 + exp_x * s8 * c8 / dr ** 8 + exp_x * s10 * c10 / dr ** 10) * m
+
+
+# kernel objects by the library's admp_sr_kind number (include/admp_hip.h)
+SR_KERNELS = {1: slater_sr_kernel, 2: TT_damping_qq_c6_kernel, 3: TT_damping_qq_kernel, 4: slater_disp_damping_kernel}
+
+
+def slater_sr_energy(positions, box, pairs, mScales, a_list, b_list):
+    """generate_pairwise_interaction(slater_sr_kernel): sum m*Ai*Aj*(1/3 (Br)^2+Br+1) e^-Br."""
+    return generate_pairwise_interaction(slater_sr_kernel)(positions, box, pairs, mScales, a_list, b_list)
 
 
 # ---------------------------------------------------------------------------

@@ -280,7 +280,23 @@ def bad_geometry(precision):
     return out
 
 
-CASES = {"bad_geometry": bad_geometry, "type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
+def pairwise(precision):
+    """admp_pairwise_compute for every short-range kind on its test system."""
+    from dmff_amd.admp.engine import Engine
+    from tests.systems import pairwise_case
+    out = {}
+    for kind in (1, 2, 3, 4):
+        pos, box, pairs, m, prm = pairwise_case(kind)
+        eng = Engine(len(pos), 0, False, False, 0.0, (0, 0, 0), 9.0, precision=precision)
+        eng.set_topology(None, None, {(0, 1): 1, (1, 0): 1, (2, 3): 2, (3, 2): 2})
+        eng.set_pairs(torch.as_tensor(pairs))
+        E, g = eng.pairwise_compute(kind, pos, box, prm, m)
+        out[str(kind)] = {"E": float(E[0]), "pos": _lst(g["pos"][0]), "box": _lst(g["box"][0]),
+                          "prm": _lst(g["prm"][0])}
+    return out
+
+
+CASES = {"pairwise": pairwise, "bad_geometry": bad_geometry, "type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
 
 if __name__ == "__main__":
     lib, case = sys.argv[1], sys.argv[2]

@@ -95,3 +95,41 @@ def neighbour_cases():
     for rc in (6.0, 9.4):
         out.append((f"tric_{rc}", p, TRICLINIC_BOX.copy(), rc))
     return out
+
+
+def pairwise_case(kind, seed=21):
+    """60 sites on a jittered 4x4x4 lattice (5 A spacing, +-1 A) in a 20 A box,
+    pairs within 9 A with two bonded pairs, scales (0.1, 0.3, 1, 1, 1, 1), and
+    per-site parameters of a short-range kind (admp_sr_kind) in realistic ranges
+    of each kernel's own units.  Returns (pos, box, pairs, mScales, prm (N, n_prm))."""
+    rng = np.random.default_rng(seed + 100 * kind)
+    g = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), axis=-1).reshape(-1, 3)[:60]
+    pos = g * 5.0 + 2.5 + rng.uniform(-1.0, 1.0, size=(60, 3))
+    box = np.eye(3) * 20.0
+    bonded = {(0, 1): 1, (1, 0): 1, (2, 3): 2, (3, 2): 2}
+    pairs = reference_pairs(pos, box, 9.0, bonded)
+    m = np.array([0.1, 0.3, 1.0, 1.0, 1.0, 1.0])
+    n = len(pos)
+    if kind == 1:    # slater: A, B (1/A)
+        prm = np.stack([rng.uniform(1e2, 3e2, n), rng.uniform(3.0, 4.5, n)], axis=1)
+    elif kind == 2:  # TT qq c6: A (hartree), B (1/bohr), Q (e), C (sqrt(kJ/mol A^6))
+        prm = np.stack([rng.uniform(1.0, 20.0, n), rng.uniform(1.5, 2.5, n), rng.uniform(-0.8, 0.8, n),
+                        rng.uniform(5.0, 40.0, n)], axis=1)
+    elif kind == 3:  # TT qq: B (1/A), Q (e)
+        prm = np.stack([rng.uniform(3.0, 4.5, n), rng.uniform(-0.8, 0.8, n)], axis=1)
+    else:            # slater damping: B (1/A), c6, c8, c10 (sqrt of kJ/mol A^n)
+        prm = np.stack([rng.uniform(3.0, 4.5, n), rng.uniform(5.0, 40.0, n), rng.uniform(10.0, 100.0, n),
+                        rng.uniform(50.0, 300.0, n)], axis=1)
+    return pos, box, pairs, m, prm
+
+
+def pairwise_oracle(kind, pos, box, pairs, m, prm):
+    """The oracle's generate_pairwise_interaction for `kind`: energy and gradients
+    with respect to positions, box and the per-site parameters."""
+    import torch
+    from oracle import admp_oracle as O
+    T = lambda x: torch.tensor(np.asarray(x, dtype=np.float64), requires_grad=True)
+    p, b, q = T(pos), T(box), T(prm)
+    e = O.generate_pairwise_interaction(O.SR_KERNELS[kind])(p, b, pairs, m, *[q[:, k] for k in range(prm.shape[1])])
+    gp, gb, gq = torch.autograd.grad(e, (p, b, q))
+    return {"E": float(e.detach()), "pos": gp.numpy(), "box": gb.numpy(), "prm": gq.numpy()}

@@ -211,3 +211,16 @@ def test_bad_geometry_is_an_argument_error(emu_lib, prec):
     tol_e, tol_g = (1e-9, 1e-6) if prec == "fp64" else (2e-5, 1e-4)
     assert abs(r["E_good"] - o["E"]) <= tol_e * abs(o["E"])
     assert rel_rms(np.asarray(r["pos_good"]), o["pos"]) <= tol_g
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_short_range_kinds_vs_oracle(emu_lib, prec):
+    from tests.systems import pairwise_case, pairwise_oracle
+    r = run(emu_lib, "pairwise", prec)
+    tol_e, tol_g = (1e-11, 1e-10) if prec == "fp64" else (1e-5, 1e-4)
+    for kind in (1, 2, 3, 4):
+        o = pairwise_oracle(kind, *pairwise_case(kind))
+        g = r[str(kind)]
+        assert abs(g["E"] - o["E"]) <= tol_e * abs(o["E"]), (kind, g["E"], o["E"])
+        for k in ("pos", "box", "prm"):
+            assert rel_rms(np.asarray(g[k]), o[k]) <= tol_g, (kind, k, rel_rms(np.asarray(g[k]), o[k]))

@@ -241,8 +241,13 @@ def test_mixed_precision_error_bounds():
     assert de < 2e-5 and df < 1e-4
 
 
-def test_dispersion_and_slater_vs_oracle():
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_dispersion_and_slater_vs_oracle(prec):
+    """Dispersion PME (pmax 10) and the Slater term; mixed runs the FP32 pair,
+    spread and FFT kernels (tolerance: energy 2e-5 relative, gradients 1e-4
+    relative RMS; FP64: 1e-9 / 1e-8)."""
     from dmff_amd.admp.engine import Engine
+    te, tg = (1e-9, 1e-8) if prec == "fp64" else (2e-5, 1e-4)
     rng = np.random.default_rng(9)
     L = 20.0
     n = 60
@@ -258,15 +263,15 @@ def test_dispersion_and_slater_vs_oracle():
     ct = T(c).requires_grad_(True)
     e = O.energy_disp_pme(p, b, pairs, ct, m, kappa, (K1, K2, K3), 10)
     gp, gb, gc = torch.autograd.grad(e, (p, b, ct))
-    eng = Engine(n, 0, False, False, 0.0, (0, 0, 0), rc, disp_kappa=kappa, dK=(K1, K2, K3), disp_pmax=10, device=DEV)
+    eng = Engine(n, 0, False, False, 0.0, (0, 0, 0), rc, disp_kappa=kappa, dK=(K1, K2, K3), disp_pmax=10,
+                 precision=prec, device=DEV)
     eng.set_topology(None, None, bonded)
     eng.set_pairs(torch.as_tensor(pairs))
     E, g = eng.disp_compute(pos, np.eye(3) * L, c, m)
-    print(f"disp: E gpu {float(E[0]):.12f} oracle {float(e):.12f}", flush=True)
-    assert abs(float(E[0]) - float(e)) < 1e-9 * abs(float(e))
-    assert rel_rms(g["pos"].cpu().numpy(), gp.numpy()) < 1e-8
-    assert rel_rms(g["box"].cpu().numpy(), gb.numpy()) < 1e-8
-    assert rel_rms(g["c"].cpu().numpy(), gc.numpy()) < 1e-8
+    errs = [abs(float(E[0]) - float(e)) / abs(float(e)), rel_rms(g["pos"].cpu().numpy(), gp.numpy()),
+            rel_rms(g["box"].cpu().numpy(), gb.numpy()), rel_rms(g["c"].cpu().numpy(), gc.numpy())]
+    print(f"disp {prec}: E gpu {float(E[0]):.12f} oracle {float(e):.12f}; errors E/pos/box/c {errs}", flush=True)
+    assert errs[0] < te and max(errs[1:]) < tg
     A = rng.uniform(1e2, 3e2, size=n)
     B = rng.uniform(3.0, 4.5, size=n)
     At, Bt = T(A).requires_grad_(True), T(B).requires_grad_(True)
@@ -275,12 +280,31 @@ def test_dispersion_and_slater_vs_oracle():
     es = O.slater_sr_energy(p2, b2, pairs, m, At, Bt)
     gp2, gb2, ga2, gbb2 = torch.autograd.grad(es, (p2, b2, At, Bt))
     E2, g2 = eng.slater_compute(pos, np.eye(3) * L, A, B, m)
-    print(f"slater: E gpu {float(E2[0]):.12f} oracle {float(es):.12f}", flush=True)
-    assert abs(float(E2[0]) - float(es)) < 1e-10 * max(1, abs(float(es)))
-    assert rel_rms(g2["pos"].cpu().numpy(), gp2.numpy()) < 1e-9
-    assert rel_rms(g2["a"].cpu().numpy(), ga2.numpy()) < 1e-9
-    assert rel_rms(g2["b"].cpu().numpy(), gbb2.numpy()) < 1e-9
-    assert rel_rms(g2["box"].cpu().numpy(), gb2.numpy()) < 1e-9
+    errs = [abs(float(E2[0]) - float(es)) / max(1, abs(float(es)))] + [
+        rel_rms(g2[k].cpu().numpy(), v.numpy()) for k, v in (("pos", gp2), ("a", ga2), ("b", gbb2), ("box", gb2))]
+    print(f"slater {prec}: E gpu {float(E2[0]):.12f} oracle {float(es):.12f}; errors {errs}", flush=True)
+    assert errs[0] < te and max(errs[1:]) < tg
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_short_range_kinds_vs_oracle(prec):
+    """admp_pairwise_compute for the four generate_pairwise_interaction kernels
+    against the oracle (FP64: energy 1e-11, gradients 1e-10; mixed: 1e-5, 1e-4)."""
+    from dmff_amd.admp.engine import Engine
+    from tests.systems import pairwise_case, pairwise_oracle
+    te, tg = (1e-11, 1e-10) if prec == "fp64" else (1e-5, 1e-4)
+    for kind in (1, 2, 3, 4):
+        pos, box, pairs, m, prm = pairwise_case(kind)
+        o = pairwise_oracle(kind, pos, box, pairs, m, prm)
+        eng = Engine(len(pos), 0, False, False, 0.0, (0, 0, 0), 9.0, precision=prec, device=DEV)
+        eng.set_topology(None, None, {(0, 1): 1, (1, 0): 1, (2, 3): 2, (3, 2): 2})
+        eng.set_pairs(torch.as_tensor(pairs))
+        E, g = eng.pairwise_compute(kind, pos, box, prm, m)
+        errs = [abs(float(E[0]) - o["E"]) / abs(o["E"])] + [rel_rms(g[k][0].cpu().numpy(), o[k])
+                                                            for k in ("pos", "box", "prm")]
+        print(f"kind {kind} {prec}: E {float(E[0]):.12e} oracle {o['E']:.12e}; errors E/pos/box/prm {errs}",
+              flush=True)
+        assert errs[0] < te and max(errs[1:]) < tg, (kind, errs)
 
 
 def test_batched_frames_equal_single():

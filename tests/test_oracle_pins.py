@@ -428,11 +428,92 @@ def test_dispersion_pme_kappa_independent_and_direct_sum():
     assert abs(e1 - ref) < 2e-5 * abs(ref), (e1, ref)
 
 
-def test_slater_closed_form():
-    pos = T([[0.0, 0.0, 0.0], [1.7, 0.4, -0.3]])
+def _sto_overlap(zeta, R):
+    """Overlap of two normalised 1s Slater orbitals sqrt(z^3/pi) e^{-z r} a distance
+    R apart, by numerical quadrature in cylindrical coordinates (rho, z) -- no
+    closed form involved.  This is the Slater-ISA basis of pairwise.py:131-169
+    (Van Vleet et al., JCTC 12, 3851)."""
+    from scipy import integrate
+    L = 40.0 / zeta
+
+    def f(rho, z):
+        ra = math.sqrt(rho * rho + z * z)
+        rb = math.sqrt(rho * rho + (z - R) * (z - R))
+        return 2 * math.pi * rho * math.exp(-zeta * (ra + rb))
+
+    tot = 0.0
+    # split z at the two nuclei so the cusps sit on interval ends
+    for z0, z1 in ((-L, 0.0), (0.0, R), (R, R + L)):
+        v, _ = integrate.dblquad(f, z0, z1, 0.0, L, epsabs=1e-14, epsrel=1e-11)
+        tot += v
+    return zeta ** 3 / math.pi * tot
+
+
+def test_slater_sr_kernel_is_the_1s_sto_overlap():
+    """slater_sr_kernel with equal exponents B_i = B_j = zeta and A_i = A_j = 1 is
+    the 1s-1s overlap S(R); a wrong polynomial coefficient fails this."""
+    for zeta, R in ((3.9, 0.7), (3.9, 1.6), (4.6, 2.5)):
+        S = _sto_overlap(zeta, R)
+        e = float(O.slater_sr_kernel(T([R]), T([1.0]), T([1.0]), T([1.0]), T([zeta]), T([zeta]))[0])
+        assert abs(e - S) < 1e-8 * S + 1e-13, (zeta, R, e, S)
+
+
+def test_slater_damping_argument_is_the_overlap_log_derivative():
+    """The damping argument of slater_disp_damping_kernel is y = -d ln S / d ln R of
+    the 1s overlap, and the damping functions are the Tang-Toennies sums
+    e^{-y} sum_{k<=n} y^k/k! = Q(n+1, y) (regularised upper incomplete gamma).
+    y comes from central differences of the quadrature overlap."""
+    from scipy.special import gammaincc
+    zeta = 3.8
+    c6, c8, c10 = 2.0, 3.0, 5.0
+    for R in (1.1, 2.4, 3.3):
+        h = 1e-4 * R
+        y = -(math.log(_sto_overlap(zeta, R + h)) - math.log(_sto_overlap(zeta, R - h))) / \
+            (math.log(R + h) - math.log(R - h))
+        want = sum(c * gammaincc(n + 1, y) / R ** n for n, c in ((6, c6), (8, c8), (10, c10)))
+        one = T([1.0])
+        got = float(O.slater_disp_damping_kernel(T([R]), one, T([zeta]), T([zeta]), T([c6]), one, T([c8]), one,
+                                                 T([c10]), one)[0])
+        assert abs(got - want) < 1e-6 * abs(want), (R, got, want)
+
+
+def test_tang_toennies_kernels_are_incomplete_gamma_sums():
+    """TT_damping_qq_kernel and TT_damping_qq_c6_kernel: the e^{-x} sum_{k<=n}
+    x^k/k! factors are Q(n+1, x) (scipy), charges scaled by DIELECTRIC (A) or by
+    2625.5 kJ/mol per hartree with r in bohr (1.889726878 bohr per A, the
+    reference's literal); A and B combine as geometric means, Q and C as products."""
+    from scipy.special import gammaincc
+    rng = np.random.default_rng(4)
+    for _ in range(5):
+        r = rng.uniform(1.5, 6.0)
+        Bi, Bj = rng.uniform(2.0, 4.5, 2)
+        Qi, Qj = rng.uniform(-0.8, 0.8, 2)
+        b, q = math.sqrt(Bi * Bj), Qi * Qj
+        e = float(O.TT_damping_qq_kernel(T([r]), T([0.7]), T([Bi]), T([Bj]), T([Qi]), T([Qj]))[0])
+        want = -0.7 * O.DIELECTRIC * q * gammaincc(2, b * r) / r
+        assert abs(e - want) < 1e-12 * abs(want), (e, want)
+        Ai, Aj = rng.uniform(1.0, 20.0, 2)
+        Ci, Cj = rng.uniform(5.0, 40.0, 2)
+        bb = rng.uniform(1.0, 2.5)  # bohr^-1
+        t = r * 1.889726878
+        e = float(O.TT_damping_qq_c6_kernel(T([r]), T([1.0]), T([Ai]), T([Aj]), T([bb]), T([bb]), T([Qi]), T([Qj]),
+                                            T([Ci]), T([Cj]))[0])
+        want = 2625.5 * math.sqrt(Ai * Aj) * math.exp(-bb * t) - 2625.5 * q * gammaincc(2, bb * t) / t + \
+            Ci * Cj * gammaincc(7, bb * t) / r ** 6
+        assert abs(e - want) < 1e-11 * (abs(want) + 1.0), (e, want)
+
+
+def test_pairwise_interaction_pair_semantics():
+    """generate_pairwise_interaction drops rows with i >= j (padding included) and
+    takes m = mScales[nbond - 1], nbond = 0 wrapping to the last entry
+    (pairwise.py:66-78, utils.py:65-84)."""
+    pos = T([[0.0, 0.0, 0.0], [1.7, 0.4, -0.3], [3.0, 1.0, 0.5]])
     box = torch.eye(3, dtype=F64) * 30
-    A, B = T([30.0, 45.0]), T([3.1, 3.9])
-    e = float(O.slater_sr_energy(pos, box, np.array([[0, 1, 0]]), np.ones(6), A, B))
-    r = float(torch.linalg.norm(pos[1]))
-    b = math.sqrt(3.1 * 3.9)
-    assert abs(e - 30 * 45 * (b * b * r * r / 3 + b * r + 1) * math.exp(-b * r)) < 1e-10
+    A, B = T([30.0, 45.0, 20.0]), T([3.1, 3.9, 3.5])
+    m = np.array([0.1, 0.2, 0.3, 0.4, 0.5, 1.0])
+    pairs = np.array([[0, 1, 2], [1, 2, 0], [2, 1, 0], [3, 3, 0], [1, 1, 0]])
+    got = float(O.slater_sr_energy(pos, box, pairs, m, A, B))
+    ker = lambda i, j, mm: float(O.slater_sr_kernel(torch.linalg.norm(pos[i] - pos[j]).reshape(1), T([mm]),
+                                                    A[i:i + 1], A[j:j + 1], B[i:i + 1], B[j:j + 1])[0])
+    want = ker(0, 1, 0.2) + ker(1, 2, 1.0)
+    assert abs(got - want) < 1e-13 * abs(want)
