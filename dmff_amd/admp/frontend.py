@@ -90,6 +90,7 @@ class Topology:
         self.elements = []
         self.types = []
         self.bonds = []
+        self.conect = []    # (i, j) bonds from CONECT records, 0-based atom indices
         self.box_nm = None
 
     @property
@@ -101,16 +102,30 @@ def read_pdb(path):
     """Minimal PDB reader: ATOM/HETATM coordinates (returned in nm) and CRYST1 box (nm)."""
     top = Topology()
     pos = []
+    serial = {}
+    model_done = False
     res_key_prev = None
     res_counter = -1
     with open(path) as f:
         for line in f:
             rec = line[:6].strip()
+            if rec == "CONECT":
+                # serial numbers in columns 7-11, 12-16, ... (bonds are added once per pair)
+                fields = [line[k:k + 5].strip() for k in range(6, min(len(line.rstrip("\n")), 31), 5)]
+                ids = [int(x) for x in fields if x]
+                if ids:
+                    a = ids[0]
+                    for b in ids[1:]:
+                        if a in serial and b in serial:
+                            i, j = serial[a], serial[b]
+                            if i != j:
+                                top.conect.append((min(i, j), max(i, j)))
+                continue
             if rec == "CRYST1":
                 a, b, c = float(line[6:15]), float(line[15:24]), float(line[24:33])
                 al, be, ga = float(line[33:40]), float(line[40:47]), float(line[47:54])
                 top.box_nm = _box_from_lengths(a, b, c, al, be, ga) / 10.0
-            elif rec in ("ATOM", "HETATM"):
+            elif rec in ("ATOM", "HETATM") and not model_done:
                 name = line[12:16].strip()
                 resn = line[17:20].strip()
                 key = (line[21], line[22:27])
@@ -123,10 +138,15 @@ def read_pdb(path):
                 top.res_names.append(resn)
                 top.res_ids.append(res_counter)
                 top.elements.append(el)
+                try:
+                    serial[int(line[6:11])] = len(pos)
+                except ValueError:
+                    pass
                 pos.append((x / 10.0, y / 10.0, z / 10.0))
-            elif rec in ("ENDMDL", "END"):
-                if pos:
-                    break
+            elif rec == "ENDMDL":
+                model_done = model_done or bool(pos)   # atoms of later models are skipped
+            elif rec == "END":
+                break
     return np.asarray(pos, dtype=np.float64), top
 
 
@@ -164,7 +184,7 @@ def assign_types(top, ff, extra_bonds=None):
     by_res = {}
     for i, rid in enumerate(top.res_ids):
         by_res.setdefault(rid, {})[top.atom_names[i]] = i
-    bonds = set()
+    bonds = set(getattr(top, "conect", []))
     for rid, names in by_res.items():
         rname = top.res_names[next(iter(names.values()))]
         pairs = list(tmpl.get(rname, {}).get("bonds", []))
