@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -352,7 +353,8 @@ extern "C" const char* admp_version(void) { return "admp-hip 0.1 (gfx950)"; }
 
 static admp_status nbr_settle(admp_ctx* ctx);
 
-extern "C" const char* admp_last_error(const admp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+static const char* context_free_error();
+extern "C" const char* admp_last_error(const admp_ctx* ctx) { return ctx ? ctx->err.c_str() : context_free_error(); }
 
 extern "C" admp_status admp_set_stream(admp_ctx* ctx, void* stream) {
     if (!ctx) return ADMP_ERR_ARG;
@@ -2075,5 +2077,213 @@ extern "C" admp_status admp_memory_bytes(const admp_ctx* ctx, int64_t* bytes) {
     int64_t t = 0;
     for (size_t b : ctx->alloc_bytes) t += (int64_t)b;
     *bytes = t;
+    return ADMP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// admp_pairwise_compute_noalloc: the pair terms of admp_pairwise_compute for
+// callers that cannot allocate (code under HIP graph capture, serving loops).
+// It takes the reference-format pair list directly (pairwise.py:47-95), issues
+// four launches on the caller's stream and nothing else: no allocation, no
+// memset, no host synchronisation.  Its only device memory besides the caller's
+// inputs and outputs is `scratch` below: the box record and the per-block
+// partial sums of the energy and the virial.  Its size does not depend on the
+// input (the pair kernel is a grid-stride loop over at most NA_MAXBLK blocks).
+// Every call on a device shares that device's copy, so calls that overlap on
+// one device race; each call rewrites what it reads.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int NA_BLOCK = 256;
+constexpr int NA_MAXBLK = 1024;
+constexpr int NA_BOX_DOUBLES = 64;                         // BoxInfo + the flag word
+constexpr int NA_SCRATCH_DOUBLES = NA_BOX_DOUBLES + NA_MAXBLK * 10;  // 82,432 bytes
+static_assert(sizeof(BoxInfo) + sizeof(int) <= NA_BOX_DOUBLES * sizeof(double), "box record");
+constexpr int NA_MAX_DEVICES = 64;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[NA_SCRATCH_DOUBLES];
+
+namespace {
+
+thread_local std::string g_err;          // last error of a context-free entry point
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NA_MAX_DEVICES] = {nullptr};
+
+// The device address of the current device's `scratch` (looked up once per
+// device; the array's name in host code is not a device address).
+admp_status scratch_ptr(double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { g_err = "hipGetDevice failed"; return ADMP_ERR_HIP; }
+    if (dev < 0 || dev >= NA_MAX_DEVICES) { g_err = "device ordinal beyond 64"; return ADMP_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            g_err = "hipGetSymbolAddress(scratch) failed";
+            return ADMP_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ADMP_OK;
+}
+
+}  // namespace
+
+static const char* context_free_error() { return g_err.empty() ? "null context" : g_err.c_str(); }
+
+namespace admp {
+
+// zero the gradient outputs and the flag word, then the box record (one thread)
+__global__ void k_na_prep(int N, int nprm, const double* __restrict__ box, double* __restrict__ sbox,
+                          double* __restrict__ dpos, double* __restrict__ dprm) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nt = (size_t)gridDim.x * blockDim.x;
+    if (dpos)
+        for (size_t k = tid; k < (size_t)N * 3; k += nt) dpos[k] = 0.0;
+    if (dprm)
+        for (size_t k = tid; k < (size_t)N * nprm; k += nt) dprm[k] = 0.0;
+    if (tid == 0) {
+        int* flags = reinterpret_cast<int*>(sbox + NA_BOX_DOUBLES - 1);
+        *flags = 0;
+        box_setup_one(box, 0, 0, 0, 0, 0, 0, reinterpret_cast<BoxInfo*>(sbox), flags);
+    }
+}
+
+// One thread per pair row (grid-stride).  Rows that are not live pairs
+// (i >= j, the (N, N) padding rows, indices outside [0, N)) contribute nothing,
+// as regularize_pairs / pair_buffer_scales make them (dmff/utils.py:65-84).
+// Forces and parameter gradients go to the outputs with atomic adds; the energy
+// and the virial to per-block partials in scratch.
+template <class R, int KIND, class I>
+__global__ void __launch_bounds__(NA_BLOCK)
+k_na_pairs(int64_t np, int N, const I* __restrict__ pairs, const double* __restrict__ pos,
+           const double* __restrict__ prm, const double* __restrict__ sbox, PairParams pp, int want_grad,
+           double* __restrict__ part, double* __restrict__ dpos, double* __restrict__ dprm) {
+    __shared__ double sh[16];
+    const BoxInfo& bx = *reinterpret_cast<const BoxInfo*>(sbox);
+    constexpr int npr = sr_nprm(KIND);
+    double E = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t r = (int64_t)blockIdx.x * NA_BLOCK + threadIdx.x; r < np; r += (int64_t)gridDim.x * NA_BLOCK) {
+        const int64_t i = (int64_t)pairs[r * 3], j = (int64_t)pairs[r * 3 + 1];
+        if (!(i >= 0 && i < j && j < N)) continue;
+        const int64_t nb64 = (int64_t)pairs[r * 3 + 2];
+        const int nb = (int)(nb64 < -64 ? -64 : (nb64 > 64 ? 64 : nb64));
+        double dx = pos[i * 3] - pos[j * 3], dy = pos[i * 3 + 1] - pos[j * 3 + 1], dz = pos[i * 3 + 2] - pos[j * 3 + 2];
+        double n0, n1, n2;
+        pbc_shift(bx, dx, dy, dz, n0, n1, n2);
+        const double rd = sqrt(dx * dx + dy * dy + dz * dz);
+        const R rr = (R)rd, rinv = (R)(1.0 / rd);
+        const R m = (R)pp.ms[scale_index(nb, pp.nscale)];
+        R pI[SR_NPRM_MAX] = {0, 0, 0, 0}, pJ[SR_NPRM_MAX] = {0, 0, 0, 0}, dI[SR_NPRM_MAX], dJ[SR_NPRM_MAX];
+        for (int q = 0; q < npr; ++q) { pI[q] = (R)prm[i * npr + q]; pJ[q] = (R)prm[j * npr + q]; }
+        R e, dedr;
+        sr_pair<R, KIND>(rr, m, pI, pJ, e, dedr, dI, dJ);
+        E += (double)e;
+        if (want_grad) {
+            const double f[3] = {(double)(dedr * (R)dx * rinv), (double)(dedr * (R)dy * rinv),
+                                 (double)(dedr * (R)dz * rinv)};
+            const double nn[3] = {n0, n1, n2};
+            for (int c = 0; c < 3; ++c) {
+                atomicAdd(&dpos[i * 3 + c], f[c]);
+                atomicAdd(&dpos[j * 3 + c], -f[c]);
+            }
+            for (int q = 0; q < npr; ++q) {
+                atomicAdd(&dprm[i * npr + q], (double)dI[q]);
+                atomicAdd(&dprm[j * npr + q], (double)dJ[q]);
+            }
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) v[p * 3 + q] -= nn[p] * f[q];
+        }
+    }
+    const double Eb = block_sum(E, sh);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * 10] = Eb;
+    for (int k = 0; k < 9; ++k) {
+        const double vb = block_sum(v[k], sh);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * 10 + 1 + k] = vb;
+    }
+}
+
+// energy and dE/dbox from the block partials, summed in block order; NaN when
+// the box was unusable (box_usable), since nothing here can return an error
+__global__ void k_na_finish(int nblk, const double* __restrict__ sbox, const double* __restrict__ part,
+                            double* __restrict__ energy, double* __restrict__ dbox) {
+    const int k = threadIdx.x;
+    if (k >= 10) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(size_t)b * 10 + k];
+    const int flags = *reinterpret_cast<const int*>(sbox + NA_BOX_DOUBLES - 1);
+    if (flags) s = __builtin_nan("");
+    if (k == 0) {
+        if (energy) energy[0] = s;
+    } else if (dbox) {
+        dbox[k - 1] = s;
+    }
+}
+
+}  // namespace admp
+
+template <class R, int KIND>
+static void na_launch(int nb, hipStream_t st, int64_t np, int N, const void* pairs, int is64, const double* pos,
+                      const double* prm, const double* sbox, const PairParams& pp, int want_grad, double* part,
+                      double* dpos, double* dprm) {
+    if (is64)
+        hipLaunchKernelGGL((k_na_pairs<R, KIND, int64_t>), dim3(nb), dim3(NA_BLOCK), 0, st, np, N,
+                           (const int64_t*)pairs, pos, prm, sbox, pp, want_grad, part, dpos, dprm);
+    else
+        hipLaunchKernelGGL((k_na_pairs<R, KIND, int32_t>), dim3(nb), dim3(NA_BLOCK), 0, st, np, N,
+                           (const int32_t*)pairs, pos, prm, sbox, pp, want_grad, part, dpos, dprm);
+}
+
+template <class R>
+static void na_dispatch(int kind, int nb, hipStream_t st, int64_t np, int N, const void* pairs, int is64,
+                        const double* pos, const double* prm, const double* sbox, const PairParams& pp,
+                        int want_grad, double* part, double* dpos, double* dprm) {
+    switch (kind) {
+        case SR_SLATER: na_launch<R, SR_SLATER>(nb, st, np, N, pairs, is64, pos, prm, sbox, pp, want_grad, part, dpos, dprm); break;
+        case SR_TT_QQ_C6: na_launch<R, SR_TT_QQ_C6>(nb, st, np, N, pairs, is64, pos, prm, sbox, pp, want_grad, part, dpos, dprm); break;
+        case SR_TT_QQ: na_launch<R, SR_TT_QQ>(nb, st, np, N, pairs, is64, pos, prm, sbox, pp, want_grad, part, dpos, dprm); break;
+        default: na_launch<R, SR_SLATER_DAMP>(nb, st, np, N, pairs, is64, pos, prm, sbox, pp, want_grad, part, dpos, dprm); break;
+    }
+}
+
+extern "C" admp_status admp_pairwise_compute_noalloc(int32_t kind, int32_t precision, int32_t n_atoms,
+                                                     const double* pos, const double* box, const void* pairs,
+                                                     int64_t n_pairs, int32_t pairs_is_int64, const double* prm,
+                                                     int32_t n_prm, const double* mScales, int32_t n_scales,
+                                                     int32_t want_grad, double* energy, double* dpos, double* dbox,
+                                                     double* dprm, void* stream) {
+    if (kind < 1 || kind > SR_KIND_MAX) { g_err = "unknown short-range kind"; return ADMP_ERR_ARG; }
+    if (n_prm != sr_nprm(kind)) { g_err = "n_prm does not match the kind's parameter count"; return ADMP_ERR_ARG; }
+    if (precision != ADMP_FP64 && precision != ADMP_MIXED) { g_err = "precision must be ADMP_FP64 or ADMP_MIXED"; return ADMP_ERR_ARG; }
+    if (n_atoms < 1 || n_atoms >= (1 << NB_SHIFT)) { g_err = "n_atoms must be in 1 .. 2^28 - 1"; return ADMP_ERR_ARG; }
+    if (n_pairs < 0 || n_pairs > ((int64_t)1 << 40)) { g_err = "n_pairs must be in 0 .. 2^40"; return ADMP_ERR_ARG; }
+    if (n_scales < 1 || n_scales > 8 || !mScales) { g_err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
+    if (!pos || !box || !prm || (n_pairs > 0 && !pairs)) { g_err = "null input"; return ADMP_ERR_ARG; }
+    if (want_grad && (!dpos || !dprm)) { g_err = "want_grad needs dpos and dprm"; return ADMP_ERR_ARG; }
+    double* sc = nullptr;
+    admp_status s = scratch_ptr(&sc);
+    if (s != ADMP_OK) return s;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    double* sbox = sc;
+    double* part = sc + NA_BOX_DOUBLES;
+    const PairParams pp = make_pp(mScales, nullptr, nullptr, n_scales, 0.0);
+    double* gp = want_grad ? dpos : nullptr;
+    double* gq = want_grad ? dprm : nullptr;
+    const int64_t want_blocks = (n_pairs + NA_BLOCK - 1) / NA_BLOCK;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(want_blocks, NA_MAXBLK));
+    const size_t nz = (size_t)n_atoms * std::max(3, (int)n_prm);
+    hipLaunchKernelGGL(k_na_prep, dim3((unsigned)std::min<size_t>(nblk(nz, 256), 1024)), dim3(256), 0, st, n_atoms,
+                       n_prm, box, sbox, gp, gq);
+    if (precision == ADMP_MIXED)
+        na_dispatch<float>(kind, nb, st, n_pairs, n_atoms, pairs, pairs_is_int64, pos, prm, sbox, pp, want_grad, part,
+                           gp, gq);
+    else
+        na_dispatch<double>(kind, nb, st, n_pairs, n_atoms, pairs, pairs_is_int64, pos, prm, sbox, pp, want_grad, part,
+                            gp, gq);
+    hipLaunchKernelGGL(k_na_finish, dim3(1), dim3(64), 0, st, nb, sbox, part, energy, want_grad ? dbox : nullptr);
+    if (hipGetLastError() != hipSuccess) { g_err = "kernel launch failed"; return ADMP_ERR_HIP; }
     return ADMP_OK;
 }

@@ -37,13 +37,11 @@ __device__ __host__ inline bool box_usable(const double* B) {
 // box (box_usable) is replaced by a 100 A cube so that nothing downstream
 // derives an index from it, and GEOM_BAD_BOX is set in *flags; the entry point
 // reports the error once its work has drained.
-__global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K1, int K2, int dK0, int dK1,
-                            int dK2, BoxInfo* __restrict__ out, int* __restrict__ flags) {
+__device__ inline void box_setup_one(const double* __restrict__ box, int K0, int K1, int K2, int dK0, int dK1,
+                                     int dK2, BoxInfo* __restrict__ out, int* __restrict__ flags) {
 #pragma clang fp contract(off)
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
     double B[9];
-    for (int k = 0; k < 9; ++k) B[k] = box[(size_t)f * 9 + k];
+    for (int k = 0; k < 9; ++k) B[k] = box[k];
     if (!box_usable(B)) {
         for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0) ? 100.0 : 0.0;
         atomicOr(flags, GEOM_BAD_BOX);
@@ -81,7 +79,14 @@ __global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K
             b.kb[c * 3 + j] = 2.0 * PI * b.Binv[c * 3 + j];
         }
     b.pad = 0;
-    out[f] = b;
+    *out = b;
+}
+
+__global__ void k_box_setup(int F, const double* __restrict__ box, int K0, int K1, int K2, int dK0, int dK1,
+                            int dK2, BoxInfo* __restrict__ out, int* __restrict__ flags) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    box_setup_one(box + (size_t)f * 9, K0, K1, K2, dK0, dK1, dK2, out + f, flags);
 }
 
 // Positions to double4.  A coordinate that is not finite or exceeds POS_MAX_ABS

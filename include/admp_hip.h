@@ -192,6 +192,31 @@ admp_status admp_pairwise_compute(admp_ctx* ctx, int32_t kind, const double* pos
                                   const double* prm, int32_t n_prm, const double* mScales, int32_t n_scales,
                                   int32_t want_grad, double* energy, double* dpos, double* dbox, double* dprm);
 
+/* No counterpart in the reference: a variant of admp_pairwise_compute for callers
+ * that cannot allocate (code under HIP graph capture, tight serving or MD loops).
+ * It needs no context and takes the reference-format pair list (n_pairs, 3)
+ * [i, j, nbond] (int32, or int64 with pairs_is_int64) of one configuration
+ * directly, as pair_int does (dmff/admp/pairwise.py:47-95); rows that are not
+ * live pairs (i >= j, (N, N) padding, indices outside [0, n_atoms)) contribute
+ * nothing.  pos (n_atoms,3), box (3,3), prm (n_atoms, n_prm), outputs energy (1),
+ * dpos (n_atoms,3), dbox (3,3), dprm (n_atoms, n_prm): device pointers.
+ * precision: ADMP_FP64 or ADMP_MIXED (FP32 pair arithmetic, FP64 sums).
+ * mScales: host.  stream: a hipStream_t (NULL = the legacy default stream).
+ * It allocates nothing and does not synchronise: four kernel launches on
+ * `stream`.  Its scratch (box record and per-block energy/virial partials) is
+ * one static device array of 82,432 bytes per GPU, held until the process
+ * ends; its size does not depend on the input.  Calls that overlap on one GPU
+ * race on it.  Forces and parameter gradients are summed with atomic adds
+ * (rounding order not fixed); energy and dbox in a fixed order.  An unusable
+ * box (non-finite, or a height below 0.5 A) yields NaN energy and dbox.
+ * Argument errors return ADMP_ERR_ARG before any launch; admp_last_error(NULL)
+ * describes the last such error of the calling thread. */
+admp_status admp_pairwise_compute_noalloc(int32_t kind, int32_t precision, int32_t n_atoms, const double* pos,
+                                          const double* box, const void* pairs, int64_t n_pairs,
+                                          int32_t pairs_is_int64, const double* prm, int32_t n_prm,
+                                          const double* mScales, int32_t n_scales, int32_t want_grad,
+                                          double* energy, double* dpos, double* dbox, double* dprm, void* stream);
+
 /* Instrumentation (no counterpart in the reference): per-kernel-family timing with
  * HIP events recorded around the library's launches on the context stream.
  * admp_profile switches recording on/off (reset != 0 clears the totals);

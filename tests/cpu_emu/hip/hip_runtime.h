@@ -19,6 +19,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {
     unsigned x, y, z;
@@ -114,6 +116,7 @@ hipError_t hipHostFree(void* p);
 hipError_t hipGetLastError();
 const char* hipGetErrorString(hipError_t e);
 hipError_t hipGetDevice(int* d);
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int d);
 hipError_t hipFuncSetAttribute(const void* f, hipFuncAttribute a, int v);
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode m);
