@@ -10,7 +10,8 @@ verbatim (they are inputs, not code).  The fixtures it writes:
   data/*.pdb, data/*.xml         inputs the reference tests and examples load
   vectors.json                   literal vectors from tests/test_admp/test_multipole.py,
                                  test_sptial.py, test_noaxistype.py
-  energies.json                  known answers (test_compute.py, examples/*/ref_out)
+  energies.json                  known answers (test_compute.py, test_slater.py, test_coul.py,
+                                 test_qeq.py, examples/*/ref_out)
   water_fullpol_uind.npy         per-atom induced dipoles, DMFF column of
                                  examples/water_fullpol/ref_out (e*nm)
 """
@@ -89,6 +90,15 @@ def main():
         "examples/fluctuated_leading_term_waterff/forcefield.xml": "fluct_forcefield.xml",
         "examples/fluctuated_leading_term_waterff/residues.xml": "fluct_residues.xml",
         "examples/fluctuated_leading_term_waterff/water_dimer.pdb": "fluct_water_dimer.pdb",
+        "tests/data/peg.xml": "peg.xml",
+        "tests/data/peg2.pdb": "peg2.pdb",
+        "tests/data/methane_water.pdb": "methane_water.pdb",
+        "tests/data/methane_water_coul.xml": "methane_water_coul.xml",
+        "tests/data/admp_charge.xml": "admp_charge.xml",
+        "tests/data/water.pdb": "water.pdb",
+        "tests/data/water_ion.pdb": "water_ion.pdb",
+        "tests/data/qeq.xml": "qeq.xml",
+        "tests/data/qeq.pdb": "qeq.pdb",
     }
     for src, dst in copies.items():
         shutil.copyfile(os.path.join(REF, src), os.path.join(DATA, dst))
@@ -135,6 +145,25 @@ def main():
         "fluct_E": "examples/fluctuated_leading_term_waterff/ref_out:2",
         "fluct_grad": "examples/fluctuated_leading_term_waterff/ref_out:4-9",
         "mpid_rc_es_disp": "examples/water_fullpol/mpid_ref/results.txt:2-9",
+    })
+    # PEG2: ADMPPme, ADMPDispPme, SlaterEx, SlaterSrEs, SlaterSrPol, SlaterSrDisp,
+    # SlaterDhf, QqTtDamping, SlaterDamping energies, decimal 4
+    peg = parametrize_tables(os.path.join(REF, "tests/test_admp/test_slater.py"))["test_admp_slater"][0]
+    en["peg2"] = {"values": peg["values"], "decimal": 4, "rc_nm": 1.49, "ethresh": 1e-4,
+                  "method": "CutoffPeriodic",
+                  "forces": ["ADMPPmeForce", "ADMPDispPmeForce", "SlaterExForce", "SlaterSrEsForce",
+                             "SlaterSrPolForce", "SlaterSrDispForce", "SlaterDhfForce", "QqTtDampingForce",
+                             "SlaterDampingForce"]}
+    coul = parametrize_tables(os.path.join(REF, "tests/test_classical/test_coul.py"))["test_coul_pme"][0]
+    en["coul_pme"] = {"value": coul["value"], "atol": 1e-2, "rc_nm": 0.5, "spacing_nm": 0.10,
+                      "box_nm": [[1.2, 0, 0], [0, 1.2, 0], [0, 0, 1.2]], "method": "gromacs"}
+    qeq_txt = open(os.path.join(REF, "tests/test_admp/test_qeq.py")).read()
+    en["qeq"] = {"value": float(re.search(r"assert_almost_equal\(energy,\s*" + num, qeq_txt).group(1)),
+                 "decimal": 3, "rc_nm": 0.6, "ethresh": 1e-3}
+    en["source"].update({
+        "peg2": "tests/test_admp/test_slater.py:13-25,74-82",
+        "coul_pme": "tests/test_classical/test_coul.py:84-129",
+        "qeq": "tests/test_admp/test_qeq.py:12-39",
     })
     with open(os.path.join(HERE, "energies.json"), "w") as f:
         json.dump(en, f, indent=1)

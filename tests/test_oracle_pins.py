@@ -517,3 +517,65 @@ def test_pairwise_interaction_pair_semantics():
                                                     A[i:i + 1], A[j:j + 1], B[i:i + 1], B[j:j + 1])[0])
     want = ker(0, 1, 0.2) + ker(1, 2, 1.0)
     assert abs(got - want) < 1e-13 * abs(want)
+
+
+# ----------------------------------------------------------------- PEG2 (test_slater.py)
+def _xml_columns(path, force, cols):
+    """Per-type columns of one force's <Atom> rows, the key attribute and mScales."""
+    import xml.etree.ElementTree as ET
+    node = ET.parse(path).getroot().find(force)
+    rows = [a.attrib for a in node if a.tag == "Atom"]
+    key = "type" if "type" in rows[0] else "class"
+    m = [float(node.get(f"mScale1{i}", [0.0, 0.0, 0.0, 1.0, 1.0][i - 2])) for i in range(2, 7)] + [1.0]
+    return [r[key] for r in rows], {c: np.array([float(r[c]) for r in rows]) for c in cols}, np.array(m)
+
+
+def peg2_oracle_energies():
+    """The nine PEG2 energies of tests/test_admp/test_slater.py by the oracle:
+    CutoffPeriodic (kappa = 0), rc 1.49 nm, pairs i<j within rc with nbond from the
+    bond graph (the PDB's CONECT records join the two halves), the generators'
+    unit conversions (dmff/generators/admp.py:369-384,504-516,638-660,768-779,822-836)."""
+    from tests.systems import golden_path
+    s = load_system("peg.xml", "peg2.pdb")
+    pos, box = s.pos_nm * 10, s.box_nm * 10
+    rc = 14.9
+    pairs = reference_pairs(pos, box, rc, s.bonded)
+    f = O.ADMPPmeForce(box, s.axis_types, s.axis_idx, rc, 1e-4, 2, True, False, None)
+    e_es, _ = f.get_energy(T(pos), T(box), pairs, T(s.pme.Q_local[s.map_atomtype]), T(s.pme.pol[s.map_poltype]),
+                           T(s.pme.thole[s.map_poltype]), s.pme.mScales, s.pme.pScales, s.pme.dScales, aux={})
+    out = {"ADMPPmeForce": float(e_es)}
+    path = golden_path("peg.xml")
+    types = s.top.types
+
+    def per_atom(force, cols):
+        keys, c, m = _xml_columns(path, force, cols)
+        idx = np.array([keys.index(t) for t in types])
+        return {k: T(v[idx]) for k, v in c.items()}, m
+
+    c, m = per_atom("ADMPDispPmeForce", ("C6", "C8", "C10"))
+    cl = torch.stack([torch.sqrt(c["C6"] * 1e6), torch.sqrt(c["C8"] * 1e8), torch.sqrt(c["C10"] * 1e10)], dim=1)
+    out["ADMPDispPmeForce"] = -float(O.energy_disp_pme(T(pos), T(box), pairs, cl, m, 0.0, (0, 0, 0), 10, lpme=False))
+    for name, sign in (("SlaterExForce", 1), ("SlaterSrEsForce", -1), ("SlaterSrPolForce", -1),
+                       ("SlaterSrDispForce", -1), ("SlaterDhfForce", -1)):
+        c, m = per_atom(name, ("A", "B"))
+        out[name] = sign * float(O.generate_pairwise_interaction(O.slater_sr_kernel)(T(pos), T(box), pairs, m,
+                                                                                      c["A"], c["B"] / 10))
+    c, m = per_atom("QqTtDampingForce", ("B", "Q"))
+    out["QqTtDampingForce"] = float(O.generate_pairwise_interaction(O.TT_damping_qq_kernel)(
+        T(pos), T(box), pairs, m, c["B"] / 10, c["Q"]))
+    c, m = per_atom("SlaterDampingForce", ("B", "C6", "C8", "C10"))
+    out["SlaterDampingForce"] = float(O.generate_pairwise_interaction(O.slater_disp_damping_kernel)(
+        T(pos), T(box), pairs, m, c["B"] / 10, torch.sqrt(c["C6"] * 1e6), torch.sqrt(c["C8"] * 1e8),
+        torch.sqrt(c["C10"] * 1e10)))
+    return out
+
+
+def test_peg2_energies_test_slater():
+    """tests/test_admp/test_slater.py:13-25,74-82: all nine energies to decimal 4.
+    ES (5.731787) pins the polarisable multipole path with a CONECT-joined
+    topology and frames across the joint; dispersion (-0.053504415) the p = 6, 8,
+    10 real-space sum and its sign; the short-range terms are below 1e-5 there."""
+    ref = EN["peg2"]
+    got = peg2_oracle_energies()
+    for name, want in zip(ref["forces"], ref["values"]):
+        assert abs(got[name] - want) < 1.5 * 10 ** -ref["decimal"], (name, got[name], want)
