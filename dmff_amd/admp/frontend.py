@@ -172,15 +172,34 @@ def load_residue_bonds(path):
     return out
 
 
+# Water residue and atom names the PDB reader of OpenMM maps to HOH / O, H1, H2
+# (its residue and atom name replacement tables); applied when the force field
+# has no template under the PDB's own name.
+WATER_RESIDUES = ("SOL", "WAT", "TIP3", "T3P", "TIP4", "T4P", "SPC", "H2O")
+WATER_ATOMS = {"OW": "O", "OH2": "O", "HW1": "H1", "HW2": "H2", "H1W": "H1", "H2W": "H2"}
+
+
+def _alias_names(top, tmpl):
+    for i in range(top.n_atoms):
+        if top.res_names[i] not in tmpl and top.res_names[i] in WATER_RESIDUES and "HOH" in tmpl:
+            top.res_names[i] = "HOH"
+            top.atom_names[i] = WATER_ATOMS.get(top.atom_names[i], top.atom_names[i])
+
+
 def assign_types(top, ff, extra_bonds=None):
     """Atom typing from the force field's <Residues> templates (residue and atom
-    names), bonds from the template or from extra residue bond definitions."""
+    names), bonds from the template or from extra residue bond definitions, and
+    per-atom charges where the template carries them (top.charges, else None)."""
     tmpl = ff.residues
+    _alias_names(top, tmpl)
+    charges = []
     for i in range(top.n_atoms):
         r = tmpl.get(top.res_names[i])
         if r is None or top.atom_names[i] not in r["types"]:
             raise DMFFException("Atom %d not matched in forcefield!" % i)
         top.types.append(r["types"][top.atom_names[i]])
+        charges.append(r.get("charges", {}).get(top.atom_names[i]))
+    top.charges = None if any(c is None for c in charges) else np.asarray(charges, dtype=np.float64)
     by_res = {}
     for i, rid in enumerate(top.res_ids):
         by_res.setdefault(rid, {})[top.atom_names[i]] = i
@@ -248,7 +267,10 @@ class ForceFieldXML:
             types = {a.get("name"): a.get("type") for a in r.iter("Atom")}
             bonds = [(names[int(b.get("from"))], names[int(b.get("to"))]) for b in r.iter("Bond")
                      if b.get("from") is not None and b.get("from").lstrip("-").isdigit()]
-            self.residues[r.get("name")] = {"types": types, "bonds": bonds}
+            bonds += [(b.get("atomName1"), b.get("atomName2")) for b in r.iter("Bond")
+                      if b.get("atomName1") is not None and b.get("atomName2") is not None]
+            charges = {a.get("name"): float(a.get("charge")) for a in r.iter("Atom") if a.get("charge") is not None}
+            self.residues[r.get("name")] = {"types": types, "bonds": bonds, "charges": charges}
 
     def force(self, name):
         return self.root.find(name)

@@ -78,3 +78,68 @@ class NeighborList:
 
 
 NeighborListFreud = NeighborList
+
+
+class NoCutoffNeighborList:
+    """dmff/common/nblist.py:73-120: every pair i < j, with the same padding and
+    nbond column.  A host enumeration, as in the reference (O(N^2), for clusters
+    and small molecules)."""
+
+    def __init__(self, cov_map, padding=True):
+        self.capacity_multiplier = None
+        self.padding = padding
+        self.cov_map = cov_map
+        self._excl = _cov_to_exclusions(cov_map)
+
+    def _candidates(self, coords):
+        n = coords.shape[0]
+        i, j = np.triu_indices(n, k=1)
+        return np.stack([i, j], axis=1).astype(np.int64)
+
+    def _finish(self, nlist, n):
+        if self.capacity_multiplier is None:
+            self.capacity_multiplier = int(nlist.shape[0] * 1.3)
+        if self.padding:
+            self.capacity_multiplier = max(self.capacity_multiplier, nlist.shape[0])
+            pad = self.capacity_multiplier - nlist.shape[0]
+            if pad > 0:
+                nlist = np.vstack([nlist, np.full((pad, 2), n, dtype=np.int64)])
+        nb = np.array([self._excl.get((int(a), int(b)), 0) for a, b in nlist], dtype=np.int64)
+        self._pairs = torch.as_tensor(np.concatenate([nlist, nb[:, None]], axis=1))
+        return self._pairs
+
+    def allocate(self, coords, box=None):
+        self._positions = coords
+        c = np.asarray(coords.detach().cpu() if isinstance(coords, torch.Tensor) else coords, dtype=np.float64)
+        return self._finish(self._candidates(c), c.shape[0])
+
+    def update(self, positions, box=None):
+        self.allocate(positions)
+
+    @property
+    def pairs(self):
+        return self._pairs
+
+    @property
+    def scaled_pairs(self):
+        return self._pairs
+
+    @property
+    def positions(self):
+        return self._positions
+
+
+class NoPeriodicNeighborList(NoCutoffNeighborList):
+    """dmff/common/nblist.py:129-160: pairs i < j with a plain (not periodic)
+    distance below rcut."""
+
+    def __init__(self, rcut, cov_map, padding=True):
+        super().__init__(cov_map, padding)
+        self.rcut = float(rcut)
+
+    def allocate(self, coords, box=None):
+        self._positions = coords
+        c = np.asarray(coords.detach().cpu() if isinstance(coords, torch.Tensor) else coords, dtype=np.float64)
+        nl = self._candidates(c)
+        d = np.linalg.norm(c[nl[:, 0]] - c[nl[:, 1]], axis=1)
+        return self._finish(nl[d < self.rcut], c.shape[0])

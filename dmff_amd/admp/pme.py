@@ -111,6 +111,7 @@ class _ADMPEnergy(torch.autograd.Function):
         ctx.g = g
         ctx.shapes = (pos.shape, box.shape, Q_local.shape,
                       pol.shape if pol is not None else None, thole.shape if thole is not None else None)
+        ctx.devs = tuple(x.device if x is not None else None for x in (pos, box, Q_local, pol, thole))
         return E
 
     @staticmethod
@@ -120,6 +121,7 @@ class _ADMPEnergy(torch.autograd.Function):
         if not g:
             return None, None, None, None, None, None
         F = gE.reshape(-1).shape[0]
+        gE = gE.to(g["pos"].device)
         w = gE.reshape(F, 1, 1)
         gp = (w * g["pos"]).reshape(sp) if ctx.needs_input_grad[0] else None
         gb = (w * g["box"]).reshape(sb) if ctx.needs_input_grad[1] else None
@@ -130,7 +132,8 @@ class _ADMPEnergy(torch.autograd.Function):
             gq[:, : q.shape[1]] = q
         ga = (gE.reshape(F, 1) * g["pol"]).sum(0).reshape(sa) if (sa is not None and ctx.needs_input_grad[3]) else None
         gt = (gE.reshape(F, 1) * g["thole"]).sum(0).reshape(st) if (st is not None and ctx.needs_input_grad[4]) else None
-        return gp, gb, gq, ga, gt, None
+        out = [x.to(d) if x is not None else None for x, d in zip((gp, gb, gq, ga, gt), ctx.devs)]
+        return (*out, None)
 
 
 class ADMPPmeForce:

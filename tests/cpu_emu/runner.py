@@ -296,7 +296,31 @@ def pairwise(precision):
     return out
 
 
-CASES = {"pairwise": pairwise, "bad_geometry": bad_geometry, "type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
+def generators(precision):
+    """PEG2 through Hamiltonian -> createPotential -> potential_fn (all nine
+    generators) and the water-dimer neighbour list, on the emulated library."""
+    from dmff_amd.admp.hamiltonian import Hamiltonian
+    from dmff_amd.admp.nblist import NeighborList
+    from dmff_amd.admp import frontend as fe
+    from tests.systems import golden_path
+    H = Hamiltonian(golden_path("peg.xml"))
+    pots = H.createPotential(golden_path("peg2.pdb"), nonbondedMethod="CutoffPeriodic", nonbondedCutoff=1.49,
+                             ethresh=1e-4, device="cpu", precision=precision)
+    pos_nm, top = fe.read_pdb(golden_path("peg2.pdb"))
+    nbl = NeighborList(top.box_nm, 1.49, pots.meta["cov_map"], device="cpu")
+    nbl.allocate(pos_nm)
+    pairs = nbl.pairs
+    out = {"pairs": _lst(pairs)}
+    P = torch.tensor(pos_nm, dtype=F64, requires_grad=True)
+    B = torch.tensor(top.box_nm, dtype=F64)
+    for name, fn in pots.dmff_potentials.items():
+        e = fn(P, B, pairs, H.paramset)
+        (g,) = torch.autograd.grad(e, P)
+        out[name] = {"E": float(e.detach()), "pos": _lst(g)}
+    return out
+
+
+CASES = {"generators": generators, "pairwise": pairwise, "bad_geometry": bad_geometry, "type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
 
 if __name__ == "__main__":
     lib, case = sys.argv[1], sys.argv[2]

@@ -1,5 +1,6 @@
 """Test-side helpers: load the reference's golden systems through the host
 frontend and build reference-format pair lists with the oracle's brute force."""
+import math
 import os
 
 import numpy as np
@@ -133,3 +134,61 @@ def pairwise_oracle(kind, pos, box, pairs, m, prm):
     e = O.generate_pairwise_interaction(O.SR_KERNELS[kind])(p, b, pairs, m, *[q[:, k] for k in range(prm.shape[1])])
     gp, gb, gq = torch.autograd.grad(e, (p, b, q))
     return {"E": float(e.detach()), "pos": gp.numpy(), "box": gb.numpy(), "prm": gq.numpy()}
+
+
+def axis_type_system(seed=8, n_units=12, L=16.0):
+    """Pyramidal XH3 units (C3v, X-H 1.05 A, H-X-H 106 deg) on a jittered lattice in
+    an L A box, every local-frame type in use: the centres cycle through ThreeFold,
+    ZBisect, Bisector, ZThenX, ZOnly and NoAxisType; hydrogens are ZThenX (z = X,
+    x = the next H) or ZOnly (z = X).  Random multipoles (harmonic, e A^l),
+    polarisabilities ~1.0 A^3, Thole 0.39; bonds X-H (1-2) and H-H (1-3).
+    Returns a dict with pos, box, axis_types, axis_idx, bonded, Q (N,9), pol,
+    thole and m/p/d scales."""
+    rng = np.random.default_rng(seed)
+    th = np.radians(106.0)
+    # angle beta between each X-H and the C3 axis: cos(th) = cos^2 b + sin^2 b cos(120)
+    b = math.acos(math.sqrt((2 * math.cos(th) + 1) / 3))
+    Hdir = np.array([[math.sin(b) * math.cos(p), math.sin(b) * math.sin(p), math.cos(b)]
+                     for p in (0.0, 2 * math.pi / 3, 4 * math.pi / 3)])
+    n_side = int(math.ceil(n_units ** (1 / 3)))
+    sites = np.array([[i, j, k] for i in range(n_side) for j in range(n_side) for k in range(n_side)])[:n_units]
+    pos, types, idx, bonded = [], [], [], {}
+    centre_types = [3, 2, 1, 0, 4, 5]
+    for u, s in enumerate(sites):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        c = (s + 0.5) * (L / n_side) + rng.uniform(-0.5, 0.5, 3)
+        base = len(pos)
+        pos.append(c)
+        for h in Hdir:
+            pos.append(c + 1.05 * (R @ h))
+        ct = centre_types[u % 6]
+        h1, h2, h3 = base + 1, base + 2, base + 3
+        types.append(ct)
+        idx.append({3: [h1, h2, h3], 2: [h1, h2, h3], 1: [h1, h2, -1], 0: [h1, h2, -1], 4: [h1, -1, -1],
+                    5: [-1, -1, -1]}[ct])
+        for k, hh in enumerate((h1, h2, h3)):
+            if (u + k) % 2 == 0:
+                types.append(0)
+                idx.append([base, (h1, h2, h3)[(k + 1) % 3], -1])
+            else:
+                types.append(4)
+                idx.append([base, -1, -1])
+        for hh in (h1, h2, h3):
+            bonded[(base, hh)] = bonded[(hh, base)] = 1
+        for a_, b_ in ((h1, h2), (h1, h3), (h2, h3)):
+            bonded[(a_, b_)] = bonded[(b_, a_)] = 2
+    pos = np.asarray(pos)
+    n = len(pos)
+    Q = np.zeros((n, 9))
+    Q[:, 0] = rng.normal(0, 0.3, n)
+    Q[:, 1:4] = rng.normal(0, 0.1, (n, 3))
+    Q[:, 4:9] = rng.normal(0, 0.1, (n, 5))
+    sc = np.array([0.0, 0.0, 1.0, 1.0, 1.0, 1.0])
+    return {"pos": pos, "box": np.eye(3) * L, "axis_types": np.asarray(types), "axis_idx": np.asarray(idx),
+            "bonded": bonded, "Q": Q, "pol": rng.uniform(0.8, 1.2, n), "thole": np.full(n, 0.39),
+            "mScales": sc, "pScales": sc.copy(), "dScales": np.ones(6)}

@@ -224,3 +224,19 @@ def test_short_range_kinds_vs_oracle(emu_lib, prec):
         assert abs(g["E"] - o["E"]) <= tol_e * abs(o["E"]), (kind, g["E"], o["E"])
         for k in ("pos", "box", "prm"):
             assert rel_rms(np.asarray(g[k]), o[k]) <= tol_g, (kind, k, rel_rms(np.asarray(g[k]), o[k]))
+
+
+def test_generators_peg2(emu_lib):
+    """Hamiltonian -> createPotential -> potential_fn for the nine PEG2 generators
+    (the reference's test_slater values to decimal 4, the oracle to 1e-9)."""
+    import json as _json
+    from tests.systems import GOLD
+    from tests.test_oracle_pins import peg2_oracle_terms
+    r = run(emu_lib, "generators", "fp64")
+    ref = _json.load(open(f"{GOLD}/energies.json"))["peg2"]
+    for name, want in zip(ref["forces"], ref["values"]):
+        assert abs(r[name]["E"] - want) < 1.5e-4, (name, r[name]["E"], want)
+    s_terms = peg2_oracle_terms()
+    for name in ref["forces"]:
+        o = float(s_terms[name])
+        assert abs(r[name]["E"] - o) <= 1e-9 * abs(o) + 1e-12, (name, r[name]["E"], o)

@@ -531,6 +531,11 @@ def _xml_columns(path, force, cols):
 
 
 def peg2_oracle_energies():
+    """Floats of peg2_oracle_terms at the PDB positions."""
+    return {k: float(v) for k, v in peg2_oracle_terms().items()}
+
+
+def peg2_oracle_terms(p=None):
     """The nine PEG2 energies of tests/test_admp/test_slater.py by the oracle:
     CutoffPeriodic (kappa = 0), rc 1.49 nm, pairs i<j within rc with nbond from the
     bond graph (the PDB's CONECT records join the two halves), the generators'
@@ -540,10 +545,11 @@ def peg2_oracle_energies():
     pos, box = s.pos_nm * 10, s.box_nm * 10
     rc = 14.9
     pairs = reference_pairs(pos, box, rc, s.bonded)
+    P = T(pos) if p is None else p
     f = O.ADMPPmeForce(box, s.axis_types, s.axis_idx, rc, 1e-4, 2, True, False, None)
-    e_es, _ = f.get_energy(T(pos), T(box), pairs, T(s.pme.Q_local[s.map_atomtype]), T(s.pme.pol[s.map_poltype]),
+    e_es, _ = f.get_energy(P, T(box), pairs, T(s.pme.Q_local[s.map_atomtype]), T(s.pme.pol[s.map_poltype]),
                            T(s.pme.thole[s.map_poltype]), s.pme.mScales, s.pme.pScales, s.pme.dScales, aux={})
-    out = {"ADMPPmeForce": float(e_es)}
+    out = {"ADMPPmeForce": e_es}
     path = golden_path("peg.xml")
     types = s.top.types
 
@@ -554,19 +560,19 @@ def peg2_oracle_energies():
 
     c, m = per_atom("ADMPDispPmeForce", ("C6", "C8", "C10"))
     cl = torch.stack([torch.sqrt(c["C6"] * 1e6), torch.sqrt(c["C8"] * 1e8), torch.sqrt(c["C10"] * 1e10)], dim=1)
-    out["ADMPDispPmeForce"] = -float(O.energy_disp_pme(T(pos), T(box), pairs, cl, m, 0.0, (0, 0, 0), 10, lpme=False))
+    out["ADMPDispPmeForce"] = -O.energy_disp_pme(P, T(box), pairs, cl, m, 0.0, (0, 0, 0), 10, lpme=False)
     for name, sign in (("SlaterExForce", 1), ("SlaterSrEsForce", -1), ("SlaterSrPolForce", -1),
                        ("SlaterSrDispForce", -1), ("SlaterDhfForce", -1)):
         c, m = per_atom(name, ("A", "B"))
-        out[name] = sign * float(O.generate_pairwise_interaction(O.slater_sr_kernel)(T(pos), T(box), pairs, m,
-                                                                                      c["A"], c["B"] / 10))
+        out[name] = sign * O.generate_pairwise_interaction(O.slater_sr_kernel)(P, T(box), pairs, m, c["A"],
+                                                                                c["B"] / 10)
     c, m = per_atom("QqTtDampingForce", ("B", "Q"))
-    out["QqTtDampingForce"] = float(O.generate_pairwise_interaction(O.TT_damping_qq_kernel)(
-        T(pos), T(box), pairs, m, c["B"] / 10, c["Q"]))
+    out["QqTtDampingForce"] = O.generate_pairwise_interaction(O.TT_damping_qq_kernel)(
+        P, T(box), pairs, m, c["B"] / 10, c["Q"])
     c, m = per_atom("SlaterDampingForce", ("B", "C6", "C8", "C10"))
-    out["SlaterDampingForce"] = float(O.generate_pairwise_interaction(O.slater_disp_damping_kernel)(
-        T(pos), T(box), pairs, m, c["B"] / 10, torch.sqrt(c["C6"] * 1e6), torch.sqrt(c["C8"] * 1e8),
-        torch.sqrt(c["C10"] * 1e10)))
+    out["SlaterDampingForce"] = O.generate_pairwise_interaction(O.slater_disp_damping_kernel)(
+        P, T(box), pairs, m, c["B"] / 10, torch.sqrt(c["C6"] * 1e6), torch.sqrt(c["C8"] * 1e8),
+        torch.sqrt(c["C10"] * 1e10))
     return out
 
 
@@ -579,3 +585,124 @@ def test_peg2_energies_test_slater():
     got = peg2_oracle_energies()
     for name, want in zip(ref["forces"], ref["values"]):
         assert abs(got[name] - want) < 1.5 * 10 ** -ref["decimal"], (name, got[name], want)
+
+
+# ----------------------------------------------------------------- local frames: every axis type
+def _rot(seed):
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _frames(pos, types, idx):
+    return O.construct_local_frames(T(pos), T(np.eye(3) * 50.0), np.asarray(types), np.asarray(idx)).numpy()
+
+
+def test_frames_threefold_is_the_c3_axis():
+    """ThreeFold (spatial.py:120-132): for three neighbours placed C3-symmetrically
+    about an axis n (unequal distances), z is n; x lies in the plane of n and the
+    x neighbour, on its side; the frame is orthonormal and right-handed."""
+    R = _rot(1)
+    n, e1, e2 = R[:, 2], R[:, 0], R[:, 1]
+    beta = math.radians(70.0)
+    c = np.array([10.0, 11.0, 12.0])
+    nb = [c + d * (math.cos(beta) * n + math.sin(beta) * (math.cos(p) * e1 + math.sin(p) * e2))
+          for d, p in ((1.0, 0.3), (1.4, 0.3 + 2 * math.pi / 3), (0.8, 0.3 + 4 * math.pi / 3))]
+    pos = np.array([c] + nb)
+    F = _frames(pos, [O.ThreeFold, 5, 5, 5], [[1, 2, 3], [-1, -1, -1], [-1, -1, -1], [-1, -1, -1]])[0]
+    assert np.allclose(F[2], n, atol=1e-12)
+    ux = (nb[1] - c) / np.linalg.norm(nb[1] - c)
+    xp = ux - n * (ux @ n)
+    assert np.allclose(F[0], xp / np.linalg.norm(xp), atol=1e-12)
+    assert np.allclose(F @ F.T, np.eye(3), atol=1e-12) and np.linalg.det(F) > 0.999999
+
+
+def test_frames_zbisect_bisects_normalised_bonds():
+    """ZBisect (spatial.py:113-118): z toward the z neighbour; x bisects the unit
+    vectors to the x and y neighbours (bond lengths 0.9 and 1.7 A differ, so an
+    unnormalised sum would tilt it), made orthogonal to z."""
+    R = _rot(2)
+    c = np.array([5.0, 6.0, 7.0])
+    pz = c + 1.1 * R[:, 2]
+    px = c + 0.9 * (R[:, 0] + R[:, 1]) / math.sqrt(2)
+    py = c + 1.7 * (R[:, 0] - R[:, 1]) / math.sqrt(2)
+    pos = np.array([c, pz, px, py])
+    F = _frames(pos, [O.ZBisect, 5, 5, 5], [[1, 2, 3], [-1, -1, -1], [-1, -1, -1], [-1, -1, -1]])[0]
+    assert np.allclose(F, R.T, atol=1e-12)   # rows x, y, z = columns of R
+
+
+def test_frames_zonly_rounding_rule():
+    """ZOnly (spatial.py:99-102): the x seed is the lab x axis while |z_x| < 1/2 and
+    the lab y axis otherwise, then orthogonalised against z."""
+    c = np.array([3.0, 3.0, 3.0])
+    for zdir, seed in (((0.3, 0.5, 0.8), np.array([1.0, 0, 0])), ((0.9, 0.1, 0.3), np.array([0, 1.0, 0]))):
+        z = np.array(zdir) / np.linalg.norm(zdir)
+        pos = np.array([c, c + 1.3 * z])
+        F = _frames(pos, [O.ZOnly, 5], [[1, -1, -1], [-1, -1, -1]])[0]
+        assert np.allclose(F[2], z, atol=1e-12)
+        # x in span{seed, z}, orthogonal to z, with a positive seed component
+        assert abs(F[0] @ z) < 1e-12 and F[0] @ seed > 0
+        assert abs(np.linalg.det(np.stack([F[0], seed, z]))) < 1e-12
+        assert np.allclose(F @ F.T, np.eye(3), atol=1e-12) and np.linalg.det(F) > 0.999999
+
+
+def test_frames_rotate_with_the_molecule():
+    """Every axis type except ZOnly (lab-fixed seed) and NoAxisType: rotating the
+    molecule rotates each frame, F' = F R^T; all frames orthonormal, right-handed."""
+    from tests.systems import axis_type_system
+    s = axis_type_system()
+    F = _frames(s["pos"], s["axis_types"], s["axis_idx"])
+    R = _rot(5)
+    ctr = s["pos"].mean(0)
+    F2 = _frames((s["pos"] - ctr) @ R.T + ctr, s["axis_types"], s["axis_idx"])
+    for k, t in enumerate(s["axis_types"]):
+        assert np.allclose(F[k] @ F[k].T, np.eye(3), atol=1e-12) and np.linalg.det(F[k]) > 0.999999
+        if t not in (O.ZOnly, O.NoAxisType):
+            assert np.allclose(F2[k], F[k] @ R.T, atol=1e-10), (k, t)
+    assert set(int(t) for t in s["axis_types"]) == {0, 1, 2, 3, 4, 5}
+
+
+# ----------------------------------------------------------------- gromacs kappa rule, charge-only PME
+def coul_pme_system():
+    """methane + 2 waters of tests/test_classical/test_coul.py:84-129: charges from the
+    residue templates, mscales (0, 0, coulomb14scale, 1, 1, 1) (generators/classical.py:878),
+    gromacs kappa with ethresh 1e-6 (the generator's default, :905) and spacing
+    0.1 nm on the PDB cell, pairs within 0.5 nm of the 1.2 nm box the test passes."""
+    from dmff_amd.admp import frontend as fe
+    from tests.systems import golden_path
+    import xml.etree.ElementTree as ET
+    c = EN["coul_pme"]
+    ff = fe.ForceFieldXML(golden_path("methane_water_coul.xml"))
+    pos_nm, top = fe.read_pdb(golden_path("methane_water.pdb"))
+    fe.assign_types(top, ff)
+    bonded = fe.bonded_distances(top.n_atoms, top.bonds)
+    s14 = float(ET.parse(golden_path("methane_water_coul.xml")).getroot().find("NonbondedForce").get("coulomb14scale"))
+    kappa_nm, K1, K2, K3 = O.setup_ewald_parameters(c["rc_nm"], 1e-6, top.box_nm, c["spacing_nm"], "gromacs")
+    box_nm = np.asarray(c["box_nm"], dtype=np.float64)
+    pairs = reference_pairs(pos_nm, box_nm, c["rc_nm"], bonded)
+    return dict(pos_nm=pos_nm, box_nm=box_nm, charges=top.charges, pairs=pairs, K=(K1, K2, K3),
+                kappa_nm=kappa_nm, mscales=np.array([0.0, 0.0, s14, 1.0, 1.0, 1.0]), bonded=bonded)
+
+
+def test_gromacs_kappa_rule():
+    """pme.py:467-488: kappa is the bisected root of erfc(kappa rc) = ethresh and
+    K_d = ceil(box_dd / spacing)."""
+    from scipy.special import erfc
+    for rc, eth in ((0.5, 1e-6), (0.9, 5e-4), (1.2, 1e-5)):
+        kappa, K1, K2, K3 = O.setup_ewald_parameters(rc, eth, np.diag([1.2, 2.05, 3.0]), 0.1, "gromacs")
+        assert abs(erfc(kappa * rc) - eth) < 1e-10 * eth
+        assert (K1, K2, K3) == (12, 21, 30)
+
+
+def test_coul_pme_test_coul():
+    """tests/test_classical/test_coul.py:84-129: -3.540451 (atol 1e-2) from the
+    lmax-0 ADMP PME path with the gromacs kappa rule."""
+    s = coul_pme_system()
+    e = O.energy_pme(T(s["pos_nm"] * 10), T(s["box_nm"] * 10), s["pairs"], T(s["charges"]).reshape(-1, 1),
+                     None, None, None, s["mscales"], None, None, None, None, s["kappa_nm"] / 10, s["K"], 0, False)
+    assert s["K"] == (12, 12, 12)
+    assert abs(float(e) - EN["coul_pme"]["value"]) < EN["coul_pme"]["atol"], float(e)
