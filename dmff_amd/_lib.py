@@ -26,7 +26,8 @@ SYMBOLS = [
     "admp_version", "admp_ctx_create", "admp_ctx_destroy", "admp_last_error", "admp_set_stream",
     "admp_set_topology", "admp_set_pairs", "admp_build_neighbors", "admp_get_pairs", "admp_compute",
     "admp_disp_compute", "admp_slater_compute", "admp_profile", "admp_profile_get", "admp_memory_bytes",
-    "admp_type_reduce", "admp_pairwise_compute", "admp_pairwise_compute_noalloc",
+    "admp_type_reduce", "admp_pairwise_compute", "admp_pairwise_compute_noalloc", "admp_local_to_global",
+    "admp_esp", "admp_add_forces_fixed",
 ]
 
 # admp_sr_kind and each kind's per-site parameter count
@@ -93,6 +94,9 @@ def load():
     lib.admp_memory_bytes.argtypes = [P, ctypes.POINTER(i64)]
     lib.admp_type_reduce.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P, ctypes.c_int32, P, P]
     lib.admp_pairwise_compute.argtypes = [P, i32, P, P, P, i32, P, i32, i32, P, P, P, P]
+    lib.admp_local_to_global.argtypes = [P, P, P, P, P]
+    lib.admp_esp.argtypes = [i32, P, P, P, i32, P, P, P]
+    lib.admp_add_forces_fixed.argtypes = [P, d, P, P, i32, i32, P]
     lib.admp_pairwise_compute_noalloc.argtypes = [i32, i32, i32, P, P, P, i64, i32, P, i32, P, i32, i32, P, P, P,
                                                   P, P]
     for name in SYMBOLS:

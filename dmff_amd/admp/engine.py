@@ -251,3 +251,17 @@ class Engine:
                                             _ptr(g.get("pos")), _ptr(g.get("box")), _ptr(g.get("prm")))
         L.check(self.h, st, "admp_pairwise_compute")
         return E, g
+
+    def local_to_global(self, pos, box, Q_local):
+        """Q_local_to_global (dmff/admp/pme.py:298-311): per-site local harmonic
+        multipoles (N, nh) -> global Cartesian (F, N, 10)."""
+        F, N = self.n_frames, self.n_atoms
+        dev = self.device
+        pos = _dev(pos, dev).reshape(F, N, 3)
+        box = _dev(box, dev).reshape(F, 3, 3)
+        Q = _dev(Q_local, dev).reshape(N, -1)[:, : self.nh].contiguous()
+        out = torch.empty((F, N, 10), dtype=F64, device=dev)
+        self._stream()
+        L.check(self.h, self.lib.admp_local_to_global(self.h, _ptr(pos), _ptr(box), _ptr(Q), _ptr(out)),
+                "admp_local_to_global")
+        return out

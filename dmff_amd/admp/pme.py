@@ -255,6 +255,54 @@ class ADMPPmeForce:
             return E, aux
         return E
 
+    def Q_local_to_global(self, positions, box, Q_local):
+        """pme.py:298-311: per-type local harmonic multipoles (expanded with
+        map_atomtype) rotated into global Cartesian [q, mu, Theta_xx, Theta_yy,
+        Theta_zz, Theta_xy, Theta_xz, Theta_yz] (N, 10).  Values only (no autograd)."""
+        Q = torch.as_tensor(np.asarray(Q_local.detach().cpu() if isinstance(Q_local, torch.Tensor) else Q_local),
+                            dtype=F64)
+        Qa = Q[torch.as_tensor(np.asarray(self.map_atomtype))]
+        eng = self._single_engine()
+        return eng.local_to_global(positions, box, Qa)[0]
+
+    def _single_engine(self):
+        if self.n_frames == 1:
+            return self.engine
+        if getattr(self, "_engine1", None) is None:
+            e = Engine(self.n_atoms, self.lmax, self.lpol, self.lpme, self.kappa, (self.K1, self.K2, self.K3), self.rc,
+                       n_frames=1, precision=self.precision, device=self.device)
+            ai = self.axis_indices if self.axis_indices is not None else np.full((self.n_atoms, 3), -1)
+            e.set_topology(self.axis_type, ai)
+            self._engine1 = e
+        return self._engine1
+
+    def generate_esp(self):
+        """pme.py:236-296: returns get_esp(positions, grids, Q_local[, U_ind]) giving
+        the potential (kJ/mol/e) of the multipoles (per-type Q_local, frames from
+        positions in a 1000-unit cubic box) and induced dipoles (e nm) at the probe
+        points, positions and probes in nm, no periodic images.  Values only."""
+        lib = L.load()
+
+        def esp(positions, grids, Q_local, U_ind=None):
+            eng = self._single_engine()
+            dev = eng.device
+            pos = _dev(positions, dev).reshape(-1, 3)
+            gr = _dev(grids, dev).reshape(-1, 3)
+            Qg = self.Q_local_to_global(pos, torch.eye(3, dtype=F64, device=dev) * 1000.0, Q_local).contiguous()
+            U = _dev(U_ind, dev).reshape(-1, 3) if (U_ind is not None and self.lpol) else None
+            out = torch.empty(gr.shape[0], dtype=F64, device=dev)
+            stream = torch.cuda.current_stream(dev).cuda_stream if dev.type == "cuda" else 0
+            st = lib.admp_esp(pos.shape[0], ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(Qg.data_ptr()),
+                              ctypes.c_void_p(U.data_ptr() if U is not None else 0), gr.shape[0],
+                              ctypes.c_void_p(gr.data_ptr()), ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream))
+            if st != L.ADMP_OK:
+                raise L.ADMPError(f"admp_esp failed with status {st}")
+            return out
+
+        if self.lpol:
+            return lambda positions, grids, Q_local, U_ind: esp(positions, grids, Q_local, U_ind)
+        return lambda positions, grids, Q_local: esp(positions, grids, Q_local)
+
     def get_forces(self, *args, **kwargs):
         """value_and_grad of get_energy with respect to positions (pme.py:339)."""
         positions = args[0]

@@ -2062,6 +2062,53 @@ extern "C" admp_status admp_pairwise_compute(admp_ctx* ctx, int32_t kind, const 
                                          dbox, dprm));
 }
 
+// Q_local_to_global (dmff/admp/pme.py:298-311): the frames' rotation of every
+// frame's per-site local harmonic multipoles to global Cartesian
+// [q, mu_x, mu_y, mu_z, Theta_xx, Theta_yy, Theta_zz, Theta_xy, Theta_xz, Theta_yz]
+static admp_status admp_local_to_global_impl(admp_ctx* ctx, const double* pos, const double* box,
+                                             const double* Q_local, double* Q_global) {
+    if (!ctx || !pos || !box || !Q_local || !Q_global) return ADMP_ERR_ARG;
+    if (!ctx->has_topo) { ctx->err = "admp_set_topology first"; return ADMP_ERR_STATE; }
+    { const admp_status ns = nbr_settle(ctx); if (ns != ADMP_OK) return ns; }
+    admp_status s = upload_geometry(ctx, pos, box);
+    if (s != ADMP_OK) return s;
+    const int N = ctx->N, F = ctx->F;
+    hipLaunchKernelGGL(k_frames_fwd, dim3(nblk(N, 128), F), dim3(128), 0, ctx->stream, N, ctx->nh, ctx->use_frames,
+                       ctx->d_pos, ctx->d_box, ctx->d_axis_type, ctx->d_axis_idx, Q_local, ctx->d_frames, ctx->d_Qc);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(Q_global, ctx->d_Qc, (size_t)N * F * 10 * sizeof(double), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    if ((s = geom_fetch(ctx)) != ADMP_OK) return s;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return geom_check(ctx);
+}
+
+extern "C" admp_status admp_local_to_global(admp_ctx* ctx, const double* pos, const double* box,
+                                            const double* Q_local, double* Q_global) {
+    ADMP_WRAP(admp_local_to_global_impl(ctx, pos, box, Q_local, Q_global));
+}
+
+extern "C" admp_status admp_esp(int32_t n_sites, const double* pos, const double* Q_global, const double* U,
+                                int32_t n_grid, const double* grids, double* esp, void* stream) {
+    if (n_sites < 1 || n_grid < 0 || !pos || !Q_global || (n_grid > 0 && (!grids || !esp))) return ADMP_ERR_ARG;
+    if (n_grid == 0) return ADMP_OK;
+    hipLaunchKernelGGL(k_esp, dim3(n_grid), dim3(256), 0, (hipStream_t)stream, n_sites, n_grid, pos, Q_global, U,
+                       grids, esp);
+    return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+}
+
+extern "C" admp_status admp_add_forces_fixed(const double* dE_dpos, double scale, const int32_t* atom_index,
+                                             int64_t* force_buffers, int32_t n_atoms, int32_t padded_n_atoms,
+                                             void* stream) {
+    if (n_atoms < 0 || padded_n_atoms < n_atoms || (n_atoms > 0 && (!dE_dpos || !force_buffers)))
+        return ADMP_ERR_ARG;
+    if (n_atoms == 0) return ADMP_OK;
+    hipLaunchKernelGGL(k_add_forces_fixed, dim3(std::min<unsigned>(nblk(n_atoms, 256), 4096)), dim3(256), 0,
+                       (hipStream_t)stream, n_atoms, padded_n_atoms, scale, dE_dpos, atom_index,
+                       reinterpret_cast<long long*>(force_buffers));
+    return hipGetLastError() == hipSuccess ? ADMP_OK : ADMP_ERR_HIP;
+}
+
 extern "C" admp_status admp_type_reduce(const double* per_site, int32_t n_frames, int32_t n_sites, int32_t d,
                                         const int32_t* type_map, int32_t n_types, double* out, void* stream) {
     if (n_frames < 1 || n_sites < 0 || d < 1 || n_types < 1 || !type_map || !out || (n_sites > 0 && !per_site))

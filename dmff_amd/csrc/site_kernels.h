@@ -1001,3 +1001,63 @@ __global__ void __launch_bounds__(256) k_copy4(CopyJobs j) {
     if (i < j.n[c]) j.dst[c][i] = j.src[c][i];
 }
 }  // namespace admp
+
+namespace admp {
+
+// Electrostatic potential at probe points from global Cartesian multipoles
+// (dmff/admp/pme.py:236-296, esp_kernel): for probe g and site i with
+// R = r_i - g (nm), r = sqrt(R.R + 1e-16),
+//   phi = q/r - (mu/10 + U).R / r^3 + 3 (Theta/300):RR / r^5,  times DIELECTRIC/10
+// (mu in e A, Theta in e A^2 as the library holds them, U in e nm).  One block
+// per probe; the sites are summed in a fixed order (per-thread strided partial
+// sums, then the block sum).
+__global__ void k_esp(int N, int G, const double* __restrict__ pos, const double* __restrict__ Qc,
+                      const double* __restrict__ U, const double* __restrict__ grid, double* __restrict__ out) {
+    __shared__ double sh[16];
+    const int g = blockIdx.x;
+    if (g >= G) return;
+    const double gx = grid[g * 3], gy = grid[g * 3 + 1], gz = grid[g * 3 + 2];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const double Rx = pos[i * 3] - gx, Ry = pos[i * 3 + 1] - gy, Rz = pos[i * 3 + 2] - gz;
+        const double r2 = Rx * Rx + Ry * Ry + Rz * Rz + 1e-16;
+        const double r = sqrt(r2);
+        const double rr1 = 1.0 / r, rr2 = rr1 * rr1, rr3 = rr1 * rr2, rr5 = 3.0 * rr3 * rr2;
+        const double* c = Qc + (size_t)i * 10;
+        double phi = c[0] * rr1;
+        double scd = (c[1] * Rx + c[2] * Ry + c[3] * Rz) / 10.0;
+        const double scu = U ? (U[i * 3] * Rx + U[i * 3 + 1] * Ry + U[i * 3 + 2] * Rz) : 0.0;
+        phi -= (scd + scu) * rr3;
+        const double qxx = c[4] / 300.0, qyy = c[5] / 300.0, qzz = c[6] / 300.0;
+        const double qxy = c[7] / 300.0, qxz = c[8] / 300.0, qyz = c[9] / 300.0;
+        double scq = Rx * (qxx * Rx + qxy * Ry + qxz * Rz);
+        scq += Ry * (qxy * Rx + qyy * Ry + qyz * Rz);
+        scq += Rz * (qxz * Rx + qyz * Ry + qzz * Rz);
+        phi += scq * rr5;
+        acc += phi * DIELECTRIC * 0.1;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[g] = s;
+}
+
+}  // namespace admp
+
+namespace admp {
+
+// OpenMM-style force accumulation (backend/openmm_dmff_plugin/platforms/cuda/src/
+// kernels/DMFFForce.cu:1-10): force = -scale * dE/dpos of the atom that
+// atom_index maps to, added to the fixed-point buffers (2^32 units) laid out as
+// [x..., y..., z...] with row length padded.  One thread per buffer slot, so
+// each slot is written by one thread (no atomics).
+__global__ void k_add_forces_fixed(int N, int padded, double scale, const double* __restrict__ dEdpos,
+                                   const int* __restrict__ atom_index, long long* __restrict__ buf) {
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += blockDim.x * gridDim.x) {
+        const int src = atom_index ? atom_index[a] : a;
+        for (int c = 0; c < 3; ++c) {
+            const double f = -scale * dEdpos[(size_t)src * 3 + c];
+            buf[a + (size_t)c * padded] += (long long)(f * 4294967296.0);
+        }
+    }
+}
+
+}  // namespace admp

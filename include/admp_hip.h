@@ -16,6 +16,9 @@
  *   dmff/admp/disp_pme.py:13-77 ADMPDispPmeForce.get_energy  admp_disp_compute
  *   dmff/admp/pairwise.py:47-95,156-169 slater_sr_kernel       admp_slater_compute
  *   dmff/admp/pairwise.py:47-169 generate_pairwise_interaction   admp_pairwise_compute
+ *   dmff/admp/pme.py:298-311  Q_local_to_global    admp_local_to_global
+ *   dmff/admp/pme.py:236-296  generate_esp         admp_local_to_global + admp_esp
+ *   openmm_dmff_plugin CudaDMFFKernels.cpp:200-220 addForces    admp_add_forces_fixed
  *
  * Conventions
  *   - Units are the reference backend's: Angstrom, e, kJ/mol (DIELECTRIC =
@@ -169,6 +172,34 @@ admp_status admp_slater_compute(admp_ctx* ctx, const double* pos, const double* 
                                 const double* a, const double* b, const double* mScales,
                                 int32_t n_scales, int32_t want_grad, double* energy,
                                 double* dpos, double* dbox, double* da, double* db);
+
+/* Q_local_to_global (dmff/admp/pme.py:298-311) for every frame: local harmonic
+ * multipoles Q_local (N, (lmax+1)^2) per site, rotated by the local frames of
+ * pos (F,N,3) / box (F,3,3), to global Cartesian Q_global (F,N,10)
+ * [q, mu_x, mu_y, mu_z, Theta_xx, Theta_yy, Theta_zz, Theta_xy, Theta_xz, Theta_yz]
+ * in the reference's C1_h2c / C2_h2c convention.  Synchronises once. */
+admp_status admp_local_to_global(admp_ctx* ctx, const double* pos, const double* box, const double* Q_local,
+                                 double* Q_global);
+
+/* generate_esp's esp_kernel (dmff/admp/pme.py:236-296): the electrostatic
+ * potential (kJ/mol/e) at n_grid probe points grids (n_grid,3) from n_sites
+ * global multipoles Q_global (n_sites,10, as admp_local_to_global writes them)
+ * at pos (n_sites,3) and induced dipoles U (n_sites,3, e nm; NULL = none);
+ * positions and probes in nm, no periodic images.  esp (n_grid,) out.  Device
+ * pointers; needs no context; asynchronous on stream. */
+admp_status admp_esp(int32_t n_sites, const double* pos, const double* Q_global, const double* U, int32_t n_grid,
+                     const double* grids, double* esp, void* stream);
+
+/* MD integration: CudaCalcDMFFForceKernel::execute's force hand-off
+ * (backend/openmm_dmff_plugin/platforms/cuda/src/CudaDMFFKernels.cpp:200-220,
+ * kernels/DMFFForce.cu:1-10).  Adds force = -scale * dE/dpos to OpenMM-style
+ * fixed-point force buffers (long long, 2^32 units per kJ/mol/nm; layout
+ * [x[padded], y[padded], z[padded]]): slot a receives the atom atom_index[a]
+ * (NULL = identity).  scale converts the library's kJ/mol/A to the buffer's
+ * unit (10 for kJ/mol/nm).  dE_dpos (n_atoms,3), atom_index, force_buffers:
+ * device pointers; asynchronous on stream; needs no context. */
+admp_status admp_add_forces_fixed(const double* dE_dpos, double scale, const int32_t* atom_index,
+                                  int64_t* force_buffers, int32_t n_atoms, int32_t padded_n_atoms, void* stream);
 
 /* Short-range pair kernels of generate_pairwise_interaction (dmff/admp/pairwise.py:47-95)
  * and their per-site parameters, in the kernel's argument order:

@@ -968,6 +968,46 @@ def slater_disp_damping_kernel(dr, m, bi, bj, c6i, c6j, c8i, c8j, c10i, c10j):
     return (exp_x * s6 * c6 / dr ** 6 + exp_x * s8 * c8 / dr ** 8 + exp_x * s10 * c10 / dr ** 10) * m
 
 
+def Q_local_to_global(positions, box, Q_local, axis_types, axis_indices, lmax=2):
+    """pme.py:298-311: frames, local->global harmonic rotation, then C1_h2c / C2_h2c
+    to Cartesian [C, D(3), Q(6)] (Q_local per atom)."""
+    frames = construct_local_frames(positions, box, axis_types, axis_indices)
+    Qg = rot_local2global(Q_local, frames, lmax)
+    n = Qg.shape[0]
+    out = torch.zeros((n, 10), dtype=F64)
+    out[:, 0] = Qg[:, 0]
+    if lmax >= 1:
+        out[:, 1:4] = Qg[:, 1:4] @ C1_c2h            # C1_h2c . D  (rows x, y, z)
+    if lmax >= 2:
+        out[:, 4:10] = convert_harm2cart(Qg, 2)[:, 4:10]
+    return out
+
+
+def esp(positions, grids, Q_global_c, U_ind=None):
+    """pme.py:240-283 (esp_kernel, esp_point, esp_grid): potential at each grid point,
+    positions/grids in nm, dipoles /10, quadrupoles /300, times DIELECTRIC * 0.1."""
+    out = []
+    for g in grids:
+        dR = positions - g
+        r2 = torch.sum(dR * dR, dim=1) + 1e-16
+        r = torch.sqrt(r2)
+        rr1 = 1.0 / r
+        rr2 = rr1 * rr1
+        rr3 = rr1 * rr2
+        pot = Q_global_c[:, 0] * rr1
+        scd = torch.sum(Q_global_c[:, 1:4] / 10.0 * dR, dim=1)
+        scu = torch.sum(U_ind * dR, dim=1) if U_ind is not None else 0.0
+        pot = pot - (scd + scu) * rr3
+        rr5 = 3.0 * rr3 * rr2
+        q = Q_global_c[:, 4:10] / 300.0
+        xx, yy, zz, xy, xz, yz = (q[:, k] for k in range(6))
+        x, y, z = dR[:, 0], dR[:, 1], dR[:, 2]
+        scq = x * (xx * x + xy * y + xz * z) + y * (xy * x + yy * y + yz * z) + z * (xz * x + yz * y + zz * z)
+        pot = pot + scq * rr5
+        out.append(torch.sum(pot * DIELECTRIC * 0.1))
+    return torch.stack(out)
+
+
 # kernel objects by the library's admp_sr_kind number (include/admp_hip.h)
 SR_KERNELS = {1: slater_sr_kernel, 2: TT_damping_qq_c6_kernel, 3: TT_damping_qq_kernel, 4: slater_disp_damping_kernel}
 
@@ -1040,5 +1080,6 @@ PINNING = {
     "disp_pme": "direct dispersion Ewald sum; g_p series; self closed form",
     "slater_sr_energy": "closed form on a pair",
     "brute_force_pairs": "pair count 15 for water dimer at 4 A (test_nblist.py:61-69); O(N^2) loop",
+    "Q_local_to_global/esp": "test_esp.py:57-63: E(water+ion) - E(water) equals the ESP at the ion (rtol 1e-3)",
     "gradients": "central finite differences of the restated energy",
 }

@@ -192,3 +192,111 @@ def axis_type_system(seed=8, n_units=12, L=16.0):
     return {"pos": pos, "box": np.eye(3) * L, "axis_types": np.asarray(types), "axis_idx": np.asarray(idx),
             "bonded": bonded, "Q": Q, "pol": rng.uniform(0.8, 1.2, n), "thole": np.full(n, 0.39),
             "mScales": sc, "pScales": sc.copy(), "dScales": np.ones(6)}
+
+
+def triclinic_water_system(n_waters=96, box=None, seed=31):
+    """Polarisable waters (the benchmark recipe's multipoles, polarisabilities,
+    Thole and scales) in a triclinic cell (rows a, b, c; TRICLINIC_BOX scaled to
+    about liquid density by default): O sites drawn uniformly in fractional
+    coordinates, redrawn while any minimum-image O-O distance is below 2.5 A,
+    random orientations.  Returns (S, positions A, box A)."""
+    from dmff_amd.admp.water import HOH, OH, _random_rotation, make_water_system
+    B = np.asarray(box if box is not None else TRICLINIC_BOX * (n_waters / 216.0) ** (1 / 3), dtype=np.float64)
+    S = make_water_system(n_waters, 1.0, seed)           # types, frames, multipoles; positions replaced below
+    rng = np.random.default_rng(seed)
+    Bi = np.linalg.inv(B)
+    h = np.array([[OH * math.sin(HOH / 2), 0.0, OH * math.cos(HOH / 2)],
+                  [-OH * math.sin(HOH / 2), 0.0, OH * math.cos(HOH / 2)]]) * 10.0
+    O = []
+    pos = []
+    for _ in range(n_waters):
+        for _t in range(5000):
+            o = rng.uniform(0, 1, 3) @ B
+            if not O:
+                break
+            d = np.asarray(O) - o
+            s = d @ Bi
+            s -= np.floor(s + 0.5)
+            d = s @ B
+            if np.min(np.einsum("ij,ij->i", d, d)) >= 2.5 ** 2:
+                break
+        else:
+            raise RuntimeError("could not place a water")
+        R = _random_rotation(rng)
+        O.append(o)
+        pos += [o, o + R @ h[0], o + R @ h[1]]
+    S.positions_nm = np.asarray(pos) / 10.0
+    S.box_nm = B / 10.0
+    return S, np.asarray(pos), B
+
+
+def cfg3_recipe_system(n_waters=512, seed=1003):
+    """Configuration 3's recipe at a reduced size: polarisable waters at liquid
+    density (L = 2.48 nm for 512 waters, so 2 rc <= L at rc 1.0 nm) with per-atom
+    sqrt(C6, C8, C10) (+-10% jitter of the O/H values, A units) and per-type Slater
+    A (prefactor A_i A_j in [1e4, 1e5] kJ/mol) and B (30-45 1/nm, in 1/A).
+    Returns (S, positions A, box A, c_list, A, B)."""
+    from dmff_amd.admp.water import make_water_system
+    L = (n_waters / 33.43) ** (1 / 3)
+    n_side = int(math.ceil(n_waters ** (1 / 3) - 1e-9))
+    S = make_water_system(n_waters, L, seed, polarizable=True, n_side=n_side)
+    rng = np.random.default_rng(seed + 1)
+    C = np.array([[2.0e-3, 1.0e-4, 6.0e-6], [1.0e-4, 3.0e-6, 1.0e-7]])
+    t = S.map_atomtype
+    Ca = C[t] * rng.uniform(0.9, 1.1, size=(S.n_atoms, 3)) * np.array([1e6, 1e8, 1e10])
+    A_t = np.sqrt(rng.uniform(1e4, 1e5, size=2))
+    B_t = rng.uniform(30.0, 45.0, size=2) / 10.0
+    return S, S.positions_nm * 10.0, S.box_nm * 10.0, np.sqrt(Ca), A_t[t], B_t[t]
+
+
+def oracle_pme(pos, box, pairs, axis_types, axis_idx, Q, pol, thole, scales, rc, lpol, lpme=True, tol=1e-10,
+               ethresh=5e-4, lmax=2):
+    """Energy and gradients (pos, box, Q, pol, thole) and U by the oracle, CG to tol."""
+    import torch
+    from oracle import admp_oracle as O
+    T = lambda x: torch.tensor(np.asarray(x, dtype=np.float64), requires_grad=True)
+    f = O.ADMPPmeForce(np.asarray(box), axis_types, axis_idx, rc, ethresh, lmax, lpol, lpme, None)
+    p, b, q = T(pos), T(box), T(Q)
+    mS, pS, dS = scales
+    out = {}
+    if lpol:
+        a, t = T(pol), T(thole)
+        e, aux = f.get_energy(p, b, pairs, q, a, t, mS, pS, dS, solver="cg", tol=tol, aux={})
+        gp, gb, gq, ga, gt = torch.autograd.grad(e, (p, b, q, a, t))
+        out.update(pol=ga.numpy(), thole=gt.numpy(), U=aux["U_ind"].detach().numpy() * 10.0)
+    else:
+        e = f.get_energy(p, b, pairs, q, mScales=mS)
+        gp, gb, gq = torch.autograd.grad(e, (p, b, q))
+    out.update(E=float(e.detach()), pos=gp.numpy(), box=gb.numpy(), Q=gq.numpy())
+    return out
+
+
+def library_pme(pos, box, axis_types, axis_idx, bonded, Q, pol, thole, scales, rc, lpol, precision="fp64",
+                device="cuda:0", pairs=None, tol=1e-10, ethresh=5e-4, lmax=2):
+    """The same through the library's Engine: pairs given (reference format) or
+    built by the library (pairs=None, then also returns them)."""
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    import torch
+    kappa, K1, K2, K3 = setup_ewald_parameters(rc, ethresh, np.asarray(box))
+    eng = Engine(len(pos), lmax, lpol, True, kappa, (K1, K2, K3), rc, precision=precision, device=device)
+    eng.set_topology(axis_types, axis_idx, bonded)
+    out = {}
+    if pairs is None:
+        eng.build_neighbors(pos, box)
+        out["pairs"] = eng.get_pairs(0).cpu().numpy().astype(np.int64)
+    else:
+        eng.set_pairs(torch.as_tensor(pairs))
+    E, g, U, info = eng.compute(pos, box, Q, pol if lpol else None, thole if lpol else None, *scales,
+                                solver="pcg", tol=tol, maxiter=1000, want_grad=True)
+    out.update(E=float(E[0]), pos=g["pos"][0].cpu().numpy(), box=g["box"][0].cpu().numpy(),
+               Q=g["Q"][0].cpu().numpy(), n_iter=info["n_iter"][0], K=(K1, K2, K3))
+    if lpol:
+        out.update(pol=g["pol"][0].cpu().numpy(), thole=g["thole"][0].cpu().numpy(), U=U[0].cpu().numpy())
+    return out
+
+
+def rel_err(a, b, floor=1e-6):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), floor))

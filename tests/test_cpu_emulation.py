@@ -240,3 +240,55 @@ def test_generators_peg2(emu_lib):
     for name in ref["forces"]:
         o = float(s_terms[name])
         assert abs(r[name]["E"] - o) <= 1e-9 * abs(o) + 1e-12, (name, r[name]["E"], o)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_every_axis_type_vs_oracle(emu_lib, prec):
+    from tests.systems import axis_type_system, oracle_pme, reference_pairs, rel_err
+    s = axis_type_system()
+    pairs = reference_pairs(s["pos"], s["box"], 7.0, s["bonded"])
+    o = oracle_pme(s["pos"], s["box"], pairs, s["axis_types"], s["axis_idx"], s["Q"], s["pol"], s["thole"],
+                   (s["mScales"], s["pScales"], s["dScales"]), 7.0, True)
+    g = run(emu_lib, "axis_types", prec)
+    te, tg = (1e-9, 1e-7) if prec == "fp64" else (2e-5, 1e-4)
+    assert abs(g["E"] - o["E"]) < te * abs(o["E"])
+    keys = ("pos", "box", "Q", "pol", "thole") if prec == "fp64" else ("pos",)
+    for k in keys:
+        assert rel_err(g[k], o[k]) < tg, (k, rel_err(g[k], o[k]))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_triclinic_box_vs_oracle(emu_lib, prec):
+    from tests.systems import oracle_pme, reference_pairs, rel_err, triclinic_water_system
+    S, pos, box = triclinic_water_system()
+    ref = reference_pairs(pos, box, 6.0, S.bonded)
+    g = run(emu_lib, "triclinic", prec)
+    got = sorted_pairs(g["pairs"])
+    assert np.array_equal(got, sorted_pairs(ref))
+    o = oracle_pme(pos, box, ref, S.axis_types, S.axis_idx, S.Q_local, S.pol, S.thole,
+                   (S.mScales, S.pScales, S.dScales), 6.0, True)
+    te, tg = (1e-9, 1e-7) if prec == "fp64" else (2e-5, 1e-4)
+    assert abs(g["E"] - o["E"]) < te * abs(o["E"])
+    keys = ("pos", "box", "Q", "pol", "thole") if prec == "fp64" else ("pos",)
+    for k in keys:
+        assert rel_err(g[k], o[k]) < tg, (k, rel_err(g[k], o[k]))
+
+
+def test_coulomb_pme_vs_oracle(emu_lib):
+    from tests.test_oracle_pins import coul_pme_system, EN as _EN
+    s = coul_pme_system()
+    r = run(emu_lib, "coul", "fp64")
+    p = torch.tensor(s["pos_nm"] * 10, requires_grad=True)
+    q = torch.tensor(s["charges"], requires_grad=True)
+    e = O.energy_pme(p, T(s["box_nm"] * 10), s["pairs"], q.reshape(-1, 1), None, None, None, s["mscales"], None,
+                     None, None, None, s["kappa_nm"] / 10, s["K"], 0, False)
+    gp, gq = torch.autograd.grad(e, (p, q))
+    assert abs(r["E"] - float(e)) < 1e-9 * abs(float(e))
+    assert abs(r["E"] - _EN["coul_pme"]["value"]) < _EN["coul_pme"]["atol"]
+    assert rel_rms(np.asarray(r["pos"]), gp.numpy() * 10) < 1e-8
+    assert rel_rms(np.asarray(r["q"]), gq.numpy()) < 1e-8
+
+
+def test_esp_consistency(emu_lib):
+    r = run(emu_lib, "esp", "fp64")
+    assert abs(r["dE"] - r["esp"]) < 1e-3 * abs(r["esp"]), r

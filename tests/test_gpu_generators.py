@@ -130,8 +130,8 @@ def test_generator_parameter_gradients_vs_oracle():
     rows = [r.attrib for r in node if r.tag == "Atom"]
     keys = [r["type"] for r in rows]
     idx = torch.as_tensor([keys.index(t) for t in s.top.types])
-    A0 = torch.tensor([float(r["A"]) for r in rows], requires_grad=True)
-    B0 = torch.tensor([float(r["B"]) for r in rows], requires_grad=True)
+    A0 = torch.tensor([float(r["A"]) for r in rows], dtype=torch.float64, requires_grad=True)
+    B0 = torch.tensor([float(r["B"]) for r in rows], dtype=torch.float64, requires_grad=True)
     m = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 1.0])
     es = O.generate_pairwise_interaction(O.slater_sr_kernel)(torch.tensor(pos), torch.tensor(bx), rp, m, A0[idx],
                                                              B0[idx] / 10)

@@ -561,6 +561,42 @@ def test_backward_central_differences_12288_atoms():
         fd = (vals[0] - vals[1]) / (2 * h)
         an = float(gQ[t, k]) if kind == "Q" else float((gP if kind == "pol" else gT)[t])
         check(an, fd, (kind, t, k))
+    # virial: dE/dbox with Cartesian positions held fixed (the reference's box
+    # gradient, tests/test_admp/test_gradDisp.py:57-58), all nine components
+    gbox = g["box"][0]
+    for p_ in range(3):
+        for q_ in range(3):
+            h = 1e-5 * float(boxt[0, 0])
+            vals = []
+            for sgn in (1, -1):
+                bx = boxt.clone()
+                bx[p_, q_] += sgn * h
+                E, _, _, _ = eng.compute(pos0, bx, Qt[mq], Pt[mp], Tt[mp], S.mScales, S.pScales, S.dScales,
+                                         solver="pcg", tol=1e-10, maxiter=1000, want_grad=False)
+                vals.append(float(E[0]))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            print(f"dE/dbox[{p_},{q_}] analytic {float(gbox[p_, q_]):.9f} fd {fd:.9f}", flush=True)
+            check(float(gbox[p_, q_]), fd, ("box", p_, q_))
+    # Thole widths enter only where pScale = 0 and dScale != 0: with dScales = 1 the
+    # intramolecular induced-induced pairs use a = thole_i + thole_j (pme.py:794-795)
+    dS1 = np.ones_like(S.dScales)
+
+    def e_d1(Th, grad=False):
+        E, gg, U, info = eng.compute(pos0, boxt, Qt[mq], Pt[mp], Th[mp], S.mScales, S.pScales, dS1,
+                                     solver="pcg", tol=1e-10, maxiter=1000, want_grad=grad)
+        return float(E[0]), gg
+
+    _, gd = e_d1(Tt, grad=True)
+    gT1 = pt(gd["thole"][0])
+    assert float(torch.linalg.norm(gT1)) > 1e-3
+    for t in range(len(Tt)):
+        h = 1e-5 * abs(float(Tt[t]))
+        Tp, Tm = Tt.clone(), Tt.clone()
+        Tp[t] += h
+        Tm[t] -= h
+        fd = (e_d1(Tp)[0] - e_d1(Tm)[0]) / (2 * h)
+        print(f"dE/dthole[{t}] (dScales 1) analytic {float(gT1[t]):.9f} fd {fd:.9f}", flush=True)
+        check(float(gT1[t]), fd, ("thole dS=1", t))
 
 
 def test_type_reduce_deterministic():

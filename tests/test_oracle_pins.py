@@ -706,3 +706,35 @@ def test_coul_pme_test_coul():
                      None, None, None, s["mscales"], None, None, None, None, s["kappa_nm"] / 10, s["K"], 0, False)
     assert s["K"] == (12, 12, 12)
     assert abs(float(e) - EN["coul_pme"]["value"]) < EN["coul_pme"]["atol"], float(e)
+
+
+# ----------------------------------------------------------------- ESP (test_esp.py)
+def esp_case():
+    """tests/test_admp/test_esp.py:11-63: water and water + Na+ (admp_charge.xml),
+    NoCutoff, all pairs i < j with their nbond; the probe is the ion's position."""
+    out = {}
+    for name, pdb in (("water", "water.pdb"), ("probe", "water_ion.pdb")):
+        s = load_system("admp_charge.xml", pdb)
+        n = s.n
+        pairs = np.array([[i, j, s.bonded.get((i, j), 0)] for i in range(n) for j in range(i + 1, n)])
+        out[name] = (s, pairs)
+    return out
+
+
+def test_esp_consistency_test_esp():
+    """E(water + ion) - E(water) = ESP of the water's multipoles at the ion (charge
+    +1), rtol 1e-3 (test_esp.py:57-63); pins Q_local_to_global and esp against the
+    kappa = 0 multipole interaction."""
+    c = esp_case()
+    E = {}
+    for name, (s, pairs) in c.items():
+        f = O.ADMPPmeForce(np.eye(3) * 1000.0, s.axis_types, s.axis_idx, 1000.0, 5e-4, s.pme.lmax, False, False, None)
+        E[name] = float(f.get_energy(T(s.pos_nm * 10), T(np.eye(3) * 1000.0), pairs,
+                                     T(s.pme.Q_local[s.map_atomtype]), mScales=s.pme.mScales))
+    s, _ = c["water"]
+    sp, _ = c["probe"]
+    Qg = O.Q_local_to_global(T(s.pos_nm), T(np.eye(3) * 1000.0), T(s.pme.Q_local[s.map_atomtype]), s.axis_types,
+                             s.axis_idx, s.pme.lmax)
+    grid = T(sp.pos_nm[3:4])
+    e = float(O.esp(T(s.pos_nm), grid, Qg)[0])
+    assert abs((E["probe"] - E["water"]) - e) < 1e-3 * abs(e), (E["probe"] - E["water"], e)
