@@ -18,6 +18,7 @@ Generators (reference file:line):
   SlaterExForce      SlaterExGenerator       :672-792
   SlaterSrEsForce, SlaterSrPolForce, SlaterSrDispForce, SlaterDhfForce
                      SlaterSrEsGenerator and subclasses :795-863 (returns -E)
+  ADMPQeqForce       ADMPQeqGenerator        dmff/generators/qeq.py:17-223
 """
 import numpy as np
 import torch
@@ -27,7 +28,8 @@ from ._autograd import as_t
 from .disp_pme import ADMPDispPmeForce
 from .pairwise import (TT_damping_qq_c6_kernel, TT_damping_qq_kernel, generate_pairwise_interaction,
                        slater_disp_damping_kernel, slater_sr_kernel)
-from .pme import ADMPPmeForce
+from .pme import ADMPPmeForce, setup_ewald_parameters
+from .qeq import ADMPQeqForce
 
 F64 = torch.float64
 
@@ -376,6 +378,73 @@ class ADMPDispGenerator(_Base):
         return potential_fn
 
 
+# ---------------------------------------------------------------------------
+class ADMPQeqGenerator(_Base):
+    """dmff/generators/qeq.py:17-223: per-type chi, J, eta; charge groups per
+    residue unless const_list/const_vals are given."""
+    name = "ADMPQeqForce"
+
+    def __init__(self, ff, paramset):
+        self.ff = ff
+        self.table = _AtomTable(ff, self.name)
+        node = self.table.node
+        self.coulomb14scale = float(node.get("coulomb14scale"))
+        self.damp_mod = int(node.get("DampMod", 3))
+        paramset.addField(self.name)
+        for attr in ("chi", "J", "eta"):
+            paramset.addParameter(self.table.column(attr), attr, field=self.name)
+        self._potential = None
+
+    def createPotential(self, top, nonbondedMethod, nonbondedCutoff, **kwargs):
+        """qeq.py:119-222.  Extra keywords (not in the reference) go to the
+        ADMPQeqForce drop-in: precision, device, tol, maxiter."""
+        m = _method_name(nonbondedMethod)
+        if m not in (PME, NoCutoff):
+            raise fe.DMFFException("Illegal nonbonded method for NonbondedForce")
+        no_cut = m == NoCutoff
+        r_cut = _cutoff_angstrom(nonbondedCutoff) / 10.0
+        if not no_cut:
+            box = np.asarray(top.box_nm, dtype=np.float64)
+            kappa, K1, K2, K3 = setup_ewald_parameters(r_cut, kwargs.get("ethresh", 1e-5), box,
+                                                       kwargs.get("PmeSpacing", 0.1),
+                                                       kwargs.get("PmeCoeffMethod", "openmm"))
+        else:
+            kappa, K1, K2, K3 = 1.0, 1, 1, 1
+        n = top.n_atoms
+        init_q = np.asarray(getattr(top, "charges", None) if getattr(top, "charges", None) is not None
+                            else np.zeros(n), dtype=np.float64)
+        map_idx = torch.as_tensor(self.table.type_map(top, self.ff))
+        if "const_list" in kwargs and "const_vals" in kwargs:
+            const_list, const_vals = kwargs["const_list"], kwargs["const_vals"]
+        else:
+            groups = {}
+            for i, rid in enumerate(top.res_ids):
+                groups.setdefault(rid, []).append(i)
+            const_list = list(groups.values())
+            const_vals = [float(np.sum(init_q[g])) for g in const_list]
+        has_aux = bool(kwargs.get("has_aux", False))
+        extra = {k: kwargs[k] for k in ("precision", "device", "tol", "maxiter") if k in kwargs}
+        force = ADMPQeqForce(init_q, r_cut, kappa, (K1, K2, K3), damp_mod=self.damp_mod, const_list=const_list,
+                             const_vals=const_vals, neutral_flag=kwargs.get("neutral", True),
+                             slab_flag=kwargs.get("slab", False), constQ=kwargs.get("constQ", True),
+                             pbc_flag=not no_cut, has_aux=has_aux, **extra)
+        self.qeq_force = force
+        qeq_energy = force.generate_get_energy()
+        mscales = np.array([0.0, 0.0, self.coulomb14scale, 1.0, 1.0, 1.0])
+
+        def potential_fn(positions, box, pairs, params, aux=None):
+            p = _params(params, self.name)
+            eta = p["eta"][map_idx.to(p["eta"].device)]
+            chi = p["chi"][map_idx.to(p["chi"].device)]
+            J = p["J"][map_idx.to(p["J"].device)]
+            if has_aux:
+                return qeq_energy(positions, box, pairs, mscales, eta, chi, J, aux)
+            return qeq_energy(positions, box, pairs, mscales, eta, chi, J)
+
+        self._potential = potential_fn
+        return potential_fn
+
+
 GENERATORS = {
     "ADMPPmeForce": ADMPPmeGenerator,
     "ADMPDispPmeForce": ADMPDispPmeGenerator,
@@ -387,4 +456,5 @@ GENERATORS = {
     "SlaterSrPolForce": SlaterSrPolGenerator,
     "SlaterSrDispForce": SlaterSrDispGenerator,
     "SlaterDhfForce": SlaterDhfGenerator,
+    "ADMPQeqForce": ADMPQeqGenerator,
 }

@@ -1066,6 +1066,159 @@ def brute_force_pairs(positions, box, rc):
     return np.concatenate(out, axis=0).astype(np.int64)
 
 
+# ---------------------------------------------------------------------------
+# Charge equilibration  (dmff/admp/qeq.py)
+# ---------------------------------------------------------------------------
+def qeq_ds_pairs(positions, box, pairs, pbc_flag):
+    """qeq.py:161-174: pair distances (minimum image when periodic), +1e-64 inside the norm."""
+    p = torch.as_tensor(np.asarray(pairs), dtype=torch.int64)
+    n = positions.shape[0]
+    i, j = p[:, 0].clamp(max=n - 1), p[:, 1].clamp(max=n - 1)   # JAX clamps out-of-range gathers
+    d = positions[i] - positions[j]
+    if pbc_flag:
+        s = d @ torch.linalg.inv(box)
+        s = s - torch.floor(s + 0.5)
+        d = s @ box
+    return torch.linalg.norm(d + 1e-64, dim=1)
+
+
+def qeq_e_sr(damp_mod, pairs, q, eta, ds):
+    """qeq.py:91-120 (E_sr, E_sr2, E_sr3; ds in Angstrom as E_full passes it).
+    Pair widths below 1e-4 give no pair term (eta_piecewise, :177-185) and widths
+    below 1e-4 no self term (etainv_piecewise, :189-197)."""
+    if damp_mod == 1:
+        return torch.zeros((), dtype=F64)
+    p = torch.as_tensor(np.asarray(pairs), dtype=torch.int64)
+    n = q.shape[0]
+    i, j = p[:, 0].clamp(max=n - 1), p[:, 1].clamp(max=n - 1)
+    mask = pair_buffer_scales(p[:, :2])
+    if damp_mod == 2:
+        e2 = torch.sqrt(2 * (eta[i] ** 2 + eta[j] ** 2))
+    else:
+        e2 = torch.sqrt(eta[i] ** 2 + eta[j] ** 2 + 1e-64)
+    safe = torch.where(e2 > 1e-4, e2, torch.ones_like(e2))
+    piece = torch.where(e2 > 1e-4, torch.special.erfc(ds / safe), torch.zeros_like(e2))
+    pair = -piece * DIELECTRIC * q[i] * q[j] / ds
+    if damp_mod == 2:
+        pair = pair * mask
+    pair = torch.where(mask < 1e-4, torch.zeros_like(pair), pair)
+    etas = torch.where(eta > 1e-4, eta, torch.ones_like(eta))
+    inv = torch.where(eta > 1e-4, 1.0 / etas, torch.zeros_like(eta))
+    self_ = inv / math.sqrt(2 * math.pi) * DIELECTRIC * q * q
+    return torch.sum(pair) + torch.sum(self_)
+
+
+def qeq_e_site(damp_mod, chi, J, q):
+    """qeq.py:123-137."""
+    if damp_mod == 1:
+        return torch.zeros((), dtype=F64)
+    if damp_mod == 2:
+        return torch.sum((chi * q + 0.5 * J * q ** 2) * 96.4869)
+    return torch.sum(chi * q * 4.184 + J * q ** 2 * DIELECTRIC * 2 * math.pi)
+
+
+def qeq_e_corr(positions, box, q, kappa, neutral_flag):
+    """qeq.py:140-158: slab correction (Angstrom, kappa in 1/A).  The reference
+    reads the slab height as norm(box[3]) of a 3x3 box, which JAX's clamped
+    indexing turns into box[2]."""
+    V = torch.linalg.det(box)
+    pre = 2 * math.pi / V * DIELECTRIC
+    z = positions[:, 2]
+    Mz = torch.sum(q * z)
+    Qt = torch.sum(q)
+    Lz = torch.linalg.norm(box[2])
+    e = pre * (Mz ** 2 - Qt * torch.sum(q * z ** 2) - Qt ** 2 * Lz ** 2 / 12)
+    if neutral_flag:
+        e = e + (-math.pi / (2 * V * kappa ** 2) * DIELECTRIC) * Qt ** 2
+    return e
+
+
+def qeq_e_const(constQ, q, lagmt, const_list, const_vals):
+    """qeq.py:61-79: constraint groups (padded with N in the reference, masked)."""
+    sums = torch.stack([torch.sum(q[list(g)]) if len(g) else torch.zeros((), dtype=F64) for g in const_list])
+    if constQ:
+        return torch.sum((sums - const_vals) * lagmt)
+    return torch.sum(sums * const_vals)
+
+
+class ADMPQeqForce:
+    """Oracle counterpart of qeq.py:200-405 (same constructor arguments; units as
+    the reference: nm for positions, box and r_cut, kappa in 1/nm).
+
+    The reference finds the root of dE/d(q, lagmt) with scipy's hybr (tol 1e-10).
+    E_full is quadratic in (q, lagmt), so the root is the solution of one linear
+    system; here its matrix is assembled column by column from gradients of the
+    restated E_full (torch autograd) and solved densely (numpy, FP64)."""
+
+    def __init__(self, init_q, r_cut, kappa, K, damp_mod=3, const_list=(), const_vals=(), neutral_flag=True,
+                 slab_flag=False, constQ=True, pbc_flag=True):
+        const_vals = np.asarray(const_vals, dtype=np.float64)
+        if neutral_flag and len(const_vals):
+            const_vals = const_vals - np.sum(const_vals) / len(const_vals)
+        assert len(const_list) == len(const_vals)
+        if damp_mod not in (1, 2, 3):
+            raise ValueError("damp_mod must be 1, 2 or 3")
+        self.const_vals = _t(const_vals)
+        self.const_list = [list(g) for g in const_list]
+        self.init_q = _t(init_q)
+        self.r_cut, self.kappa, self.K = r_cut, float(kappa) if pbc_flag else 0.0, tuple(K)
+        self.damp_mod, self.neutral_flag, self.slab_flag = damp_mod, neutral_flag, slab_flag
+        self.constQ, self.pbc_flag = constQ, pbc_flag
+
+    def coul_energy(self, positions, box, pairs, q, mscales):
+        """qeq.py:265-316 (positions, box in nm)."""
+        if self.pbc_flag:
+            return energy_pme(positions * 10, box * 10, pairs, q.reshape(-1, 1), None, None, None, mscales,
+                              None, None, None, None, self.kappa / 10, self.K, 0, False)
+        p = _live_pairs(pairs)
+        i, j, nb = p[:, 0], p[:, 1], p[:, 2]
+        msc = _t(mscales)[nb - 1]
+        d = positions[i] - positions[j]
+        r = torch.linalg.norm(d + 1e-64, dim=1)
+        return torch.sum(q[i] * q[j] * msc * DIELECTRIC * 0.1 / r)
+
+    def E_full(self, q, lagmt, chi, J, positions, box, pairs, eta, mscales):
+        """qeq.py:323-337."""
+        ds = qeq_ds_pairs(positions, box, pairs, self.pbc_flag)
+        e = qeq_e_const(self.constQ, q, lagmt, self.const_list, self.const_vals)
+        e = e + qeq_e_sr(self.damp_mod, pairs, q, eta, ds * 10)
+        e = e + qeq_e_site(self.damp_mod, chi, J, q)
+        e = e + self.coul_energy(positions, box, pairs, q, mscales)
+        if self.slab_flag:
+            e = e + qeq_e_corr(positions * 10, box * 10, q, self.kappa / 10, self.neutral_flag)
+        return e
+
+    def solve(self, chi, J, positions, box, pairs, eta, mscales):
+        """The stationary point (q, lagmt) of E_full: gradient g(x) = H x + c."""
+        n, m = self.init_q.shape[0], len(self.const_list)
+        nx = n + (m if self.constQ else 0)
+
+        def grad(x):
+            x = x.clone().requires_grad_(True)
+            lag = x[n:] if self.constQ else torch.ones(m, dtype=F64)
+            e = self.E_full(x[:n], lag, chi, J, positions, box, pairs, eta, mscales)
+            return torch.autograd.grad(e, x)[0].detach()
+
+        c = grad(torch.zeros(nx, dtype=F64))
+        H = np.empty((nx, nx))
+        for k in range(nx):
+            ek = torch.zeros(nx, dtype=F64)
+            ek[k] = 1.0
+            H[:, k] = (grad(ek) - c).numpy()
+        H = 0.5 * (H + H.T)
+        x = np.linalg.solve(H, -c.numpy())
+        q = _t(x[:n])
+        lag = _t(x[n:]) if self.constQ else torch.ones(m, dtype=F64)
+        return q, lag
+
+    def get_energy(self, positions, box, pairs, mscales, eta, chi, J):
+        """qeq.py:352-405: energy at the solution, which is held fixed
+        (stop_gradient) so gradients are partial derivatives."""
+        q, lag = self.solve(chi.detach(), J.detach(), positions.detach(), box.detach(), pairs, eta.detach(), mscales)
+        e = self.E_full(q, lag, chi, J, positions, box, pairs, eta, mscales)
+        return e, q, lag
+
+
 PINNING = {
     "setup_ewald_parameters": "BASELINE table kappa/K values (tests/test_oracle_pins.py)",
     "pbc_shift": "reference test vectors tests/test_admp/test_sptial.py:43-68",
@@ -1082,4 +1235,5 @@ PINNING = {
     "brute_force_pairs": "pair count 15 for water dimer at 4 A (test_nblist.py:61-69); O(N^2) loop",
     "Q_local_to_global/esp": "test_esp.py:57-63: E(water+ion) - E(water) equals the ESP at the ion (rtol 1e-3)",
     "gradients": "central finite differences of the restated energy",
+    "ADMPQeqForce": "test_qeq.py:12-39 (-37.84692763, decimal 3) and :42-89 (4817.295171, decimal 2)",
 }

@@ -401,7 +401,28 @@ def cfg3s(precision):
             "e_pol": info["e_pol"], "n_iter": info["n_iter"][0], "pos": _lst(g["pos"][0])}
 
 
-CASES = {"cfg3s": cfg3s, "axis_types": axis_types, "triclinic": triclinic, "coul": coul, "esp": esp, "generators": generators, "pairwise": pairwise, "bad_geometry": bad_geometry, "type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
+def qeq(variant):
+    """ADMPQeqForce on the small charge-equilibration case: periodic (damp_mod 3,
+    slab correction) or not (damp_mod 2): energy, charges, and gradients with
+    respect to positions, box, chi, J and eta."""
+    from dmff_amd.admp.qeq import ADMPQeqForce
+    from tests.systems import qeq_small_system
+    pbc = variant == "pbc"
+    s = qeq_small_system(pbc)
+    f = ADMPQeqForce(s["init_q"], s["rc_nm"], s["kappa"], s["K"], damp_mod=3 if pbc else 2,
+                     const_list=s["const_list"], const_vals=s["const_vals"], slab_flag=pbc, pbc_flag=pbc,
+                     has_aux=True, device="cpu")
+    P = torch.tensor(s["pos_nm"], dtype=F64, requires_grad=True)
+    B = torch.tensor(s["box_nm"], dtype=F64, requires_grad=True)
+    chi, J, eta = (torch.tensor(s[k], dtype=F64, requires_grad=True) for k in ("chi", "J", "eta"))
+    e, aux = f.generate_get_energy()(P, B, torch.as_tensor(s["pairs"]), s["mscales"], eta, chi, J, {
+        "q": torch.zeros(len(s["init_q"]), dtype=F64), "lagmt": torch.ones(len(s["const_list"]), dtype=F64)})
+    g = torch.autograd.grad(e, (P, B, chi, J, eta))
+    return {"E": float(e.detach()), "q": _lst(aux["q"]), "iters": f.last_iterations,
+            "pos": _lst(g[0]), "box": _lst(g[1]), "chi": _lst(g[2]), "J": _lst(g[3]), "eta": _lst(g[4])}
+
+
+CASES = {"qeq": qeq, "cfg3s": cfg3s, "axis_types": axis_types, "triclinic": triclinic, "coul": coul, "esp": esp, "generators": generators, "pairwise": pairwise, "bad_geometry": bad_geometry, "type_reduce": type_reduce, "small_box": small_box, "frames": frames, "disp_slater": disp_slater, "neighbors": neighbors}
 
 if __name__ == "__main__":
     lib, case = sys.argv[1], sys.argv[2]

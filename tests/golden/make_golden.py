@@ -99,6 +99,8 @@ def main():
         "tests/data/water_ion.pdb": "water_ion.pdb",
         "tests/data/qeq.xml": "qeq.xml",
         "tests/data/qeq.pdb": "qeq.pdb",
+        "tests/data/qeq2.xml": "qeq2.xml",
+        "tests/data/qeq2.pdb": "qeq2.pdb",
     }
     for src, dst in copies.items():
         shutil.copyfile(os.path.join(REF, src), os.path.join(DATA, dst))
@@ -158,12 +160,16 @@ def main():
     en["coul_pme"] = {"value": coul["value"], "atol": 1e-2, "rc_nm": 0.5, "spacing_nm": 0.10,
                       "box_nm": [[1.2, 0, 0], [0, 1.2, 0], [0, 0, 1.2]], "method": "gromacs"}
     qeq_txt = open(os.path.join(REF, "tests/test_admp/test_qeq.py")).read()
-    en["qeq"] = {"value": float(re.search(r"assert_almost_equal\(energy,\s*" + num, qeq_txt).group(1)),
-                 "decimal": 3, "rc_nm": 0.6, "ethresh": 1e-3}
+    qeq_vals = [float(v) for v in re.findall(r"assert_almost_equal\(energy,\s*" + num, qeq_txt)]
+    en["qeq"] = {"value": qeq_vals[0], "decimal": 3, "rc_nm": 0.6, "ethresh": 1e-3}
+    # second case: electrode (144 atoms) + one water, two neutral groups, zero start
+    en["qeq2"] = {"value": qeq_vals[1], "decimal": 2, "rc_nm": 0.6, "ethresh": 1e-3,
+                  "const_list": [list(range(144)), [144, 145, 146]], "const_vals": [0.0, 0.0]}
     en["source"].update({
         "peg2": "tests/test_admp/test_slater.py:13-25,74-82",
         "coul_pme": "tests/test_classical/test_coul.py:84-129",
         "qeq": "tests/test_admp/test_qeq.py:12-39",
+        "qeq2": "tests/test_admp/test_qeq.py:42-89",
     })
     with open(os.path.join(HERE, "energies.json"), "w") as f:
         json.dump(en, f, indent=1)

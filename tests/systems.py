@@ -300,3 +300,87 @@ def rel_err(a, b, floor=1e-6):
     a = np.asarray(a, dtype=np.float64).ravel()
     b = np.asarray(b, dtype=np.float64).ravel()
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), floor))
+
+
+def qeq_system(case="qeq", water_bonds=True):
+    """The charge-equilibration cases of tests/test_admp/test_qeq.py: an electrode
+    of 144 atoms (qeq: one neutral group, template charges as the start; qeq2: plus
+    one water, two neutral groups, zero start).  Types follow the XML residue
+    particles in order, as the reference test assigns them; the water's O-H bonds
+    come from OpenMM's standard template for H2O (HOH).  Returns positions and box
+    in nm, per-atom chi/J/eta, constraint groups, the PME setup and the pairs."""
+    import json
+    import xml.etree.ElementTree as ET
+    from dmff_amd.admp import frontend as fe
+    from oracle.admp_oracle import setup_ewald_parameters
+    c = json.load(open(os.path.join(GOLD, "energies.json")))[case]
+    xml = golden_path(f"{case}.xml")
+    pos_nm, top = fe.read_pdb(golden_path(f"{case}.pdb"))
+    root = ET.parse(xml).getroot()
+    parts = [(a.get("type"), float(a.get("charge"))) for r in root.find("Residues") for a in r if a.tag == "Atom"]
+    n = top.n_atoms
+    assert len(parts) == n
+    node = root.find("ADMPQeqForce")
+    rows = [a.attrib for a in node if a.tag == "Atom"]
+    key = "type" if "type" in rows[0] else "class"
+    keys = [r[key] for r in rows]
+    tmap = np.array([keys.index(t) for t, _ in parts], dtype=np.int64)
+    col = lambda a: np.array([float(r[a]) for r in rows], dtype=np.float64)
+    charges = np.array([q for _, q in parts], dtype=np.float64)
+    bonds = []
+    if case == "qeq2":
+        charges[:] = 0.0
+        const_list, const_vals = c["const_list"], c["const_vals"]
+        if water_bonds:
+            bonds = [(144, 145), (144, 146)]
+    else:
+        groups = {}
+        for i, rid in enumerate(top.res_ids):
+            groups.setdefault(rid, []).append(i)
+        const_list = list(groups.values())
+        const_vals = [float(np.sum(charges[g])) for g in const_list]
+    bonded = fe.bonded_distances(n, bonds)
+    box_nm = np.asarray(top.box_nm, dtype=np.float64)
+    kappa, K1, K2, K3 = setup_ewald_parameters(c["rc_nm"], c["ethresh"], box_nm)
+    s14 = float(node.get("coulomb14scale"))
+    return dict(pos_nm=pos_nm, box_nm=box_nm, tmap=tmap, chi_t=col("chi"), J_t=col("J"), eta_t=col("eta"),
+                init_q=charges, const_list=const_list, const_vals=const_vals, kappa=kappa, K=(K1, K2, K3),
+                rc_nm=c["rc_nm"], damp_mod=int(node.get("DampMod", 3)),
+                mscales=np.array([0.0, 0.0, s14, 1.0, 1.0, 1.0]), bonded=bonded,
+                pairs=reference_pairs(pos_nm, box_nm, c["rc_nm"], bonded), value=c["value"], decimal=c["decimal"])
+
+
+def qeq_small_system(pbc=True, seed=11):
+    """A small charge-equilibration case with every term live: 18 sites (six
+    3-site groups, neutral) in a 1.0 nm box, eta 0.9-1.6 (so the erfc pair and
+    self terms matter at Angstrom distances), J > 0, chi of both signs, 1-2 and
+    1-3 exclusions inside each group; rc 0.45 nm and K 10 when periodic."""
+    from oracle.admp_oracle import setup_ewald_parameters
+    rng = np.random.default_rng(seed)
+    n = 18
+    L = 1.0
+    pos = []
+    while len(pos) < n:
+        p = rng.uniform(0.05, 0.95, size=3) * L
+        if all(np.linalg.norm(((p - q) + L / 2) % L - L / 2) > 0.15 for q in pos):
+            pos.append(p)
+    pos = np.array(pos)
+    box = np.eye(3) * L
+    groups = [list(range(3 * k, 3 * k + 3)) for k in range(6)]
+    bonds = [(g[0], g[1]) for g in groups] + [(g[0], g[2]) for g in groups]
+    bonded = fe.bonded_distances(n, bonds)
+    chi = rng.uniform(-50.0, 150.0, size=n)
+    J = rng.uniform(0.2, 0.6, size=n)
+    eta = rng.uniform(0.9, 1.6, size=n)
+    rc = 0.45
+    if pbc:
+        kappa, K1, K2, K3 = setup_ewald_parameters(rc, 5e-4, box)
+        K = (max(K1, 10), max(K2, 10), max(K3, 10))
+        pairs = reference_pairs(pos, box, rc, bonded)
+    else:
+        kappa, K = 0.0, (1, 1, 1)
+        p = np.array([[i, j] for i in range(n) for j in range(i + 1, n)])
+        pairs = np.concatenate([p, nbond_column(p, bonded)[:, None]], axis=1)
+    return dict(pos_nm=pos, box_nm=box, chi=chi, J=J, eta=eta, const_list=groups, const_vals=[0.0] * 6,
+                init_q=np.zeros(n), rc_nm=rc, kappa=kappa, K=K, pairs=pairs,
+                mscales=np.array([0.0, 0.0, 0.5, 1.0, 1.0, 1.0]), bonded=bonded)

@@ -292,3 +292,28 @@ def test_coulomb_pme_vs_oracle(emu_lib):
 def test_esp_consistency(emu_lib):
     r = run(emu_lib, "esp", "fp64")
     assert abs(r["dE"] - r["esp"]) < 1e-3 * abs(r["esp"]), r
+
+
+@pytest.mark.parametrize("variant", ["pbc", "nopbc"])
+def test_qeq_vs_oracle(emu_lib, variant):
+    """ADMPQeqForce (MINRES on the library's lmax-0 Coulomb) against the oracle's
+    dense solve of the same stationary point: periodic with damp_mod 3 and the
+    slab correction, and the plain pair sum with damp_mod 2.  Energy 1e-12,
+    charges 1e-8, gradients (positions, box, chi, J, eta) 1e-7 relative."""
+    from tests.systems import qeq_small_system
+    pbc = variant == "pbc"
+    r = run(emu_lib, "qeq", variant)
+    s = qeq_small_system(pbc)
+    f = O.ADMPQeqForce(s["init_q"], s["rc_nm"], s["kappa"], s["K"], 3 if pbc else 2, s["const_list"],
+                       s["const_vals"], neutral_flag=True, slab_flag=pbc, constQ=True, pbc_flag=pbc)
+    P, B = T(s["pos_nm"]).requires_grad_(True), T(s["box_nm"]).requires_grad_(True)
+    chi, J, eta = (T(s[k]).requires_grad_(True) for k in ("chi", "J", "eta"))
+    e, q, lag = f.get_energy(P, B, s["pairs"], s["mscales"], eta, chi, J)
+    g = torch.autograd.grad(e, (P, B, chi, J, eta), allow_unused=True)
+    assert abs(r["E"] - float(e.detach())) < 1e-12 * abs(float(e.detach()))
+    assert np.abs(np.asarray(r["q"]) - q.numpy()).max() < 1e-8
+    for k, gg in zip(("pos", "box", "chi", "J", "eta"), g):
+        if gg is None:
+            assert np.abs(np.asarray(r[k])).max() == 0.0, k
+        else:
+            assert rel_rms(np.asarray(r[k]), gg.numpy()) < 1e-7, k

@@ -738,3 +738,28 @@ def test_esp_consistency_test_esp():
     grid = T(sp.pos_nm[3:4])
     e = float(O.esp(T(s.pos_nm), grid, Qg)[0])
     assert abs((E["probe"] - E["water"]) - e) < 1e-3 * abs(e), (E["probe"] - E["water"], e)
+
+
+def qeq_oracle(case):
+    """The oracle's ADMPQeqForce on a case of tests/test_admp/test_qeq.py
+    (neutral, slab, constQ as the reference test asks): (energy, charges, lagmt, system)."""
+    from tests.systems import qeq_system
+    s = qeq_system(case)
+    f = O.ADMPQeqForce(s["init_q"], s["rc_nm"], s["kappa"], s["K"], s["damp_mod"], s["const_list"], s["const_vals"],
+                       neutral_flag=True, slab_flag=True, constQ=True, pbc_flag=True)
+    tm = s["tmap"]
+    e, q, lag = f.get_energy(T(s["pos_nm"]), T(s["box_nm"]), s["pairs"], s["mscales"], T(s["eta_t"][tm]),
+                             T(s["chi_t"][tm]), T(s["J_t"][tm]))
+    return float(e), q, lag, s
+
+
+@pytest.mark.parametrize("case", ["qeq", "qeq2"])
+def test_qeq_reference_energies(case):
+    """tests/test_admp/test_qeq.py:12-39 (-37.84692763, decimal 3) and :42-89
+    (4817.295171, decimal 2; the water's O-H bonds matter: without them the
+    energy is 3973.7).  The constraint groups hold their charges exactly."""
+    e, q, lag, s = qeq_oracle(case)
+    assert abs(e - s["value"]) < 1.5 * 10.0 ** (-s["decimal"]), (case, e, s["value"])
+    vals = np.asarray(s["const_vals"]) - np.mean(s["const_vals"])
+    for g, v in zip(s["const_list"], vals):
+        assert abs(float(q[g].sum()) - v) < 1e-10
