@@ -332,14 +332,16 @@ def leg_large(steps=2, warmup=1, precision="mixed"):
             "E_elec": float(E[0]), "E_disp_backend": float(Ed[0]), "E_slater": float(Es[0])}
 
 
-def leg_serving(stream, precision, slots=4):
-    """Poisson request stream through the continuous-batching server; latency per request."""
+def leg_serving(stream, precision, slots=4, max_batch=8, time_scale=1.0):
+    """A request stream through the continuous-batching server: arrivals at the
+    stream's times x time_scale; latency per request, throughput over the span."""
     import torch
     from dmff_amd.admp.serving import ADMPServer, Request, latency_summary
-    srv = ADMPServer(device="cuda:0", precision=precision, slots=slots, tol=1e-6)
+    srv = ADMPServer(device="cuda:0", precision=precision, slots=slots, tol=1e-6, max_batch=max_batch)
     futs = []
     t0 = time.perf_counter()
     for rid, (t, spec, pos, box) in enumerate(stream):
+        t = t * time_scale
         now = time.perf_counter() - t0
         if t > now:
             time.sleep(t - now)
@@ -350,7 +352,8 @@ def leg_serving(stream, precision, slots=4):
     del srv
     torch.cuda.empty_cache()
     out = latency_summary(res)
-    out.update({"evals_per_s": len(res) / span, "slots": slots, "precision": precision,
+    out.update({"evals_per_s": len(res) / span, "offered_qps": len(res) / (stream[-1][0] * time_scale),
+                "slots": slots, "max_batch": max_batch, "precision": precision,
                 "warm_frames": int(sum(r.warm for r in res))})
     return out, res
 
@@ -453,20 +456,44 @@ def main():
         out["large"] = leg_large()
         log(f"large: {out['large']['ms_per_eval']:.1f} ms/eval ({out['large']['n_atoms']} atoms)")
     if "serving" in legs:
-        from dmff_amd.admp.serving import make_stream
-        stream = make_stream(32, 20.0, 1004)
+        from dmff_amd.admp.serving import make_fitting_stream, make_stream
+        t_gen = time.perf_counter()
+        stream = make_stream(500, 20.0, 1004)
+        log(f"serving stream: 500 requests generated in {time.perf_counter() - t_gen:.1f} s")
         sv = {}
         res = {}
         for prec in ("mixed", "fp64"):
             sv[prec], res[prec] = leg_serving(stream, prec)
             log(f"serving {prec}: p50 {sv[prec]['p50_ms']:.1f} ms, p99 {sv[prec]['p99_ms']:.1f} ms, "
-                f"{sv[prec]['evals_per_s']:.1f} evals/s")
+                f"{sv[prec]['evals_per_s']:.1f} evals/s, {sv[prec].get('ctx_created', 0)} contexts created "
+                f"(mean {sv[prec].get('ctx_ms_mean', 0.0):.1f} ms)")
         import torch as _t
         errs = [float(_t.linalg.norm(a.dpos - f.dpos) / _t.linalg.norm(f.dpos))
                 for a, f in zip(res["mixed"], res["fp64"])]
         sv["mixed_force_rel_rms_max"] = max(errs)
-        sv["stream"] = ("32 requests, Poisson arrivals at 20/s, water boxes of 64-8192 molecules (log-uniform), "
-                        "30% follow-on frames of a live system, 4 slots, solve to 1e-6")
+        sv["stream"] = ("500 requests, Poisson arrivals at 20/s, water boxes of 64-8192 molecules (log-uniform), "
+                        "30% follow-on frames of a live system, 4 slots, batches of up to 8 frames of one class, "
+                        "solve to 1e-6")
+        # the same stream offered faster, up to saturation (mixed)
+        sweep = []
+        for qps in (80.0, 320.0, 1280.0):
+            r, _ = leg_serving(stream, "mixed", time_scale=20.0 / qps)
+            sweep.append({k: r[k] for k in ("offered_qps", "evals_per_s", "p50_ms", "p99_ms", "mean_batch")})
+            log(f"serving mixed at {qps:.0f}/s offered: {r['evals_per_s']:.1f} evals/s, p50 {r['p50_ms']:.1f} ms, "
+                f"p99 {r['p99_ms']:.1f} ms")
+        sv["qps_sweep_mixed"] = sweep
+        # trajectory-fitting pattern: frames of 8 same-size boxes (one class) batch into multi-frame launches
+        fit = []
+        for qps in (100.0, 400.0, 1600.0, 6400.0):
+            r, _ = leg_serving(make_fitting_stream(300, qps, 5), "mixed")
+            fit.append({k: r[k] for k in ("offered_qps", "evals_per_s", "p50_ms", "p99_ms", "mean_batch")})
+            log(f"fitting stream at {qps:.0f}/s offered: {r['evals_per_s']:.1f} evals/s, mean batch "
+                f"{r['mean_batch']:.2f}, p99 {r['p99_ms']:.1f} ms")
+        r1, _ = leg_serving(make_fitting_stream(300, 6400.0, 5), "mixed", max_batch=1)
+        log(f"fitting stream at 6400/s, no batching: {r1['evals_per_s']:.1f} evals/s")
+        sv["fitting_sweep_mixed"] = fit
+        sv["fitting_unbatched_6400qps"] = {k: r1[k] for k in ("evals_per_s", "p50_ms", "p99_ms", "mean_batch")}
+        sv["fitting_stream"] = "300 frames of 8 boxes of 512 waters (one force field), sigma 0.02 A, Poisson arrivals"
         out["serving"] = sv
     if not args.no_cpu:
         # the oracle's CG is preconditioned by the diagonal: its iteration count to the

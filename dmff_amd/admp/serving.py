@@ -1,26 +1,39 @@
 """Continuous-batching request loop for ADMP PME evaluations on one GPU.
 
 A request is one configuration (positions, box) of a periodic system with its
-force-field data.  The server keeps every live system on the device -- its
-native context (PME grids, FFT tables, neighbour lists, solver scratch) and the
-last induced dipoles as the warm start of its next frame -- and runs requests
-concurrently in ``slots`` worker threads, each on its own stream.  The library
-releases the interpreter lock inside every native call and each context works
-on streams of its own, so the work of different requests overlaps on the GPU.
+force-field data.  The server keeps every live system's induced dipoles on the
+device as the warm start of its next frame, and keeps native contexts (PME
+grids, FFT tables, neighbour lists, solver scratch, captured solver graphs)
+alive between requests.  Requests run concurrently in ``slots`` worker threads,
+each on its own stream; the library releases the interpreter lock inside every
+native call, so the work of different requests overlaps on the GPU.
 
-Admission (continuous batching): requests wait in arrival order; whenever a
-slot is free the scheduler takes the first waiting request whose system is idle
-and whose HBM footprint fits the budget (evicting idle systems, least recently
-used first, when that makes room).  A request never overtakes an earlier
-request of the same system, so the warm-start chain of a system -- and with it
-every result -- is the same whether it runs alone or in a mix (all reductions
-in the library have a fixed order).
+Batching: systems with the same force-field data, size and PME grid (one
+"class": same atom count, topology, parameters, cutoff, kappa and K) share
+native contexts, and up to ``max_batch`` waiting requests of one class --
+one frame per system -- run as the frames of ONE multi-frame launch sequence
+(every kernel of the evaluation, the solver's captured graphs included, covers
+all frames at once).  Requests of different classes never share a launch: a
+different K or atom count would have to be padded to a common grid, which
+changes the PME discretisation and so the result.
 
-Per-request latency (arrival -> result ready) and queueing time are recorded;
-``latency_summary`` gives p50/p99.
+Admission: requests wait in arrival order; whenever a slot is free the
+scheduler takes the first waiting request whose system is idle, gathers the
+later waiting requests of its class (systems idle, one frame each, none
+overtaking an earlier request of its own system), and admits the batch when a
+context for that class and frame count exists and is idle or its HBM footprint
+fits the budget (evicting idle contexts, least recently used first).  Frames of
+one system keep their order, so the warm-start chain of a system is the same
+alone and in a mix; within a batch each frame's solve is independent, so
+results agree with a lone run to the solver tolerance (to rounding when the
+batch holds one frame).
+
+Per-request latency (arrival -> result ready), queueing time, batch size and
+context-creation time are recorded; ``latency_summary`` gives p50/p99.
 """
 import collections
 import concurrent.futures
+import hashlib
 import math
 import threading
 import time
@@ -45,6 +58,19 @@ class SystemSpec:
         self.Q_local, self.pol, self.thole = Q_local, pol, thole
         self.scales = (mScales, pScales, dScales)
         self.rc, self.lmax, self.lpol, self.ethresh = float(rc), int(lmax), bool(lpol), float(ethresh)
+        self._class = None
+
+    def class_key(self):
+        """Digest of everything a shared context depends on besides the box."""
+        if self._class is None:
+            h = hashlib.sha1(repr((self.n_atoms, self.rc, self.lmax, self.lpol, self.ethresh)).encode())
+            for a in (self.axis_types, self.axis_idx, self.Q_local, self.pol, self.thole, *self.scales):
+                if a is not None:
+                    h.update(np.ascontiguousarray(np.asarray(a, dtype=np.float64)).tobytes())
+            if self.exclusions:
+                h.update(repr(sorted(self.exclusions.items())).encode())
+            self._class = h.hexdigest()
+        return self._class
 
 
 class Request:
@@ -55,29 +81,44 @@ class Request:
         self.box = box            # (3, 3) Angstrom, lattice vectors in rows
         self.arrival = time.perf_counter() if arrival is None else arrival
         self.future = concurrent.futures.Future()
+        self._bkey = None
 
 
 class Result:
-    def __init__(self, rid, key, energy, dpos, dbox, n_iter, residual, warm, t_arrival, t_start, t_end):
+    def __init__(self, rid, key, energy, dpos, dbox, n_iter, residual, warm, t_arrival, t_start, t_end,
+                 batch=1, ctx_ms=0.0):
         self.rid, self.key = rid, key
         self.energy, self.dpos, self.dbox = energy, dpos, dbox
         self.n_iter, self.residual, self.warm = n_iter, residual, warm
         self.queued_ms = (t_start - t_arrival) * 1e3
         self.latency_ms = (t_end - t_arrival) * 1e3
+        self.batch = int(batch)          # frames in the launch that served this request
+        self.ctx_ms = float(ctx_ms)      # native context creation inside this request's latency
 
 
 class _Live:
-    def __init__(self, spec, engine, nbytes, kappa, K):
-        self.spec, self.engine, self.nbytes = spec, engine, nbytes
-        self.kappa, self.K = kappa, K
+    """Per-system state: the warm start of the next frame."""
+
+    def __init__(self, spec):
+        self.spec = spec
         self.U = None
         self.busy = False
         self.last_used = time.perf_counter()
+
+
+class _Ctx:
+    """A native context for one class and frame count."""
+
+    def __init__(self, key, nbytes, kappa, K):
+        self.key, self.nbytes, self.kappa, self.K = key, nbytes, kappa, K
+        self.engine = None
         self.dev = {}
+        self.busy = False
+        self.last_used = time.perf_counter()
 
 
 def estimate_bytes(n_atoms, K, rc, precision="mixed", density=0.1003):
-    """Upper estimate of a context's device footprint before it is created."""
+    """Upper estimate of a one-frame context's device footprint before it is created."""
     w = 4 if precision == "mixed" else 8
     cap = int(1.5 * density * 4.18879 * rc ** 3) + 64
     G = K[0] * K[1] * K[2]
@@ -89,17 +130,19 @@ def estimate_bytes(n_atoms, K, rc, precision="mixed", density=0.1003):
 
 class ADMPServer:
     def __init__(self, device="cuda:0", precision="mixed", slots=4, hbm_budget=None, tol=1e-6, maxiter=500,
-                 runner=None):
-        """runner(req, live, t_start) -> Result replaces the GPU evaluation (host-logic tests)."""
+                 max_batch=8, runner=None, max_systems=4096):
+        """runner(reqs, lives, t_start) -> [Result] replaces the GPU evaluation (host-logic tests)."""
         self.device = torch.device(device)
         self.runner = runner
         self.precision, self.tol, self.maxiter = precision, float(tol), int(maxiter)
+        self.max_batch, self.max_systems = max(1, int(max_batch)), int(max_systems)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             hbm_budget = int(0.5 * free)
         self.budget = int(hbm_budget)
         self.used = 0
-        self.systems = {}
+        self.systems = collections.OrderedDict()   # key -> _Live, least recently used first
+        self.ctxs = {}                              # (class key, K, frames) -> _Ctx
         self.pending = collections.deque()
         self.cv = threading.Condition()
         self.closed = False
@@ -138,19 +181,37 @@ class ADMPServer:
         return True
 
     # ------------------------------------------------------------ scheduling
-    def _footprint(self, spec, box):
-        kappa, K1, K2, K3 = setup_ewald_parameters(spec.rc, spec.ethresh, np.asarray(box))
-        return estimate_bytes(spec.n_atoms, (K1, K2, K3), spec.rc, self.precision), kappa, (K1, K2, K3)
+    def _batch_key(self, req):
+        """(class, K) of a request, with its one-frame footprint and kappa."""
+        if req._bkey is None:
+            spec = req.spec
+            kappa, K1, K2, K3 = setup_ewald_parameters(spec.rc, spec.ethresh, np.asarray(req.box))
+            K = (K1, K2, K3)
+            req._bkey = ((spec.class_key(), K), estimate_bytes(spec.n_atoms, K, spec.rc, self.precision), kappa)
+        return req._bkey
 
     def _evict_for_locked(self, need):
-        idle = sorted((lv for lv in self.systems.values() if not lv.busy), key=lambda lv: lv.last_used)
-        for lv in idle:
+        idle = sorted((c for c in self.ctxs.values() if not c.busy), key=lambda c: c.last_used)
+        for c in idle:
             if self.used + need <= self.budget:
                 break
-            self.used -= lv.nbytes
-            del self.systems[lv.spec.key]
-            lv.engine = None
+            self.used -= c.nbytes
+            del self.ctxs[c.key]
+            c.engine = None
         return self.used + need <= self.budget
+
+    def _live_locked(self, spec):
+        lv = self.systems.get(spec.key)
+        if lv is None:
+            lv = _Live(spec)
+            self.systems[spec.key] = lv
+            while len(self.systems) > self.max_systems:
+                old = next(iter(self.systems.values()))
+                if old.busy:
+                    break
+                del self.systems[old.spec.key]
+        self.systems.move_to_end(spec.key)
+        return lv
 
     def _admit_locked(self):
         blocked = set()
@@ -158,29 +219,49 @@ class ADMPServer:
         while self.free_slots > 0 and i < len(self.pending):
             req = self.pending[i]
             key = req.spec.key
-            if key in blocked:
+            lv = self.systems.get(key)
+            if key in blocked or (lv is not None and lv.busy):
+                blocked.add(key)   # keep the frame order of a system
                 i += 1
                 continue
-            lv = self.systems.get(key)
-            if lv is not None:
-                if lv.busy:
-                    blocked.add(key)   # keep the frame order of a system
-                    i += 1
-                    continue
-                lv.busy = True
-            else:
-                need, kappa, K = self._footprint(req.spec, req.box)
+            bkey, need1, kappa = self._batch_key(req)
+            batch, members, skip = [i], {key}, set(blocked)
+            j = i + 1
+            while len(batch) < self.max_batch and j < len(self.pending):
+                r2 = self.pending[j]
+                k2 = r2.spec.key
+                lv2 = self.systems.get(k2)
+                if k2 not in members and k2 not in skip and not (lv2 is not None and lv2.busy) \
+                        and self._batch_key(r2)[0] == bkey:
+                    batch.append(j)
+                    members.add(k2)
+                else:
+                    skip.add(k2)
+                j += 1
+            ck = (bkey[0], bkey[1], len(batch))
+            ctx = self.ctxs.get(ck)
+            if ctx is not None and ctx.busy:
+                blocked.add(key)
+                i += 1
+                continue
+            if ctx is None:
+                need = need1 * len(batch)
                 if self.used + need > self.budget and not self._evict_for_locked(need):
                     blocked.add(key)
                     i += 1
                     continue
-                lv = _Live(req.spec, None, need, kappa, K)
-                lv.busy = True
-                self.systems[key] = lv
+                ctx = _Ctx(ck, need, kappa, bkey[1])
+                self.ctxs[ck] = ctx
                 self.used += need
-            del self.pending[i]
+            ctx.busy = True
+            reqs = [self.pending[k] for k in batch]
+            for k in reversed(batch):
+                del self.pending[k]
+            lives = [self._live_locked(r.spec) for r in reqs]
+            for x in lives:
+                x.busy = True
             self.free_slots -= 1
-            self.work.append((req, lv))
+            self.work.append((reqs, lives, ctx))
             self.cv.notify_all()
 
     # --------------------------------------------------------------- workers
@@ -192,59 +273,90 @@ class ADMPServer:
                     self.cv.wait()
                 if not self.work and self.closed:
                     return
-                req, lv = self.work.popleft()
+                reqs, lives, ctx = self.work.popleft()
             t_start = time.perf_counter()
             try:
                 if self.runner is not None:
-                    res = self.runner(req, lv, t_start)
+                    res = self.runner(reqs, lives, t_start)
                 else:
                     with torch.cuda.device(self.device), torch.cuda.stream(stream):
-                        res = self._run(req, lv, t_start)
+                        res = self._run(reqs, lives, ctx, t_start)
                         stream.synchronize()
-                req.future.set_result(res)
-            except Exception as e:  # noqa: BLE001 - reported through the future
-                req.future.set_exception(e)
+                for r, x in zip(reqs, res):
+                    r.future.set_result(x)
+            except Exception as e:  # noqa: BLE001 - reported through the futures
+                for r in reqs:
+                    if not r.future.done():
+                        r.future.set_exception(e)
             with self.cv:
-                lv.busy = False
-                lv.last_used = time.perf_counter()
+                now = time.perf_counter()
+                ctx.busy = False
+                ctx.last_used = now
+                for x in lives:
+                    x.busy = False
+                    x.last_used = now
                 self.free_slots += 1
-                if req.future.done() and not req.future.exception():
-                    self.results.append(req.future.result())
+                for r in reqs:
+                    if r.future.done() and not r.future.exception():
+                        self.results.append(r.future.result())
                 self._admit_locked()
                 self.cv.notify_all()
 
-    def _run(self, req, lv, t_start):
-        spec = req.spec
-        if lv.engine is None:
-            eng = Engine(spec.n_atoms, spec.lmax, spec.lpol, True, lv.kappa, lv.K, spec.rc,
+    def _run(self, reqs, lives, ctx, t_start):
+        spec = reqs[0].spec
+        F, N = len(reqs), spec.n_atoms
+        t_ctx = 0.0
+        if ctx.engine is None:
+            t0 = time.perf_counter()
+            eng = Engine(N, spec.lmax, spec.lpol, True, ctx.kappa, ctx.K, spec.rc, n_frames=F,
                          precision=self.precision, device=self.device)
             eng.set_topology(spec.axis_types, spec.axis_idx, spec.exclusions)
-            lv.engine = eng
+            t = lambda x: torch.as_tensor(np.asarray(x), dtype=F64, device=self.device)
+            ctx.dev = {"Q": t(spec.Q_local), "pol": t(spec.pol), "thole": t(spec.thole)}
+            ctx.engine = eng
             nb = eng.memory_bytes()
             with self.cv:
-                self.used += nb - lv.nbytes
-                lv.nbytes = nb
-            t = lambda x: torch.as_tensor(np.asarray(x), dtype=F64, device=self.device)
-            lv.dev = {"Q": t(spec.Q_local), "pol": t(spec.pol), "thole": t(spec.thole)}
-        eng = lv.engine
-        pos = torch.as_tensor(np.asarray(req.pos), dtype=F64).to(self.device, non_blocking=False)
-        box = torch.as_tensor(np.asarray(req.box), dtype=F64).to(self.device, non_blocking=False)
+                self.used += nb - ctx.nbytes
+                ctx.nbytes = nb
+            t_ctx = time.perf_counter() - t0
+        eng = ctx.engine
+        pos = torch.as_tensor(np.stack([np.asarray(r.pos) for r in reqs]), dtype=F64).to(self.device)
+        box = torch.as_tensor(np.stack([np.asarray(r.box) for r in reqs]), dtype=F64).to(self.device)
         eng.build_neighbors(pos, box)
-        warm = lv.U is not None
-        E, g, U, info = eng.compute(pos, box, lv.dev["Q"], lv.dev["pol"], lv.dev["thole"], *spec.scales,
-                                    U_init=lv.U, solver="pcg", tol=self.tol, maxiter=self.maxiter, want_grad=True)
-        lv.U = U
+        warm = [x.U is not None for x in lives]
+        U0 = None
+        if any(warm):
+            # a cold frame starts from zero dipoles, which is the library's cold start
+            U0 = torch.stack([x.U if x.U is not None else torch.zeros((N, 3), dtype=F64, device=self.device)
+                              for x in lives])
+        E, g, U, info = eng.compute(pos, box, ctx.dev["Q"], ctx.dev["pol"], ctx.dev["thole"], *spec.scales,
+                                    U_init=U0, solver="pcg", tol=self.tol, maxiter=self.maxiter, want_grad=True)
+        for f, x in enumerate(lives):
+            x.U = U[f].clone()
         t_end = time.perf_counter()
-        return Result(req.rid, spec.key, float(E[0]), g["pos"][0], g["box"][0], info["n_iter"][0],
-                      info["residual"][0], warm, req.arrival, t_start, t_end)
+        Eh = E.cpu().tolist()
+        return [Result(r.rid, r.spec.key, Eh[f], g["pos"][f], g["box"][f], info["n_iter"][f], info["residual"][f],
+                       warm[f], r.arrival, t_start, t_end, batch=F, ctx_ms=t_ctx * 1e3)
+                for f, r in enumerate(reqs)]
 
 
 def latency_summary(results):
     lat = np.array([r.latency_ms for r in results])
     if len(lat) == 0:
         return {}
-    return {"n": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}
+    out = {"n": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}
+    ctx = np.array([getattr(r, "ctx_ms", 0.0) for r in results])
+    if ctx.any():
+        reuse = lat[ctx == 0.0]
+        out.update({"ctx_created": int((ctx > 0).sum()), "ctx_ms_mean": float(ctx[ctx > 0].mean()),
+                    "ctx_ms_p50": float(np.percentile(ctx[ctx > 0], 50))})
+        if len(reuse):
+            out.update({"reused_ctx_p50_ms": float(np.percentile(reuse, 50)),
+                        "reused_ctx_p99_ms": float(np.percentile(reuse, 99))})
+    b = np.array([getattr(r, "batch", 1) for r in results])
+    out["mean_batch"] = float(b.mean())
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -254,14 +366,17 @@ WATER_DENSITY_NM3 = 33.43
 
 
 def water_spec(key, n_waters, seed, rc=8.0):
-    """A polarisable water box of n_waters at liquid density with cutoff rc.
-    Boxes shorter than 2 rc keep rc: the pair set is then the reference's
-    single minimum image, each pair once at its rounded-fractional image
-    (dmff/admp/spatial.py:13-32), and kappa follows rc as in pme.py:460-466."""
-    from .water import make_water_system
+    """A polarisable water box of n_waters at liquid density with cutoff rc,
+    carrying configuration 1's multipoles (every box has the same force field;
+    the seed draws the geometry).  Boxes shorter than 2 rc keep rc: the pair set
+    is then the reference's single minimum image, each pair once at its
+    rounded-fractional image (dmff/admp/spatial.py:13-32), and kappa follows rc
+    as in pme.py:460-466."""
+    from .water import config1_multipoles, make_water_system
     L_nm = (n_waters / WATER_DENSITY_NM3) ** (1.0 / 3.0)
     n_side = int(math.ceil(n_waters ** (1.0 / 3.0) - 1e-9))
-    S = make_water_system(n_waters, L_nm, seed, polarizable=True, n_side=n_side)
+    S = make_water_system(n_waters, L_nm, seed, polarizable=True, n_side=n_side,
+                          Q_local_types=config1_multipoles())
     box = S.box_nm * 10.0
     spec = SystemSpec(key, S.n_atoms, S.axis_types, S.axis_idx, S.bonded, S.Q_local, S.pol, S.thole,
                       S.mScales, S.pScales, S.dScales, rc)
@@ -285,4 +400,19 @@ def make_stream(n_requests, qps, seed, n_min=64, n_max=8192, follow_frac=0.3, si
             spec, pos, box = water_spec(f"sys{len(systems)}", nw, int(rng.integers(1 << 30)))
             systems.append((spec, pos, box))
         out.append((t, spec, pos, box))
+    return out
+
+
+def make_fitting_stream(n_requests, qps, seed, n_systems=8, n_waters=512, sigma_A=0.02):
+    """Poisson arrivals of frames of n_systems same-size boxes (one force field,
+    different geometries): the trajectory-fitting pattern, in which waiting
+    frames of different systems share one class and run as one batch."""
+    rng = np.random.default_rng(seed)
+    systems = [water_spec(f"fit{k}", n_waters, int(rng.integers(1 << 30))) for k in range(n_systems)]
+    t = 0.0
+    out = []
+    for k in range(n_requests):
+        t += rng.exponential(1.0 / qps)
+        spec, pos0, box = systems[int(rng.integers(n_systems))]
+        out.append((t, spec, pos0 + rng.normal(0.0, sigma_A, pos0.shape), box))
     return out

@@ -35,28 +35,46 @@ def _random_rotation(rng):
 
 
 def water_positions(n_side, L, rng, jitter=0.02, min_oo=0.25, n_waters=None):
-    """Positions (nm), shape (3*nw, 3); O on a jittered n_side^3 lattice."""
+    """Positions (nm), shape (3*nw, 3); O on a jittered n_side^3 lattice.
+
+    An oxygen within min_oo of a new one sits at a lattice site at most
+    floor((min_oo + 2 jitter) / cell) + 1 cells away along each axis, so only
+    those sites are checked; the accept/redraw decisions, and with them the
+    random stream, are those of a check against every placed oxygen."""
     h = np.array([[OH * math.sin(HOH / 2), 0.0, OH * math.cos(HOH / 2)],
                   [-OH * math.sin(HOH / 2), 0.0, OH * math.cos(HOH / 2)]])
-    sites = np.array([[i, j, k] for i in range(n_side) for j in range(n_side) for k in range(n_side)],
-                     dtype=np.float64) * (L / n_side)
-    if n_waters is not None and n_waters < len(sites):
-        sites = sites[np.sort(rng.choice(len(sites), n_waters, replace=False))]
-    O = []
-    out = []
+    ijk = np.array([[i, j, k] for i in range(n_side) for j in range(n_side) for k in range(n_side)],
+                   dtype=np.int64)
+    if n_waters is not None and n_waters < len(ijk):
+        ijk = ijk[np.sort(rng.choice(len(ijk), n_waters, replace=False))]
+    sites = ijk.astype(np.float64) * (L / n_side)
     cell = L / n_side
-    for s in sites:
+    m = int(math.floor((min_oo + 2 * jitter) / cell)) + 1
+    if 2 * m + 1 >= n_side:
+        offs = None     # the window wraps the whole lattice: check every placed oxygen
+    else:
+        r = range(-m, m + 1)
+        offs = [(a, b, c) for a in r for b in r for c in r]
+    placed = {}
+    out = []
+    for s, key in zip(sites, map(tuple, ijk)):
+        if offs is None:
+            cand = list(placed.values())
+        else:
+            cand = [placed[q] for q in (((key[0] + a) % n_side, (key[1] + b) % n_side, (key[2] + c) % n_side)
+                                        for a, b, c in offs) if q in placed]
+        cand = np.asarray(cand) if cand else None
         for _ in range(1000):
             o = s + rng.uniform(-jitter, jitter, size=3) + cell * 0.25
             ok = True
-            if O:
-                d = np.asarray(O) - o
+            if cand is not None:
+                d = cand - o
                 d -= L * np.round(d / L)
                 ok = np.min(np.einsum("ij,ij->i", d, d)) >= min_oo * min_oo
             if ok:
                 break
         R = _random_rotation(rng)
-        O.append(o)
+        placed[key] = o
         out.append(o)
         out.append(o + R @ h[0])
         out.append(o + R @ h[1])
@@ -67,7 +85,8 @@ class WaterSystem:
     pass
 
 
-def make_water_system(n_waters, L_nm, seed, polarizable=True, n_side=None):
+def make_water_system(n_waters, L_nm, seed, polarizable=True, n_side=None, Q_local_types=None):
+    """Q_local_types: per-type harmonic multipoles to use instead of the seeded draw."""
     rng = np.random.default_rng(seed)
     if n_side is None:
         n_side = int(round(n_waters ** (1.0 / 3.0)))
@@ -91,7 +110,7 @@ def make_water_system(n_waters, L_nm, seed, polarizable=True, n_side=None):
     S.n_atoms = 3 * nw
     S.map_atomtype = np.tile(np.array([0, 1, 1]), nw)
     S.map_poltype = S.map_atomtype.copy()
-    S.Q_local_types = cart2harm(cart, 2)
+    S.Q_local_types = cart2harm(cart, 2) if Q_local_types is None else np.array(Q_local_types, dtype=np.float64)
     S.Q_local = S.Q_local_types[S.map_atomtype]
     S.pol_types = np.array([0.00088, 0.00050]) * 1000.0
     S.thole_types = np.array([0.39, 0.39])
@@ -129,3 +148,13 @@ CONFIGS = {
 def config_system(cfg):
     c = CONFIGS[cfg]
     return make_water_system(c["n_waters"], c["L"], 1000 + cfg, polarizable=c["polarizable"]), c
+
+
+_CFG1_Q = []
+
+
+def config1_multipoles():
+    """Configuration 1's per-type local multipoles (harmonic, e*A units)."""
+    if not _CFG1_Q:
+        _CFG1_Q.append(config_system(1)[0].Q_local_types)
+    return _CFG1_Q[0]

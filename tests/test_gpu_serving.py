@@ -59,3 +59,66 @@ def test_results_alone_equal_results_in_mix():
         assert d <= 1e-10, (rid, d)
     # follow-on frames started from the previous frame's induced dipoles
     assert any(r.warm for r in alone.values())
+
+
+def _serve(reqs, slots, max_batch, sequential, precision="fp64", tol=1e-10):
+    from dmff_amd.admp.serving import ADMPServer, Request
+    srv = ADMPServer(device=DEV, precision=precision, slots=slots, tol=tol, max_batch=max_batch)
+    out = {}
+    try:
+        futs = []
+        for rid, (spec, pos, box) in enumerate(reqs):
+            f = srv.submit(Request(rid, spec, pos, box))
+            if sequential:
+                out[rid] = f.result(timeout=120)
+            futs.append((rid, f))
+        for rid, f in futs:
+            out[rid] = f.result(timeout=120)
+    finally:
+        srv.close()
+    return out
+
+
+def test_same_class_frames_batched_equal_alone():
+    """Four same-size boxes (one class) with two frames each: served one at a
+    time (one-frame launches) and all at once (batches of up to four frames in
+    one launch sequence).  Each frame's solve is independent inside a batch, so
+    results agree to the solver tolerance (1e-10 here): energy 1e-9, forces 1e-8
+    relative."""
+    from dmff_amd.admp.serving import water_spec
+    rng = np.random.default_rng(3)
+    systems = [water_spec(f"c{k}", 64, 500 + k) for k in range(4)]
+    reqs = [(s, p, b) for s, p, b in systems]
+    reqs += [(s, p + rng.normal(0.0, 0.02, p.shape), b) for s, p, b in systems]
+    alone = _serve(reqs, 1, 1, True)
+    batched = _serve(reqs, 1, 4, False)
+    sizes = sorted({r.batch for r in batched.values()})
+    print(f"batch sizes served: {sizes}", flush=True)
+    assert max(sizes) > 1
+    for rid in alone:
+        a, m = alone[rid], batched[rid]
+        assert a.warm == m.warm
+        assert abs(a.energy - m.energy) <= 1e-9 * abs(a.energy), (rid, a.energy, m.energy)
+        d = float(torch.linalg.norm(a.dpos - m.dpos) / torch.linalg.norm(a.dpos))
+        assert d <= 1e-8, (rid, d)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_request_vs_oracle(precision):
+    """One served request (64 waters, L 12.4 A < 2 rc: the reference's single
+    minimum image) against the oracle on the same pairs: FP64 energy 1e-8 and
+    forces 1e-6 relative (solve to 1e-10), mixed 2e-5 / 1e-4 (solve to 1e-6)."""
+    from dmff_amd.admp.serving import water_spec
+    from tests.systems import oracle_pme, reference_pairs, rel_err
+    spec, pos, box = water_spec("o", 64, 77)
+    tol = 1e-10 if precision == "fp64" else 1e-6
+    r = _serve([(spec, pos, box)], 1, 1, True, precision=precision, tol=tol)[0]
+    pairs = reference_pairs(pos, box, spec.rc, spec.exclusions)
+    o = oracle_pme(pos, box, pairs, spec.axis_types, spec.axis_idx, spec.Q_local, spec.pol, spec.thole, spec.scales,
+                   spec.rc, True)
+    de = abs(r.energy - o["E"]) / abs(o["E"])
+    df = rel_err(r.dpos.cpu().numpy(), o["pos"])
+    print(f"served {precision}: E {r.energy:.10f} oracle {o['E']:.10f}; E rel {de:.2e}, force rel RMS {df:.2e}",
+          flush=True)
+    te, tf = (1e-8, 1e-6) if precision == "fp64" else (2e-5, 1e-4)
+    assert de < te and df < tf
