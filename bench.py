@@ -272,6 +272,8 @@ def leg_batch(D, precision, n_frames=64, steps=2, warmup=1, sigma_nm=0.005, prof
         res["kernel_us_per_launch"] = {k: v[0] / v[1] * 1e3 for k, v in prof.items() if v[1]}
         res["roofline"] = {k: roofline(k, prof[k], Db, precision, frames=n_frames, prof=prof)
                            for k in ("spread", "fft", "influence", "gather", "real_matvec") if prof.get(k, (0, 0))[1]}
+        for r in res["roofline"].values():
+            r["traffic"], r["traffic_source"] = measured_traffic(r["kernel"].split("/")[0], batch=True)
     return res
 
 
@@ -421,6 +423,8 @@ def main():
     dom = max(cand.items(), key=lambda x: x[1][0])
     roof = roofline(dom[0], dom[1], D, "mixed", prof=kprof)
     roof["traffic"], roof["traffic_source"] = measured_traffic(roof["kernel"])
+    rf = kprof.get("real_final", (0, 0))
+    roof["pair_kernel_valu"] = pair_kernel_valu(rf[0] / rf[1] * 1e3 if rf[1] else None)
     out = {
         "metric": "ADMP E+F+grad evals/s @12,288 atoms (mixed precision); FP64 beside it",
         "value": 1.0 / mix["dt"],
@@ -511,25 +515,61 @@ def main():
 
 
 COUNTER_SUMMARY = os.path.join("profiles", "admp_12288_hbm_counters.md")
+BATCH_COUNTER_SUMMARY = os.path.join("profiles", "admp_12288x64_batch_hbm_counters.md")
+VALU_SUMMARY = os.path.join("profiles", "admp_12288_pair_kernel_valu.md")
+FP64_VALU_PEAK_TFLOPS, FP32_VALU_PEAK_TFLOPS = 78.6, 157.3
 
 
-def measured_traffic(kname):
-    """HBM bytes per launch of kernel `kname` at this workload, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors
-    read off a 256 MiB copy), from the committed summary of two rocprofv3 --pmc runs of this script
-    (`--legs none --steps 1 --warmup 0 --calibrate-copy`, one counter each).  (None, None) without a row for it."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), COUNTER_SUMMARY)
+def _summary_rows(rel):
+    """Table rows (lists of cells) of a committed profile summary; [] when it is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), rel)
     try:
         lines = open(path).read().splitlines()
     except OSError:
-        return None, None
-    for line in lines:
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0].startswith("`" + kname + "<"):
+        return []
+    return [[c.strip() for c in ln.strip().strip("|").split("|")] for ln in lines if ln.startswith("| `")]
+
+
+def measured_traffic(kname, batch=False):
+    """HBM bytes per launch of kernel `kname` at this workload, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors
+    read off a 256 MiB copy), from the committed summary of two rocprofv3 --pmc runs of this script, one counter
+    each (single frame: `--legs none --steps 1 --warmup 0 --calibrate-copy`; 64-frame batch: `--legs batch ...`).
+    (None, None) without a row for it."""
+    rel, ncol, col = (BATCH_COUNTER_SUMMARY, 9, 3) if batch else (COUNTER_SUMMARY, 7, 4)
+    for cells in _summary_rows(rel):
+        if len(cells) == ncol and cells[0].startswith("`" + kname + "<"):
             try:
-                return int(cells[4]) * 1024, COUNTER_SUMMARY + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+                return int(cells[col]) * 1024, rel + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
             except ValueError:
                 return None, None
     return None, None
+
+
+def pair_kernel_valu(us_mixed):
+    """VALU FLOP rate of the real-space final pair kernel: lane FLOPs per launch (64 x the SQ_INSTS_VALU_FLOPS_*
+    counters, committed summary of two rocprofv3 --pmc runs of `--legs none`) over the launch duration, the
+    mixed path's measured live (HIP events, graphs off), the FP64 path's from the summary's kernel trace."""
+    out = {}
+    for cells in _summary_rows(VALU_SUMMARY):
+        if len(cells) != 11:
+            continue
+        try:
+            g64, g32, us = float(cells[6]), float(cells[7]), float(cells[8])
+        except ValueError:
+            continue
+        if cells[0].startswith("`k_real_final<float"):
+            t = us_mixed if us_mixed else us
+            ach = g32 * 1e9 / (t * 1e-6) / 1e12
+            out = {"kernel": "k_real_final<float, true>", "bound": "valu", "achieved": ach,
+                   "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP32_VALU_PEAK_TFLOPS,
+                   "frac_of_unpacked_fp32": ach / FP64_VALU_PEAK_TFLOPS, "fp32_flop_per_launch": g32 * 1e9,
+                   "fp64_flop_per_launch": g64 * 1e9, "avg_launch_us": t, "source": VALU_SUMMARY, **out}
+        elif cells[0].startswith("`k_real_final<double"):
+            ach = g64 * 1e9 / (us * 1e-6) / 1e12
+            out["fp64_path"] = {"kernel": "k_real_final<double, true>", "achieved": ach,
+                                "peak": FP64_VALU_PEAK_TFLOPS, "frac": ach / FP64_VALU_PEAK_TFLOPS,
+                                "fp64_flop_per_launch": g64 * 1e9, "avg_launch_us": us}
+    return out or None
 
 
 def roofline(name, tm, D, prec, frames=1, prof=None):
