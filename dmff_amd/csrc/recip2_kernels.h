@@ -380,9 +380,36 @@ k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R*
             const PcgState* __restrict__ skip, CgCheck chk) {
     static_assert(BRX == 4 && BRY == 4 && BRZ == 4, "column spread assumes 4x4x4 bricks");
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
+    constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
+    constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
+    const int f = blockIdx.y;
+    const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY, nbx = (K0 + BRX - 1) / BRX;
+    const int nbr = nbx * nby * nbz;
+    const int b = xcd_item(blockIdx.x, nbr);
+    if (b >= nbr) return;
+    const int t = threadIdx.x;
+    const int lo = bstart[(size_t)f * (nbr + 1) + b];
+    const int total = (int)min((long)bstart[(size_t)f * (nbr + 1) + b + 1], cap) - lo;
+    // A lane's candidate for the next staging round is fetched into registers one
+    // round ahead (the first one before the convergence test), so its global
+    // loads are in flight while the wavefront works on the current round.
+    int2 ne = make_int2(0, 0);
+    R nw[3] = {0, 0, 0}, nq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) nq[q] = 0;
+    auto fetch = [&](int c0) {
+        if (t < min(SP_CHUNK, total - c0)) {
+            ne = blist[(size_t)f * cap + lo + c0 + t];
+            const size_t a = (size_t)f * N + ne.x;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) nw[d] = wfrac[a * 3 + d];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) nq[q] = fq[a * 10 + q];
+        }
+    };
+    if (total > 0) fetch(0);
     if (chk.on) {
         // every block reduces the same partials in the same order: one decision
-        const int f = blockIdx.y;
         const double* mxp = chk.parts.mx(f, chk.cur);
         double m = 0.0;
         for (int k = threadIdx.x; k < chk.parts.nb; k += 64) m = fmax(m, mxp[k]);
@@ -395,33 +422,21 @@ k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R*
             return;
         }
     }
-    constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
-    constexpr int NT = LM == 0 ? 1 : (LM == 1 ? 2 : 3);
     __shared__ int s_r[SP_CHUNK];
     __shared__ R s_t[SP_CHUNK][3][NT][6];
     __shared__ R s_q[SP_CHUNK][NQ];
-    const int f = blockIdx.y;
-    const int nbz = (K2 + BRZ - 1) / BRZ, nby = (K1 + BRY - 1) / BRY, nbx = (K0 + BRX - 1) / BRX;
-    const int nbr = nbx * nby * nbz;
-    const int b = xcd_item(blockIdx.x, nbr);
-    if (b >= nbr) return;
     const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
-    const int t = threadIdx.x;
     const int g = t >> 4, c = t & 15;
     const int lx = c >> 2, ly = c & 3;
     const int px = bx * BRX + lx, py = by * BRY + ly, pz0 = bz * BRZ;
-    const int lo = bstart[(size_t)f * (nbr + 1) + b];
-    const int total = (int)min((long)bstart[(size_t)f * (nbr + 1) + b + 1], cap) - lo;
     R acc[4] = {0, 0, 0, 0};
     for (int c0 = 0; c0 < total; c0 += SP_CHUNK) {
         const int nchunk = min(SP_CHUNK, total - c0);
         if (t < nchunk) {
-            const int2 e = blist[(size_t)f * cap + lo + c0 + t];
-            const size_t a = (size_t)f * N + e.x;
-            s_r[t] = e.y;
+            s_r[t] = ne.y;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const R w = wfrac[a * 3 + d];
+                const R w = nw[d];
 #pragma unroll
                 for (int k = 0; k < 6; ++k) {
                     R v, d1, d2;
@@ -432,9 +447,10 @@ k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R*
                 }
             }
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) s_q[t][q] = fq[a * 10 + q];
+            for (int q = 0; q < NQ; ++q) s_q[t][q] = nq[q];
         }
         __syncthreads();
+        if (c0 + SP_CHUNK < total) fetch(c0 + SP_CHUNK);
         for (int s2 = g; s2 < nchunk; s2 += 4) {
             const int e = s_r[s2];
             const int k0 = lx + 5 - (e & 15), k1 = ly + 5 - ((e >> 4) & 15);

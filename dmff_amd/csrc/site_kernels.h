@@ -732,6 +732,31 @@ __device__ inline void parts_total3(const double* __restrict__ a, const double* 
     __syncthreads();
 }
 
+// The CG update's three totals in one pass: max of the max|r| partials and sum
+// of the r.z partials (nb slots each), sum of the z.w partials (nw slots); each
+// total is formed in the same order as parts_total3 forms it; sh >= 48 doubles
+__device__ inline void parts_total_cg(const double* __restrict__ mxp, const double* __restrict__ rzp, int nb,
+                                      const double* __restrict__ zwp, int nw, double* sh, double& res,
+                                      double& gamma, double& delta) {
+    double v0 = 0, v1 = 0, v2 = 0;
+    const int n = nb > nw ? nb : nw;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        if (k < nb) {
+            v0 = fmax(v0, mxp[k]);
+            v1 += rzp[k];
+        }
+        if (k < nw) v2 += zwp[k];
+    }
+    v0 = wave_allmax(v0); v1 = wave_allsum(v1); v2 = wave_allsum(v2);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sh[w] = v0; sh[16 + w] = v1; sh[32 + w] = v2; }
+    __syncthreads();
+    res = sh[0]; gamma = sh[16]; delta = sh[32];
+    const int nwv = (int)(blockDim.x >> 6);
+    for (int k = 1; k < nwv; ++k) { res = fmax(res, sh[k]); gamma += sh[16 + k]; delta += sh[32 + k]; }
+    __syncthreads();
+}
+
 // block reduction of one value into every thread
 template <bool MAX>
 __device__ inline double block_red(double v, double* sh) {
@@ -887,25 +912,7 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
                                                          PcgState* __restrict__ st) {
     __shared__ double sh[48];
     const int f = blockIdx.y;
-    if (st[f].done) return;
     const int nx = cur ^ 1;
-    double res, gamma, delta, unused;
-    parts_total3(parts.mx(f, cur), parts.rz(f, cur), parts.rz(f, cur), parts.nb, sh, res, gamma, unused);
-    // every array of a call is read over that call's bound: the z.w slots (nwu)
-    parts_total3(parts.zw(f), parts.zw(f), parts.zw(f), nwu, sh, unused, delta, unused);
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (res < tol) {
-        if (lead) {
-            st[f].res = res;
-            st[f].done = 1;
-        }
-        return;
-    }
-    const double gp = st[f].gam[nx], ap = st[f].alp[nx];
-    const bool restart = !(gp < INFINITY) || gp == 0.0 || ap == 0.0;
-    const double beta = restart ? 0.0 : gamma / gp;
-    const double den = restart ? delta : delta - beta * gamma / ap;
-    const double alpha = den != 0.0 ? gamma / den : 0.0;
     const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
     // r and s have two buffers: this update reads buffer cur and writes cur ^ 1,
     // so a site can form its partner's new residual from values no thread writes
@@ -914,31 +921,73 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
     double* rn = r + (size_t)nx * nf3;
     const double* sc = s + (size_t)cur * nf3;
     double* sn = s + (size_t)nx * nf3;
-    double lrz = 0, lmx = 0;
+    // Every load that does not depend on the step lengths is issued before the
+    // reductions that give them, so the wave waits for memory once, not four
+    // times in a row (state, partial sums, site vectors, partner vectors).
+    const double gp = st[f].gam[nx], ap = st[f].alp[nx];
+    const int done = st[f].done;
+    const size_t a = (size_t)f * N + (i < N ? i : 0);
+    double zv[3] = {0, 0, 0}, pv0[3] = {0, 0, 0}, wv[3] = {0, 0, 0}, sv0[3] = {0, 0, 0}, rv0[3] = {0, 0, 0};
+    double wj[3] = {0, 0, 0}, sj[3] = {0, 0, 0}, rj0[3] = {0, 0, 0};
+    double poli = 0.0;
+    int j = -1;
     if (i < N) {
-        const size_t a = (size_t)f * N + i;
-        double u[3], rv[3], rj[3] = {0, 0, 0};
         for (int k = 0; k < 3; ++k) {
             const size_t e = a * 3 + k;
-            const double pv = restart ? z[e] : z[e] + beta * p[e];
-            const double sv = restart ? w[e] : w[e] + beta * sc[e];
-            p[e] = pv;
-            sn[e] = sv;
-            x[e] += alpha * pv;
-            rv[k] = rc[e] - alpha * sv;
-            rn[e] = rv[k];
-            if (pol[i] > 0.001) lmx = fmax(lmx, fabs(rv[k]));
+            zv[k] = z[e];
+            pv0[k] = p[e];
+            wv[k] = w[e];
+            sv0[k] = sc[e];
+            rv0[k] = rc[e];
         }
-        const int j = pcj ? pcj[a] : -1;
+        poli = pol[i];
+        j = pcj ? pcj[a] : -1;
         if (j >= 0) {
             const size_t aj = (size_t)f * N + j;
             for (int k = 0; k < 3; ++k) {
                 const size_t e = aj * 3 + k;
-                const double sv = restart ? w[e] : w[e] + beta * sc[e];
-                rj[k] = rc[e] - alpha * sv;
+                wj[k] = w[e];
+                sj[k] = sc[e];
+                rj0[k] = rc[e];
             }
         }
-        pc_apply(j, j >= 0 ? pcd + a * 8 : nullptr, inv_diag(pol[i]), rv, rj, u);
+    }
+    if (done) return;   // the same value in every thread of the block
+    double res, gamma, delta;
+    parts_total_cg(parts.mx(f, cur), parts.rz(f, cur), parts.nb, parts.zw(f), nwu, sh, res, gamma, delta);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (res < tol) {
+        if (lead) {
+            st[f].res = res;
+            st[f].done = 1;
+        }
+        return;
+    }
+    const bool restart = !(gp < INFINITY) || gp == 0.0 || ap == 0.0;
+    const double beta = restart ? 0.0 : gamma / gp;
+    const double den = restart ? delta : delta - beta * gamma / ap;
+    const double alpha = den != 0.0 ? gamma / den : 0.0;
+    double lrz = 0, lmx = 0;
+    if (i < N) {
+        double u[3], rv[3], rj[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) {
+            const size_t e = a * 3 + k;
+            const double pv = restart ? zv[k] : zv[k] + beta * pv0[k];
+            const double sv = restart ? wv[k] : wv[k] + beta * sv0[k];
+            p[e] = pv;
+            sn[e] = sv;
+            x[e] += alpha * pv;
+            rv[k] = rv0[k] - alpha * sv;
+            rn[e] = rv[k];
+            if (poli > 0.001) lmx = fmax(lmx, fabs(rv[k]));
+        }
+        if (j >= 0) {
+            for (int k = 0; k < 3; ++k) {
+                const double sv = restart ? wj[k] : wj[k] + beta * sj[k];
+                rj[k] = rj0[k] - alpha * sv;
+            }
+        }
+        pc_apply(j, j >= 0 ? pcd + a * 8 : nullptr, inv_diag(poli), rv, rj, u);
         for (int k = 0; k < 3; ++k) {
             z[a * 3 + k] = u[k];
             lrz += rv[k] * u[k];
