@@ -76,16 +76,22 @@ def evaluate(eng, D, tol=1e-6):
 
 
 def time_path(eng, D, steps, warmup):
+    """Seconds per evaluation over `steps` timed evaluations (no instrumentation in
+    the timed region), then the per-family breakdown from as many evaluations with
+    the library's event timing on (one event pair per launch, which costs time)."""
     import torch
     for _ in range(warmup):
         out = evaluate(eng, D)
     torch.cuda.synchronize()
-    eng.profile(True, True)
     t0 = time.perf_counter()
     for _ in range(steps):
         out = evaluate(eng, D)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    eng.profile(True, True)
+    for _ in range(steps):
+        evaluate(eng, D)
+    torch.cuda.synchronize()
     prof = eng.profile_get()
     eng.profile(False, True)
     return (t1 - t0) / steps, out, prof

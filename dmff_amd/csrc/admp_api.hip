@@ -1547,16 +1547,27 @@ static admp_status solver_rhs(admp_ctx* ctx, const double* pol, const double* th
     R* site = reinterpret_cast<R*>(ctx->d_site);
     hipLaunchKernelGGL(k_pack_site<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_Qc, pol, thole,
                        (const double*)nullptr, site);
-    { ProfScope ps_(ctx, 1);
-    hipLaunchKernelGGL((k_real_permfield<R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, st, N,
+    // the real-space permanent field (and the stored coefficients and preconditioner
+    // blocks) runs on the side stream beside the reciprocal part: they share only
+    // inputs; k_rhs_finish joins them
+    const hipStream_t sr = c.lpme ? ctx->side : st;
+    if (c.lpme) {
+        HIPCHK(hipEventRecord(ctx->ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(sr, ctx->ev_fork, 0));
+    }
+    { ProfScope ps_(ctx, 1, sr);
+    hipLaunchKernelGGL((k_real_permfield<R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, sr, N,
                        ctx->cap, ctx->d_pos, ctx->d_box, site, ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_b,
                        reinterpret_cast<R*>(ctx->d_pcoef), ctx->d_pnear);
-    hipLaunchKernelGGL((k_pc_pairs<R>), dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box,
+    hipLaunchKernelGGL((k_pc_pairs<R>), dim3(nblk(N, 256), F), dim3(256), 0, sr, N, ctx->cap, ctx->d_pos, ctx->d_box,
                        site, pol, ctx->d_nbr, pp, ctx->d_pnear, ctx->d_pcj, ctx->d_pcd); }
     HIPCHK(hipGetLastError());
     if (c.lpme) {
+        HIPCHK(hipEventRecord(ctx->ev_join, sr));
         admp_status s = recip_run<RB>(ctx, ctx->eg, 0, ctx->d_Qc, nullptr, 1, lmax_eff, 0, DIELECTRIC, kappa, 1,
                                       ctx->d_t2, nullptr, nullptr, nullptr, false);
+        // join before any return, so that no error leaves the side stream unjoined
+        HIPCHK(hipStreamWaitEvent(st, ctx->ev_join, 0));
         if (s != ADMP_OK) return s;
         hipLaunchKernelGGL(k_rhs_finish, dim3(nblk(N, 256), F), dim3(256), 0, st, N, f1, ctx->d_Qc, ctx->d_t2,
                            ctx->d_b);
@@ -1713,23 +1724,32 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     }
     // ---------------- final pass: energy and gradients with U fixed ----------------
     hipLaunchKernelGGL(k_pack_site<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_Qc, pol, thole, U, site);
+    // the real-space pass runs on the side stream beside the reciprocal one (their
+    // outputs are separate buffers; k_self, which adds to both, comes after the join)
+    const hipStream_t sr = lpme ? ctx->side : st;
+    if (lpme) {
+        HIPCHK(hipEventRecord(ctx->ev_fork, st));
+        HIPCHK(hipStreamWaitEvent(sr, ctx->ev_fork, 0));
+    }
     if (lpol)
-        { ProfScope ps_(ctx, 0); hipLaunchKernelGGL((k_real_final<R, true>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box,
+        { ProfScope ps_(ctx, 0, sr); hipLaunchKernelGGL((k_real_final<R, true>), pgrid, dim3(PAIR_BLOCK), 0, sr, N, ctx->cap, ctx->d_pos, ctx->d_box,
                            site, ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_e_atom, ctx->d_gpos, ctx->d_gq, ctx->d_vir_atom,
                            ctx->d_gpol, ctx->d_gth); }
     else
-        { ProfScope ps_(ctx, 0); hipLaunchKernelGGL((k_real_final<R, false>), pgrid, dim3(PAIR_BLOCK), 0, st, N, ctx->cap, ctx->d_pos, ctx->d_box,
+        { ProfScope ps_(ctx, 0, sr); hipLaunchKernelGGL((k_real_final<R, false>), pgrid, dim3(PAIR_BLOCK), 0, sr, N, ctx->cap, ctx->d_pos, ctx->d_box,
                            site, ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_e_atom, ctx->d_gpos, ctx->d_gq, ctx->d_vir_atom,
                            ctx->d_gpol, ctx->d_gth); }
     HIPCHK(hipGetLastError());
     if (!lpol) {
-        HIPCHK(hipMemsetAsync(ctx->d_gpol, 0, NF * sizeof(double), st));
-        HIPCHK(hipMemsetAsync(ctx->d_gth, 0, NF * sizeof(double), st));
+        HIPCHK(hipMemsetAsync(ctx->d_gpol, 0, NF * sizeof(double), sr));
+        HIPCHK(hipMemsetAsync(ctx->d_gth, 0, NF * sizeof(double), sr));
     }
     if (lpme) {
-        if ((s = recip_run<R>(ctx, ctx->eg, 0, ctx->d_Qc, U, lpol ? 3 : 1, lmax_eff, 0, DIELECTRIC, kappa, 0, nullptr,
-                              ctx->d_gq_rec, ctx->d_gpos_rec, ctx->d_vir_rec, true)) != ADMP_OK)
-            return s;
+        HIPCHK(hipEventRecord(ctx->ev_join, sr));
+        s = recip_run<R>(ctx, ctx->eg, 0, ctx->d_Qc, U, lpol ? 3 : 1, lmax_eff, 0, DIELECTRIC, kappa, 0, nullptr,
+                         ctx->d_gq_rec, ctx->d_gpos_rec, ctx->d_vir_rec, true);
+        HIPCHK(hipStreamWaitEvent(st, ctx->ev_join, 0));
+        if (s != ADMP_OK) return s;
     } else {
         HIPCHK(hipMemsetAsync(ctx->d_gq_rec, 0, NF * 10 * sizeof(double), st));
         HIPCHK(hipMemsetAsync(ctx->d_gpos_rec, 0, NF * 3 * sizeof(double), st));
