@@ -27,7 +27,7 @@ SYMBOLS = [
     "admp_set_topology", "admp_set_pairs", "admp_build_neighbors", "admp_get_pairs", "admp_compute",
     "admp_disp_compute", "admp_slater_compute", "admp_profile", "admp_profile_get", "admp_memory_bytes",
     "admp_type_reduce", "admp_pairwise_compute", "admp_pairwise_compute_noalloc", "admp_local_to_global",
-    "admp_esp", "admp_add_forces_fixed",
+    "admp_esp", "admp_add_forces_fixed", "admp_set_stream_mode",
 ]
 
 # admp_sr_kind and each kind's per-site parameter count
@@ -81,6 +81,7 @@ def load():
     lib.admp_ctx_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(P)]
     lib.admp_ctx_destroy.argtypes = [P]
     lib.admp_set_stream.argtypes = [P, P]
+    lib.admp_set_stream_mode.argtypes = [P, i32]
     lib.admp_set_topology.argtypes = [P, P, P, i64, P, P, P]
     lib.admp_set_pairs.argtypes = [P, i32, P, i64, i32]
     lib.admp_build_neighbors.argtypes = [P, P, P]

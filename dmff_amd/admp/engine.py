@@ -62,7 +62,14 @@ class Engine:
             pass
 
     def _stream(self):
-        self.lib.admp_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        L.check(self.h, self.lib.admp_set_stream(
+            self.h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)), "admp_set_stream")
+
+    def set_stream_mode(self, caller_mode):
+        """caller_mode True: the context works directly on torch's current stream of
+        each call (no stream of its own), for many contexts driven from a few
+        worker streams; False (default): its own stream, ordered against the caller's."""
+        L.check(self.h, self.lib.admp_set_stream_mode(self.h, int(bool(caller_mode))), "admp_set_stream_mode")
 
     def memory_bytes(self):
         """Device bytes held by the native context."""

@@ -311,6 +311,10 @@ class ADMPServer:
             eng = Engine(N, spec.lmax, spec.lpol, True, ctx.kappa, ctx.K, spec.rc, n_frames=F,
                          precision=self.precision, device=self.device)
             eng.set_topology(spec.axis_types, spec.axis_idx, spec.exclusions)
+            # the context works on the worker's stream: with a stream per context,
+            # hundreds of contexts share the GPU's few hardware queues and wait on
+            # each other's unrelated work
+            eng.set_stream_mode(True)
             t = lambda x: torch.as_tensor(np.asarray(x), dtype=F64, device=self.device)
             ctx.dev = {"Q": t(spec.Q_local), "pol": t(spec.pol), "thole": t(spec.thole)}
             ctx.engine = eng

@@ -40,6 +40,8 @@ struct admp_ctx {
     admp_config cfg;
     hipStream_t stream = nullptr;       // the context's own stream (capturable)
     hipStream_t user_stream = nullptr;  // caller's stream: ordered with events on entry and exit
+    hipStream_t own = nullptr;          // the context's own stream (stream is own, or the caller's in caller mode)
+    int caller_mode = 0;                // admp_set_stream_mode: work directly on the caller's stream
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipStream_t side = nullptr;         // second context stream: real-space products beside the reciprocal chain
     PcgState* h_pcg = nullptr;          // pinned host copies of the PCG state (two slots of F)
@@ -356,14 +358,35 @@ static admp_status nbr_settle(admp_ctx* ctx);
 static const char* context_free_error();
 extern "C" const char* admp_last_error(const admp_ctx* ctx) { return ctx ? ctx->err.c_str() : context_free_error(); }
 
+// The active stream: the context's own one, or in caller mode the caller's
+// (when it is not the null stream, on which graphs cannot be captured).  ev_out
+// marks the end of the context's last work (recorded by every exit and at
+// creation), so a change of the active stream orders the new one after that
+// work only, not after unrelated work queued later on the old stream.
+static admp_status apply_stream(admp_ctx* ctx) {
+    hipStream_t want = (ctx->caller_mode && ctx->user_stream) ? ctx->user_stream : ctx->own;
+    if (want != ctx->stream) {
+        HIPCHK(hipStreamWaitEvent(want, ctx->ev_out, 0));
+        ctx->stream = want;
+    }
+    return ADMP_OK;
+}
+
 extern "C" admp_status admp_set_stream(admp_ctx* ctx, void* stream) {
     if (!ctx) return ADMP_ERR_ARG;
     ctx->user_stream = reinterpret_cast<hipStream_t>(stream);
-    return ADMP_OK;
+    return apply_stream(ctx);
+}
+
+extern "C" admp_status admp_set_stream_mode(admp_ctx* ctx, int32_t caller_mode) {
+    if (!ctx) return ADMP_ERR_ARG;
+    ctx->caller_mode = caller_mode != 0;
+    return apply_stream(ctx);
 }
 
 // work on the context stream starts after everything queued on the caller's stream
 static admp_status enter(admp_ctx* ctx) {
+    if (ctx->stream == ctx->user_stream) return ADMP_OK;
     HIPCHK(hipEventRecord(ctx->ev_in, ctx->user_stream));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_in, 0));
     return ADMP_OK;
@@ -372,6 +395,7 @@ static admp_status enter(admp_ctx* ctx) {
 // and the caller's stream continues after the context's work
 static admp_status leave(admp_ctx* ctx) {
     HIPCHK(hipEventRecord(ctx->ev_out, ctx->stream));
+    if (ctx->stream == ctx->user_stream) return ADMP_OK;
     HIPCHK(hipStreamWaitEvent(ctx->user_stream, ctx->ev_out, 0));
     return ADMP_OK;
 }
@@ -398,6 +422,12 @@ extern "C" admp_status admp_profile_get(admp_ctx* ctx, int32_t key, double* tota
 
 extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     if (!ctx) return ADMP_OK;
+    // back on the own stream, after the context's last work only (never a wait on
+    // the caller's stream, which may carry other work by now)
+    if (ctx->own && ctx->stream != ctx->own) {
+        if (ctx->ev_out) (void)hipStreamWaitEvent(ctx->own, ctx->ev_out, 0);
+        ctx->stream = ctx->own;
+    }
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& p : ctx->ev_pending) { ctx->ev_pool.push_back(p.second.first); ctx->ev_pool.push_back(p.second.second); }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -417,7 +447,7 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
     if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);   // never the caller's stream
     delete ctx;
     return ADMP_OK;
 }
@@ -451,7 +481,8 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     const size_t NF = (size_t)ctx->N * ctx->F;
     const bool mixed = c.precision == ADMP_MIXED;
     admp_status st;
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+    if (hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking) != hipSuccess ||
+        (ctx->stream = ctx->own, false) ||
         hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_state[0], hipEventDisableTiming) != hipSuccess ||
@@ -558,6 +589,11 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_catoms, NF);
     ALLOC(ctx->d_spos, NF);
 #undef ALLOC
+    // the zero fills of creation are the context's first work
+    if (hipEventRecord(ctx->ev_out, ctx->own) != hipSuccess) {
+        ctx->err = "event record failed";
+        return fail(ADMP_ERR_HIP);
+    }
     *out = ctx;
     return ADMP_OK;
 }

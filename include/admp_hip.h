@@ -118,6 +118,17 @@ const char* admp_last_error(const admp_ctx* ctx);
 /* stream: a hipStream_t (NULL = the legacy default stream). */
 admp_status admp_set_stream(admp_ctx* ctx, void* stream);
 
+/* No counterpart in the reference (a serving and MD option).  caller_mode 1:
+ * the context works directly on the stream last given to admp_set_stream
+ * (when it is not the null stream) instead of on its own stream ordered by
+ * events against it.  For callers that drive many contexts from a few worker
+ * streams: HIP maps streams onto a few hardware queues, so hundreds of
+ * contexts with streams of their own share queues and wait on each other's
+ * unrelated work.  A change of stream is ordered after the context's last
+ * work (an event recorded by every entry point); captured solver graphs are
+ * replayed on whichever stream is active.  Default 0. */
+admp_status admp_set_stream_mode(admp_ctx* ctx, int32_t caller_mode);
+
 /* Local frames (axis_type (N,), axis_idx (N,3) [z,x,y], -1 = none; spatial.py:44-156)
  * and the covalent exclusions (i, j, nbond) for nbond in 1..6
  * (dmff/api/topology.py:391-418, each unordered pair once).  All (host). */
