@@ -252,6 +252,13 @@ static std::vector<int> factorize(int K) {
     for (int p = 3; p * p <= n; p += 2)
         while (n % p == 0) { f.push_back(p); n /= p; }
     if (n > 1) f.push_back(n);
+    // a 2 and a 5 become one radix-10 stage (one LDS pass and barrier fewer)
+    for (;;) {
+        auto i2 = std::find(f.begin(), f.end(), 2), i5 = std::find(f.begin(), f.end(), 5);
+        if (i2 == f.end() || i5 == f.end()) break;
+        *i5 = 10;
+        f.erase(i2);
+    }
     return f;
 }
 

@@ -589,6 +589,44 @@ __device__ __forceinline__ void fft_bfly(const Cplx<R>* in, Cplx<R>* out, int es
         O(ob + 4 * Ns, csub(a1, b1));
         O(ob + 2 * Ns, cadd(a2, b2));
         O(ob + 3 * Ns, csub(a2, b2));
+    } else if (r == 10) {
+        // 10 points as two 5-point transforms of the even and the odd inputs
+        // (Cooley-Tukey with q = 2a + b): E_k, O_k for k < 5, then
+        // y_k = E_k + W10^k O_k and y_{k+5} = E_k - W10^k O_k, W10 = tw[K / 10]
+        const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+        const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
+        Cplx<R> x[10];
+        x[0] = I(b);
+#pragma unroll
+        for (int q = 1; q < 10; ++q) x[q] = cmul(I(b + q * nb), twq(q));
+        auto dft5 = [&](const Cplx<R>& a0, const Cplx<R>& a1, const Cplx<R>& a2, const Cplx<R>& a3,
+                        const Cplx<R>& a4, Cplx<R>* y) {
+            const Cplx<R> t1c = cadd(a1, a4), t2 = cadd(a2, a3), t3 = csub(a1, a4), t4 = csub(a2, a3);
+            const Cplx<R> u1 = Cplx<R>{a0.x + c1 * t1c.x + c2 * t2.x, a0.y + c1 * t1c.y + c2 * t2.y};
+            const Cplx<R> u2 = Cplx<R>{a0.x + c2 * t1c.x + c1 * t2.x, a0.y + c2 * t1c.y + c1 * t2.y};
+            const Cplx<R> v1 = mul_mi(Cplx<R>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, sg);
+            const Cplx<R> v2 = mul_mi(Cplx<R>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, sg);
+            y[0] = Cplx<R>{a0.x + t1c.x + t2.x, a0.y + t1c.y + t2.y};
+            y[1] = cadd(u1, v1);
+            y[4] = csub(u1, v1);
+            y[2] = cadd(u2, v2);
+            y[3] = csub(u2, v2);
+        };
+        Cplx<R> E[5], Od[5];
+        dft5(x[0], x[2], x[4], x[6], x[8], E);
+        dft5(x[1], x[3], x[5], x[7], x[9], Od);
+        const int k10 = K / 10;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Cplx<R> t = Od[k];
+            if (k > 0) {
+                Cplx<R> w = tw[k * k10];
+                w.y *= sg;
+                t = cmul(t, w);
+            }
+            O(ob + k * Ns, cadd(E[k], t));
+            O(ob + (k + 5) * Ns, csub(E[k], t));
+        }
     } else {
         // direct r-point sum per output: y_k = sum_q x_q W_{Ns r}^{q kj} W_r^{q k}
         const int kr = K / r;  // W_r = tw[K / r]

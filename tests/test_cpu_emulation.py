@@ -138,10 +138,12 @@ def test_neighbour_lists_bit_exact(emu_lib):
 
 
 @pytest.mark.parametrize("K,prec", [((104, 104, 104), "fp64"), ((104, 104, 104), "mixed"), ((6, 7, 8), "fp64"),
-                                    ((20, 24, 15), "fp64"), ((32, 27, 25), "mixed")])
+                                    ((20, 24, 15), "fp64"), ((32, 27, 25), "mixed"), ((50, 30, 40), "fp64"),
+                                    ((10, 70, 50), "mixed")])
 def test_grid_paths(emu_lib, K, prec):
     """104^3 in FP64 takes the line-major FFT passes (its plane exceeds the LDS
-    limit); in mixed precision the fused plane pass."""
+    limit); in mixed precision the fused plane pass.  50, 30, 40, 10 and 70 have
+    radix-10 stages (with 5, 3, 4 and 7 beside them)."""
     s, pos, box = small_water_box()
     pairs = reference_pairs(pos, box, 6.0, s.bonded)
     kappa = O.setup_ewald_parameters(6.0, 5e-4, box)[0]

@@ -442,7 +442,8 @@ def test_mixed_batched_frames_converge(fp32_products):
         assert np.linalg.norm(Gm[k] - Gd[k]) / np.linalg.norm(Gd[k]) < 1e-4
 
 
-@pytest.mark.parametrize("K", [(16, 16, 16), (20, 24, 15), (32, 27, 25), (21, 22, 26), (6, 7, 8), (104, 104, 104)])
+@pytest.mark.parametrize("K", [(16, 16, 16), (20, 24, 15), (32, 27, 25), (21, 22, 26), (6, 7, 8), (104, 104, 104),
+                               (50, 30, 40), (10, 70, 50)])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_grid_sizes_vs_oracle(K, prec):
     """Energies and forces on grids whose sizes exercise every FFT radix path
