@@ -451,11 +451,24 @@ k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R*
         }
         __syncthreads();
         if (c0 + SP_CHUNK < total) fetch(c0 + SP_CHUNK);
+        // the next candidate's packed offsets are read one step ahead, and the z
+        // table is read at a clamped index for all four points at once (a point
+        // outside the support takes a zero weight), so a candidate waits for LDS
+        // twice instead of six times
+        int en = g < nchunk ? s_r[g] : 0;
         for (int s2 = g; s2 < nchunk; s2 += 4) {
-            const int e = s_r[s2];
+            const int e = en;
+            if (s2 + 4 < nchunk) en = s_r[s2 + 4];
             const int k0 = lx + 5 - (e & 15), k1 = ly + 5 - ((e >> 4) & 15);
             if ((unsigned)k0 > 5u || (unsigned)k1 > 5u) continue;
             const int kz = 5 - ((e >> 8) & 15);   // z table index of the brick's first point
+            R tz[4][NT];
+#pragma unroll
+            for (int lz = 0; lz < 4; ++lz) {
+                const int kc = min(max(kz + lz, 0), 5);
+#pragma unroll
+                for (int j = 0; j < NT; ++j) tz[lz][j] = s_t[s2][2][j][kc];
+            }
             const R vx = s_t[s2][0][0][k0], vy = s_t[s2][1][0][k1];
             R C0 = s_q[s2][0] * vx * vy, C1 = 0, C2 = 0;
             if (LM >= 1) {
@@ -472,12 +485,10 @@ k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R*
             }
 #pragma unroll
             for (int lz = 0; lz < 4; ++lz) {
-                const int k2 = kz + lz;
-                if ((unsigned)k2 > 5u) continue;
-                R w = C0 * s_t[s2][2][0][k2];
-                if (LM >= 1) w += C1 * s_t[s2][2][NT > 1 ? 1 : 0][k2];
-                if (LM >= 2) w += C2 * s_t[s2][2][NT > 2 ? 2 : 0][k2];
-                acc[lz] += w;
+                R w = C0 * tz[lz][0];
+                if (LM >= 1) w += C1 * tz[lz][NT > 1 ? 1 : 0];
+                if (LM >= 2) w += C2 * tz[lz][NT > 2 ? 2 : 0];
+                if ((unsigned)(kz + lz) <= 5u) acc[lz] += w;
             }
         }
         __syncthreads();
