@@ -1115,6 +1115,18 @@ k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, cons
     const double* A = use_dA ? b.dA : b.A;
     const int4 m = mbase[a];
     const R* gbase = rgrid + (size_t)f * K0 * K1 * K2;
+    // the CG epilogue's inputs do not depend on the grid: fetched before it
+    double e_pol = 0.0, e_real[3] = {0, 0, 0}, e_z[3] = {0, 0, 0};
+    if constexpr (MODE == 2) {
+        if (active && lane == 0) {
+            e_pol = epi.pol[ii];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                e_real[c] = epi.real[a * 3 + c];
+                e_z[c] = U[a * 3 + c];
+            }
+        }
+    }
     // fp[c][aa][bb]: d^(c,aa,bb) of the fractional potential, c + aa + bb < ND
     double fp[ND][ND][ND];
 #pragma unroll
@@ -1187,13 +1199,13 @@ k_gather_ml(int N, int K0, int K1, int K2, const double4* __restrict__ pos, cons
         double zw = 0;
         if (active && lane == 0) {
             const double q1[3] = {fp[1][0][0], fp[0][1][0], fp[0][0][1]};
-            const double ap = epi.pol[ii] < 1e-8 ? 1e-8 : epi.pol[ii];
+            const double ap = e_pol < 1e-8 ? 1e-8 : e_pol;
             const double d = DIELECTRIC / ap - 2.0 * DIELECTRIC * epi.f1;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const double dm = A[c * 3 + 0] * q1[0] + A[c * 3 + 1] * q1[1] + A[c * 3 + 2] * q1[2];
-                const double zv = U[a * 3 + c];
-                const double wv = epi.real[a * 3 + c] + dm + d * zv;
+                const double zv = e_z[c];
+                const double wv = e_real[c] + dm + d * zv;
                 out3[a * 3 + c] = wv;
                 zw += zv * wv;
             }
