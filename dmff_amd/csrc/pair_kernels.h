@@ -386,7 +386,10 @@ k_real_final(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __r
 // Solver set-up: real-space part of -dE/dU at U = 0 (the permanent field felt
 // by each induced dipole, p-scaled and Thole damped), and the two coefficients
 // of the induced-induced coupling of each directed pair, stored for the matvec:
-//   dE_dd/dU_I = sum_J c1 U_J + c2 (z.U_J) z,  c1 = udud_m1, c2 = udud_m0 - udud_m1.
+//   dE_dd/dU_I = sum_J c1 U_J + c2 (z.U_J) z,  c1 = udud_m1, c2 = udud_m0 - udud_m1,
+// with z = d / r for the minimum-image separation d; the second coefficient is
+// stored as c2 / r^2, so that the product forms c2 (z.U) z = (c2 / r^2)(d.U) d
+// without a square root or a division.
 // ---------------------------------------------------------------------------
 template <class R>
 __global__ void __launch_bounds__(PAIR_BLOCK)
@@ -457,7 +460,7 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
         const R gzI = s * zz + dud1 * mJz_ - k1 * udq1 * tJz;
         bx_ -= (double)gxI; by_ -= (double)gyI; bz_ -= (double)gzI;
         pc[2 * k] = ud1;
-        pc[2 * k + 1] = ud0 - ud1;
+        pc[2 * k + 1] = (ud0 - ud1) * rinv * rinv;
     }
     bx_ = group_sum(bx_, TPA); by_ = group_sum(by_, TPA); bz_ = group_sum(bz_, TPA);
     // smallest (distance, atom index) over the lanes of the atom
@@ -535,7 +538,9 @@ k_pc_pairs(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __res
     pcj[a] = j;
 }
 
-// out_I = sum_J c1 V_J + c2 (z.V_J) z   (induced-induced real-space coupling)
+// out_I = sum_J c1 V_J + c2 (z.V_J) z   (induced-induced real-space coupling),
+// formed as c1 V_J + (c2 / r^2)(d.V_J) d from the stored (c1, c2 / r^2) and the
+// minimum-image separation d (no square root or division per pair).
 // RC: stored coefficient type, R: arithmetic type of the products (double for the
 // FP64 residual of the mixed-precision solve)
 // One-dimensional grid over all (frame, atom block) items, renumbered so that
@@ -543,6 +548,50 @@ k_pc_pairs(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __res
 // holds the positions and v of few frames).  nbf: atom blocks per frame.
 // item: (frame, atom block of BS threads) pair, numbered frame-major; nbf atom
 // blocks per frame
+template <bool ORTHO, class RC, class R>
+__device__ __forceinline__ void real_matvec_pairs(int sub, int cnt, const BoxInfo& bx, const double4& pi_,
+                                                  const double4* __restrict__ posf, const R* __restrict__ vf,
+                                                  const int* __restrict__ nl, const RC* __restrict__ pc, double& ox,
+                                                  double& oy, double& oz) {
+    // two neighbours per trip, loads issued before either is used; the second of
+    // a trip past the end gets zero coefficients (adds exactly 0), so the sum
+    // runs in the same order as one neighbour per trip.  posf and vf are the
+    // frame's bases, so the per-neighbour offsets are 32-bit.
+    auto term = [&](int j, RC c1r, RC c2r) {
+        const double4 pj = posf[j];
+        double dx = pi_.x - pj.x, dy = pi_.y - pj.y, dz = pi_.z - pj.z;
+        if constexpr (ORTHO) {
+            // the general minimum image of pbc_shift with zero off-diagonal
+            // terms (which add exact zeros there): the same value
+            const double t0 = dx * bx.Binv[0], t1 = dy * bx.Binv[4], t2 = dz * bx.Binv[8];
+            dx = (t0 - floor(t0 + 0.5)) * bx.B[0];
+            dy = (t1 - floor(t1 + 0.5)) * bx.B[4];
+            dz = (t2 - floor(t2 + 0.5)) * bx.B[8];
+        } else {
+            double n0, n1, n2;
+            pbc_shift(bx, dx, dy, dz, n0, n1, n2);
+        }
+        const R ex = (R)dx, ey = (R)dy, ez = (R)dz;
+        const R* vj = vf + 3 * j;
+        const R vx = vj[0], vy = vj[1], vz = vj[2];
+        const R c1 = (R)c1r, c2 = (R)c2r;
+        const R p = c2 * (ex * vx + ey * vy + ez * vz);
+        ox += (double)(c1 * vx + p * ex);
+        oy += (double)(c1 * vy + p * ey);
+        oz += (double)(c1 * vz + p * ez);
+    };
+    for (int k = sub; k < cnt; k += 2 * TPA) {
+        const int k2 = k + TPA;
+        const bool h2 = k2 < cnt;
+        const int j1 = nl[k] & NB_MASK;
+        const int j2 = h2 ? (nl[k2] & NB_MASK) : j1;
+        const RC a1 = pc[2 * k], b1 = pc[2 * k + 1];
+        const RC a2 = h2 ? pc[2 * k2] : (RC)0, b2 = h2 ? pc[2 * k2 + 1] : (RC)0;
+        term(j1, a1, b1);
+        term(j2, a2, b2);
+    }
+}
+
 template <class RC, class R, int BS>
 __device__ __forceinline__ void real_matvec_body(int item, int N, int nbf, int cap, const double4* __restrict__ pos,
                                                  const BoxInfo* __restrict__ boxes, const int* __restrict__ nbr,
@@ -557,38 +606,17 @@ __device__ __forceinline__ void real_matvec_body(int item, int N, int nbf, int c
     const int ii = active ? i : 0;
     const BoxInfo& bx = boxes[f];
     const size_t fo = (size_t)f * N;
-    const double4 pi_ = pos[fo + ii];
+    const double4* posf = pos + fo;
+    const R* vf = v + fo * 3;
+    const double4 pi_ = posf[ii];
     double ox = 0, oy = 0, oz = 0;
     const int cnt = active ? ncnt[fo + ii] : 0;
     const int* nl = nbr + (fo + ii) * (size_t)cap;
     const RC* pc = pcoef + (fo + ii) * (size_t)cap * 2;
-    // two neighbours per trip, loads issued before either is used; the second of
-    // a trip past the end gets zero coefficients (adds exactly 0), so the sum
-    // runs in the same order as one neighbour per trip
-    auto term = [&](int j, RC c1r, RC c2r) {
-        const double4 pj = pos[fo + j];
-        double dx = pi_.x - pj.x, dy = pi_.y - pj.y, dz = pi_.z - pj.z, n0, n1, n2;
-        pbc_shift(bx, dx, dy, dz, n0, n1, n2);
-        const double rinv = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-        const R zx = (R)(dx * rinv), zy = (R)(dy * rinv), zz = (R)(dz * rinv);
-        const R* vj = v + (fo + j) * 3;
-        const R vx = vj[0], vy = vj[1], vz = vj[2];
-        const R c1 = (R)c1r, c2 = (R)c2r;
-        const R p = c2 * (zx * vx + zy * vy + zz * vz);
-        ox += (double)(c1 * vx + p * zx);
-        oy += (double)(c1 * vy + p * zy);
-        oz += (double)(c1 * vz + p * zz);
-    };
-    for (int k = sub; k < cnt; k += 2 * TPA) {
-        const int k2 = k + TPA;
-        const bool h2 = k2 < cnt;
-        const int j1 = nl[k] & NB_MASK;
-        const int j2 = h2 ? (nl[k2] & NB_MASK) : j1;
-        const RC a1 = pc[2 * k], b1 = pc[2 * k + 1];
-        const RC a2 = h2 ? pc[2 * k2] : (RC)0, b2 = h2 ? pc[2 * k2 + 1] : (RC)0;
-        term(j1, a1, b1);
-        term(j2, a2, b2);
-    }
+    if (bx.ortho)
+        real_matvec_pairs<true>(sub, cnt, bx, pi_, posf, vf, nl, pc, ox, oy, oz);
+    else
+        real_matvec_pairs<false>(sub, cnt, bx, pi_, posf, vf, nl, pc, ox, oy, oz);
     ox = group_sum(ox, TPA); oy = group_sum(oy, TPA); oz = group_sum(oz, TPA);
     if (active && sub == 0) {
         const size_t a = fo + i;
