@@ -19,7 +19,7 @@ HOST_MEMORY = False
 ADMP_OK = 0
 ADMP_WARN_NOT_CONVERGED = 5
 ADMP_FP64, ADMP_MIXED = 0, 1
-ADMP_PRECOND_PAIRS, ADMP_PRECOND_DIAGONAL = 0, 1
+ADMP_PRECOND_CLUSTERS, ADMP_PRECOND_DIAGONAL, ADMP_PRECOND_PAIRS = 0, 1, 2
 SOLVER_REFERENCE, SOLVER_STEPS, SOLVER_PCG = 0, 1, 2
 
 SYMBOLS = [

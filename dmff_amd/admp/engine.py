@@ -141,10 +141,12 @@ class Engine:
 
     def compute(self, pos, box, Q_local, pol=None, thole=None, mScales=None, pScales=None, dScales=None,
                 U_init=None, solver="pcg", tol=1e-6, maxiter=500, steps=0, want_grad=True, fp32_products=False,
-                precond="pairs"):
+                precond="clusters"):
         """Energies (F,) and, with want_grad, dE/d{pos, box, Q_local, pol, thole}.
-        U_init / returned U are global Cartesian e*A (F,N,3).  precond (PCG): "pairs"
-        (6x6 blocks of mutually nearest coupled sites within 2 A) or "diagonal"."""
+        U_init / returned U are global Cartesian e*A (F,N,3).  precond (PCG):
+        "clusters" (exact blocks of up to 4 sites: a pair of mutually nearest coupled
+        sites within 2 A and the sites whose nearest coupled neighbour lies in it),
+        "pairs" (the pairs alone) or "diagonal"."""
         F, N = self.n_frames, self.n_atoms
         dev = self.device
         pos = _dev(pos, dev).reshape(F, N, 3)
@@ -168,7 +170,8 @@ class Engine:
         opts.kind = {"reference": L.SOLVER_REFERENCE, "steps": L.SOLVER_STEPS, "pcg": L.SOLVER_PCG}[solver]
         opts.maxiter, opts.steps, opts.tol, opts.warm_start = int(maxiter), int(steps), float(tol), warm
         opts.fp32_products = int(bool(fp32_products))
-        opts.precond = {"pairs": L.ADMP_PRECOND_PAIRS, "diagonal": L.ADMP_PRECOND_DIAGONAL}[precond]
+        opts.precond = {"clusters": L.ADMP_PRECOND_CLUSTERS, "pairs": L.ADMP_PRECOND_PAIRS,
+                        "diagonal": L.ADMP_PRECOND_DIAGONAL}[precond]
         E = torch.empty(F, dtype=F64, device=dev)
         g = {}
         if want_grad:

@@ -111,9 +111,11 @@ struct admp_ctx {
     PcgState* d_pcg = nullptr;
     double* d_s = nullptr;      // CG: s = A p by recurrence (r and s: two buffers each)
     int* d_pnear = nullptr;     // nearest coupled neighbour slot per site (-1: none)
-    int* d_pcj = nullptr;       // pair-block preconditioner partner (-1: diagonal)
-    double* d_pcd = nullptr;    // its coefficients, 8 per site
-    bool pc_pairs = true;       // pair blocks on (admp_solver_opts.precond)
+    int* d_pcj = nullptr;       // block preconditioner partners, 3 per site (-1: unused)
+    double* d_pcd = nullptr;    // the site's rows of its block inverse, 3 x 12 per site
+    int* d_pcn = nullptr;       // children per pair site (k_pc_children)
+    int* d_pcc = nullptr;       // the children, PC_MAXCH per site
+    int pc_maxm = PC_MAXM;      // sites per preconditioner block (1: diagonal; admp_solver_opts.precond)
     CgCheck spread_chk{};       // convergence test of the next solver product (set by pcg_apply)
     int* d_active = nullptr;
     int dots_blocks = 64;
@@ -579,7 +581,8 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ctx->cg_nw = std::max(ctx->pcg_nb, (ctx->N + 31) / 32);
     ALLOC(ctx->d_pparts, (size_t)ctx->F * (ctx->cg_nw + 4 * ctx->pcg_nb));
     ALLOC(ctx->d_s, NF * 6);
-    ALLOC(ctx->d_pnear, NF); ALLOC(ctx->d_pcj, NF); ALLOC(ctx->d_pcd, NF * 8);
+    ALLOC(ctx->d_pnear, NF); ALLOC(ctx->d_pcj, NF * 3); ALLOC(ctx->d_pcd, NF * 3 * PC_ROW);
+    ALLOC(ctx->d_pcn, NF); ALLOC(ctx->d_pcc, NF * PC_MAXCH);
     ALLOC(ctx->d_swfrac, NF * 3);
     ALLOC(ctx->d_sfq, NF * 10);
     ALLOC(ctx->d_res, NF * 3);
@@ -1328,8 +1331,8 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1, int cu
     return ADMP_OK;
 }
 
-// partners of the pair-block preconditioner, or null for the diagonal one
-static const int* pc_j(const admp_ctx* ctx) { return ctx->pc_pairs ? ctx->d_pcj : nullptr; }
+// partners of the block preconditioner, or null for the diagonal one
+static const int* pc_j(const admp_ctx* ctx) { return ctx->pc_maxm > 1 ? ctx->d_pcj : nullptr; }
 
 // k_cg_update of parity cur with products of type R (z copy for the pair kernel
 // and fractional dipoles typed R)
@@ -1433,7 +1436,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         admp_ctx::ChunkGraph* cg = nullptr;
         for (auto& g : ctx->graphs)
             if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1 &&
-                g.pc == ctx->pc_pairs)
+                g.pc == (ctx->pc_maxm > 1))
                 cg = &g;
         if (!cg) {
             admp_ctx::ChunkGraph* slot = nullptr;
@@ -1465,7 +1468,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
                 return iterate();
             }
             slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
-            slot->tol = tol; slot->f1 = f1; slot->pc = ctx->pc_pairs;
+            slot->tol = tol; slot->f1 = f1; slot->pc = ctx->pc_maxm > 1;
             cg = slot;
         }
         ProfScope ps_(ctx, 12);
@@ -1601,9 +1604,14 @@ static admp_status solver_rhs(admp_ctx* ctx, const double* pol, const double* th
     { ProfScope ps_(ctx, 1, sr);
     hipLaunchKernelGGL((k_real_permfield<R>), dim3(nblk((size_t)N * TPA, PAIR_BLOCK), F), dim3(PAIR_BLOCK), 0, sr, N,
                        ctx->cap, ctx->d_pos, ctx->d_box, site, ctx->d_nbr, ctx->d_ncnt, pp, ctx->d_b,
-                       reinterpret_cast<R*>(ctx->d_pcoef), ctx->d_pnear);
-    hipLaunchKernelGGL((k_pc_pairs<R>), dim3(nblk(N, 256), F), dim3(256), 0, sr, N, ctx->cap, ctx->d_pos, ctx->d_box,
-                       site, pol, ctx->d_nbr, pp, ctx->d_pnear, ctx->d_pcj, ctx->d_pcd); }
+                       reinterpret_cast<R*>(ctx->d_pcoef), ctx->d_pnear, ctx->d_pcn);
+    if (ctx->pc_maxm > 1) {
+        hipLaunchKernelGGL(k_pc_children, dim3(nblk(N, 256), F), dim3(256), 0, sr, N, ctx->cap, ctx->d_nbr,
+                           ctx->d_pnear, ctx->d_pcn, ctx->d_pcc);
+        hipLaunchKernelGGL((k_pc_clusters<R>), dim3(nblk(N, PC_BLOCK), F), dim3(PC_BLOCK), 0, sr, N, ctx->cap,
+                           ctx->pc_maxm, ctx->d_pos, ctx->d_box, site, pol, ctx->d_nbr, ctx->d_pnear, ctx->d_pcn,
+                           ctx->d_pcc, pp, ctx->d_pcj, ctx->d_pcd);
+    } }
     HIPCHK(hipGetLastError());
     if (c.lpme) {
         HIPCHK(hipEventRecord(ctx->ev_join, sr));
@@ -1634,7 +1642,10 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     const double tol = o ? o->tol : 1e-6;
     const int maxiter = o ? (kind == ADMP_SOLVER_STEPS ? o->steps : o->maxiter) : 500;
     const bool mixed = sizeof(R) == 4;
-    ctx->pc_pairs = !(o && o->precond == ADMP_PRECOND_DIAGONAL);
+    {
+        const int pk = o ? o->precond : ADMP_PRECOND_CLUSTERS;
+        ctx->pc_maxm = pk == ADMP_PRECOND_DIAGONAL ? 1 : pk == ADMP_PRECOND_PAIRS ? 2 : PC_MAXM;
+    }
     const bool warm = o && o->warm_start && U_in;
     if (warm)
         HIPCHK(hipMemcpyAsync(U, U_in, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -2209,6 +2220,7 @@ constexpr int NA_MAXBLK = 1024;
 constexpr int NA_BOX_DOUBLES = 64;                         // BoxInfo + the flag word
 constexpr int NA_SCRATCH_DOUBLES = NA_BOX_DOUBLES + NA_MAXBLK * 10;  // 82,432 bytes
 static_assert(sizeof(BoxInfo) + sizeof(int) <= NA_BOX_DOUBLES * sizeof(double), "box record");
+static_assert(PC_ROW == 3 * PC_MAXM, "preconditioner rows hold one 3x3 block per cluster site");
 constexpr int NA_MAX_DEVICES = 64;
 
 }  // namespace

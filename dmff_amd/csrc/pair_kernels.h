@@ -23,7 +23,7 @@ namespace admp {
 
 constexpr int TPA = 16;           // lanes per atom
 constexpr int PAIR_BLOCK = 256;   // 16 atoms per block
-constexpr double PC_RPAIR = 2.0;  // A: pair-block preconditioner partner range
+constexpr double PC_RPAIR = 2.0;  // A: block preconditioner neighbour range
 
 // site record (R precision), 16 words per atom
 // 0 q | 1-3 mu | 4-9 Theta xx yy zz xy xz yz | 10 alpha^(1/6) | 11 thole | 12 alpha | 13-15 U
@@ -395,7 +395,8 @@ template <class R>
 __global__ void __launch_bounds__(PAIR_BLOCK)
 k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
                  const R* __restrict__ site, const int* __restrict__ nbr, const int* __restrict__ ncnt,
-                 PairParams pp, double* __restrict__ field, R* __restrict__ pcoef, int* __restrict__ pnear) {
+                 PairParams pp, double* __restrict__ field, R* __restrict__ pcoef, int* __restrict__ pnear,
+                 int* __restrict__ nchild) {
     const int f = blockIdx.y;
     const int gt = blockIdx.x * PAIR_BLOCK + threadIdx.x;
     const int i = gt / TPA, sub = gt % TPA;
@@ -412,7 +413,7 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
     const int* nl = nbr + (fo + ii) * (size_t)cap;
     R* pc = pcoef + (fo + ii) * (size_t)cap * 2;
     // nearest neighbour with induced-induced coupling (d-scale > 0) inside
-    // PC_RPAIR: candidate partner of the pair-block preconditioner
+    // PC_RPAIR: the nearest-neighbour link of the block preconditioner
     double nr = PC_RPAIR;
     int nj = -1, nk = -1;
     for (int k = sub; k < cnt; k += TPA) {
@@ -473,69 +474,218 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
         const size_t a = fo + i;
         field[a * 3 + 0] = bx_; field[a * 3 + 1] = by_; field[a * 3 + 2] = bz_;
         if (pnear) pnear[a] = nk;
+        if (nchild) nchild[a] = 0;   // counters of k_pc_children, which runs next
     }
 }
 
 // ---------------------------------------------------------------------------
-// Pair-block preconditioner (set-up once per solve; positions are fixed).
-// Two sites that are each other's nearest coupled neighbour inside PC_RPAIR
-// form a pair; its 6x6 block of the polarisation matrix is approximated by
-//   [[d_i I, B], [B, d_j I]],  d = DIELECTRIC / alpha,
-//   B = c1 I + c2 z z^T  (the Thole-damped dipole-dipole tensor without Ewald
-//   screening, d-scaled, with the Thole width of the induced-induced product),
-// and its inverse is exact in closed form: along z (b = c1 + c2) and across it
-// (b = c1) the block is [[d_i, b], [b, d_j]] with inverse
-// [[d_j, -b], [-b, d_i]] / (d_i d_j - b^2).  Row i keeps
-//   z, a_par = d_j / D_par, b_par = -b_par / D_par, a_perp, b_perp
-// so that (M^-1 r)_i = a_perp r_i + b_perp r_j
-//                      + ((a_par - a_perp) z.r_i + (b_par - b_perp) z.r_j) z.
-// Sites without a partner (or with a nearly singular block) keep the diagonal
-// preconditioner 1/d_i.  The blocks of close contacts may be indefinite; they
-// carry the local modes that slow a diagonal-preconditioned solve most.
+// Block preconditioner (set-up once per solve; positions are fixed).
+// Every site has a nearest coupled neighbour n(i) inside PC_RPAIR (or none).
+// Two sites with n(i) = j and n(j) = i form a mutual pair; a site c whose
+// n(c) = p lies in a mutual pair (p, q), c outside it, is a child of p.  A
+// cluster is a mutual pair plus up to maxm - 2 children of its two sites, the
+// nearest to their parent first (site index on ties); a site whose parent has
+// more than PC_MAXCH children takes no child of it (the recorded ones depend
+// on atomic order).  In water the clusters are an oxygen with the hydrogens
+// hydrogen-bonded to it inside PC_RPAIR.  The cluster's block of the
+// polarisation matrix is approximated by
+//   diagonal blocks d_i I, d = DIELECTRIC / alpha,
+//   for the pair and each child-parent link: c1 I + c2 z z^T  (the
+//   Thole-damped dipole-dipole tensor without Ewald screening, d-scaled, with
+//   the Thole width of the induced-induced product and the bonded distance of
+//   the link's neighbour-list entry); the other couplings inside a cluster are
+//   left out (numpy on the dense matrix: the same iteration count),
+// and inverted exactly (Gaussian elimination in LDS).  Each member solves the
+// same system for its own three rows, so every member holds the same inverse.
+// A pivot below 1e-3 of its row's d (nearly singular block) makes the cluster
+// fall back to its mutual pair, and that pair to the diagonal 1/d.  The blocks
+// of close contacts may be indefinite; they carry the local modes that slow a
+// diagonal-preconditioned solve most.  Output per site: the partner sites
+// pcj[3] (-1: unused; pcj[0] = -1: diagonal) and its three rows of the inverse,
+// pcd[3][12] (column blocks: the site itself, then the partners in pcj order).
 // ---------------------------------------------------------------------------
-template <class R>
+constexpr int PC_MAXM = 4;    // sites per cluster
+constexpr int PC_MAXCH = 4;   // children recorded per pair site
+constexpr int PC_BLOCK = 32;  // threads per block of k_pc_clusters (one 12 x 15 system each in LDS)
+
+// n(i): the nearest coupled neighbour of site i inside PC_RPAIR, -1 if none
+__device__ __forceinline__ int pc_nn(const int* __restrict__ nbr, const int* __restrict__ pnear, size_t fo,
+                                     int cap, int i) {
+    const int k = pnear[fo + i];
+    return k < 0 ? -1 : (nbr[(fo + i) * (size_t)cap + k] & NB_MASK);
+}
+
+// records each child c of a mutual-pair site p in child[p][slot]; nchild
+// (cleared by k_real_permfield) counts them, also past PC_MAXCH
 __global__ void __launch_bounds__(256)
-k_pc_pairs(int N, int cap, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
-           const R* __restrict__ site, const double* __restrict__ pol, const int* __restrict__ nbr,
-           PairParams pp, const int* __restrict__ pnear, int* __restrict__ pcj, double* __restrict__ pcd) {
-    const int f = blockIdx.y;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    const size_t fo = (size_t)f * N, a = fo + i;
-    pcj[a] = -1;
-    const int k = pnear[a];
-    if (k < 0) return;
-    const int w = nbr[a * (size_t)cap + k];
-    const int j = w & NB_MASK, nb = w >> NB_SHIFT;
-    const int kj = pnear[fo + j];
-    if (kj < 0 || (nbr[(fo + j) * (size_t)cap + kj] & NB_MASK) != i) return;   // not mutual
+k_pc_children(int N, int cap, const int* __restrict__ nbr, const int* __restrict__ pnear, int* __restrict__ nchild,
+              int* __restrict__ child) {
+    const int f = blockIdx.y, c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const size_t fo = (size_t)f * N;
+    const int p = pc_nn(nbr, pnear, fo, cap, c);
+    if (p < 0) return;
+    const int q = pc_nn(nbr, pnear, fo, cap, p);
+    if (q < 0 || q == c || pc_nn(nbr, pnear, fo, cap, q) != p) return;
+    const int slot = atomicAdd(nchild + fo + p, 1);
+    if (slot < PC_MAXCH) child[(fo + p) * PC_MAXCH + slot] = c;
+}
+
+template <class R>
+__global__ void __launch_bounds__(PC_BLOCK)
+k_pc_clusters(int N, int cap, int maxm, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
+              const R* __restrict__ site, const double* __restrict__ pol, const int* __restrict__ nbr,
+              const int* __restrict__ pnear, const int* __restrict__ nchild, const int* __restrict__ child,
+              PairParams pp, int* __restrict__ pcj, double* __restrict__ pcd) {
+    constexpr int NR = 3 * PC_MAXM, NC = NR + 3;
+    __shared__ double lds[NR * NC * PC_BLOCK];
+    const int f = blockIdx.y, a = blockIdx.x * PC_BLOCK + threadIdx.x;
+    if (a >= N) return;  // no barriers below: each thread works on its own slice of lds
+    auto M = [&](int r, int c) -> double& { return lds[(r * NC + c) * PC_BLOCK + threadIdx.x]; };
+    const size_t fo = (size_t)f * N, ga = fo + a;
+    int* oj = pcj + ga * 3;
+    oj[0] = -1; oj[1] = -1; oj[2] = -1;
+    // the mutual pair (p, q) of a, as one of its sites or as a child
+    int p, q;
+    {
+        const int j = pc_nn(nbr, pnear, fo, cap, a);
+        if (j < 0) return;
+        const int jj = pc_nn(nbr, pnear, fo, cap, j);
+        if (jj == a) { p = a; q = j; }
+        else if (jj >= 0 && pc_nn(nbr, pnear, fo, cap, jj) == j) { p = j; q = jj; }
+        else return;
+    }
     const BoxInfo& bx = boxes[f];
-    const double4 pi_ = pos[a], pj = pos[fo + j];
-    double dx = pi_.x - pj.x, dy = pi_.y - pj.y, dz = pi_.z - pj.z, n0, n1, n2;
-    pbc_shift(bx, dx, dy, dz, n0, n1, n2);
-    const double r = sqrt(dx * dx + dy * dy + dz * dz), ri = 1.0 / r;
-    const int sidx = scale_index(nb, pp.nscale);
-    const double dsc = pp.ds[sidx];
-    const R* si = site + a * SITE;
-    const R* sj = site + (fo + j) * SITE;
-    const double th_a = pp.ps[sidx] > 0.0 ? 5.0 : (double)si[11] + (double)sj[11];
-    double dmp = (double)si[10] * (double)sj[10];
-    if (dmp < 1e-8) dmp = 1e-8;
-    double u = r / dmp;
-    if (!(u < 1e8)) u = 1e8;
-    Thole<double> th;
-    thole_f<double>(th_a * u, th);
-    const double P3 = DIELECTRIC * ri * ri * ri;
-    const double c1 = P3 * dsc * th.c, c2 = P3 * dsc * (-2.0 * th.d0 - th.c);
-    const double di = DIELECTRIC / (pol[i] < 1e-8 ? 1e-8 : pol[i]);
-    const double dj = DIELECTRIC / (pol[j] < 1e-8 ? 1e-8 : pol[j]);
-    const double bpar = c1 + c2, bperp = c1, dd = di * dj;
-    const double Dpar = dd - bpar * bpar, Dperp = dd - bperp * bperp;
-    if (fabs(Dpar) < 1e-3 * dd || fabs(Dperp) < 1e-3 * dd) return;   // nearly singular: keep the diagonal
-    double* o = pcd + a * 8;
-    o[0] = dx * ri; o[1] = dy * ri; o[2] = dz * ri; o[3] = 0.0;
-    o[4] = dj / Dpar; o[5] = -bpar / Dpar; o[6] = dj / Dperp; o[7] = -bperp / Dperp;
-    pcj[a] = j;
+    auto sep = [&](int x, int y, double& dx, double& dy, double& dz) {
+        const double4 px = pos[fo + x], py = pos[fo + y];
+        dx = px.x - py.x; dy = px.y - py.y; dz = px.z - py.z;
+        double n0, n1, n2;
+        pbc_shift(bx, dx, dy, dz, n0, n1, n2);
+    };
+    int mem[PC_MAXM];
+    int m = 2;
+    mem[0] = p; mem[1] = q;
+    if (maxm > 2) {
+        int cand[2 * PC_MAXCH];
+        double cd[2 * PC_MAXCH];
+        int nc = 0;
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int par = s2 ? q : p;
+            const int n = nchild[fo + par];
+            if (n > PC_MAXCH) continue;
+            for (int t = 0; t < n; ++t) {
+                const int c = child[(fo + par) * PC_MAXCH + t];
+                double dx, dy, dz;
+                sep(par, c, dx, dy, dz);
+                cand[nc] = c;
+                cd[nc] = dx * dx + dy * dy + dz * dz;
+                ++nc;
+            }
+        }
+        while (m < maxm && nc > 0) {
+            int b = 0;
+            for (int t = 1; t < nc; ++t)
+                if (cd[t] < cd[b] || (cd[t] == cd[b] && cand[t] < cand[b])) b = t;
+            mem[m++] = cand[b];
+            cand[b] = cand[nc - 1];
+            cd[b] = cd[nc - 1];
+            --nc;
+        }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            if (m == 2 || (a != p && a != q)) return;   // the pair block failed, or a child of a failed cluster
+            m = 2; mem[0] = p; mem[1] = q;
+        }
+        int self = -1;
+        for (int t = 0; t < m; ++t) self = mem[t] == a ? t : self;
+        if (self < 0) return;  // a child left out of its pair's cluster
+        // canonical member order (ascending site index): every member builds the same system
+        for (int t = 1; t < m; ++t)
+            for (int u = t; u > 0 && mem[u - 1] > mem[u]; --u) {
+                const int x = mem[u]; mem[u] = mem[u - 1]; mem[u - 1] = x;
+            }
+        for (int t = 0; t < m; ++t) self = mem[t] == a ? t : self;
+        const int n3 = 3 * m;
+        double dref[PC_MAXM];
+        for (int t = 0; t < m; ++t) {
+            const double al = pol[mem[t]];
+            dref[t] = DIELECTRIC / (al < 1e-8 ? 1e-8 : al);
+        }
+        for (int r = 0; r < n3; ++r)
+            for (int c = 0; c < n3 + 3; ++c) M(r, c) = 0.0;
+        for (int t = 0; t < m; ++t) {
+            for (int k = 0; k < 3; ++k) M(3 * t + k, 3 * t + k) = dref[t];
+            for (int u = t + 1; u < m; ++u) {
+                const int x = mem[t], y = mem[u];
+                // only the links of the nearest-neighbour graph (the pair, each
+                // child to its parent) are coupled, with the bonded distance of
+                // their neighbour-list entry; other couplings inside the cluster
+                // are left out of the block
+                int nbl = -1;
+                if (pc_nn(nbr, pnear, fo, cap, x) == y) nbl = nbr[(fo + x) * (size_t)cap + pnear[fo + x]] >> NB_SHIFT;
+                else if (pc_nn(nbr, pnear, fo, cap, y) == x) nbl = nbr[(fo + y) * (size_t)cap + pnear[fo + y]] >> NB_SHIFT;
+                if (nbl < 0) continue;
+                double dx, dy, dz;
+                sep(x, y, dx, dy, dz);
+                const double r = sqrt(dx * dx + dy * dy + dz * dz), ri = 1.0 / r;
+                const int sidx = scale_index(nbl, pp.nscale);
+                const double dsc = pp.ds[sidx];
+                const R* sx = site + (fo + x) * SITE;
+                const R* sy = site + (fo + y) * SITE;
+                const double th_a = pp.ps[sidx] > 0.0 ? 5.0 : (double)sx[11] + (double)sy[11];
+                double dmp = (double)sx[10] * (double)sy[10];
+                if (dmp < 1e-8) dmp = 1e-8;
+                double uu = r / dmp;
+                if (!(uu < 1e8)) uu = 1e8;
+                Thole<double> th;
+                thole_f<double>(th_a * uu, th);
+                const double P3 = DIELECTRIC * ri * ri * ri;
+                const double c1 = P3 * dsc * th.c, c2 = P3 * dsc * (-2.0 * th.d0 - th.c);
+                const double z[3] = {dx * ri, dy * ri, dz * ri};
+                for (int k = 0; k < 3; ++k)
+                    for (int l = 0; l < 3; ++l) {
+                        const double v = (k == l ? c1 : 0.0) + c2 * z[k] * z[l];
+                        M(3 * t + k, 3 * u + l) = v;
+                        M(3 * u + l, 3 * t + k) = v;
+                    }
+            }
+        }
+        for (int k = 0; k < 3; ++k) M(3 * self + k, n3 + k) = 1.0;
+        // elimination without pivoting (the diagonal blocks dominate unless
+        // the block is nearly singular, which the pivot test catches)
+        bool ok = true;
+        for (int k = 0; k < n3 && ok; ++k) {
+            const double piv = M(k, k);
+            if (!(fabs(piv) >= 1e-3 * dref[k / 3])) { ok = false; break; }
+            const double ip = 1.0 / piv;
+            for (int r = k + 1; r < n3; ++r) {
+                const double l = M(r, k) * ip;
+                if (l == 0.0) continue;
+                for (int c = k + 1; c < n3 + 3; ++c) M(r, c) -= l * M(k, c);
+            }
+        }
+        if (!ok) continue;
+        for (int k = n3 - 1; k >= 0; --k)
+            for (int c = n3; c < n3 + 3; ++c) {
+                double v = M(k, c);
+                for (int l = k + 1; l < n3; ++l) v -= M(k, l) * M(l, c);
+                M(k, c) = v / M(k, k);
+            }
+        // row k of a's rows = column k of the (symmetric) inverse = solution column k
+        double* o = pcd + ga * (3 * NR);
+        int slot = 0;
+        for (int t = 0; t < m; ++t) {
+            const int blk = t == self ? 0 : ++slot;
+            if (t != self) oj[blk - 1] = mem[t];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) o[r * NR + 3 * blk + c] = M(3 * t + c, n3 + r);
+        }
+        for (int b = m; b < PC_MAXM; ++b)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) o[r * NR + 3 * b + c] = 0.0;
+        return;
+    }
 }
 
 // out_I = sum_J c1 V_J + c2 (z.V_J) z   (induced-induced real-space coupling),

@@ -785,26 +785,31 @@ __device__ inline void cg_derived(const BoxInfo* __restrict__ boxes, int f, size
     for (int j = 0; j < 3; ++j) o[1 + j] = (R)(A[0 * 3 + j] * u[0] + A[1 * 3 + j] * u[1] + A[2 * 3 + j] * u[2]);
 }
 
-// z = M^-1 r at site a: the pair block of a (partner j >= 0 with its row of
-// coefficients pd, see k_pc_pairs) or the diagonal m = alpha / DIELECTRIC
-__device__ inline void pc_apply(int j, const double* __restrict__ pd, double m, const double* ri,
-                                const double* rj, double* u) {
-    if (j < 0) {
+// z = M^-1 r at site a: its three rows of the cluster inverse (pd: 3 rows of 12,
+// column blocks a, then the partners pj[0..2]; see k_pc_clusters) or, without
+// a cluster (pj[0] < 0), the diagonal m = alpha / DIELECTRIC.  rj: the partners'
+// r (zero for unused partners, whose coefficients are zero too).
+constexpr int PC_ROW = 12;
+__device__ inline void pc_apply(const int* pj, const double* __restrict__ pd, double m, const double* ri,
+                                const double (*rj)[3], double* u) {
+    if (pj[0] < 0) {
         for (int k = 0; k < 3; ++k) u[k] = m * ri[k];
         return;
     }
-    const double zx = pd[0], zy = pd[1], zz = pd[2];
-    const double pi = zx * ri[0] + zy * ri[1] + zz * ri[2];
-    const double pj = zx * rj[0] + zy * rj[1] + zz * rj[2];
-    const double t = (pd[4] - pd[6]) * pi + (pd[5] - pd[7]) * pj;
-    u[0] = pd[6] * ri[0] + pd[7] * rj[0] + t * zx;
-    u[1] = pd[6] * ri[1] + pd[7] * rj[1] + t * zy;
-    u[2] = pd[6] * ri[2] + pd[7] * rj[2] + t * zz;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double* row = pd + k * PC_ROW;
+        double v = row[0] * ri[0] + row[1] * ri[1] + row[2] * ri[2];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+            v += row[3 * t + 3] * rj[t][0] + row[3 * t + 4] * rj[t][1] + row[3 * t + 5] * rj[t][2];
+        u[k] = v;
+    }
 }
 
 // r = b - Ax (Ax null: x = 0, r = b), z = M^-1 r and the product inputs derived from z (with the spline
 // coordinates wfrac of the solve's precision R); r into buffer 0 of its two,
-// partials into buffers 0.  pcj/pcd: pair-block preconditioner (null: diagonal).
+// partials into buffers 0.  pcj/pcd: block preconditioner (pcj null: diagonal).
 template <class R>
 __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __restrict__ boxes,
                                                         const double* __restrict__ wd, const double* __restrict__ pol,
@@ -819,19 +824,22 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __
     double rz = 0, mx = 0;
     if (i < N) {
         const size_t a = (size_t)f * N + i;
-        double rv[3], rj[3] = {0, 0, 0}, u[3];
+        double rv[3], rj[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, u[3];
         for (int k = 0; k < 3; ++k) {
             rv[k] = Ax ? b[a * 3 + k] - Ax[a * 3 + k] : b[a * 3 + k];
             r[a * 3 + k] = rv[k];
             if (pol[i] > 0.001) mx = fmax(mx, fabs(rv[k]));
             if (wd) wfrac[a * 3 + k] = (R)wd[a * 3 + k];
         }
-        const int j = pcj ? pcj[a] : -1;
-        if (j >= 0) {
-            const size_t aj = (size_t)f * N + j;
-            for (int k = 0; k < 3; ++k) rj[k] = Ax ? b[aj * 3 + k] - Ax[aj * 3 + k] : b[aj * 3 + k];
-        }
-        pc_apply(j, j >= 0 ? pcd + a * 8 : nullptr, inv_diag(pol[i]), rv, rj, u);
+        int pj[3] = {-1, -1, -1};
+        if (pcj)
+            for (int t = 0; t < 3; ++t) pj[t] = pcj[a * 3 + t];
+        for (int t = 0; t < 3; ++t)
+            if (pj[t] >= 0) {
+                const size_t aj = (size_t)f * N + pj[t];
+                for (int k = 0; k < 3; ++k) rj[t][k] = Ax ? b[aj * 3 + k] - Ax[aj * 3 + k] : b[aj * 3 + k];
+            }
+        pc_apply(pj, pcd + a * 3 * PC_ROW, inv_diag(pol[i]), rv, rj, u);
         for (int k = 0; k < 3; ++k) {
             z[a * 3 + k] = u[k];
             rz += rv[k] * u[k];
@@ -928,9 +936,9 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
     const int done = st[f].done;
     const size_t a = (size_t)f * N + (i < N ? i : 0);
     double zv[3] = {0, 0, 0}, pv0[3] = {0, 0, 0}, wv[3] = {0, 0, 0}, sv0[3] = {0, 0, 0}, rv0[3] = {0, 0, 0};
-    double wj[3] = {0, 0, 0}, sj[3] = {0, 0, 0}, rj0[3] = {0, 0, 0};
+    double wj[3][3] = {}, sj[3][3] = {}, rj0[3][3] = {};
     double poli = 0.0;
-    int j = -1;
+    int pj[3] = {-1, -1, -1};
     if (i < N) {
         for (int k = 0; k < 3; ++k) {
             const size_t e = a * 3 + k;
@@ -941,16 +949,18 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
             rv0[k] = rc[e];
         }
         poli = pol[i];
-        j = pcj ? pcj[a] : -1;
-        if (j >= 0) {
-            const size_t aj = (size_t)f * N + j;
-            for (int k = 0; k < 3; ++k) {
-                const size_t e = aj * 3 + k;
-                wj[k] = w[e];
-                sj[k] = sc[e];
-                rj0[k] = rc[e];
+        if (pcj)
+            for (int t = 0; t < 3; ++t) pj[t] = pcj[a * 3 + t];
+        for (int t = 0; t < 3; ++t)
+            if (pj[t] >= 0) {
+                const size_t aj = (size_t)f * N + pj[t];
+                for (int k = 0; k < 3; ++k) {
+                    const size_t e = aj * 3 + k;
+                    wj[t][k] = w[e];
+                    sj[t][k] = sc[e];
+                    rj0[t][k] = rc[e];
+                }
             }
-        }
     }
     if (done) return;   // the same value in every thread of the block
     double res, gamma, delta;
@@ -969,7 +979,7 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
     const double alpha = den != 0.0 ? gamma / den : 0.0;
     double lrz = 0, lmx = 0;
     if (i < N) {
-        double u[3], rv[3], rj[3] = {0, 0, 0};
+        double u[3], rv[3], rj[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         for (int k = 0; k < 3; ++k) {
             const size_t e = a * 3 + k;
             const double pv = restart ? zv[k] : zv[k] + beta * pv0[k];
@@ -981,13 +991,13 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_update(int N, int nwu, int cur
             rn[e] = rv[k];
             if (poli > 0.001) lmx = fmax(lmx, fabs(rv[k]));
         }
-        if (j >= 0) {
-            for (int k = 0; k < 3; ++k) {
-                const double sv = restart ? wj[k] : wj[k] + beta * sj[k];
-                rj[k] = rj0[k] - alpha * sv;
-            }
-        }
-        pc_apply(j, j >= 0 ? pcd + a * 8 : nullptr, inv_diag(poli), rv, rj, u);
+        for (int t = 0; t < 3; ++t)
+            if (pj[t] >= 0)
+                for (int k = 0; k < 3; ++k) {
+                    const double sv = restart ? wj[t][k] : wj[t][k] + beta * sj[t][k];
+                    rj[t][k] = rj0[t][k] - alpha * sv;
+                }
+        pc_apply(pj, pcd + a * 3 * PC_ROW, inv_diag(poli), rv, rj, u);
         for (int k = 0; k < 3; ++k) {
             z[a * 3 + k] = u[k];
             lrz += rv[k] * u[k];

@@ -63,7 +63,7 @@ typedef enum { ADMP_FP64 = 0, ADMP_MIXED = 1 } admp_precision;
  *  ADMP_SOLVER_PCG: preconditioned conjugate gradients to the same fixed point,
  *      stop on the same max-norm criterion (used for tight tolerances). */
 typedef enum { ADMP_SOLVER_REFERENCE = 0, ADMP_SOLVER_STEPS = 1, ADMP_SOLVER_PCG = 2 } admp_solver_kind;
-typedef enum { ADMP_PRECOND_PAIRS = 0, ADMP_PRECOND_DIAGONAL = 1 } admp_precond;
+typedef enum { ADMP_PRECOND_CLUSTERS = 0, ADMP_PRECOND_DIAGONAL = 1, ADMP_PRECOND_PAIRS = 2 } admp_precond;
 
 typedef struct {
     int32_t n_atoms;
@@ -93,9 +93,12 @@ typedef struct {
                                   FP32 until the true residual fell 1000x (or 32
                                   iterations), then a restart with FP64 products;
                                   0 = FP64 products throughout (default)      */
-    int32_t precond;           /* PCG preconditioner: ADMP_PRECOND_PAIRS (default,
-                                  exact 6x6 blocks of mutually nearest coupled
-                                  sites within 2 A, diagonal elsewhere) or
+    int32_t precond;           /* PCG preconditioner: ADMP_PRECOND_CLUSTERS
+                                  (default: exact blocks of up to 4 sites, a
+                                  pair of mutually nearest coupled sites within
+                                  2 A with the sites whose nearest coupled
+                                  neighbour is in the pair; diagonal elsewhere),
+                                  ADMP_PRECOND_PAIRS (the pairs alone) or
                                   ADMP_PRECOND_DIAGONAL (alpha / DIELECTRIC) */
 } admp_solver_opts;
 

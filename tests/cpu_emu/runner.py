@@ -199,7 +199,7 @@ def waters(precision, n_waters):
 
 
 def precond(precision, n_waters):
-    """The same solve with the diagonal and the pair-block preconditioner."""
+    """The same solve with the diagonal, the pair-block and the cluster-block preconditioner."""
     from dmff_amd.admp.engine import Engine
     from dmff_amd.admp.pme import setup_ewald_parameters
     from dmff_amd.admp.serving import water_spec
@@ -209,7 +209,7 @@ def precond(precision, n_waters):
     eng.set_topology(spec.axis_types, spec.axis_idx, spec.exclusions)
     eng.build_neighbors(pos, box)
     out = {}
-    for pc in ("diagonal", "pairs"):
+    for pc in ("diagonal", "pairs", "clusters"):
         E, g, U, info = eng.compute(pos, box, spec.Q_local, spec.pol, spec.thole, *spec.scales, solver="pcg",
                                     tol=1e-6, maxiter=500, want_grad=True, precond=pc)
         out[pc] = {"E": float(E[0]), "pos": _lst(g["pos"][0]), "U": _lst(U[0]), "n_iter": info["n_iter"][0],

@@ -167,16 +167,21 @@ def test_water_box_several_solver_blocks(emu_lib):
     assert abs(rmx["E"] - r64["E"]) < 2e-5 * abs(r64["E"])
 
 
-def test_pair_preconditioner_same_solution_fewer_iterations(emu_lib):
-    """512 waters: the pair-block preconditioner converges to the same induced
-    dipoles as the diagonal one (both to max|dE/dU| < 1e-6) in fewer iterations."""
+def test_block_preconditioners_same_solution_fewer_iterations(emu_lib):
+    """512 waters: the pair-block and the cluster-block preconditioners converge
+    to the same induced dipoles as the diagonal one (all to max|dE/dU| < 1e-6),
+    the pairs in fewer iterations than the diagonal, the clusters in no more
+    than the pairs."""
     r = run(emu_lib, "precond", "fp64", 512)
-    d, p = r["diagonal"], r["pairs"]
-    assert d["residual"] < 1e-6 and p["residual"] < 1e-6
+    d, p, c = r["diagonal"], r["pairs"], r["clusters"]
+    print({k: v["n_iter"] for k, v in r.items()})
+    assert d["residual"] < 1e-6 and p["residual"] < 1e-6 and c["residual"] < 1e-6
     assert p["n_iter"] < d["n_iter"], (p["n_iter"], d["n_iter"])
-    assert abs(p["E"] - d["E"]) < 1e-12 * abs(d["E"])
-    assert rel_rms(p["pos"], d["pos"]) < 1e-8
-    assert np.abs(np.asarray(p["U"]) - np.asarray(d["U"])).max() < 1e-8
+    assert c["n_iter"] <= p["n_iter"], (c["n_iter"], p["n_iter"])
+    for x in (p, c):
+        assert abs(x["E"] - d["E"]) < 1e-12 * abs(d["E"])
+        assert rel_rms(x["pos"], d["pos"]) < 1e-8
+        assert np.abs(np.asarray(x["U"]) - np.asarray(d["U"])).max() < 1e-8
 
 
 def test_type_reduce(emu_lib):

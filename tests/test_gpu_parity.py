@@ -361,10 +361,11 @@ def test_batched_frames_side_stream_product():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_pair_preconditioner_same_solution_fewer_iterations(prec):
-    """4096 waters (the 12,288-site system): the pair-block preconditioner and the
-    diagonal one reach the same induced dipoles (max|dE/dU| < 1e-6 each); the pair
-    blocks take fewer iterations.  The small-box tests check both against the oracle."""
+def test_block_preconditioners_same_solution_fewer_iterations(prec):
+    """4096 waters (the 12,288-site system): the cluster-block, pair-block and
+    diagonal preconditioners reach the same induced dipoles (max|dE/dU| < 1e-6
+    each); the pair blocks take fewer iterations than the diagonal, the clusters
+    no more than the pairs.  The small-box tests check them against the oracle."""
     from dmff_amd.admp.engine import Engine
     from dmff_amd.admp.water import config_system
     S, c = config_system(1)
@@ -375,19 +376,22 @@ def test_pair_preconditioner_same_solution_fewer_iterations(prec):
     pos = S.positions_nm * 10.0
     eng.build_neighbors(pos, box)
     out = {}
-    for pc in ("diagonal", "pairs"):
+    for pc in ("diagonal", "pairs", "clusters"):
         E, g, U, info = eng.compute(pos, box, S.Q_local, S.pol, S.thole, S.mScales, S.pScales, S.dScales,
                                     solver="pcg", tol=1e-6, maxiter=500, want_grad=True, precond=pc)
         out[pc] = (float(E[0]), g["pos"][0].cpu().numpy(), U[0].cpu().numpy(), info)
         print(f"{prec} {pc}: E={float(E[0]):.10f} iters={info['n_iter'][0]} res={info['residual'][0]:.2e}",
               flush=True)
     (Ed, gd, Ud, idg), (Ep, gp, Up, ip) = out["diagonal"], out["pairs"]
-    assert idg["residual"][0] < 1e-6 and ip["residual"][0] < 1e-6
+    ic = out["clusters"][3]
+    assert idg["residual"][0] < 1e-6 and ip["residual"][0] < 1e-6 and ic["residual"][0] < 1e-6
     assert ip["n_iter"][0] < idg["n_iter"][0]
+    assert ic["n_iter"][0] <= ip["n_iter"][0]
     etol, ftol = (1e-11, 1e-8) if prec == "fp64" else (2e-8, 1e-6)
-    assert abs(Ep - Ed) < etol * abs(Ed)
-    assert rel_rms(gp, gd) < ftol
-    assert np.abs(Up - Ud).max() < 1e-7
+    for E2, g2, U2, _ in (out["pairs"], out["clusters"]):
+        assert abs(E2 - Ed) < etol * abs(Ed)
+        assert rel_rms(g2, gd) < ftol
+        assert np.abs(U2 - Ud).max() < 1e-7
 
 
 def test_small_box_diagonal_preconditioner_vs_oracle():
