@@ -1608,8 +1608,8 @@ static admp_status solver_rhs(admp_ctx* ctx, const double* pol, const double* th
     if (ctx->pc_maxm > 1) {
         hipLaunchKernelGGL(k_pc_children, dim3(nblk(N, 256), F), dim3(256), 0, sr, N, ctx->cap, ctx->d_nbr,
                            ctx->d_pnear, ctx->d_pcn, ctx->d_pcc);
-        hipLaunchKernelGGL((k_pc_clusters<R>), dim3(nblk(N, PC_BLOCK), F), dim3(PC_BLOCK), 0, sr, N, ctx->cap,
-                           ctx->pc_maxm, ctx->d_pos, ctx->d_box, site, pol, ctx->d_nbr, ctx->d_pnear, ctx->d_pcn,
+        hipLaunchKernelGGL((k_pc_clusters<R>), dim3(nblk((size_t)N * PC_LANES, 256), F), dim3(256), 0, sr, N, ctx->cap,
+                           ctx->pc_maxm, ctx->d_pos, ctx->d_box, site, pol, ctx->d_nbr, ctx->d_ncnt, ctx->d_pnear, ctx->d_pcn,
                            ctx->d_pcc, pp, ctx->d_pcj, ctx->d_pcd);
     } }
     HIPCHK(hipGetLastError());

@@ -481,8 +481,8 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
 // ---------------------------------------------------------------------------
 // Block preconditioner (set-up once per solve; positions are fixed).
 // Every site has a nearest coupled neighbour n(i) inside PC_RPAIR (or none).
-// Two sites with n(i) = j and n(j) = i form a mutual pair; a site c whose
-// n(c) = p lies in a mutual pair (p, q), c outside it, is a child of p.  A
+// Two sites with n(i) = j and n(j) = i form a mutual pair (p < q); a site c
+// whose n(c) lies in a mutual pair, c outside it, is a child of that site.  A
 // cluster is a mutual pair plus up to maxm - 2 children of its two sites, the
 // nearest to their parent first (site index on ties); a site whose parent has
 // more than PC_MAXCH children takes no child of it (the recorded ones depend
@@ -490,23 +490,28 @@ k_real_permfield(int N, int cap, const double4* __restrict__ pos, const BoxInfo*
 // hydrogen-bonded to it inside PC_RPAIR.  The cluster's block of the
 // polarisation matrix is approximated by
 //   diagonal blocks d_i I, d = DIELECTRIC / alpha,
-//   for the pair and each child-parent link: c1 I + c2 z z^T  (the
+//   for the pair and each child-parent link: B = c1 I + c2 z z^T  (the
 //   Thole-damped dipole-dipole tensor without Ewald screening, d-scaled, with
 //   the Thole width of the induced-induced product and the bonded distance of
 //   the link's neighbour-list entry); the other couplings inside a cluster are
 //   left out (numpy on the dense matrix: the same iteration count),
-// and inverted exactly (Gaussian elimination in LDS).  Each member solves the
-// same system for its own three rows, so every member holds the same inverse.
-// A pivot below 1e-3 of its row's d (nearly singular block) makes the cluster
-// fall back to its mutual pair, and that pair to the diagonal 1/d.  The blocks
-// of close contacts may be indefinite; they carry the local modes that slow a
-// diagonal-preconditioned solve most.  Output per site: the partner sites
-// pcj[3] (-1: unused; pcj[0] = -1: diagonal) and its three rows of the inverse,
-// pcd[3][12] (column blocks: the site itself, then the partners in pcj order).
+// a tree, whose inverse is exact in 3x3 blocks: eliminating the children
+// leaves the pair with P_x = d_x I - sum_{c -> x} B_c B_c / d_c, then
+//   S = P_q - B_pq P_p^-1 B_pq,  K_qq = S^-1,  K_pq = -P_p^-1 B_pq S^-1,
+//   K_pp = P_p^-1 + P_p^-1 B_pq S^-1 B_pq P_p^-1,
+// and for children (parent x, y any pair site, c' another child):
+//   M^-1_cy = -B_c K_xy / d_c,  M^-1_cc' = delta I / d_c + B_c K_xx' B_c' / (d_c d_c').
+// Every member evaluates the same expressions in the same order, so every
+// member holds the same inverse.  |det| of P_p or S below 1e-3 d^3 (a nearly
+// singular block) makes the cluster fall back to its pair, and the pair to the
+// diagonal 1/d.  The blocks of close contacts may be indefinite; they carry the
+// local modes that slow a diagonal-preconditioned solve most.  Output per site:
+// the partner sites pcj[3] (-1: unused; pcj[0] = -1: diagonal) and its three
+// rows of the inverse, pcd[3][12] (column blocks: the site itself, then the
+// partners in pcj order).
 // ---------------------------------------------------------------------------
 constexpr int PC_MAXM = 4;    // sites per cluster
 constexpr int PC_MAXCH = 4;   // children recorded per pair site
-constexpr int PC_BLOCK = 32;  // threads per block of k_pc_clusters (one 12 x 15 system each in LDS)
 
 // n(i): the nearest coupled neighbour of site i inside PC_RPAIR, -1 if none
 __device__ __forceinline__ int pc_nn(const int* __restrict__ nbr, const int* __restrict__ pnear, size_t fo,
@@ -531,160 +536,194 @@ k_pc_children(int N, int cap, const int* __restrict__ nbr, const int* __restrict
     if (slot < PC_MAXCH) child[(fo + p) * PC_MAXCH + slot] = c;
 }
 
+// 16 lanes per site: the site's cluster block is solved by Gauss-Jordan
+// elimination with one matrix column per lane (lanes 0..11) and the three unit
+// right-hand sides of the site's own rows in lanes 12..14; pivot columns are
+// broadcast by shuffles.
+constexpr int PC_LANES = 16;
+
 template <class R>
-__global__ void __launch_bounds__(PC_BLOCK)
+__global__ void __launch_bounds__(256)
 k_pc_clusters(int N, int cap, int maxm, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes,
               const R* __restrict__ site, const double* __restrict__ pol, const int* __restrict__ nbr,
-              const int* __restrict__ pnear, const int* __restrict__ nchild, const int* __restrict__ child,
-              PairParams pp, int* __restrict__ pcj, double* __restrict__ pcd) {
-    constexpr int NR = 3 * PC_MAXM, NC = NR + 3;
-    __shared__ double lds[NR * NC * PC_BLOCK];
-    const int f = blockIdx.y, a = blockIdx.x * PC_BLOCK + threadIdx.x;
-    if (a >= N) return;  // no barriers below: each thread works on its own slice of lds
-    auto M = [&](int r, int c) -> double& { return lds[(r * NC + c) * PC_BLOCK + threadIdx.x]; };
+              const int* __restrict__ ncnt, const int* __restrict__ pnear, const int* __restrict__ nchild,
+              const int* __restrict__ child, PairParams pp, int* __restrict__ pcj, double* __restrict__ pcd) {
+    static_assert(PC_MAXM == 4 && 256 % PC_LANES == 0, "four sites per cluster, whole lane groups per block");
+    constexpr int NR = 3 * PC_MAXM;
+    const int f = blockIdx.y;
+    const int gt = blockIdx.x * 256 + threadIdx.x;
+    const int a = gt / PC_LANES, lane = gt % PC_LANES;
+    if (a >= N) return;  // the whole lane group
     const size_t fo = (size_t)f * N, ga = fo + a;
-    int* oj = pcj + ga * 3;
-    oj[0] = -1; oj[1] = -1; oj[2] = -1;
-    // the mutual pair (p, q) of a, as one of its sites or as a child
-    int p, q;
-    {
-        const int j = pc_nn(nbr, pnear, fo, cap, a);
-        if (j < 0) return;
-        const int jj = pc_nn(nbr, pnear, fo, cap, j);
-        if (jj == a) { p = a; q = j; }
-        else if (jj >= 0 && pc_nn(nbr, pnear, fo, cap, jj) == j) { p = j; q = jj; }
-        else return;
-    }
     const BoxInfo& bx = boxes[f];
-    auto sep = [&](int x, int y, double& dx, double& dy, double& dz) {
-        const double4 px = pos[fo + x], py = pos[fo + y];
-        dx = px.x - py.x; dy = px.y - py.y; dz = px.z - py.z;
-        double n0, n1, n2;
-        pbc_shift(bx, dx, dy, dz, n0, n1, n2);
-    };
-    int mem[PC_MAXM];
-    int m = 2;
-    mem[0] = p; mem[1] = q;
-    if (maxm > 2) {
-        int cand[2 * PC_MAXCH];
-        double cd[2 * PC_MAXCH];
-        int nc = 0;
-        for (int s2 = 0; s2 < 2; ++s2) {
-            const int par = s2 ? q : p;
-            const int n = nchild[fo + par];
-            if (n > PC_MAXCH) continue;
-            for (int t = 0; t < n; ++t) {
-                const int c = child[(fo + par) * PC_MAXCH + t];
-                double dx, dy, dz;
-                sep(par, c, dx, dy, dz);
-                cand[nc] = c;
-                cd[nc] = dx * dx + dy * dy + dz * dz;
-                ++nc;
+    int outj[3] = {-1, -1, -1};
+    // Every condition below depends on the site only, so the 16 lanes of a
+    // group take the same branches (the shuffles need the whole group).
+    [&]() {
+        // the mutual pair (p < q) of a, as one of its sites or as a child
+        int p, q;
+        {
+            const int j = pc_nn(nbr, pnear, fo, cap, a);
+            if (j < 0) return;
+            const int jj = pc_nn(nbr, pnear, fo, cap, j);
+            if (jj == a) { p = a; q = j; }
+            else if (jj >= 0 && pc_nn(nbr, pnear, fo, cap, jj) == j) { p = j; q = jj; }
+            else return;
+            if (p > q) { const int t = p; p = q; q = t; }
+        }
+        // children: up to maxm - 2, nearest to their parent first (index on ties),
+        // then in ascending index order
+        int ch0 = -1, ch1 = -1, nch = 0;
+        if (maxm > 2) {
+            int cand[2 * PC_MAXCH];
+            double cd[2 * PC_MAXCH];
+            int nc = 0;
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const int par = s2 ? q : p;
+                const int n = nchild[fo + par];
+                if (n > PC_MAXCH) continue;
+                for (int t = 0; t < n; ++t) {
+                    const int c = child[(fo + par) * PC_MAXCH + t];
+                    const double4 pa = pos[fo + par], pc = pos[fo + c];
+                    double dx = pa.x - pc.x, dy = pa.y - pc.y, dz = pa.z - pc.z, n0, n1, n2;
+                    pbc_shift(bx, dx, dy, dz, n0, n1, n2);
+                    cand[nc] = c;
+                    cd[nc] = dx * dx + dy * dy + dz * dz;
+                    ++nc;
+                }
             }
+            for (int pick = 0; pick < maxm - 2 && nc > 0; ++pick) {
+                int b = 0;
+                for (int t = 1; t < nc; ++t)
+                    if (cd[t] < cd[b] || (cd[t] == cd[b] && cand[t] < cand[b])) b = t;
+                if (pick == 0) ch0 = cand[b]; else ch1 = cand[b];
+                ++nch;
+                cand[b] = cand[nc - 1];
+                cd[b] = cd[nc - 1];
+                --nc;
+            }
+            if (nch == 2 && ch1 < ch0) { const int t = ch0; ch0 = ch1; ch1 = t; }
         }
-        while (m < maxm && nc > 0) {
-            int b = 0;
-            for (int t = 1; t < nc; ++t)
-                if (cd[t] < cd[b] || (cd[t] == cd[b] && cand[t] < cand[b])) b = t;
-            mem[m++] = cand[b];
-            cand[b] = cand[nc - 1];
-            cd[b] = cd[nc - 1];
-            --nc;
-        }
-    }
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) {
-            if (m == 2 || (a != p && a != q)) return;   // the pair block failed, or a child of a failed cluster
-            m = 2; mem[0] = p; mem[1] = q;
-        }
-        int self = -1;
-        for (int t = 0; t < m; ++t) self = mem[t] == a ? t : self;
+        int self = a == p ? 0 : a == q ? 1 : a == ch0 ? 2 : a == ch1 ? 3 : -1;
         if (self < 0) return;  // a child left out of its pair's cluster
-        // canonical member order (ascending site index): every member builds the same system
-        for (int t = 1; t < m; ++t)
-            for (int u = t; u > 0 && mem[u - 1] > mem[u]; --u) {
-                const int x = mem[u]; mem[u] = mem[u - 1]; mem[u - 1] = x;
-            }
-        for (int t = 0; t < m; ++t) self = mem[t] == a ? t : self;
-        const int n3 = 3 * m;
-        double dref[PC_MAXM];
-        for (int t = 0; t < m; ++t) {
-            const double al = pol[mem[t]];
-            dref[t] = DIELECTRIC / (al < 1e-8 ? 1e-8 : al);
-        }
-        for (int r = 0; r < n3; ++r)
-            for (int c = 0; c < n3 + 3; ++c) M(r, c) = 0.0;
-        for (int t = 0; t < m; ++t) {
-            for (int k = 0; k < 3; ++k) M(3 * t + k, 3 * t + k) = dref[t];
-            for (int u = t + 1; u < m; ++u) {
-                const int x = mem[t], y = mem[u];
-                // only the links of the nearest-neighbour graph (the pair, each
-                // child to its parent) are coupled, with the bonded distance of
-                // their neighbour-list entry; other couplings inside the cluster
-                // are left out of the block
-                int nbl = -1;
-                if (pc_nn(nbr, pnear, fo, cap, x) == y) nbl = nbr[(fo + x) * (size_t)cap + pnear[fo + x]] >> NB_SHIFT;
-                else if (pc_nn(nbr, pnear, fo, cap, y) == x) nbl = nbr[(fo + y) * (size_t)cap + pnear[fo + y]] >> NB_SHIFT;
-                if (nbl < 0) continue;
-                double dx, dy, dz;
-                sep(x, y, dx, dy, dz);
-                const double r = sqrt(dx * dx + dy * dy + dz * dz), ri = 1.0 / r;
-                const int sidx = scale_index(nbl, pp.nscale);
-                const double dsc = pp.ds[sidx];
-                const R* sx = site + (fo + x) * SITE;
-                const R* sy = site + (fo + y) * SITE;
-                const double th_a = pp.ps[sidx] > 0.0 ? 5.0 : (double)sx[11] + (double)sy[11];
-                double dmp = (double)sx[10] * (double)sy[10];
-                if (dmp < 1e-8) dmp = 1e-8;
-                double uu = r / dmp;
-                if (!(uu < 1e8)) uu = 1e8;
-                Thole<double> th;
-                thole_f<double>(th_a * uu, th);
-                const double P3 = DIELECTRIC * ri * ri * ri;
-                const double c1 = P3 * dsc * th.c, c2 = P3 * dsc * (-2.0 * th.d0 - th.c);
-                const double z[3] = {dx * ri, dy * ri, dz * ri};
-                for (int k = 0; k < 3; ++k)
-                    for (int l = 0; l < 3; ++l) {
-                        const double v = (k == l ? c1 : 0.0) + c2 * z[k] * z[l];
-                        M(3 * t + k, 3 * u + l) = v;
-                        M(3 * u + l, 3 * t + k) = v;
+        const int mem[PC_MAXM] = {p, q, ch0, ch1};
+        // bonded distance of every member pair (t < u) from the neighbour-list
+        // entry the product uses (-1: no entry, uncoupled): the pnear entry
+        // for a nearest-neighbour link, else a scan of the row shared by the lanes
+        int nbp[6];
+        {
+            int e = 0;
+#pragma unroll
+            for (int t = 0; t < PC_MAXM; ++t)
+#pragma unroll
+                for (int u = t + 1; u < PC_MAXM; ++u, ++e) {
+                    // every group runs the same sequence of shuffles (n = 0 where no scan is needed)
+                    int found = -1, n = 0;
+                    const int* row = nbr;
+                    const int x = mem[t], y = mem[u];
+                    if (u < 2 + nch) {
+                        const int kx = pnear[fo + x], ky = pnear[fo + y];
+                        const int wx = kx >= 0 ? nbr[(fo + x) * (size_t)cap + kx] : -1;
+                        const int wy = ky >= 0 ? nbr[(fo + y) * (size_t)cap + ky] : -1;
+                        if (kx >= 0 && (wx & NB_MASK) == y) found = wx >> NB_SHIFT;
+                        else if (ky >= 0 && (wy & NB_MASK) == x) found = wy >> NB_SHIFT;
+                        else { row = nbr + (fo + x) * (size_t)cap; n = ncnt[fo + x]; }
                     }
-            }
+                    for (int k = lane; k < n; k += PC_LANES)
+                        if ((row[k] & NB_MASK) == y) found = row[k] >> NB_SHIFT;
+                    for (int o = PC_LANES >> 1; o > 0; o >>= 1) found = max(found, __shfl_xor(found, o, PC_LANES));
+                    nbp[e] = found;
+                }
         }
-        for (int k = 0; k < 3; ++k) M(3 * self + k, n3 + k) = 1.0;
-        // elimination without pivoting (the diagonal blocks dominate unless
-        // the block is nearly singular, which the pivot test catches)
-        bool ok = true;
-        for (int k = 0; k < n3 && ok; ++k) {
-            const double piv = M(k, k);
-            if (!(fabs(piv) >= 1e-3 * dref[k / 3])) { ok = false; break; }
-            const double ip = 1.0 / piv;
-            for (int r = k + 1; r < n3; ++r) {
-                const double l = M(r, k) * ip;
-                if (l == 0.0) continue;
-                for (int c = k + 1; c < n3 + 3; ++c) M(r, c) -= l * M(k, c);
+        auto dinv = [&](int x) {
+            const double al = pol[x];
+            return DIELECTRIC / (al < 1e-8 ? 1e-8 : al);
+        };
+        // (k, l) element of the coupling block of members x, y with bonded distance nbl
+        auto coupling = [&](int x, int y, int nbl, int k, int l) {
+            const double4 px = pos[fo + x], py = pos[fo + y];
+            double dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z, n0, n1, n2;
+            pbc_shift(bx, dx, dy, dz, n0, n1, n2);
+            const double r = sqrt(dx * dx + dy * dy + dz * dz), ri = 1.0 / r;
+            const int sidx = scale_index(nbl, pp.nscale);
+            const double dsc = pp.ds[sidx];
+            const R* sx = site + (fo + x) * SITE;
+            const R* sy = site + (fo + y) * SITE;
+            const double th_a = pp.ps[sidx] > 0.0 ? 5.0 : (double)sx[11] + (double)sy[11];
+            double dmp = (double)sx[10] * (double)sy[10];
+            if (dmp < 1e-8) dmp = 1e-8;
+            double u = r / dmp;
+            if (!(u < 1e8)) u = 1e8;
+            Thole<double> th;
+            thole_f<double>(th_a * u, th);
+            const double P3 = DIELECTRIC * ri * ri * ri;
+            const double c1 = P3 * dsc * th.c, c2 = P3 * dsc * (-2.0 * th.d0 - th.c);
+            const double z[3] = {dx * ri, dy * ri, dz * ri};
+            return (k == l ? c1 : 0.0) + c2 * z[k] * z[l];
+        };
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1) {
+                if (nch == 0 || self >= 2) return;  // the pair block failed, or a child of a failed cluster
+                nch = 0;
             }
-        }
-        if (!ok) continue;
-        for (int k = n3 - 1; k >= 0; --k)
-            for (int c = n3; c < n3 + 3; ++c) {
-                double v = M(k, c);
-                for (int l = k + 1; l < n3; ++l) v -= M(k, l) * M(l, c);
-                M(k, c) = v / M(k, k);
+            const int ncl = 2 + nch;
+            double dref[PC_MAXM];
+#pragma unroll
+            for (int t = 0; t < PC_MAXM; ++t) dref[t] = t < ncl ? dinv(mem[t]) : 1.0;
+            // this lane's column: matrix column j = 3 u + l, or a unit right-hand side
+            double col[NR];
+            if (lane < NR) {
+                const int u = lane / 3, l = lane % 3;
+#pragma unroll
+                for (int t = 0; t < PC_MAXM; ++t) {
+                    const int e = t < u ? t * (2 * PC_MAXM - t - 1) / 2 + (u - t - 1)
+                                        : u * (2 * PC_MAXM - u - 1) / 2 + (t - u - 1);
+                    const bool cpl = t != u && t < ncl && u < ncl && nbp[t == u ? 0 : e] >= 0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        col[3 * t + k] = t == u ? (k == l ? dref[t] : 0.0)
+                                                : (cpl ? coupling(mem[t], mem[u], nbp[e], k, l) : 0.0);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) col[r] = (lane - NR < 3 && r == 3 * self + (lane - NR)) ? 1.0 : 0.0;
             }
-        // row k of a's rows = column k of the (symmetric) inverse = solution column k
-        double* o = pcd + ga * (3 * NR);
-        int slot = 0;
-        for (int t = 0; t < m; ++t) {
-            const int blk = t == self ? 0 : ++slot;
-            if (t != self) oj[blk - 1] = mem[t];
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) o[r * NR + 3 * blk + c] = M(3 * t + c, n3 + r);
+            // Gauss-Jordan without pivoting (the diagonal blocks dominate unless
+            // the block is nearly singular, which the pivot test catches)
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                double ck[NR];
+#pragma unroll
+                for (int r = 0; r < NR; ++r) ck[r] = __shfl(col[r], k, PC_LANES);
+                const double piv = ck[k];
+                ok = ok && fabs(piv) >= 1e-3 * dref[k / 3];
+                const double fk = col[k] / piv;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) col[r] = r == k ? fk : col[r] - ck[r] * fk;
+            }
+            if (!ok) continue;
+            // lanes 12..14: column r of the inverse = a's row r (the block is symmetric)
+            if (lane >= NR && lane < NR + 3) {
+                const int r = lane - NR;
+                double* o = pcd + ga * (3 * NR) + r * NR;
+                int slot = 0;
+#pragma unroll
+                for (int t = 0; t < PC_MAXM; ++t) {
+                    const int blk = t == self ? 0 : ++slot;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) o[3 * blk + c] = t < ncl ? col[3 * t + c] : 0.0;
+                }
+            }
+            int slot = 0;
+            for (int t = 0; t < PC_MAXM; ++t)
+                if (t != self) { if (t < ncl) outj[slot] = mem[t]; ++slot; }
+            return;
         }
-        for (int b = m; b < PC_MAXM; ++b)
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) o[r * NR + 3 * b + c] = 0.0;
-        return;
+    }();
+    if (lane == 0) {
+        int* oj = pcj + ga * 3;
+        oj[0] = outj[0]; oj[1] = outj[1]; oj[2] = outj[2];
     }
 }
 
