@@ -141,12 +141,13 @@ class Engine:
 
     def compute(self, pos, box, Q_local, pol=None, thole=None, mScales=None, pScales=None, dScales=None,
                 U_init=None, solver="pcg", tol=1e-6, maxiter=500, steps=0, want_grad=True, fp32_products=False,
-                precond="clusters"):
+                precond="clusters", stats=True):
         """Energies (F,) and, with want_grad, dE/d{pos, box, Q_local, pol, thole}.
         U_init / returned U are global Cartesian e*A (F,N,3).  precond (PCG):
         "clusters" (exact blocks of up to 4 sites: a pair of mutually nearest coupled
         sites within 2 A and the sites whose nearest coupled neighbour lies in it),
-        "pairs" (the pairs alone) or "diagonal"."""
+        "pairs" (the pairs alone) or "diagonal".  stats=False: no solver statistics
+        (info is None), and the call need not wait for the device at its end."""
         F, N = self.n_frames, self.n_atoms
         dev = self.device
         pos = _dev(pos, dev).reshape(F, N, 3)
@@ -161,7 +162,8 @@ class Engine:
         sc = L.check_scales(mScales, pScales, dScales)
         ns = sc[0].shape[0]
         sc = [s if s is not None else np.ones(ns) for s in sc]
-        U = torch.zeros((F, N, 3), dtype=F64, device=dev)
+        # a polarisable call writes all of U (the solve's result)
+        U = (torch.empty if lpol else torch.zeros)((F, N, 3), dtype=F64, device=dev)
         warm = 0
         if U_init is not None and lpol:
             U.copy_(_dev(U_init, dev).reshape(F, N, 3))
@@ -181,15 +183,18 @@ class Engine:
             if lpol:
                 g["pol"] = torch.empty((F, N), dtype=F64, device=dev)
                 g["thole"] = torch.empty((F, N), dtype=F64, device=dev)
-        stats = L.Stats()
+        st_out = L.Stats() if stats else None
         self._stream()
         st = self.lib.admp_compute(
             self.h, _ptr(pos), _ptr(box), _ptr(Q), _ptr(polt), _ptr(tht),
             sc[0].ctypes.data_as(ctypes.c_void_p), sc[1].ctypes.data_as(ctypes.c_void_p),
             sc[2].ctypes.data_as(ctypes.c_void_p), ns, _ptr(U) if lpol else ctypes.c_void_p(0),
             ctypes.byref(opts), int(want_grad), _ptr(E), _ptr(g.get("pos")), _ptr(g.get("box")), _ptr(g.get("Q")),
-            _ptr(g.get("pol")), _ptr(g.get("thole")), ctypes.byref(stats))
+            _ptr(g.get("pol")), _ptr(g.get("thole")), ctypes.byref(st_out) if stats else None)
         L.check(self.h, st, "admp_compute")
+        if not stats:
+            return E, g, U, None
+        stats = st_out
         info = {
             "n_iter": list(stats.n_iter[:F]), "converged": [bool(x) for x in stats.converged[:F]],
             "residual": list(stats.residual[:F]), "e_real": stats.e_real, "e_recip": stats.e_recip,

@@ -52,6 +52,7 @@ struct admp_ctx {
     int* d_geom = nullptr;              // geometry flags of the last upload_geometry
     double* h_comp = nullptr;           // pinned energy components (3 F real/self/pol, 7 F reciprocal)
     hipEvent_t ev_nbr = nullptr;
+    hipEvent_t ev_geom = nullptr;      // after the compute's geometry flags were copied to h_geom
     hipEvent_t ev_state[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
@@ -452,6 +453,7 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     if (ctx->h_geom) (void)hipHostFree(ctx->h_geom);
     if (ctx->h_comp) (void)hipHostFree(ctx->h_comp);
     if (ctx->ev_nbr) (void)hipEventDestroy(ctx->ev_nbr);
+    if (ctx->ev_geom) (void)hipEventDestroy(ctx->ev_geom);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
@@ -501,6 +503,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         hipHostMalloc((void**)&ctx->h_geom, sizeof(int), 0) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_comp, 10 * sizeof(double) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_nbr, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_geom, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess) {
@@ -1810,8 +1813,10 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         HIPCHK(hipMemsetAsync(ctx->d_vir_rec, 0, NF * 9 * sizeof(double), st));
         HIPCHK(hipMemsetAsync(ctx->d_rec, 0, F * 7 * sizeof(double), st));
     }
+    // the self terms, and (with gradients) the reciprocal dE/dQ_global added to the real one
     hipLaunchKernelGGL(k_self, dim3(nblk(N, 256), F), dim3(256), 0, st, N, kappa, lpme, lpol, ctx->d_Qc, U, pol,
-                       ctx->d_e_self, ctx->d_e_pol, ctx->d_gq, lpol ? ctx->d_gpol : nullptr);
+                       ctx->d_e_self, ctx->d_e_pol, ctx->d_gq, lpol ? ctx->d_gpol : nullptr,
+                       want_grad ? ctx->d_gq_rec : (const double*)nullptr);
     // per-frame energy components: real, self, polarisation penalty -> d_red [0..F), [F..2F), [2F..3F)
     hipLaunchKernelGGL(k_reduce3, dim3(3, F), dim3(RED_BLOCK), 0, st, N,
                        Reduce3{{ctx->d_e_atom, ctx->d_e_self, ctx->d_e_pol}}, ctx->d_red);
@@ -1820,19 +1825,15 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         hipLaunchKernelGGL(k_energy_total, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_red, ctx->d_rec, energy);
     if (want_grad) {
         // dE/dQ_global total -> frames backward
-        hipLaunchKernelGGL(k_add3, dim3(nblk(NF * 10, 256)), dim3(256), 0, st, NF * 10, ctx->d_gq, ctx->d_gq_rec,
-                           (const double*)nullptr, ctx->d_gq);
         { ProfScope ps_(ctx, 9); hipLaunchKernelGGL(k_frames_bwd, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->nh, ctx->use_frames, ctx->d_pos,
                            ctx->d_box, ctx->d_axis_type, ctx->d_axis_idx, Q_local, ctx->d_gq, ctx->d_gQl,
                            ctx->d_contrib, ctx->d_vir_fr); }
-        hipLaunchKernelGGL(k_add3, dim3(nblk(NF * 3, 256)), dim3(256), 0, st, NF * 3, ctx->d_gpos, ctx->d_gpos_rec,
-                           (const double*)nullptr, ctx->d_gpos);
+        // forces: real + reciprocal + the frames' contributions
         hipLaunchKernelGGL(k_frames_gather, dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->d_rev_off, ctx->d_rev_ent,
-                           ctx->d_contrib, ctx->d_gpos);
+                           ctx->d_contrib, ctx->d_gpos_rec, ctx->d_gpos);
         // box: direct terms (real pairs + frames), reciprocal A-terms and C_k terms
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9a, 0);
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_fr, ctx->d_red9a, 1);
-        hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_rec, ctx->d_red9b, 0);
+        hipLaunchKernelGGL(k_reduce_vir, dim3(9, F, 2), dim3(RED_BLOCK), 0, st, N, ctx->d_vir_atom, ctx->d_vir_fr,
+                           ctx->d_vir_rec, ctx->d_red9a, ctx->d_red9b);
         if (dbox)
             hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9a,
                                lpme ? ctx->d_red9b : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr,
@@ -1847,11 +1848,24 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         if (nmax) hipLaunchKernelGGL(k_copy4, dim3(nblk(nmax, 256), 4), dim3(256), 0, st, cj);
         HIPCHK(hipGetLastError());
     }
-    // energy components for the stats: one pinned copy after everything is queued
-    HIPCHK(hipMemcpyAsync(ctx->h_comp, ctx->d_red, 3 * F * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(ctx->h_comp + 3 * F, ctx->d_rec, 7 * F * sizeof(double), hipMemcpyDeviceToHost, st));
-    if ((s = geom_fetch(ctx)) != ADMP_OK) return s;
-    HIPCHK(hipStreamSynchronize(st));
+    // Without stats the call returns once everything is queued, provided the
+    // geometry flags have already reached the host (the solve's convergence
+    // reads waited on the device after they were copied); otherwise, and with
+    // stats (energy components: one pinned copy after everything is queued),
+    // it waits for the stream.
+    bool geom_known = false;
+    if (!stats) {
+        const hipError_t q = hipEventQuery(ctx->ev_geom);
+        if (q == hipSuccess) geom_known = true;
+        else if (q != hipErrorNotReady) HIPCHK(q);
+    }
+    if (!geom_known) {
+        if (stats) {
+            HIPCHK(hipMemcpyAsync(ctx->h_comp, ctx->d_red, 3 * F * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(ctx->h_comp + 3 * F, ctx->d_rec, 7 * F * sizeof(double), hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    }
     if ((s = geom_check(ctx)) != ADMP_OK) return s;
     const double* comp = ctx->h_comp;
     const double* rec = ctx->h_comp + 3 * F;
@@ -1898,6 +1912,10 @@ static admp_status admp_compute_impl(admp_ctx* ctx, const double* pos, const dou
     if (n_scales < 1 || n_scales > 8) { ctx->err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
     admp_status s = upload_geometry(ctx, pos, box);
     if (s != ADMP_OK) return s;
+    // the geometry flags go to the host early: a call whose solve already waited
+    // on the device checks them without a final synchronisation
+    if ((s = geom_fetch(ctx)) != ADMP_OK) return s;
+    HIPCHK(hipEventRecord(ctx->ev_geom, ctx->stream));
     PairParams pp = make_pp(mScales, pScales, dScales, n_scales, ctx->cfg.lpme ? ctx->cfg.kappa : 0.0);
     if (ctx->cfg.precision == ADMP_MIXED)
         return compute_impl<float>(ctx, Q_local, pol, thole, pp, U_inout, opts, want_grad, energy, dpos, dbox, dQ_local,

@@ -337,8 +337,11 @@ __global__ void k_frames_bwd(int N, int nh, int use_frames, const double4* __res
 
 // dpos[j] += own contribution + contributions of sites whose frame uses j
 // (rev_off/rev_ent: static reverse map, entry = site*4 + slot).
+// gpos = (gpos + add) + the frame contributions of the sites that use atom j
+// (add: e.g. the reciprocal part, may be null)
 __global__ void k_frames_gather(int N, const int* __restrict__ rev_off, const int* __restrict__ rev_ent,
-                                const double* __restrict__ contrib, double* __restrict__ gpos) {
+                                const double* __restrict__ contrib, const double* __restrict__ add,
+                                double* __restrict__ gpos) {
     const int f = blockIdx.y;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
@@ -350,9 +353,17 @@ __global__ void k_frames_gather(int N, const int* __restrict__ rev_off, const in
         const double* c = contrib + (fo + site) * 12 + slot * 3;
         s0 += c[0]; s1 += c[1]; s2 += c[2];
     }
-    gpos[(fo + j) * 3 + 0] += s0;
-    gpos[(fo + j) * 3 + 1] += s1;
-    gpos[(fo + j) * 3 + 2] += s2;
+    double* g = gpos + (fo + j) * 3;
+    if (add) {
+        const double* ad = add + (fo + j) * 3;
+        g[0] = (g[0] + ad[0]) + s0;
+        g[1] = (g[1] + ad[1]) + s1;
+        g[2] = (g[2] + ad[2]) + s2;
+    } else {
+        g[0] += s0;
+        g[1] += s1;
+        g[2] += s2;
+    }
 }
 
 // ----------------------------------------------------------------- site packing
@@ -383,9 +394,11 @@ template <class R> __global__ void k_cast(size_t n, const double* __restrict__ i
 
 // Self energy (pme.py:1183-1222) on Q + U and polarisation penalty (pme.py:1225-1240):
 // per-site energy, and their gradients added into gQc / gpol.
+// gq_add (may be null): added to gQc after the self terms (the reciprocal part)
 __global__ void k_self(int N, double kappa, int lpme, int lpol, const double* __restrict__ Qc,
                        const double* __restrict__ U, const double* __restrict__ pol, double* __restrict__ e_self,
-                       double* __restrict__ e_pol, double* __restrict__ gQc, double* __restrict__ gpol) {
+                       double* __restrict__ e_pol, double* __restrict__ gQc, double* __restrict__ gpol,
+                       const double* __restrict__ gq_add) {
     const int f = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -405,6 +418,11 @@ __global__ void k_self(int N, double kappa, int lpme, int lpol, const double* __
         g[0] += -2.0 * DIELECTRIC * f0 * c[0];
         for (int k = 0; k < 3; ++k) g[1 + k] += -2.0 * DIELECTRIC * f1 * mu[k];
         for (int k = 0; k < 6; ++k) g[4 + k] += -(4.0 / 3.0) * DIELECTRIC * f2 * c[4 + k];
+    }
+    if (gq_add) {
+        double* g = gQc + a * 10;
+        const double* ad = gq_add + a * 10;
+        for (int k = 0; k < 10; ++k) g[k] = g[k] + ad[k];
     }
     e_self[a] = es;
     if (lpol) {
@@ -443,6 +461,41 @@ __global__ void __launch_bounds__(RED_BLOCK) k_reduce_cols(int N, int ncol, cons
     if (threadIdx.x == 0) {
         if (accumulate) out[(size_t)f * ncol + c] += s;
         else out[(size_t)f * ncol + c] = s;
+    }
+}
+
+// the virial sums of the final pass in one launch: block (c, f, 0) writes
+// outa = sum(in_a col c) + sum(in_b col c), block (c, f, 1) outb = sum(in_c col c)
+// (each sum in the order of k_reduce_cols)
+__device__ __forceinline__ double reduce_col(int N, int ncol, const double* __restrict__ col, double* sh) {
+    const int st = blockDim.x;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int i = threadIdx.x;
+    for (; i + 3 * st < N; i += 4 * st) {
+        s0 += col[(size_t)i * ncol];
+        s1 += col[(size_t)(i + st) * ncol];
+        s2 += col[(size_t)(i + 2 * st) * ncol];
+        s3 += col[(size_t)(i + 3 * st) * ncol];
+    }
+    for (; i < N; i += st) s0 += col[(size_t)i * ncol];
+    double s = (s0 + s1) + (s2 + s3);
+    return block_sum(s, sh);
+}
+__global__ void __launch_bounds__(RED_BLOCK) k_reduce_vir(int N, const double* __restrict__ in_a,
+                                                          const double* __restrict__ in_b,
+                                                          const double* __restrict__ in_c, double* __restrict__ outa,
+                                                          double* __restrict__ outb) {
+    __shared__ double sh[16];
+    const int c = blockIdx.x, f = blockIdx.y;
+    const size_t off = (size_t)f * N * 9 + c;
+    if (blockIdx.z == 0) {
+        const double sa = reduce_col(N, 9, in_a + off, sh);
+        __syncthreads();
+        const double sb = reduce_col(N, 9, in_b + off, sh);
+        if (threadIdx.x == 0) outa[(size_t)f * 9 + c] = sa + sb;
+    } else {
+        const double sc = reduce_col(N, 9, in_c + off, sh);
+        if (threadIdx.x == 0) outb[(size_t)f * 9 + c] = sc;
     }
 }
 

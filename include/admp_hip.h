@@ -164,7 +164,11 @@ admp_status admp_get_pairs(admp_ctx* ctx, int32_t frame, int32_t* out, int64_t c
  *   energy (F,) out.  With want_grad: dpos (F,N,3), dbox (F,3,3), dQ_local (F,N,9),
  *   dpol (F,N), dthole (F,N) out (any may be NULL to skip writing it).
  *   Gradients hold U fixed (Hellmann-Feynman, pme.py:364-371).
- *   Synchronises the stream once at the end (solver statistics). */
+ *   With stats: synchronises the stream once at the end (solver statistics,
+ *   energy components).  stats NULL: returns as soon as the work is queued
+ *   when the call has already waited on the device after its geometry check
+ *   was copied back (a polarisable CG solve does, for its convergence reads);
+ *   the outputs are ordered on the stream as always. */
 admp_status admp_compute(admp_ctx* ctx, const double* pos, const double* box,
                          const double* Q_local, const double* pol, const double* thole,
                          const double* mScales, const double* pScales, const double* dScales,

@@ -83,7 +83,7 @@ template <class A, class B> inline auto max(A a, B b) -> typename std::common_ty
 
 // ------------------------------------------------------------------ runtime
 typedef int hipError_t;
-enum { hipSuccess = 0, hipErrorUnknown = 999, hipErrorNotSupported = 801, hipErrorInvalidValue = 1 };
+enum { hipSuccess = 0, hipErrorUnknown = 999, hipErrorNotSupported = 801, hipErrorInvalidValue = 1, hipErrorNotReady = 600 };
 typedef struct emu_stream* hipStream_t;
 typedef struct emu_event* hipEvent_t;
 typedef struct emu_graph* hipGraph_t;
@@ -109,6 +109,7 @@ hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned f);
 hipError_t hipEventCreate(hipEvent_t* e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned f);

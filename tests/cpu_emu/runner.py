@@ -264,6 +264,7 @@ def bad_geometry(precision):
     nanbox[1, 1] = np.inf
     kw = dict(solver="pcg", tol=1e-8, maxiter=200, want_grad=True)
     attempt("compute_nan_pos", lambda: eng.compute(bad_nan, box, *args, **kw))
+    attempt("compute_nan_pos_nostats", lambda: eng.compute(bad_nan, box, *args, stats=False, **kw))
     attempt("compute_far_pos", lambda: eng.compute(bad_far, box, *args, **kw))
     attempt("compute_flat_box", lambda: eng.compute(pos, flat, *args, **kw))
     attempt("compute_inf_box", lambda: eng.compute(pos, nanbox, *args, **kw))

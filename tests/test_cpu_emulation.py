@@ -208,7 +208,7 @@ def test_small_box_other_spread_variants(emu_lib, variant):
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_bad_geometry_is_an_argument_error(emu_lib, prec):
     r = run(emu_lib, "bad_geometry", prec)
-    for k in ("compute_nan_pos", "compute_far_pos", "slater_nan_pos", "after_bad_build"):
+    for k in ("compute_nan_pos", "compute_nan_pos_nostats", "compute_far_pos", "slater_nan_pos", "after_bad_build"):
         assert "status 1" in r[k] and "positions" in r[k], (k, r[k])
     for k in ("compute_flat_box", "compute_inf_box", "build_flat_box"):
         assert "status 1" in r[k] and "box" in r[k], (k, r[k])

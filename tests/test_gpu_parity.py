@@ -411,6 +411,36 @@ def test_small_box_diagonal_preconditioner_vs_oracle():
     assert np.abs(U[0].cpu().numpy() - o["U"]).max() < 1e-8
 
 
+def test_compute_without_stats_same_results_and_checks():
+    """admp_compute without solver statistics (the form that need not wait for
+    the device at its end): bit-identical energy, gradients and U to the call with
+    them, and a non-finite position is still an argument error."""
+    import dmff_amd._lib as L
+    from dmff_amd.admp.engine import Engine
+    s, pos, box = _small_waters()
+    pairs = reference_pairs(pos, box, 6.0, s.bonded)
+    kappa, K1, K2, K3 = O.setup_ewald_parameters(6.0, 5e-4, box)
+    eng = Engine(len(pos), 2, True, True, kappa, (K1, K2, K3), 6.0, device=DEV)
+    eng.set_topology(s.axis_types, s.axis_idx, s.bonded)
+    eng.set_pairs(torch.as_tensor(pairs, device=DEV))
+    args = (s.pme.Q_local[s.map_atomtype], s.pme.pol[s.map_poltype], s.pme.thole[s.map_poltype],
+            s.pme.mScales, s.pme.pScales, s.pme.dScales)
+    kw = dict(solver="pcg", tol=1e-9, maxiter=500, want_grad=True)
+    E1, g1, U1, info = eng.compute(pos, box, *args, **kw)
+    E2, g2, U2, none = eng.compute(pos, box, *args, stats=False, **kw)
+    torch.cuda.synchronize()
+    assert none is None and info["n_iter"][0] > 0
+    assert torch.equal(E1, E2) and torch.equal(U1, U2)
+    for k in g1:
+        assert torch.equal(g1[k], g2[k]), k
+    bad = pos.copy()
+    bad[5, 1] = np.nan
+    with pytest.raises(L.ADMPError, match="positions"):
+        eng.compute(bad, box, *args, stats=False, **kw)
+    E3, _, _, _ = eng.compute(pos, box, *args, stats=False, **kw)
+    assert torch.equal(E1, E3)
+
+
 @pytest.mark.parametrize("fp32_products", [False, True])
 def test_mixed_batched_frames_converge(fp32_products):
     """Mixed precision on a batch: frames converge at different iterations (and,
