@@ -1832,11 +1832,14 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         hipLaunchKernelGGL(k_frames_gather, dim3(nblk(N, 256), F), dim3(256), 0, st, N, ctx->d_rev_off, ctx->d_rev_ent,
                            ctx->d_contrib, ctx->d_gpos_rec, ctx->d_gpos);
         // box: direct terms (real pairs + frames), reciprocal A-terms and C_k terms
-        hipLaunchKernelGGL(k_reduce_vir, dim3(9, F, 2), dim3(RED_BLOCK), 0, st, N, ctx->d_vir_atom, ctx->d_vir_fr,
-                           ctx->d_vir_rec, ctx->d_red9a, ctx->d_red9b);
+        // (real pairs and frames are summed separately and added in k_box_grad, in
+        // that order)
+        hipLaunchKernelGGL(k_reduce_vir, dim3(9, F, 3), dim3(RED_BLOCK), 0, st, N,
+                           Vir3{{ctx->d_vir_atom, ctx->d_vir_fr, ctx->d_vir_rec},
+                                {ctx->d_red9a, ctx->d_red9c, ctx->d_red9b}});
         if (dbox)
             hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9a,
-                               lpme ? ctx->d_red9b : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr,
+                               ctx->d_red9c, lpme ? ctx->d_red9b : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr,
                                c.K1, c.K2, c.K3, 0, dbox, 0);
         // the four gradient outputs in one launch
         CopyJobs cj{};
@@ -1981,6 +1984,7 @@ static admp_status disp_impl(admp_ctx* ctx, const double* c_list, const PairPara
         hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9b, 0);
         if (dbox)
             hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9b,
+                               (const double*)nullptr,
                                lpme ? ctx->d_red9c : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr, c.dK1,
                                c.dK2, c.dK3, 1, dbox, 0);
         if (dpos) HIPCHK(hipMemcpyAsync(dpos, ctx->d_gpos, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -2044,6 +2048,7 @@ static admp_status pairwise_impl(admp_ctx* ctx, int kind, const double* prm, con
         hipLaunchKernelGGL(k_reduce_cols, dim3(9, F), dim3(RED_BLOCK), 0, st, N, 9, ctx->d_vir_atom, ctx->d_red9b, 0);
         if (dbox)
             hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9b,
+                               (const double*)nullptr,
                                (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, 1, 1, 1, 0, dbox, 0);
         if (dpos) HIPCHK(hipMemcpyAsync(dpos, ctx->d_gpos, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
         if (dprm)

@@ -60,6 +60,27 @@ struct CellGrid {
     int ncell;
 };
 
+// atomicAdd(&ctr[key], 1) for every active lane of the wavefront with one atomic
+// per distinct key (consecutive atoms mostly share a cell, so same-address
+// atomics would serialise); returns the lane's old value: the key's base plus
+// its rank among the lanes with that key.  Blocks are whole wavefronts.
+__device__ __forceinline__ int wave_agg_inc(int* ctr, int key) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(1);
+    int res = 0;
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lkey = __shfl(key, leader);
+        const unsigned long long same = __ballot(key == lkey) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lkey], __popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) res = base + __popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return res;
+}
+
 __global__ void k_cell_assign(int N, const double4* __restrict__ pos, const BoxInfo* __restrict__ boxes, CellGrid cg,
                               int* __restrict__ cell, int* __restrict__ ccount) {
     const int f = blockIdx.y;
@@ -78,7 +99,7 @@ __global__ void k_cell_assign(int N, const double4* __restrict__ pos, const BoxI
     }
     const int id = (c[0] * cg.nc[1] + c[1]) * cg.nc[2] + c[2];
     cell[(size_t)f * N + i] = id;
-    atomicAdd(&ccount[(size_t)f * cg.ncell + id], 1);
+    wave_agg_inc(ccount + (size_t)f * cg.ncell, id);
 }
 
 // exclusive scan of ncell counts per frame (single block per frame)
@@ -203,7 +224,7 @@ __global__ void k_cell_fill(int N, int ncell, const int* __restrict__ cell, cons
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int id = cell[(size_t)f * N + i];
-    const int slot = atomicAdd(&cfill[(size_t)f * ncell + id], 1);
+    const int slot = wave_agg_inc(cfill + (size_t)f * ncell, id);
     catoms[(size_t)f * N + cstart[(size_t)f * (ncell + 1) + id] + slot] = i;
 }
 

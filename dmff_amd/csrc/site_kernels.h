@@ -464,9 +464,9 @@ __global__ void __launch_bounds__(RED_BLOCK) k_reduce_cols(int N, int ncol, cons
     }
 }
 
-// the virial sums of the final pass in one launch: block (c, f, 0) writes
-// outa = sum(in_a col c) + sum(in_b col c), block (c, f, 1) outb = sum(in_c col c)
-// (each sum in the order of k_reduce_cols)
+// the virial column sums of the final pass in one launch: block (c, f, z) writes
+// the sum of column c of frame f of in_z into out_z (each in the order of
+// k_reduce_cols)
 __device__ __forceinline__ double reduce_col(int N, int ncol, const double* __restrict__ col, double* sh) {
     const int st = blockDim.x;
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -481,22 +481,15 @@ __device__ __forceinline__ double reduce_col(int N, int ncol, const double* __re
     double s = (s0 + s1) + (s2 + s3);
     return block_sum(s, sh);
 }
-__global__ void __launch_bounds__(RED_BLOCK) k_reduce_vir(int N, const double* __restrict__ in_a,
-                                                          const double* __restrict__ in_b,
-                                                          const double* __restrict__ in_c, double* __restrict__ outa,
-                                                          double* __restrict__ outb) {
+struct Vir3 {
+    const double* in[3];
+    double* out[3];
+};
+__global__ void __launch_bounds__(RED_BLOCK) k_reduce_vir(int N, Vir3 v) {
     __shared__ double sh[16];
-    const int c = blockIdx.x, f = blockIdx.y;
-    const size_t off = (size_t)f * N * 9 + c;
-    if (blockIdx.z == 0) {
-        const double sa = reduce_col(N, 9, in_a + off, sh);
-        __syncthreads();
-        const double sb = reduce_col(N, 9, in_b + off, sh);
-        if (threadIdx.x == 0) outa[(size_t)f * 9 + c] = sa + sb;
-    } else {
-        const double sc = reduce_col(N, 9, in_c + off, sh);
-        if (threadIdx.x == 0) outb[(size_t)f * 9 + c] = sc;
-    }
+    const int c = blockIdx.x, f = blockIdx.y, z = blockIdx.z;
+    const double s = reduce_col(N, 9, v.in[z] + (size_t)f * N * 9 + c, sh);
+    if (threadIdx.x == 0) v.out[z][(size_t)f * 9 + c] = s;
 }
 
 // three single-column reductions in one launch: block (k, f) sums in[k] of frame
@@ -605,11 +598,12 @@ __global__ void k_sub(size_t n, const double* __restrict__ b, double* __restrict
     if (i < n) x[i] -= b[i];
 }
 
-// dE/dbox assembly: vir (F,9) holds the real+frame+A-term sums (A term as dE/dA);
+// dE/dbox assembly: vir_direct (+ vir_direct2, may be null) (F,9) hold the direct
+// sums (real pairs, frames), dEdA the reciprocal A term;
 // here: dbox = vir_direct - Binv^T H Binv^T + (-E_rec Binv^T + 2 Binv^T W)
 // H = (diag(K) dEdA)^T.
 __global__ void k_box_grad(int F, const BoxInfo* __restrict__ boxes, const double* __restrict__ vir_direct,
-                           const double* __restrict__ dEdA, const double* __restrict__ erec,
+                           const double* __restrict__ vir_direct2, const double* __restrict__ dEdA, const double* __restrict__ erec,
                            const double* __restrict__ W6, int K0, int K1, int K2, int use_dA, double* __restrict__ dbox,
                            int accumulate) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -620,7 +614,10 @@ __global__ void k_box_grad(int F, const BoxInfo* __restrict__ boxes, const doubl
         for (int q = 0; q < 3; ++q) BiT[p][q] = b.Binv[q * 3 + p];
     double out[3][3];
     for (int p = 0; p < 3; ++p)
-        for (int q = 0; q < 3; ++q) out[p][q] = vir_direct ? vir_direct[f * 9 + p * 3 + q] : 0.0;
+        for (int q = 0; q < 3; ++q) {
+            out[p][q] = vir_direct ? vir_direct[f * 9 + p * 3 + q] : 0.0;
+            if (vir_direct2) out[p][q] += vir_direct2[f * 9 + p * 3 + q];
+        }
     if (dEdA) {
         const int K[3] = {K0, K1, K2};
         double H[3][3];  // H[c][j] = dEdA[j][c] K_j

@@ -74,6 +74,7 @@ template <class T> inline T __shfl_up(T v, unsigned delta, int width = 64) {
 inline unsigned long long __ballot(int pred) { return (unsigned long long)emu::wave_op(3, pred ? 1 : 0, 0, 64); }
 inline int __builtin_amdgcn_readlane(int v, int lane) { return (int)emu::wave_op(4, v, lane, 64); }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 template <class T> inline T atomicAdd(T* p, T v) { T o = *p; *p = o + v; return o; }
 inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
 inline int atomicExch(int* p, int v) { int o = *p; *p = v; return o; }
