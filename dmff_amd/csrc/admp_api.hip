@@ -62,6 +62,10 @@ struct admp_ctx {
     std::vector<char> frame_nbr;  // per frame: its pair list is in d_nbr / d_ncnt
     bool nbr_built = false;       // lists (and d_nhalf) come from admp_build_neighbors
     CellGrid nbr_cg{};            // cell grid of the last list build
+    bool cg_host = false;         // nbr_cg was sized on the host from a box read back
+    bool cg_spec = false;         // the pending build took nbr_cg over unchecked (k_cell_grid_check)
+    int* d_cgflag = nullptr;      // per frame: the box needs another cell grid
+    int* h_cgflag = nullptr;      // pinned copy
     int use_frames = 0;
     // topology
     int *d_axis_type = nullptr, *d_axis_idx = nullptr, *d_xoff = nullptr, *d_xent = nullptr;
@@ -451,6 +455,7 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     if (ctx->h_pcg) (void)hipHostFree(ctx->h_pcg);
     if (ctx->h_maxcnt) (void)hipHostFree(ctx->h_maxcnt);
     if (ctx->h_geom) (void)hipHostFree(ctx->h_geom);
+    if (ctx->h_cgflag) (void)hipHostFree(ctx->h_cgflag);
     if (ctx->h_comp) (void)hipHostFree(ctx->h_comp);
     if (ctx->ev_nbr) (void)hipEventDestroy(ctx->ev_nbr);
     if (ctx->ev_geom) (void)hipEventDestroy(ctx->ev_geom);
@@ -501,6 +506,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         hipHostMalloc((void**)&ctx->h_pcg, 2 * sizeof(PcgState) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_maxcnt, sizeof(int), 0) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_geom, sizeof(int), 0) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_cgflag, sizeof(int) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_comp, 10 * sizeof(double) * (size_t)std::max(1, c.n_frames), 0) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_nbr, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_geom, hipEventDisableTiming) != hipSuccess ||
@@ -596,6 +602,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_active, ctx->F);
     ALLOC(ctx->d_maxcnt, 1);
     ALLOC(ctx->d_geom, 1);
+    ALLOC(ctx->d_cgflag, ctx->F);
     ALLOC(ctx->d_ncnt, NF);
     ALLOC(ctx->d_nhalf, NF);
     ALLOC(ctx->d_cell, NF);
@@ -838,11 +845,34 @@ static admp_status launch_nbr_build(admp_ctx* ctx) {
 // the cell tables it used: when a row overflowed the capacity, grow it and build
 // again from the same geometry (this one waits).  Every entry point that reads
 // or replaces them calls this first.
+static CellGrid host_cell_grid(const admp_ctx* ctx, const std::vector<double>& hb);
+static admp_status launch_cell_sort(admp_ctx* ctx, const CellGrid& cg);
+
 static admp_status nbr_settle(admp_ctx* ctx) {
     if (!ctx->nbr_pending) return ADMP_OK;
     ctx->nbr_pending = false;
     HIPCHK(hipEventSynchronize(ctx->ev_nbr));
     { const admp_status g = geom_check(ctx); if (g != ADMP_OK) return g; }
+    if (ctx->cg_spec) {
+        ctx->cg_spec = false;
+        bool regrid = false;
+        for (int f = 0; f < ctx->F; ++f) regrid = regrid || ctx->h_cgflag[f] != 0;
+        if (regrid) {
+            // a box outgrew the cell grid taken over from the earlier build: size
+            // it from the uploaded boxes and build again (this one waits)
+            std::vector<BoxInfo> bi(ctx->F);
+            HIPCHK(hipMemcpyAsync(bi.data(), ctx->d_box, ctx->F * sizeof(BoxInfo), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            std::vector<double> hb(9 * (size_t)ctx->F);
+            for (int f = 0; f < ctx->F; ++f)
+                for (int k = 0; k < 9; ++k) hb[9 * (size_t)f + k] = bi[f].B[k];
+            admp_status s = launch_cell_sort(ctx, host_cell_grid(ctx, hb));
+            if (s != ADMP_OK) return s;
+            if ((s = launch_nbr_build(ctx)) != ADMP_OK) return s;
+            ctx->nbr_pending = false;
+            HIPCHK(hipEventSynchronize(ctx->ev_nbr));
+        }
+    }
     const int mx = *ctx->h_maxcnt;
     if (mx <= ctx->cap) return ADMP_OK;
     admp_status s = ensure_cap(ctx, mx);
@@ -854,27 +884,13 @@ static admp_status nbr_settle(admp_ctx* ctx) {
     return ADMP_OK;
 }
 
-static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, const double* box) {
-    if (!ctx || !pos || !box) return ADMP_ERR_ARG;
-    if (!ctx->has_topo) { ctx->err = "admp_set_topology first"; return ADMP_ERR_STATE; }
-    const int N = ctx->N, F = ctx->F;
-    admp_status s = nbr_settle(ctx);
-    if (s != ADMP_OK) return s;
-    s = upload_geometry(ctx, pos, box);
-    if (s != ADMP_OK) return s;
-    // cell grid from frame boxes (host copy)
-    std::vector<double> hb(9 * F);
-    HIPCHK(hipMemcpyAsync(hb.data(), box, 9 * F * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (int f = 0; f < F; ++f)
-        if (!box_usable(hb.data() + 9 * f)) {
-            ctx->err = "box not finite or degenerate (each height must be >= 0.5 A)";
-            return ADMP_ERR_ARG;
-        }
+// cell grid for the frames' boxes hb (9 per frame, host): the most cells per
+// axis whose slabs stay at least rc wide in every frame
+static CellGrid host_cell_grid(const admp_ctx* ctx, const std::vector<double>& hb) {
     const double rc = ctx->cfg.rc;
     CellGrid cg;
     cg.nc[0] = cg.nc[1] = cg.nc[2] = 1 << 20;
-    for (int f = 0; f < F; ++f) {
+    for (int f = 0; f < ctx->F; ++f) {
         const double* B = hb.data() + 9 * f;
         // width of the cell slab along reciprocal direction d: 1/|Binv column d|
         const double det = B[0] * (B[4] * B[8] - B[5] * B[7]) - B[1] * (B[3] * B[8] - B[5] * B[6]) +
@@ -894,7 +910,15 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
         }
     }
     cg.ncell = cg.nc[0] * cg.nc[1] * cg.nc[2];
+    return cg;
+}
+
+// cell sort of the uploaded positions on grid cg (stored as nbr_cg)
+static admp_status launch_cell_sort(admp_ctx* ctx, const CellGrid& cg) {
+    const int N = ctx->N, F = ctx->F;
     if (cg.ncell > ctx->ncell_alloc) {
+        // the tables may still be read by queued work of an earlier build
+        HIPCHK(hipStreamSynchronize(ctx->stream));
         dfree(ctx, ctx->d_ccount); dfree(ctx, ctx->d_cstart); dfree(ctx, ctx->d_cfill);
         HIPCHK(dalloc(ctx, &ctx->d_ccount, (size_t)cg.ncell * F));
         HIPCHK(dalloc(ctx, &ctx->d_cstart, (size_t)(cg.ncell + 1) * F));
@@ -905,7 +929,8 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
     HIPCHK(hipMemsetAsync(ctx->d_cfill, 0, (size_t)cg.ncell * F * sizeof(int), ctx->stream));
     hipLaunchKernelGGL(k_cell_assign, dim3(nblk(N, 256), F), dim3(256), 0, ctx->stream, N, ctx->d_pos, ctx->d_box, cg,
                        ctx->d_cell, ctx->d_ccount);
-    if ((s = scan_rows(ctx, ctx->d_ccount, ctx->d_cstart, cg.ncell, F, ctx->stream)) != ADMP_OK) return s;
+    admp_status s = scan_rows(ctx, ctx->d_ccount, ctx->d_cstart, cg.ncell, F, ctx->stream);
+    if (s != ADMP_OK) return s;
     hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256), F), dim3(256), 0, ctx->stream, N, cg.ncell, ctx->d_cell,
                        ctx->d_cstart, ctx->d_cfill, ctx->d_catoms);
     hipLaunchKernelGGL(k_cell_sort_block, dim3(cg.ncell, F), dim3(256), 0, ctx->stream, cg.ncell, N, ctx->d_cstart,
@@ -913,8 +938,46 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
     hipLaunchKernelGGL(k_cell_pos, dim3(nblk(N, 256), F), dim3(256), 0, ctx->stream, N, ctx->d_pos, ctx->d_catoms,
                        ctx->d_spos);
     HIPCHK(hipGetLastError());
+    ctx->nbr_cg = cg;
+    return ADMP_OK;
+}
+
+// The cell grid depends on the boxes, which live on the device.  The first
+// build (and one after a grid change) reads them back and waits; a later build
+// takes the previous grid over without waiting, and k_cell_grid_check flags a
+// box that needs another one: nbr_settle then sizes the grid from the boxes and
+// builds again before anything reads the lists.
+static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, const double* box) {
+    if (!ctx || !pos || !box) return ADMP_ERR_ARG;
+    if (!ctx->has_topo) { ctx->err = "admp_set_topology first"; return ADMP_ERR_STATE; }
+    const int N = ctx->N, F = ctx->F;
+    admp_status s = nbr_settle(ctx);
+    if (s != ADMP_OK) return s;
+    s = upload_geometry(ctx, pos, box);
+    if (s != ADMP_OK) return s;
+    CellGrid cg;
+    std::vector<double> hb;
+    const bool spec = ctx->cg_host && ctx->d_nbr;
+    if (spec) {
+        cg = ctx->nbr_cg;
+        hipLaunchKernelGGL(k_cell_grid_check, dim3(nblk(F, 64)), dim3(64), 0, ctx->stream, F, ctx->d_box,
+                           ctx->cfg.rc, cg, ctx->d_cgflag);
+        HIPCHK(hipMemcpyAsync(ctx->h_cgflag, ctx->d_cgflag, F * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        hb.resize(9 * (size_t)F);
+        HIPCHK(hipMemcpyAsync(hb.data(), box, 9 * F * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (int f = 0; f < F; ++f)
+            if (!box_usable(hb.data() + 9 * f)) {
+                ctx->err = "box not finite or degenerate (each height must be >= 0.5 A)";
+                return ADMP_ERR_ARG;
+            }
+        cg = host_cell_grid(ctx, hb);
+    }
+    if ((s = launch_cell_sort(ctx, cg)) != ADMP_OK) return s;
     if (!ctx->d_nbr) {
         // first estimate from density: 1.5 x expected full-list count + margin
+        const double rc = ctx->cfg.rc;
         double vol = 0;
         {
             const double* B = hb.data();
@@ -926,7 +989,8 @@ static admp_status admp_build_neighbors_impl(admp_ctx* ctx, const double* pos, c
         est = std::min(est, N);
         if ((s = ensure_cap(ctx, std::max(est, 1))) != ADMP_OK) return s;
     }
-    ctx->nbr_cg = cg;
+    ctx->cg_host = true;
+    ctx->cg_spec = spec;
     if ((s = geom_fetch(ctx)) != ADMP_OK) return s;  // read back with the build's longest row
     if ((s = launch_nbr_build(ctx)) != ADMP_OK) return s;
     ctx->frame_nbr.assign(ctx->F, 1);

@@ -60,6 +60,25 @@ struct CellGrid {
     int ncell;
 };
 
+// Validity of a cell grid taken over from an earlier build: every frame's cell
+// slabs must stay at least rc wide (nc[d] <= floor(width_d / rc), width_d =
+// 1 / |column d of Binv|, the rule the host applies when it sizes the grid).
+// flag[f] = 1 where frame f's box needs another grid.
+__global__ void k_cell_grid_check(int F, const BoxInfo* __restrict__ boxes, double rc, CellGrid cg,
+                                  int* __restrict__ flag) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const double* Bi = boxes[f].Binv;
+    int bad = 0;
+    for (int d = 0; d < 3; ++d) {
+        const double n2 = Bi[d] * Bi[d] + Bi[3 + d] * Bi[3 + d] + Bi[6 + d] * Bi[6 + d];
+        const double width = 1.0 / sqrt(n2);
+        const double n = floor(width / rc);
+        if (!(n >= (double)cg.nc[d])) bad = 1;
+    }
+    flag[f] = bad;
+}
+
 // atomicAdd(&ctr[key], 1) for every active lane of the wavefront with one atomic
 // per distinct key (consecutive atoms mostly share a cell, so same-address
 // atomics would serialise); returns the lane's old value: the key's base plus

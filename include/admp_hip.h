@@ -148,7 +148,11 @@ admp_status admp_set_pairs(admp_ctx* ctx, int32_t frame, const void* pairs, int6
 
 /* Cell-list neighbour build on the GPU for all frames: the pair set
  * {i<j : min-image |r_i - r_j| < rc} (nblist.py:27-55), distances in FP64 with
- * a fixed, unfused operation order.  pos (F,N,3), box (F,3,3).  Synchronises. */
+ * a fixed, unfused operation order.  pos (F,N,3), box (F,3,3).  The first
+ * build reads the boxes back to size the cell grid and synchronises; later
+ * builds take that grid over without waiting (a box it no longer fits is
+ * rebuilt on a fresh grid by the next call that reads the lists, which also
+ * reports a bad box or bad positions). */
 admp_status admp_build_neighbors(admp_ctx* ctx, const double* pos, const double* box);
 
 /* Write the i<j pairs of `frame` in the reference format (int32 (n,3) [i,j,nbond])

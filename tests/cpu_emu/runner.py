@@ -159,6 +159,14 @@ def neighbors(precision):
         e.set_topology(None, None, {})
         e.build_neighbors(p, b)
         out[name] = e.get_pairs(0).numpy().astype(np.int64).tolist()
+    # one context, successive builds: the second takes the first cell grid over,
+    # the shrunken box needs a coarser one (rebuilt at the next read), the
+    # grown box keeps it (still valid)
+    e = Engine(len(pos), 0, False, False, 0.0, (0, 0, 0), 4.0)
+    e.set_topology(None, None, s.bonded)
+    for name, sc in (("seq_1.0", 1.0), ("seq_1.0b", 1.0), ("seq_0.7", 0.7), ("seq_1.3", 1.3)):
+        e.build_neighbors(pos * sc, box * sc)
+        out[name] = e.get_pairs(0).numpy().astype(np.int64).tolist()
     return out
 
 

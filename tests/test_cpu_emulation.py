@@ -135,6 +135,10 @@ def test_neighbour_lists_bit_exact(emu_lib):
         ref = sorted_pairs(reference_pairs(p, b, rc, {}))
         got = sorted_pairs(r[name])
         assert len(ref) > 0 and ref.shape == got.shape and np.array_equal(ref, got), name
+    for name, sc in (("seq_1.0", 1.0), ("seq_1.0b", 1.0), ("seq_0.7", 0.7), ("seq_1.3", 1.3)):
+        ref = sorted_pairs(reference_pairs(pos * sc, box * sc, 4.0, s.bonded))
+        got = sorted_pairs(r[name])
+        assert len(ref) > 0 and ref.shape == got.shape and np.array_equal(ref, got), name
 
 
 @pytest.mark.parametrize("K,prec", [((104, 104, 104), "fp64"), ((104, 104, 104), "mixed"), ((6, 7, 8), "fp64"),
