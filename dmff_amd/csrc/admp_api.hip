@@ -932,7 +932,7 @@ static admp_status launch_cell_sort(admp_ctx* ctx, const CellGrid& cg) {
     admp_status s = scan_rows(ctx, ctx->d_ccount, ctx->d_cstart, cg.ncell, F, ctx->stream);
     if (s != ADMP_OK) return s;
     hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256), F), dim3(256), 0, ctx->stream, N, cg.ncell, ctx->d_cell,
-                       ctx->d_cstart, ctx->d_cfill, ctx->d_catoms);
+                       ctx->d_cstart, ctx->d_cfill, ctx->d_catoms, 1);
     hipLaunchKernelGGL(k_cell_sort_block, dim3(cg.ncell, F), dim3(256), 0, ctx->stream, cg.ncell, N, ctx->d_cstart,
                        ctx->d_catoms);
     hipLaunchKernelGGL(k_cell_pos, dim3(nblk(N, 256), F), dim3(256), 0, ctx->stream, N, ctx->d_pos, ctx->d_catoms,
@@ -1049,7 +1049,7 @@ static admp_status recip_positions(admp_ctx* ctx, const Grid& g, int use_dA) {
         admp_status s2 = scan_rows(ctx, ctx->d_gccount, ctx->d_gcstart, (int)G, F, st);
         if (s2 != ADMP_OK) return s2;
         hipLaunchKernelGGL(k_cell_fill, dim3(nblk(N, 256), F), dim3(256), 0, st, N, (int)G, ctx->d_gcell,
-                           ctx->d_gcstart, ctx->d_gcfill, ctx->d_gcatoms);
+                           ctx->d_gcstart, ctx->d_gcfill, ctx->d_gcatoms, 0);
         hipLaunchKernelGGL(k_cell_sort, dim3(nblk(G, 256), F), dim3(256), 0, st, (int)G, N, ctx->d_gcstart,
                            ctx->d_gcatoms);
         // per-brick candidate lists for the spreads of this evaluation

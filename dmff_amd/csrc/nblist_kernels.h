@@ -237,13 +237,15 @@ __global__ void __launch_bounds__(1024) k_scan_small(int n, const int* __restric
     if (threadIdx.x == 0) o[n] = carry;
 }
 
+// agg: one atomic per distinct cell per wavefront (for cells holding many
+// consecutive atoms); 0: one per atom (many cells, few atoms each)
 __global__ void k_cell_fill(int N, int ncell, const int* __restrict__ cell, const int* __restrict__ cstart,
-                            int* __restrict__ cfill, int* __restrict__ catoms) {
+                            int* __restrict__ cfill, int* __restrict__ catoms, int agg) {
     const int f = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int id = cell[(size_t)f * N + i];
-    const int slot = wave_agg_inc(cfill + (size_t)f * ncell, id);
+    const int slot = agg ? wave_agg_inc(cfill + (size_t)f * ncell, id) : atomicAdd(&cfill[(size_t)f * ncell + id], 1);
     catoms[(size_t)f * N + cstart[(size_t)f * (ncell + 1) + id] + slot] = i;
 }
 
