@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <initializer_list>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -140,8 +141,19 @@ struct admp_ctx {
         bool pc = true;
     };
     ChunkGraph graphs[4];
-    bool use_graphs = true;   // ADMP_NO_GRAPH=1 runs the chunk launches eagerly
-    double* d_pol = nullptr;  // context-owned copy of the polarisabilities (stable graph argument)
+    // the solver set-up (right-hand side, preconditioner, CG start) and the
+    // final pass, captured on their second use with the same arguments
+    struct PhaseGraph {
+        hipGraphExec_t exec = nullptr;
+        std::string key, seen;
+    };
+    PhaseGraph g_setup, g_final;
+    bool use_graphs = true;   // ADMP_NO_GRAPH=1 runs the chunk and phase launches eagerly
+    double* d_pol = nullptr;  // context-owned copies of the caller's inputs (stable graph arguments):
+    double* d_thole = nullptr;  // polarisabilities, Thole widths, local multipoles
+    double* d_Qloc = nullptr;
+    double* d_eout = nullptr;   // energy per frame and dE/dbox, copied to the caller's outputs
+    double* d_bout = nullptr;
     // per-kernel timing (HIP events around launches on the context stream)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -159,7 +171,12 @@ static void clear_graphs(admp_ctx* ctx) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = admp_ctx::ChunkGraph();
     }
+    for (admp_ctx::PhaseGraph* g : {&ctx->g_setup, &ctx->g_final}) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        *g = admp_ctx::PhaseGraph();
+    }
 }
+
 
 static hipEvent_t ev_get(admp_ctx* ctx) {
     if (!ctx->ev_pool.empty()) {
@@ -215,6 +232,55 @@ static void prof_resolve(admp_ctx* ctx) {
             return ADMP_ERR_HIP;                                                   \
         }                                                                          \
     } while (0)
+
+// Run body() through a captured graph cached under key: eagerly on the first
+// use of a key (its lazy allocations happen then, outside any capture), captured
+// and launched on the second, replayed afterwards.  Eager whenever graphs are
+// off, profiling is on, or capture is unavailable (the caller's stream being
+// captured already, for instance).
+template <class Body>
+static admp_status run_phase(admp_ctx* ctx, admp_ctx::PhaseGraph& pg, const std::string& key, Body body) {
+    if (!ctx->use_graphs || ctx->prof) return body();
+    if (pg.exec && pg.key == key) {
+        HIPCHK(hipGraphLaunch(pg.exec, ctx->stream));
+        return ADMP_OK;
+    }
+    if (pg.seen != key) {
+        pg.seen = key;
+        return body();
+    }
+    if (pg.exec) {
+        (void)hipGraphExecDestroy(pg.exec);
+        pg.exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return body();
+    }
+    const admp_status s = body();
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    hipGraphExec_t exec = nullptr;
+    hipError_t ie = hipErrorUnknown;
+    if (s == ADMP_OK && ce == hipSuccess && graph) ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        // nothing of the capture ran: report the body's error, or run it eagerly
+        (void)hipGetLastError();
+        if (s != ADMP_OK) return s;
+        ctx->err.clear();
+        return body();
+    }
+    pg.exec = exec;
+    pg.key = key;
+    HIPCHK(hipGraphLaunch(pg.exec, ctx->stream));
+    return ADMP_OK;
+}
+
+// key bytes of a value (phase graph keys)
+template <class T> static void key_add(std::string& k, const T& v) {
+    k.append(reinterpret_cast<const char*>(&v), sizeof(T));
+}
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
@@ -598,6 +664,10 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_dU, NF * 3);
     ALLOC(ctx->d_pcg, ctx->F);
     ALLOC(ctx->d_pol, ctx->N);
+    ALLOC(ctx->d_thole, ctx->N);
+    ALLOC(ctx->d_Qloc, (size_t)ctx->N * ctx->nh);
+    ALLOC(ctx->d_eout, ctx->F);
+    ALLOC(ctx->d_bout, (size_t)ctx->F * 9);
     ALLOC(ctx->d_coef, ctx->F * 4);
     ALLOC(ctx->d_active, ctx->F);
     ALLOC(ctx->d_maxcnt, 1);
@@ -1458,7 +1528,7 @@ static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x
 template <class RC, class R>
 static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double* pol, double f1, double tol,
                        int maxiter, std::vector<int>& it_out, std::vector<double>& res_out, bool fp32 = false,
-                       bool cold = false) {
+                       bool cold = false, bool started = false) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     // optional first phase with FP32 reciprocal products (fp32 on a mixed context)
@@ -1466,7 +1536,8 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
     int fp32_iters = 0;
     std::vector<int> it_base(F, 0);
     admp_status s;
-    if ((s = pcg_restart<RC>(ctx, rhs, x, pol, f1, tol, 1, dbl, cold)) != ADMP_OK) return s;
+    // (started: the caller already ran this first restart, in its set-up graph)
+    if (!started && (s = pcg_restart<RC>(ctx, rhs, x, pol, f1, tol, 1, dbl, cold)) != ADMP_OK) return s;
     const int chunk = 4;  // FP32-phase chunk (host reads the state after each)
     std::vector<PcgState> hs(F);
     int done_iters = 0;
@@ -1721,8 +1792,25 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     admp_status s;
     // right-hand side and the stored real-space dipole coupling in FP64 on both
     // paths: the near-field coupling is what makes close contacts ill-conditioned,
-    // and FP32 rounding of it changes the solution beyond the residual tolerance
-    if ((s = solver_rhs<double, double>(ctx, pol, thole, pp)) != ADMP_OK) return s;
+    // and FP32 rounding of it changes the solution beyond the residual tolerance.
+    // A cold FP64-product CG also starts here (residual, preconditioner, first
+    // direction): the set-up is one captured graph from its second use on.
+    const bool fp32 = mixed && o && o->fp32_products;
+    const bool started = kind == ADMP_SOLVER_PCG && !fp32 && !warm;
+    if (started) {
+        std::string key;
+        key_add(key, pp);
+        key_add(key, tol);
+        key_add(key, ctx->pc_maxm);
+        s = run_phase(ctx, ctx->g_setup, key, [&]() -> admp_status {
+            const admp_status s2 = solver_rhs<double, double>(ctx, pol, thole, pp);
+            if (s2 != ADMP_OK) return s2;
+            return pcg_restart<double>(ctx, ctx->d_b, U, pol, f1, tol, 1, true, true);
+        });
+        if (s != ADMP_OK) return s;
+    } else if ((s = solver_rhs<double, double>(ctx, pol, thole, pp)) != ADMP_OK) {
+        return s;
+    }
     info.n_iter.assign(F, 0);
     info.conv.assign(F, 0);
     info.res.assign(F, 0.0);
@@ -1736,7 +1824,8 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     if (kind == ADMP_SOLVER_PCG && !mixed) {
         std::vector<int> it;
         std::vector<double> rr;
-        if ((s = pcg<double, double>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr, false, !warm)) != ADMP_OK)
+        if ((s = pcg<double, double>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr, false, !warm, started)) !=
+            ADMP_OK)
             return s;
         for (int f = 0; f < F; ++f) {
             info.n_iter[f] = it[f];
@@ -1750,8 +1839,8 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
         // reciprocal products in FP32 with FP64 true residuals (reliable updates)
         std::vector<int> it;
         std::vector<double> rr;
-        const bool fp32 = o && o->fp32_products;
-        if ((s = pcg<double, R>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr, fp32, !warm)) != ADMP_OK) return s;
+        if ((s = pcg<double, R>(ctx, ctx->d_b, U, pol, f1, tol, maxiter, it, rr, fp32, !warm, started)) != ADMP_OK)
+            return s;
         for (int f = 0; f < F; ++f) info.n_iter[f] = it[f];
         if (!fp32) {
             for (int f = 0; f < F; ++f) {
@@ -1844,6 +1933,8 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         if (U_inout) HIPCHK(hipMemcpyAsync(U_inout, U, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     }
     // ---------------- final pass: energy and gradients with U fixed ----------------
+    // (energy and dE/dbox land in context buffers; one launch copies every output)
+    auto final_body = [&]() -> admp_status {
     hipLaunchKernelGGL(k_pack_site<R>, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->d_Qc, pol, thole, U, site);
     // the real-space pass runs on the side stream beside the reciprocal one (their
     // outputs are separate buffers; k_self, which adds to both, comes after the join)
@@ -1865,12 +1956,13 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         HIPCHK(hipMemsetAsync(ctx->d_gpol, 0, NF * sizeof(double), sr));
         HIPCHK(hipMemsetAsync(ctx->d_gth, 0, NF * sizeof(double), sr));
     }
+    admp_status s2 = ADMP_OK;
     if (lpme) {
         HIPCHK(hipEventRecord(ctx->ev_join, sr));
-        s = recip_run<R>(ctx, ctx->eg, 0, ctx->d_Qc, U, lpol ? 3 : 1, lmax_eff, 0, DIELECTRIC, kappa, 0, nullptr,
-                         ctx->d_gq_rec, ctx->d_gpos_rec, ctx->d_vir_rec, true);
+        s2 = recip_run<R>(ctx, ctx->eg, 0, ctx->d_Qc, U, lpol ? 3 : 1, lmax_eff, 0, DIELECTRIC, kappa, 0, nullptr,
+                          ctx->d_gq_rec, ctx->d_gpos_rec, ctx->d_vir_rec, true);
         HIPCHK(hipStreamWaitEvent(st, ctx->ev_join, 0));
-        if (s != ADMP_OK) return s;
+        if (s2 != ADMP_OK) return s2;
     } else {
         HIPCHK(hipMemsetAsync(ctx->d_gq_rec, 0, NF * 10 * sizeof(double), st));
         HIPCHK(hipMemsetAsync(ctx->d_gpos_rec, 0, NF * 3 * sizeof(double), st));
@@ -1885,8 +1977,7 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     hipLaunchKernelGGL(k_reduce3, dim3(3, F), dim3(RED_BLOCK), 0, st, N,
                        Reduce3{{ctx->d_e_atom, ctx->d_e_self, ctx->d_e_pol}}, ctx->d_red);
     // total energy per frame on the device (real + self + polarisation + reciprocal)
-    if (energy)
-        hipLaunchKernelGGL(k_energy_total, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_red, ctx->d_rec, energy);
+    hipLaunchKernelGGL(k_energy_total, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_red, ctx->d_rec, ctx->d_eout);
     if (want_grad) {
         // dE/dQ_global total -> frames backward
         { ProfScope ps_(ctx, 9); hipLaunchKernelGGL(k_frames_bwd, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->nh, ctx->use_frames, ctx->d_pos,
@@ -1901,18 +1992,36 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
         hipLaunchKernelGGL(k_reduce_vir, dim3(9, F, 3), dim3(RED_BLOCK), 0, st, N,
                            Vir3{{ctx->d_vir_atom, ctx->d_vir_fr, ctx->d_vir_rec},
                                 {ctx->d_red9a, ctx->d_red9c, ctx->d_red9b}});
-        if (dbox)
-            hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9a,
-                               ctx->d_red9c, lpme ? ctx->d_red9b : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr,
-                               c.K1, c.K2, c.K3, 0, dbox, 0);
-        // the four gradient outputs in one launch
+        hipLaunchKernelGGL(k_box_grad, dim3(nblk(F, 64)), dim3(64), 0, st, F, ctx->d_box, ctx->d_red9a,
+                           ctx->d_red9c, lpme ? ctx->d_red9b : nullptr, (const double*)nullptr, lpme ? ctx->d_rec : nullptr,
+                           c.K1, c.K2, c.K3, 0, ctx->d_bout, 0);
+    }
+    HIPCHK(hipGetLastError());
+    return ADMP_OK;
+    };
+    if (lpol && lpme) {
+        // graph key: every value the captured launches take by value
+        std::string key;
+        key_add(key, want_grad);
+        key_add(key, pp);
+        if ((s = run_phase(ctx, ctx->g_final, key, final_body)) != ADMP_OK) return s;
+    } else if ((s = final_body()) != ADMP_OK) {
+        return s;
+    }
+    {
+        // every requested output in one launch
         CopyJobs cj{};
-        cj.src[0] = ctx->d_gpos; cj.dst[0] = dpos; cj.n[0] = dpos ? NF * 3 : 0;
-        cj.src[1] = ctx->d_gQl; cj.dst[1] = dQ_local; cj.n[1] = dQ_local ? NF * ctx->nh : 0;
-        cj.src[2] = ctx->d_gpol; cj.dst[2] = dpol; cj.n[2] = dpol ? NF : 0;
-        cj.src[3] = ctx->d_gth; cj.dst[3] = dthole; cj.n[3] = dthole ? NF : 0;
-        const size_t nmax = std::max(std::max(cj.n[0], cj.n[1]), std::max(cj.n[2], cj.n[3]));
-        if (nmax) hipLaunchKernelGGL(k_copy4, dim3(nblk(nmax, 256), 4), dim3(256), 0, st, cj);
+        cj.src[0] = ctx->d_eout; cj.dst[0] = energy; cj.n[0] = energy ? F : 0;
+        if (want_grad) {
+            cj.src[1] = ctx->d_gpos; cj.dst[1] = dpos; cj.n[1] = dpos ? NF * 3 : 0;
+            cj.src[2] = ctx->d_gQl; cj.dst[2] = dQ_local; cj.n[2] = dQ_local ? NF * ctx->nh : 0;
+            cj.src[3] = ctx->d_gpol; cj.dst[3] = dpol; cj.n[3] = dpol ? NF : 0;
+            cj.src[4] = ctx->d_gth; cj.dst[4] = dthole; cj.n[4] = dthole ? NF : 0;
+            cj.src[5] = ctx->d_bout; cj.dst[5] = dbox; cj.n[5] = dbox ? (size_t)F * 9 : 0;
+        }
+        size_t nmax = 0;
+        for (int k = 0; k < 6; ++k) nmax = std::max(nmax, cj.n[k]);
+        if (nmax) hipLaunchKernelGGL(k_copy4, dim3(nblk(nmax, 256), 6), dim3(256), 0, st, cj);
         HIPCHK(hipGetLastError());
     }
     // Without stats the call returns once everything is queued, provided the
@@ -1972,9 +2081,20 @@ static admp_status admp_compute_impl(admp_ctx* ctx, const double* pos, const dou
         return ADMP_ERR_STATE;
     }
     if (ctx->cfg.lpol && (!pol || !thole)) { ctx->err = "pol and thole required"; return ADMP_ERR_ARG; }
-    if (ctx->cfg.lpol) {
-        HIPCHK(hipMemcpyAsync(ctx->d_pol, pol, ctx->N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        pol = ctx->d_pol;
+    {
+        // the caller's per-atom inputs into context buffers (stable arguments of
+        // the captured phases), in one launch
+        CopyJobs cj{};
+        cj.src[0] = Q_local; cj.dst[0] = ctx->d_Qloc; cj.n[0] = (size_t)ctx->N * ctx->nh;
+        if (ctx->cfg.lpol) {
+            cj.src[1] = pol; cj.dst[1] = ctx->d_pol; cj.n[1] = ctx->N;
+            cj.src[2] = thole; cj.dst[2] = ctx->d_thole; cj.n[2] = ctx->N;
+            pol = ctx->d_pol;
+            thole = ctx->d_thole;
+        }
+        hipLaunchKernelGGL(k_copy4, dim3(nblk(cj.n[0], 256), ctx->cfg.lpol ? 3 : 1), dim3(256), 0, ctx->stream, cj);
+        HIPCHK(hipGetLastError());
+        Q_local = ctx->d_Qloc;
     }
     if (n_scales < 1 || n_scales > 8) { ctx->err = "n_scales must be 1..8"; return ADMP_ERR_ARG; }
     admp_status s = upload_geometry(ctx, pos, box);

@@ -1098,11 +1098,11 @@ __global__ void k_energy_total(int F, const double* __restrict__ red, const doub
     if (f < F) e[f] = red[f] + red[F + f] + red[2 * F + f] + rec[7 * f];
 }
 
-// up to four device-to-device copies in one launch (grid.y selects the copy)
+// up to six device-to-device copies in one launch (grid.y selects the copy)
 struct CopyJobs {
-    const double* src[4];
-    double* dst[4];
-    size_t n[4];
+    const double* src[6];
+    double* dst[6];
+    size_t n[6];
 };
 __global__ void __launch_bounds__(256) k_copy4(CopyJobs j) {
     const int c = blockIdx.y;

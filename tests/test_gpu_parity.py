@@ -414,7 +414,10 @@ def test_small_box_diagonal_preconditioner_vs_oracle():
 def test_compute_without_stats_same_results_and_checks():
     """admp_compute without solver statistics (the form that need not wait for
     the device at its end): bit-identical energy, gradients and U to the call with
-    them, and a non-finite position is still an argument error."""
+    them, and a non-finite position is still an argument error.  The first call
+    of a context runs its set-up and final pass eagerly, the second captures them
+    as graphs and later calls replay those: every call is compared with the
+    first, with host synchronisations and other work between them."""
     import dmff_amd._lib as L
     from dmff_amd.admp.engine import Engine
     s, pos, box = _small_waters()
@@ -439,6 +442,14 @@ def test_compute_without_stats_same_results_and_checks():
         eng.compute(bad, box, *args, stats=False, **kw)
     E3, _, _, _ = eng.compute(pos, box, *args, stats=False, **kw)
     assert torch.equal(E1, E3)
+    for k in range(4):
+        torch.cuda.synchronize()
+        _ = torch.randn(1000, device=DEV).sum().item()   # other work between replays
+        Ek, gk, Uk, _ = eng.compute(pos, box, *args, stats=bool(k % 2), **kw)
+        torch.cuda.synchronize()
+        assert torch.equal(E1, Ek) and torch.equal(U1, Uk), k
+        for name in g1:
+            assert torch.equal(g1[name], gk[name]), (k, name)
 
 
 @pytest.mark.parametrize("fp32_products", [False, True])
