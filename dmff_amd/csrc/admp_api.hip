@@ -1488,10 +1488,11 @@ static void launch_update(admp_ctx* ctx, int cur, double* x, double tol, const d
 // iterations use FP64 products.  The next update has parity 0.
 template <class RC>
 static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x, const double* pol, double f1,
-                               double tol, int mode, bool dbl, bool x_zero = false) {
+                               double tol, int mode, bool dbl, bool x_zero = false, double* set_x_zero = nullptr) {
     const int N = ctx->N, F = ctx->F;
     hipStream_t st = ctx->stream;
     admp_status s;
+    double* xz = set_x_zero;  // the start kernel zeroes x (a cold start whose x was not cleared)
     // from x = 0 the residual is the right-hand side: no product
     const double* Ax = x_zero ? nullptr : ctx->d_res;
     if (!x_zero && (s = matvec<RC, double>(ctx, x, ctx->d_res, pol, f1)) != ADMP_OK) return s;
@@ -1502,13 +1503,13 @@ static admp_status pcg_restart(admp_ctx* ctx, const double* rhs, const double* x
                            rhs, Ax, ctx->d_r, ctx->d_z, (double*)nullptr,
                            reinterpret_cast<double*>(ctx->d_swfrac),
                            ctx->cfg.lpme ? reinterpret_cast<double*>(ctx->d_sfq) : nullptr, pc_j(ctx), ctx->d_pcd,
-                           cg_parts(ctx));
+                           cg_parts(ctx), xz);
     else
         hipLaunchKernelGGL(k_cg_start<float>, dim3(ctx->pcg_nb, F), dim3(PCG_BLOCK), 0, st, N, ctx->d_box, wd, pol,
                            rhs, Ax, ctx->d_r, ctx->d_z, reinterpret_cast<float*>(ctx->d_vR),
                            reinterpret_cast<float*>(ctx->d_swfrac),
                            ctx->cfg.lpme ? reinterpret_cast<float*>(ctx->d_sfq) : nullptr, pc_j(ctx), ctx->d_pcd,
-                           cg_parts(ctx));
+                           cg_parts(ctx), xz);
     hipLaunchKernelGGL(k_cg_publish, dim3(F), dim3(PCG_BLOCK), 0, st, mode, tol, cg_parts(ctx), ctx->d_pcg);
     HIPCHK(hipGetLastError());
     return ADMP_OK;
@@ -1765,6 +1766,16 @@ static admp_status solver_rhs(admp_ctx* ctx, const double* pol, const double* th
     return ADMP_OK;
 }
 
+// local frames and global multipoles of the context's copy of Q_local
+static admp_status launch_frames_fwd(admp_ctx* ctx) {
+    const int N = ctx->N, F = ctx->F;
+    { ProfScope ps_(ctx, 9); hipLaunchKernelGGL(k_frames_fwd, dim3(nblk(N, 128), F), dim3(128), 0, ctx->stream, N, ctx->nh,
+                       ctx->use_frames, ctx->d_pos, ctx->d_box, ctx->d_axis_type, ctx->d_axis_idx, ctx->d_Qloc,
+                       ctx->d_frames, ctx->d_Qc); }
+    HIPCHK(hipGetLastError());
+    return ADMP_OK;
+}
+
 // Induced-dipole solve: U (ctx->d_U) from A U = b.
 template <class R>
 static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, const PairParams& pp,
@@ -1785,9 +1796,11 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
         ctx->pc_maxm = pk == ADMP_PRECOND_DIAGONAL ? 1 : pk == ADMP_PRECOND_PAIRS ? 2 : PC_MAXM;
     }
     const bool warm = o && o->warm_start && U_in;
+    const bool fp32 = mixed && o && o->fp32_products;
+    const bool started = kind == ADMP_SOLVER_PCG && !fp32 && !warm;   // see below
     if (warm)
         HIPCHK(hipMemcpyAsync(U, U_in, NF * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    else
+    else if (!started)
         HIPCHK(hipMemsetAsync(U, 0, NF * 3 * sizeof(double), st));
     admp_status s;
     // right-hand side and the stored real-space dipole coupling in FP64 on both
@@ -1795,21 +1808,21 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
     // and FP32 rounding of it changes the solution beyond the residual tolerance.
     // A cold FP64-product CG also starts here (residual, preconditioner, first
     // direction): the set-up is one captured graph from its second use on.
-    const bool fp32 = mixed && o && o->fp32_products;
-    const bool started = kind == ADMP_SOLVER_PCG && !fp32 && !warm;
     if (started) {
         std::string key;
         key_add(key, pp);
         key_add(key, tol);
         key_add(key, ctx->pc_maxm);
         s = run_phase(ctx, ctx->g_setup, key, [&]() -> admp_status {
-            const admp_status s2 = solver_rhs<double, double>(ctx, pol, thole, pp);
+            admp_status s2 = launch_frames_fwd(ctx);
             if (s2 != ADMP_OK) return s2;
-            return pcg_restart<double>(ctx, ctx->d_b, U, pol, f1, tol, 1, true, true);
+            if ((s2 = solver_rhs<double, double>(ctx, pol, thole, pp)) != ADMP_OK) return s2;
+            return pcg_restart<double>(ctx, ctx->d_b, U, pol, f1, tol, 1, true, true, U);
         });
         if (s != ADMP_OK) return s;
-    } else if ((s = solver_rhs<double, double>(ctx, pol, thole, pp)) != ADMP_OK) {
-        return s;
+    } else {
+        if ((s = launch_frames_fwd(ctx)) != ADMP_OK) return s;
+        if ((s = solver_rhs<double, double>(ctx, pol, thole, pp)) != ADMP_OK) return s;
     }
     info.n_iter.assign(F, 0);
     info.conv.assign(F, 0);
@@ -1921,9 +1934,8 @@ static admp_status compute_impl(admp_ctx* ctx, const double* Q_local, const doub
     const int lmax_eff = (lpol && c.lmax == 0) ? 1 : c.lmax;
     R* site = reinterpret_cast<R*>(ctx->d_site);
     const dim3 pgrid(nblk((size_t)N * TPA, PAIR_BLOCK), F);
-    { ProfScope ps_(ctx, 9); hipLaunchKernelGGL(k_frames_fwd, dim3(nblk(N, 128), F), dim3(128), 0, st, N, ctx->nh, ctx->use_frames, ctx->d_pos,
-                       ctx->d_box, ctx->d_axis_type, ctx->d_axis_idx, Q_local, ctx->d_frames, ctx->d_Qc); }
-    HIPCHK(hipGetLastError());
+    // local frames and global multipoles (inside the solver's set-up when polarisable)
+    if (!lpol) { const admp_status sf = launch_frames_fwd(ctx); if (sf != ADMP_OK) return sf; }
     SolveInfo info;
     admp_status s;
     const double* U = nullptr;

@@ -860,6 +860,7 @@ __device__ inline void pc_apply(const int* pj, const double* __restrict__ pd, do
 // r = b - Ax (Ax null: x = 0, r = b), z = M^-1 r and the product inputs derived from z (with the spline
 // coordinates wfrac of the solve's precision R); r into buffer 0 of its two,
 // partials into buffers 0.  pcj/pcd: block preconditioner (pcj null: diagonal).
+// xzero (may be null): the iterate, set to zero here (a cold start).
 template <class R>
 __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __restrict__ boxes,
                                                         const double* __restrict__ wd, const double* __restrict__ pol,
@@ -867,7 +868,7 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __
                                                         double* __restrict__ r, double* __restrict__ z,
                                                         R* __restrict__ zR, R* __restrict__ wfrac, R* __restrict__ fq,
                                                         const int* __restrict__ pcj, const double* __restrict__ pcd,
-                                                        CgParts parts) {
+                                                        CgParts parts, double* __restrict__ xzero) {
     __shared__ double sh[16];
     const int f = blockIdx.y;
     const int i = blockIdx.x * PCG_BLOCK + threadIdx.x;
@@ -875,6 +876,8 @@ __global__ void __launch_bounds__(PCG_BLOCK) k_cg_start(int N, const BoxInfo* __
     if (i < N) {
         const size_t a = (size_t)f * N + i;
         double rv[3], rj[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, u[3];
+        if (xzero)
+            for (int k = 0; k < 3; ++k) xzero[a * 3 + k] = 0.0;
         for (int k = 0; k < 3; ++k) {
             rv[k] = Ax ? b[a * 3 + k] - Ax[a * 3 + k] : b[a * 3 + k];
             r[a * 3 + k] = rv[k];
