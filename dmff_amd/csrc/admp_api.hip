@@ -135,12 +135,12 @@ struct admp_ctx {
     // captured PCG iteration chunks (replayed until the frames converge)
     struct ChunkGraph {
         hipGraphExec_t exec = nullptr;
-        int type = -1, nrun = 0;
+        int type = -1, nrun = 0, par = 0;
         const void *rhs = nullptr, *x = nullptr;
         double tol = 0, f1 = 0;
         bool pc = true;
     };
-    ChunkGraph graphs[4];
+    ChunkGraph graphs[12];
     // the solver set-up (right-hand side, preconditioner, CG start) and the
     // final pass, captured on their second use with the same arguments
     struct PhaseGraph {
@@ -166,11 +166,15 @@ static const char* PROF_NAMES[16] = {"real_final", "real_permfield", "real_matve
                                      "fft", "influence", "gather", "nbr_build", "frames", "pair_short",
                                      "solver_vec", "solver_graph", "plane_matvec", "", ""};
 
-static void clear_graphs(admp_ctx* ctx) {
+static void clear_chunk_graphs(admp_ctx* ctx) {
     for (auto& g : ctx->graphs) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = admp_ctx::ChunkGraph();
     }
+}
+
+static void clear_graphs(admp_ctx* ctx) {
+    clear_chunk_graphs(ctx);
     for (admp_ctx::PhaseGraph* g : {&ctx->g_setup, &ctx->g_final}) {
         if (g->exec) (void)hipGraphExecDestroy(g->exec);
         *g = admp_ctx::PhaseGraph();
@@ -1555,10 +1559,12 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
     };
     // launch nrun iterations (FP64 or FP32 reciprocal products), eagerly or as a
     // captured HIP graph replayed on later calls
-    auto launch_chunk = [&](int nrun, bool use_dbl) -> admp_status {
+    // (par: buffer parity of the chunk's first iteration; r and s alternate
+    // between two buffers, and a restart leaves its state in parity 0)
+    auto launch_chunk = [&](int nrun, bool use_dbl, int par) -> admp_status {
         auto iterate = [&]() -> admp_status {
             for (int c = 0; c < nrun; ++c) {
-                const int cur = c & 1;  // chunks have even length (but a last one cut by maxiter)
+                const int cur = (par + c) & 1;
                 admp_status s2;
                 if (use_dbl) s2 = pcg_apply<RC, double>(ctx, pol, f1, cur, tol);
                 else s2 = pcg_apply<RC, float>(ctx, pol, f1, cur, tol);
@@ -1574,14 +1580,14 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         const int type = (sizeof(RC) == 8 ? 1 : 0) + (use_dbl ? 2 : 0);
         admp_ctx::ChunkGraph* cg = nullptr;
         for (auto& g : ctx->graphs)
-            if (g.exec && g.type == type && g.nrun == nrun && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1 &&
+            if (g.exec && g.type == type && g.nrun == nrun && g.par == par && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1 &&
                 g.pc == (ctx->pc_maxm > 1))
                 cg = &g;
         if (!cg) {
             admp_ctx::ChunkGraph* slot = nullptr;
             for (auto& g : ctx->graphs)
                 if (!g.exec) { slot = &g; break; }
-            if (!slot) { clear_graphs(ctx); slot = &ctx->graphs[0]; }
+            if (!slot) { clear_chunk_graphs(ctx); slot = &ctx->graphs[0]; }
             const bool prof = ctx->prof;
             ctx->prof = false;
             hipGraph_t graph = nullptr;
@@ -1606,7 +1612,7 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
                 ctx->err.clear();
                 return iterate();
             }
-            slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->rhs = rhs; slot->x = x;
+            slot->exec = exec; slot->type = type; slot->nrun = nrun; slot->par = par; slot->rhs = rhs; slot->x = x;
             slot->tol = tol; slot->f1 = f1; slot->pc = ctx->pc_maxm > 1;
             cg = slot;
         }
@@ -1643,7 +1649,8 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         if (dbl) break;  // FP64 products from here on: pipelined loop below
         const int nrun = std::min(chunk, maxiter - done_iters);
         fp32_iters += nrun;
-        if ((s = launch_chunk(nrun, false)) != ADMP_OK) return s;
+        // (FP32 chunks have even length but a last one cut by maxiter: parity 0)
+        if ((s = launch_chunk(nrun, false, 0)) != ADMP_OK) return s;
         done_iters += nrun;
         if ((s = read_state(worst, all)) != ADMP_OK) return s;
     }
@@ -1653,20 +1660,23 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         // chunk boundaries.  A chunk launched after convergence costs little: the
         // product kernels return at once for converged frames.
         const int ch = 4;
-        int q[2], nq = 0, last_slot = 1;
+        int q[2], qn[2], nq = 0, last_slot = 1, par = 0;  // (qn: iterations of each chunk in flight)
         auto post = [&](int sl) -> admp_status {
             HIPCHK(hipMemcpyAsync(ctx->h_pcg + (size_t)sl * F, ctx->d_pcg, F * sizeof(PcgState),
                                   hipMemcpyDeviceToHost, st));
             HIPCHK(hipEventRecord(ctx->ev_state[sl], st));
             return ADMP_OK;
         };
-        auto launch_next = [&]() -> admp_status {
-            const int nrun = std::min(ch, maxiter - done_iters);
+        // nrun: iterations of the next chunk (at most ch)
+        auto launch_next = [&](int want = 1 << 30) -> admp_status {
+            const int nrun = std::min(std::min(ch, want), maxiter - done_iters);
             admp_status s2;
-            if ((s2 = launch_chunk(nrun, true)) != ADMP_OK) return s2;
+            if ((s2 = launch_chunk(nrun, true, par)) != ADMP_OK) return s2;
             done_iters += nrun;
+            par = (par + nrun) & 1;
             last_slot ^= 1;
             if ((s2 = post(last_slot)) != ADMP_OK) return s2;
+            qn[nq] = nrun;
             q[nq++] = last_slot;
             return ADMP_OK;
         };
@@ -1677,24 +1687,40 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         // prediction costs one host round trip before the next chunk starts.
         std::vector<PcgState> older = hs;
         bool have_older = false, have_hs = read_start;
-        auto finishes_in_flight = [&]() -> bool {
-            if (!have_older) return false;
+        // the most iterations any frame is predicted to need from the last state
+        // read (geometric extrapolation from its last two states); -1: no prediction
+        auto predicted_need = [&]() -> double {
+            if (!have_older) return -1.0;
+            double worst_need = 0.0;
             for (int f = 0; f < F; ++f) {
                 if (hs[f].done) continue;
                 const int di = hs[f].iter - older[f].iter;
-                if (di <= 0 || !(hs[f].res > 0.0) || !(hs[f].res < older[f].res)) return false;
+                if (di <= 0 || !(hs[f].res > 0.0) || !(hs[f].res < older[f].res)) return -1.0;
                 const double rate = std::log(hs[f].res / older[f].res) / di;   // per iteration, < 0
-                const double need = std::log(tol / hs[f].res) / rate;          // iterations still needed
-                if (!(need <= ch - 1.5)) return false;
+                worst_need = std::max(worst_need, std::log(tol / hs[f].res) / rate);
             }
-            return true;
+            return worst_need;
+        };
+        // (both with a margin of 1.5 iterations; called with one chunk in flight)
+        auto finishes_in_flight = [&]() -> bool {
+            const double need = predicted_need();
+            return need >= 0.0 && need <= qn[0] - 1.5;
+        };
+        // a chunk launched ahead of the one in flight is cut to the predicted
+        // remainder, so that the solve does not end in iterations that only
+        // return (each still costs its launches)
+        auto ahead_size = [&]() -> int {
+            const double need = predicted_need();
+            if (need < 0.0) return ch;
+            return std::max(1, std::min(ch, (int)std::ceil(need - qn[0] + 1.5)));
         };
         if ((s = launch_next()) != ADMP_OK) return s;
         while (nq > 0) {
             if (done_iters < maxiter && nq < 2 && !finishes_in_flight())
-                if ((s = launch_next()) != ADMP_OK) return s;
+                if ((s = launch_next(ahead_size())) != ADMP_OK) return s;
             const int sl = q[0];
             q[0] = q[1];
+            qn[0] = qn[1];
             --nq;
             HIPCHK(hipEventSynchronize(ctx->ev_state[sl]));
             older = hs;
