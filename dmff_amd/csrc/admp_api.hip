@@ -43,7 +43,7 @@ struct admp_ctx {
     hipStream_t user_stream = nullptr;  // caller's stream: ordered with events on entry and exit
     hipStream_t own = nullptr;          // the context's own stream (stream is own, or the caller's in caller mode)
     int caller_mode = 0;                // admp_set_stream_mode: work directly on the caller's stream
-    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_done = nullptr;
     hipStream_t side = nullptr;         // second context stream: real-space products beside the reciprocal chain
     PcgState* h_pcg = nullptr;          // pinned host copies of the PCG state (two slots of F)
     // neighbour build whose row-count check is still pending (resolved by nbr_settle)
@@ -445,12 +445,17 @@ extern "C" const char* admp_last_error(const admp_ctx* ctx) { return ctx ? ctx->
 // The active stream: the context's own one, or in caller mode the caller's
 // (when it is not the null stream, on which graphs cannot be captured).  ev_out
 // marks the end of the context's last work (recorded by every exit and at
-// creation), so a change of the active stream orders the new one after that
-// work only, not after unrelated work queued later on the old stream.
+// creation); in caller mode the own stream takes that end over at every exit
+// (ev_done).  A change of the active stream orders the new one after the
+// context's last work only, not after unrelated work queued later on the old
+// stream, and always through an event of the own stream: a wait on an event of a
+// caller's stream that another thread is capturing at that moment (a server's
+// worker, capturing the graphs of another context on its stream) would join that
+// capture or be refused.
 static admp_status apply_stream(admp_ctx* ctx) {
     hipStream_t want = (ctx->caller_mode && ctx->user_stream) ? ctx->user_stream : ctx->own;
     if (want != ctx->stream) {
-        HIPCHK(hipStreamWaitEvent(want, ctx->ev_out, 0));
+        HIPCHK(hipStreamWaitEvent(want, ctx->stream == ctx->own ? ctx->ev_out : ctx->ev_done, 0));
         ctx->stream = want;
     }
     return ADMP_OK;
@@ -479,6 +484,11 @@ static admp_status enter(admp_ctx* ctx) {
 // and the caller's stream continues after the context's work
 static admp_status leave(admp_ctx* ctx) {
     HIPCHK(hipEventRecord(ctx->ev_out, ctx->stream));
+    if (ctx->stream != ctx->own) {
+        // caller mode: the own stream takes the end of this work over
+        HIPCHK(hipStreamWaitEvent(ctx->own, ctx->ev_out, 0));
+        HIPCHK(hipEventRecord(ctx->ev_done, ctx->own));
+    }
     if (ctx->stream == ctx->user_stream) return ADMP_OK;
     HIPCHK(hipStreamWaitEvent(ctx->user_stream, ctx->ev_out, 0));
     return ADMP_OK;
@@ -506,12 +516,10 @@ extern "C" admp_status admp_profile_get(admp_ctx* ctx, int32_t key, double* tota
 
 extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     if (!ctx) return ADMP_OK;
-    // back on the own stream, after the context's last work only (never a wait on
-    // the caller's stream, which may carry other work by now)
-    if (ctx->own && ctx->stream != ctx->own) {
-        if (ctx->ev_out) (void)hipStreamWaitEvent(ctx->own, ctx->ev_out, 0);
-        ctx->stream = ctx->own;
-    }
+    // back on the own stream, which in caller mode took the end of the context's
+    // last work over at its exit (never a wait on the caller's stream, which may
+    // carry other work or be captured by another thread by now)
+    if (ctx->own && ctx->stream != ctx->own) ctx->stream = ctx->own;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& p : ctx->ev_pending) { ctx->ev_pool.push_back(p.second.first); ctx->ev_pool.push_back(p.second.second); }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -533,6 +541,7 @@ extern "C" admp_status admp_ctx_destroy(admp_ctx* ctx) {
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
     if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
+    if (ctx->ev_done) (void)hipEventDestroy(ctx->ev_done);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);   // never the caller's stream
     delete ctx;
     return ADMP_OK;
@@ -582,7 +591,8 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
         hipEventCreateWithFlags(&ctx->ev_geom, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_done, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "stream/event creation failed";
         return fail(ADMP_ERR_HIP);
     }
@@ -684,7 +694,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_spos, NF);
 #undef ALLOC
     // the zero fills of creation are the context's first work
-    if (hipEventRecord(ctx->ev_out, ctx->own) != hipSuccess) {
+    if (hipEventRecord(ctx->ev_out, ctx->own) != hipSuccess || hipEventRecord(ctx->ev_done, ctx->own) != hipSuccess) {
         ctx->err = "event record failed";
         return fail(ADMP_ERR_HIP);
     }
