@@ -128,8 +128,10 @@ admp_status admp_set_stream(admp_ctx* ctx, void* stream);
  * streams: HIP maps streams onto a few hardware queues, so hundreds of
  * contexts with streams of their own share queues and wait on each other's
  * unrelated work.  A change of stream is ordered after the context's last
- * work (an event recorded by every entry point); captured solver graphs are
- * replayed on whichever stream is active.  Default 0. */
+ * work (an event recorded by every entry point; in caller mode carried over to
+ * the context's own stream, so that the wait never touches a stream another
+ * thread may be capturing); captured solver graphs are replayed on whichever
+ * stream is active.  Default 0. */
 admp_status admp_set_stream_mode(admp_ctx* ctx, int32_t caller_mode);
 
 /* Local frames (axis_type (N,), axis_idx (N,3) [z,x,y], -1 = none; spatial.py:44-156)
