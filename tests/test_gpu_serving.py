@@ -122,3 +122,80 @@ def test_served_request_vs_oracle(precision):
           flush=True)
     te, tf = (1e-8, 1e-6) if precision == "fp64" else (2e-5, 1e-4)
     assert de < te and df < tf
+
+
+def test_caller_mode_contexts_handed_between_capturing_workers():
+    """Contexts in caller stream mode are handed between two worker threads, each
+    on a stream of its own, while the other worker captures graphs of another
+    context on its stream (set-up and final pass on the second use of a key,
+    solver chunks on their first): every call succeeds and every result equals
+    the same context's result for the same input computed from one thread (the
+    library's reductions have a fixed order; 1e-12 relative is written here).
+    Regression test of a change of stream that waited on an event of the other
+    worker's stream and was refused while that stream was being captured."""
+    import queue
+    import threading
+    from dmff_amd.admp.engine import Engine
+    from dmff_amd.admp.pme import setup_ewald_parameters
+    from dmff_amd.admp.serving import water_spec
+    F64 = torch.float64
+    systems = [water_spec(f"h{k}", n, 300 + k) for k, n in enumerate((64, 96, 128))]
+    engs = []
+    for spec, pos0, box in systems:
+        kappa, K1, K2, K3 = setup_ewald_parameters(spec.rc, spec.ethresh, np.asarray(box))
+        e = Engine(spec.n_atoms, spec.lmax, spec.lpol, True, kappa, (K1, K2, K3), spec.rc, n_frames=1,
+                   precision="fp64", device=DEV)
+        e.set_topology(spec.axis_types, spec.axis_idx, spec.exclusions)
+        e.set_stream_mode(True)
+        dev = {k: torch.as_tensor(np.asarray(v), dtype=F64, device=DEV)
+               for k, v in (("Q", spec.Q_local), ("pol", spec.pol), ("thole", spec.thole))}
+        engs.append((spec, e, dev, threading.Lock()))
+    rng = np.random.default_rng(11)
+    tasks = [(k, systems[k][1] + rng.normal(0.0, 0.01, systems[k][1].shape), systems[k][2])
+             for rep in range(10) for k in range(len(systems))]
+
+    def run(k, pos, box):
+        spec, e, dev, _ = engs[k]
+        p = torch.as_tensor(np.asarray(pos)[None], dtype=F64, device=DEV)
+        b = torch.as_tensor(np.asarray(box)[None], dtype=F64, device=DEV)
+        e.build_neighbors(p, b)
+        E, g, U, info = e.compute(p, b, dev["Q"], dev["pol"], dev["thole"], *spec.scales, solver="pcg", tol=1e-8,
+                                  maxiter=200, want_grad=True)
+        return float(E[0]), g["pos"][0].detach().cpu().numpy()
+
+    ref = [run(*t) for t in tasks]
+    torch.cuda.synchronize()
+    order = list(range(len(tasks)))
+    rng.shuffle(order)
+    work = queue.Queue()
+    for i in order:
+        work.put(i)
+    out, errors = {}, []
+
+    def worker():
+        s = torch.cuda.Stream(device=DEV)
+        while True:
+            try:
+                i = work.get_nowait()
+            except queue.Empty:
+                return
+            k = tasks[i][0]
+            try:
+                with engs[k][3], torch.cuda.device(DEV), torch.cuda.stream(s):
+                    out[i] = run(*tasks[i])
+                    s.synchronize()
+            except Exception as exc:  # noqa: BLE001 - reported below
+                errors.append(repr(exc))
+
+    threads = [threading.Thread(target=worker) for _ in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=300)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors[:3]
+    assert len(out) == len(tasks)
+    for i, (E0, g0) in enumerate(ref):
+        E1, g1 = out[i]
+        assert abs(E1 - E0) <= 1e-12 * abs(E0), (i, E0, E1)
+        assert np.max(np.abs(g1 - g0)) <= 1e-12 * np.max(np.abs(g0)), i
