@@ -287,6 +287,8 @@ template <class T> static void key_add(std::string& k, const T& v) {
 }
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+// blocks of the gather (one z.w partial of the CG epilogue each)
+static inline int gather_blocks(int N) { return (N + GATHER_SITES_PER_BLOCK - 1) / GATHER_SITES_PER_BLOCK; }
 
 template <class T> static hipError_t dalloc(admp_ctx* ctx, T** p, size_t n) {
     void* q = nullptr;
@@ -667,7 +669,7 @@ extern "C" admp_status admp_ctx_create(const admp_config* cfg, admp_ctx** out) {
     ALLOC(ctx->d_dpart, (size_t)ctx->dots_blocks * ctx->F * 3);
     ALLOC(ctx->d_dots, ctx->F * 3);
     ctx->pcg_nb = (int)nblk(ctx->N, PCG_BLOCK);
-    ctx->cg_nw = std::max(ctx->pcg_nb, (ctx->N + 31) / 32);
+    ctx->cg_nw = std::max(ctx->pcg_nb, gather_blocks(ctx->N));
     ALLOC(ctx->d_pparts, (size_t)ctx->F * (ctx->cg_nw + 4 * ctx->pcg_nb));
     ALLOC(ctx->d_s, NF * 6);
     ALLOC(ctx->d_pnear, NF); ALLOC(ctx->d_pcj, NF * 3); ALLOC(ctx->d_pcd, NF * 3 * PC_ROW);
@@ -1347,15 +1349,15 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     { ProfScope ps_(ctx, 7);
     const CgEpi none{};
     if (gather_mode == 2)
-        hipLaunchKernelGGL((k_gather_ml<R, 2>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,
+        hipLaunchKernelGGL((k_gather_ml<R, 2>), dim3(xcd_blocks(gather_blocks(N)), F), dim3(8 * GATHER_SITES_PER_BLOCK), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
                            vir, ctx->d_gcatoms, skip, *epi);
     else if (gather_mode == 1)
-        hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,
+        hipLaunchKernelGGL((k_gather_ml<R, 1>), dim3(xcd_blocks(gather_blocks(N)), F), dim3(8 * GATHER_SITES_PER_BLOCK), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
                            vir, ctx->d_gcatoms, skip, none);
     else
-        hipLaunchKernelGGL((k_gather_ml<R, 0>), dim3(xcd_blocks((N + 31) / 32), F), dim3(256), 0, st, N, K0, K1, K2,
+        hipLaunchKernelGGL((k_gather_ml<R, 0>), dim3(xcd_blocks(gather_blocks(N)), F), dim3(8 * GATHER_SITES_PER_BLOCK), 0, st, N, K0, K1, K2,
                            ctx->d_pos, ctx->d_box, use_dA, ctx->d_mbase, wfrac, fq, Qc, U, qmode, rg, out3, gq, gpos,
                            vir, ctx->d_gcatoms, skip, none); }
     HIPCHK(hipGetLastError());
@@ -1458,12 +1460,12 @@ static admp_status pcg_apply(admp_ctx* ctx, const double* pol, double f1, int cu
         const MatvecJob mv{ctx->d_z, ctx->d_t1};
         s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_z, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 2, ctx->d_Ap,
                          nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg, &epi, nullptr, &mv);
-        ctx->cg_nwu = (N + 31) / 32;
+        ctx->cg_nwu = gather_blocks(N);
     } else if (fused) {
         const CgEpi epi{ctx->d_t1, pol, f1, cg_parts(ctx)};
         s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_z, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 2, ctx->d_Ap,
                          nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg, &epi, ctx->ev_join);
-        ctx->cg_nwu = (N + 31) / 32;
+        ctx->cg_nwu = gather_blocks(N);
     } else if (ctx->cfg.lpme) {
         s = recip_run<R>(ctx, ctx->eg, 0, nullptr, ctx->d_z, 2, 1, 0, DIELECTRIC, ctx->cfg.kappa, 1, ctx->d_t2,
                          nullptr, nullptr, nullptr, false, ctx->d_swfrac, ctx->d_sfq, ctx->d_pcg);
