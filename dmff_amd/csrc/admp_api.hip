@@ -1858,6 +1858,9 @@ static admp_status solve(admp_ctx* ctx, const double* pol, const double* thole, 
             return pcg_restart<double>(ctx, ctx->d_b, U, pol, f1, tol, 1, true, true, U);
         });
         if (s != ADMP_OK) return s;
+        // a replayed set-up graph sorted the sites for the spread as well (the host
+        // state its capture left): later launches and captures must not sort again
+        if (c.lpme) ctx->prepped = &ctx->eg;
     } else {
         if ((s = launch_frames_fwd(ctx)) != ADMP_OK) return s;
         if ((s = solver_rhs<double, double>(ctx, pol, thole, pp)) != ADMP_OK) return s;
