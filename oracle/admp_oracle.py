@@ -1220,7 +1220,7 @@ class ADMPQeqForce:
 
 
 PINNING = {
-    "setup_ewald_parameters": "BASELINE table kappa/K values (tests/test_oracle_pins.py)",
+    "setup_ewald_parameters": "kappa/K of the 49.67 A and 99.30 A water boxes computed by hand from the pme.py:460-466 rule (tests/test_oracle_pins.py)",
     "pbc_shift": "reference test vectors tests/test_admp/test_sptial.py:43-68",
     "construct_local_frames": "reference test vectors test_sptial.py:70-142, test_noaxistype.py:12-49",
     "build_quasi_internal": "reference test vectors test_sptial.py:11-41",

@@ -88,7 +88,7 @@ def test_harmonic_rotation_equals_cartesian_rotation():
 
 
 def test_ewald_parameters_rule():
-    # BASELINE table: cfg1 L=49.67 A, rc 8 -> kappa 0.328533, K 50; cfg3 L=99.30, rc 10 -> 0.262826, 80
+    # by hand from pme.py:460-466: L=49.67 A, rc 8 -> kappa 0.328533, K 50; L=99.30, rc 10 -> 0.262826, 80
     k, K1, _, _ = O.setup_ewald_parameters(8.0, 5e-4, np.eye(3) * 49.67)
     assert abs(k - 0.328533) < 1e-6 and K1 == 50
     k, K1, _, _ = O.setup_ewald_parameters(10.0, 5e-4, np.eye(3) * 99.30)
