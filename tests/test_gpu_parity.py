@@ -452,7 +452,8 @@ def test_compute_without_stats_same_results_and_checks():
             assert torch.equal(g1[name], gk[name]), (k, name)
 
 
-def test_graph_replays_equal_eager_over_varied_positions():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_equal_eager_over_varied_positions(precision):
     """Twelve evaluations of displaced geometries through a context that captures
     and replays its set-up, final-pass and solver-chunk graphs (chunks of 1-4
     iterations, the last one cut to the predicted remainder, starting on either
@@ -465,7 +466,7 @@ def test_graph_replays_equal_eager_over_varied_positions():
     kappa, K1, K2, K3 = O.setup_ewald_parameters(6.0, 5e-4, box)
 
     def make():
-        e = Engine(len(pos), 2, True, True, kappa, (K1, K2, K3), 6.0, device=DEV)
+        e = Engine(len(pos), 2, True, True, kappa, (K1, K2, K3), 6.0, precision=precision, device=DEV)
         e.set_topology(s.axis_types, s.axis_idx, s.bonded)
         e.set_pairs(torch.as_tensor(pairs, device=DEV))
         return e
