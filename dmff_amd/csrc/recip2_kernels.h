@@ -82,6 +82,10 @@ __global__ void k_grid_cell(int N, int K0, int K1, int K2, const int4* __restric
 
 constexpr int BRX = 4, BRY = 4, BRZ = 4;   // 64 grid points per wavefront
 constexpr int SP_CHUNK = 32;               // sites staged in LDS per round
+// the column spread of a single frame stages 48 (two rounds for the ~72
+// candidates of a brick at liquid density): 1.5 % faster per evaluation at
+// 12,288 sites, while 64-frame batches run 6 % slower with it than with 32
+constexpr int SP_CHUNK_ONE_FRAME = 48;
 
 // ---------------------------------------------------------------------------
 // Per-brick candidate lists.  The candidates of a brick are the sites whose
@@ -373,11 +377,14 @@ k_spread_col(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R
 // bitwise reproducible.  Per candidate a lane reads 4 x-y and up to 12 z table
 // values for 4 points, where the one-point-per-lane spread reads 9 for one.
 // ---------------------------------------------------------------------------
-template <class R, int LM>
+// CH: sites staged in LDS per round (a multiple of four, so each candidate keeps
+// its quarter of the wavefront and the sums their order)
+template <class R, int LM, int CH = SP_CHUNK>
 __global__ void __launch_bounds__(64)
 k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R* __restrict__ fq,
             const int* __restrict__ bstart, const int2* __restrict__ blist, long cap, R* __restrict__ rgrid,
             const PcgState* __restrict__ skip, CgCheck chk) {
+    static_assert(CH % 4 == 0 && CH <= 64, "one staged site per lane, whole quarters");
     static_assert(BRX == 4 && BRY == 4 && BRZ == 4, "column spread assumes 4x4x4 bricks");
     if (skip && skip[blockIdx.y].done) return;  // frame already converged
     constexpr int NQ = LM == 0 ? 1 : (LM == 1 ? 4 : 10);
@@ -398,7 +405,7 @@ k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R*
 #pragma unroll
     for (int q = 0; q < NQ; ++q) nq[q] = 0;
     auto fetch = [&](int c0) {
-        if (t < min(SP_CHUNK, total - c0)) {
+        if (t < min(CH, total - c0)) {
             ne = blist[(size_t)f * cap + lo + c0 + t];
             const size_t a = (size_t)f * N + ne.x;
 #pragma unroll
@@ -422,16 +429,16 @@ k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R*
             return;
         }
     }
-    __shared__ int s_r[SP_CHUNK];
-    __shared__ R s_t[SP_CHUNK][3][NT][6];
-    __shared__ R s_q[SP_CHUNK][NQ];
+    __shared__ int s_r[CH];
+    __shared__ R s_t[CH][3][NT][6];
+    __shared__ R s_q[CH][NQ];
     const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
     const int g = t >> 4, c = t & 15;
     const int lx = c >> 2, ly = c & 3;
     const int px = bx * BRX + lx, py = by * BRY + ly, pz0 = bz * BRZ;
     R acc[4] = {0, 0, 0, 0};
-    for (int c0 = 0; c0 < total; c0 += SP_CHUNK) {
-        const int nchunk = min(SP_CHUNK, total - c0);
+    for (int c0 = 0; c0 < total; c0 += CH) {
+        const int nchunk = min(CH, total - c0);
         if (t < nchunk) {
             s_r[t] = ne.y;
 #pragma unroll
@@ -450,7 +457,7 @@ k_spread_cw(int N, int K0, int K1, int K2, const R* __restrict__ wfrac, const R*
             for (int q = 0; q < NQ; ++q) s_q[t][q] = nq[q];
         }
         __syncthreads();
-        if (c0 + SP_CHUNK < total) fetch(c0 + SP_CHUNK);
+        if (c0 + CH < total) fetch(c0 + CH);
         // the next candidate's packed offsets are read one step ahead, and the z
         // table is read at a clamped index for all four points at once (a point
         // outside the support takes a zero weight), so a candidate waits for LDS
