@@ -593,8 +593,8 @@ def pair_kernel_valu(us_mixed):
     return out or None
 
 
-DOMINANT_KERNEL_SOURCE = ("k_plane_matvec<double>: 19.5 % of the kernel time of the timed evaluations, ahead of "
-                          "k_spread_cw<double, 1> (17.2 %), profiles/admp_12288_kernel_stats.md")
+DOMINANT_KERNEL_SOURCE = ("k_plane_matvec<double>: 19.6 % of the kernel time of the timed evaluations, ahead of "
+                          "k_spread_cw<double, 1, 48> (16.0 %), profiles/admp_12288_kernel_stats.md")
 
 
 def roofline(name, tm, D, prec, frames=1, prof=None):
