@@ -141,6 +141,8 @@ struct admp_ctx {
         bool pc = true;
     };
     ChunkGraph graphs[12];
+    ChunkGraph chunk_seen[16];   // chunk variants met once (captured on their second use)
+    int chunk_seen_next = 0;
     // the solver set-up (right-hand side, preconditioner, CG start) and the
     // final pass, captured on their second use with the same arguments
     struct PhaseGraph {
@@ -171,6 +173,7 @@ static void clear_chunk_graphs(admp_ctx* ctx) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = admp_ctx::ChunkGraph();
     }
+    for (auto& g : ctx->chunk_seen) g = admp_ctx::ChunkGraph();
 }
 
 static void clear_graphs(admp_ctx* ctx) {
@@ -1594,10 +1597,27 @@ static admp_status pcg(admp_ctx* ctx, const double* rhs, double* x, const double
         if (!ctx->use_graphs) return iterate();
         const int type = (sizeof(RC) == 8 ? 1 : 0) + (use_dbl ? 2 : 0);
         admp_ctx::ChunkGraph* cg = nullptr;
+        auto same = [&](const admp_ctx::ChunkGraph& g) {
+            return g.type == type && g.nrun == nrun && g.par == par && g.rhs == rhs && g.x == x && g.tol == tol &&
+                   g.f1 == f1 && g.pc == (ctx->pc_maxm > 1);
+        };
         for (auto& g : ctx->graphs)
-            if (g.exec && g.type == type && g.nrun == nrun && g.par == par && g.rhs == rhs && g.x == x && g.tol == tol && g.f1 == f1 &&
-                g.pc == (ctx->pc_maxm > 1))
-                cg = &g;
+            if (g.exec && same(g)) cg = &g;
+        if (!cg) {
+            // captured on the second use of a variant: a chunk length a context meets
+            // once (a server's many short-lived contexts) runs eagerly instead of
+            // paying for a capture and an instantiation
+            bool seen = false;
+            for (auto& g : ctx->chunk_seen)
+                if (g.nrun > 0 && same(g)) seen = true;
+            if (!seen) {
+                admp_ctx::ChunkGraph& k = ctx->chunk_seen[ctx->chunk_seen_next++ % 16];
+                k = admp_ctx::ChunkGraph();
+                k.type = type; k.nrun = nrun; k.par = par; k.rhs = rhs; k.x = x; k.tol = tol; k.f1 = f1;
+                k.pc = ctx->pc_maxm > 1;
+                return iterate();
+            }
+        }
         if (!cg) {
             admp_ctx::ChunkGraph* slot = nullptr;
             for (auto& g : ctx->graphs)
