@@ -1263,7 +1263,7 @@ static admp_status recip_run(admp_ctx* ctx, const Grid& g, int use_dA, const dou
     if (variant == 2)                                                                                                \
         hipLaunchKernelGGL((k_spread_col<R, LMv>), dim3(nb4, F), dim3(64), 0, st, N, K0, K1, K2, wfrac, fq,           \
                            ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip);                                   \
-    else if (variant == 3 && F == 1)                                                                                 \
+    else if (variant == 3 && F == 1 && (LMv) <= 1)                                                                  \
         hipLaunchKernelGGL((k_spread_cw<R, LMv, SP_CHUNK_ONE_FRAME>), dim3(nb1, F), dim3(64), 0, st, N, K0, K1, K2,    \
                            wfrac, fq, ctx->d_bstart, ctx->d_blist, ctx->blist_cap, rg, skip, ctx->spread_chk);       \
     else if (variant == 3)                                                                                           \

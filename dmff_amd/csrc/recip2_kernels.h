@@ -82,9 +82,10 @@ __global__ void k_grid_cell(int N, int K0, int K1, int K2, const int4* __restric
 
 constexpr int BRX = 4, BRY = 4, BRZ = 4;   // 64 grid points per wavefront
 constexpr int SP_CHUNK = 32;               // sites staged in LDS per round
-// the column spread of a single frame stages 48 (two rounds for the ~72
-// candidates of a brick at liquid density): 1.5 % faster per evaluation at
-// 12,288 sites, while 64-frame batches run 6 % slower with it than with 32
+// the solver's column spread of a single frame (dipoles only) stages 48 (two
+// rounds for the ~72 candidates of a brick at liquid density): 1.5 % faster per
+// evaluation at 12,288 sites, while 64-frame batches run 6 % slower with it and
+// the quadrupole spreads (three times the tables per site) 4 us slower
 constexpr int SP_CHUNK_ONE_FRAME = 48;
 
 // ---------------------------------------------------------------------------
